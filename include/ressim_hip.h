@@ -1,0 +1,119 @@
+/* ressim_hip.h - C-ABI of the MI355X (gfx950) linear-solve library for the black-oil Newton loop.
+ *
+ * This library replaces the device side of ressim::GpuBlockSolver (engine/include/ressim/bsr.h:143-151): the
+ * engine's Newton loop (engine/src/simulator.cpp:3358-3378) hands every Schur-reduced BSR system to
+ *     bool GpuBlockSolver::solve(const BSR& A, const std::vector<double>& b, std::vector<double>& x,
+ *                                double tol, int maxIt, int& iterations)
+ * registered through setDistributedSolver (FP64, bsr.h:154-157) or setGpuBlockSolver (float inner solve inside
+ * FP64 refinement, bsr.h:152-153).  The C++ adapter in include/ressim_hip_solver.hpp implements that virtual
+ * interface on top of the entry points below; INTEGRATION.md shows the registration.
+ *
+ * Conventions (mirroring the reference C API, engine/include/ressim/capi.h:5-7): opaque handles, int status
+ * (>= 0 success, < 0 error), a thread-local message from rsh_last_error().  Plain pointers and sizes only.
+ * Matrices are BSR: n block rows, nb x nb row-major blocks (row = equation, column = unknown, column 0 =
+ * pressure), int32 row_ptr[n+1], col[nnzb] sorted per row with the diagonal present, double val[nnzb*nb*nb].
+ * Vectors are n*nb doubles, cell-major (the layout of ressim::BSR::spmv, bsr.cpp:109-126).
+ *
+ * Threading: every context owns one HIP stream; different contexts may be used from different threads at the
+ * same time (the reference hook is global and called from concurrent Simulators, study.cpp:345-361).  One
+ * context must not be used by two threads at once.
+ */
+#ifndef RESSIM_HIP_H
+#define RESSIM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSH_OK 0
+#define RSH_ERR_ARG (-1)          /* invalid argument (sizes, pattern, options) */
+#define RSH_ERR_DEVICE (-2)       /* HIP runtime failure or no device */
+#define RSH_ERR_NOT_CONVERGED (-3)/* the Krylov method stopped before reaching tol (x holds the last iterate) */
+#define RSH_ERR_BREAKDOWN (-4)    /* Krylov breakdown (rho or r^T v below 1e-300) */
+#define RSH_ERR_NONFINITE (-5)    /* non-finite solution: the caller falls back to its CPU path */
+#define RSH_ERR_UNSUPPORTED (-6)  /* block size or option not built */
+
+/* preconditioners (bsr.h:50-250) */
+#define RSH_PREC_BILU0 1          /* natural-order block ILU(0), level-scheduled sweeps (BILU0, bsr.cpp:347-596) */
+#define RSH_PREC_MCBILU0 2        /* multicolour block ILU(0) (MCBILU0, bsr.cpp:599-730) */
+#define RSH_PREC_CPR 3            /* BlockCPR: quasi-IMPES weights, AMG V-cycle on the pressure system, BILU0 */
+#define RSH_PREC_CPR_MC 4         /* BlockCPR with the multicolour BILU0 second stage */
+/* Krylov methods (linsolve.h:99-106) */
+#define RSH_KRYLOV_BICGSTAB 0     /* right-preconditioned BiCGSTAB from x0 = 0 (linsolve.cpp:159-216) */
+#define RSH_KRYLOV_FGMRES 1       /* flexible restarted GMRES (linsolve.cpp:218-273) */
+/* precision */
+#define RSH_FP64 0                /* every kernel in double: the drop-in for setDistributedSolver */
+#define RSH_MIXED 1               /* FP32 matrix / ILU factors / AMG hierarchy / smoothers inside an FP64 outer
+                                     iterative-refinement loop over FGMRES; FP64 true residual, dots,
+                                     orthogonalisation and coarsest-level solve */
+/* AMG smoothers */
+#define RSH_SMOOTH_GS 0           /* level-scheduled Gauss-Seidel: forward pre-, backward post-sweep, the exact
+                                     arithmetic of AMG::cycle (amg.cpp:303-329) */
+#define RSH_SMOOTH_JACOBI 1       /* damped Jacobi, omega 2/3, one pre and one post sweep (DeviceCprAmg,
+                                     bsr.h:158-179): fully parallel, different iteration counts */
+
+typedef struct rsh_context rsh_context;
+typedef struct rsh_system rsh_system;
+
+typedef struct rsh_options {
+    int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
+    int krylov;         /* RSH_KRYLOV_* (default BiCGSTAB; RSH_MIXED always uses FGMRES inside refinement) */
+    int precision;      /* RSH_FP64 or RSH_MIXED */
+    int smoother;       /* RSH_SMOOTH_* */
+    int restart;        /* FGMRES restart length (default 30) */
+    int max_outer;      /* RSH_MIXED: iterative-refinement steps (default 10) */
+    double inner_tol;   /* RSH_MIXED: relative tolerance of each inner FGMRES solve (default 1e-5) */
+    int reuse_amg;      /* 1: keep the AMG aggregates of the last setup while the pattern is unchanged and only
+                           recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
+    int reserved[7];
+} rsh_options;
+
+/* ---- library / context */
+int rsh_version(void);
+const char* rsh_last_error(void);
+int rsh_device_count(void);
+void rsh_default_options(rsh_options* opt);
+int rsh_context_create(int device, rsh_context** out);
+int rsh_context_destroy(rsh_context* ctx);
+int rsh_context_synchronize(rsh_context* ctx);
+
+/* ---- a resident system: pattern, values and preconditioner stay on the device across Newton iterations */
+int rsh_system_create(rsh_context* ctx, int n, int nb, const int* row_ptr, const int* col, rsh_system** out);
+int rsh_system_destroy(rsh_system* sys);
+/* new values on the same pattern (host or device pointer to nnzb*nb*nb doubles) */
+int rsh_system_set_values(rsh_system* sys, const double* val);
+int rsh_system_set_values_device(rsh_system* sys, const double* val_dev);
+/* build (first call) or refresh the preconditioner for the current values */
+int rsh_system_setup(rsh_system* sys, const rsh_options* opt);
+/* solve A x = b to relative residual tol (FP64 true residual for RSH_MIXED, the recursive residual of BiCGSTAB
+ * for RSH_FP64 as linsolve.cpp:210-212).  Host pointers (n*nb doubles) or device pointers. */
+int rsh_solve(rsh_system* sys, const double* b, double* x, double tol, int max_it, int* iterations, double* rel_res);
+int rsh_solve_device(rsh_system* sys, const double* b_dev, double* x_dev, double tol, int max_it, int* iterations,
+                     double* rel_res);
+
+/* ---- kernel-level entry points (host vectors), for parity checks against BSR::spmv / Preconditioner::apply */
+int rsh_spmv(rsh_system* sys, const double* x, double* y);
+int rsh_precond_apply(rsh_system* sys, const double* r, double* z);
+int rsh_amg_apply(rsh_system* sys, const double* r, double* z);          /* CPR only: one V-cycle, n doubles */
+int rsh_cpr_weights(rsh_system* sys, double* w);                         /* CPR only: n*nb quasi-IMPES weights */
+int rsh_bilu0_factors(rsh_system* sys, double* lu);                      /* natural-order factors, S's layout */
+int rsh_amg_levels(rsh_system* sys, int* nlevels, int* sizes, int* nnz, int cap);
+int rsh_system_info(rsh_system* sys, char* json, int cap);               /* levels, colours, bytes, last solve */
+
+/* ---- device memory helpers (benchmarks: inputs resident in HBM) */
+int rsh_device_alloc(rsh_context* ctx, size_t bytes, void** ptr);
+int rsh_device_free(rsh_context* ctx, void* ptr);
+int rsh_memcpy_h2d(rsh_context* ctx, void* dst, const void* src, size_t bytes);
+int rsh_memcpy_d2h(rsh_context* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- measurement: average duration (ms) of `reps` back-to-back launches of one kernel class on the context's
+ * stream, bracketed by HIP events.  kind: 0 BSR SpMV (precision of the system setup), 1 preconditioner apply,
+ * 2 ILU forward+backward sweeps, 3 AMG V-cycle, 4 copy kernel of `bytes` (bandwidth calibration). */
+int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

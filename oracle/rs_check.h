@@ -1,0 +1,54 @@
+/* Checker API shared by the oracle restatement (oracle/oracle.cpp -> oracle/liboracle.so) and the shim over the
+ * compiled reference sources (oracle/ref_shim.cpp -> oracle/_ref/libressim_ref.so).  TEST INFRASTRUCTURE ONLY:
+ * nothing in the product library links, loads or calls these functions.
+ *
+ * Matrices are BSR with int32 indices: n block rows, nb x nb row-major blocks, rp[n+1], ci[nnzb] (sorted per row,
+ * diagonal present), v[nnzb*nb*nb].  Vectors are n*nb doubles.  Handles own copies of what they need.
+ *
+ * Preconditioner ids for chk_krylov:
+ *   1 BILU0 (double factors)                       bsr.cpp:347-596
+ *   2 BILU0, float32 factors after a double factorisation, double vectors (C38)   bsr.cpp:413-419, 446-481
+ *   3 BILU0f: float32 factors and float32 sweeps (C16)                            bsr.cpp:1087-1121
+ *   4 MCBILU0 (multicolour)                         bsr.cpp:599-730
+ *   5 BlockCPR, quasi-IMPES weights, double BILU0 second stage                    bsr.cpp:993-1084
+ * Methods: 0 BiCGSTAB (linsolve.cpp:159-216), 1 flexible restarted GMRES (linsolve.cpp:218-273).
+ */
+#ifndef RS_CHECK_H
+#define RS_CHECK_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int chk_spmv(int n, int nb, const int* rp, const int* ci, const double* v, const double* x, double* y);
+
+void* chk_bilu0_new(int n, int nb, const int* rp, const int* ci, const double* v, int mode);
+int chk_bilu0_apply(void* h, const double* r, double* z);
+int chk_bilu0_regularised(void* h);
+void chk_bilu0_free(void* h);
+/* oracle only: the natural-order factors on the input pattern (L, U_ii^-1 on the diagonal, U) */
+int chk_bilu0_factors(void* h, double* lu);
+
+void* chk_mcbilu0_new(int n, int nb, const int* rp, const int* ci, const double* v);
+int chk_mcbilu0_apply(void* h, const double* r, double* z);
+int chk_mcbilu0_colours(void* h);
+int chk_mcbilu0_colouring(void* h, int* colour, int* perm);
+void chk_mcbilu0_free(void* h);
+
+void* chk_cpr_new(int n, int nb, const int* rp, const int* ci, const double* v);
+int chk_cpr_refresh(void* h, const double* v);
+int chk_cpr_apply(void* h, const double* r, double* z);
+int chk_cpr_weights(void* h, double* w);
+int chk_cpr_nlevels(void* h);
+int chk_cpr_level(void* h, int l, int* n, int* nnz);
+int chk_cpr_level_matrix(void* h, int l, int* rp, int* ci, double* v);
+int chk_cpr_aggregates(void* h, int l, int* agg);
+int chk_cpr_amg_apply(void* h, const double* r, double* z);
+void chk_cpr_free(void* h);
+
+int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond,
+               int method, int restart, double tol, int maxit, int* its, double* relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,304 @@
+"""ressimstudio_amd - the linear-solve hot path of ResSimStudio's black-oil Newton loop on MI355X (gfx950).
+
+The product is ``libressim_hip.so`` (C-ABI declared in include/ressim_hip.h, HIP kernels in csrc/hip/).  This
+module is its Python mirror of the reference's solver interface, for tests, benchmarks and scripting:
+
+  * :class:`BSR`                  the block matrix (ressim::BSR, engine/include/ressim/bsr.h:23-37)
+  * :class:`HipBlockSolver`       ressim::GpuBlockSolver (bsr.h:143-151): ``solve(A, b, tol, max_it)`` returns
+                                  ``(ok, x, iterations)`` - false on device failure or non-finite results, the
+                                  caller then falls back to its CPU path (simulator.cpp:3366, 3376-3377)
+  * :class:`System`               a device-resident system: pattern, values, preconditioner and Krylov method
+  * :func:`bicgstab`, :func:`gmres`  the Krylov entry points (linsolve.h:99-106) running on the device
+
+There is no CPU fallback: every call goes through the HIP library and raises when it is not built or no GPU is
+present.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libressim_hip.so")
+
+PREC_BILU0, PREC_MCBILU0, PREC_CPR, PREC_CPR_MC = 1, 2, 3, 4
+KRYLOV_BICGSTAB, KRYLOV_FGMRES = 0, 1
+FP64, MIXED = 0, 1
+SMOOTH_GS, SMOOTH_JACOBI = 0, 1
+PRECONDITIONERS = {"bilu0": PREC_BILU0, "mcbilu0": PREC_MCBILU0, "cpr": PREC_CPR, "cpr_mc": PREC_CPR_MC}
+
+OK, ERR_ARG, ERR_DEVICE, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_NONFINITE, ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+
+EXPORTED = [
+    "rsh_version", "rsh_last_error", "rsh_device_count", "rsh_default_options", "rsh_context_create", "rsh_context_destroy",
+    "rsh_context_synchronize", "rsh_system_create", "rsh_system_destroy", "rsh_system_set_values", "rsh_system_set_values_device",
+    "rsh_system_setup", "rsh_solve", "rsh_solve_device", "rsh_spmv", "rsh_precond_apply", "rsh_amg_apply", "rsh_cpr_weights",
+    "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
+    "rsh_memcpy_d2h", "rsh_time_kernel",
+]
+
+
+class RshError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precond", ctypes.c_int), ("krylov", ctypes.c_int), ("precision", ctypes.c_int), ("smoother", ctypes.c_int),
+                ("restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("inner_tol", ctypes.c_double), ("reuse_amg", ctypes.c_int),
+                ("reserved", ctypes.c_int * 7)]
+
+
+_LIB = None
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load the HIP library (raises when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rsh_version.restype = ctypes.c_int
+        L.rsh_last_error.restype = ctypes.c_char_p
+        L.rsh_device_count.restype = ctypes.c_int
+        L.rsh_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.rsh_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        L.rsh_context_destroy.argtypes = [_vp]
+        L.rsh_context_synchronize.argtypes = [_vp]
+        L.rsh_system_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.POINTER(_vp)]
+        L.rsh_system_destroy.argtypes = [_vp]
+        L.rsh_system_set_values.argtypes = [_vp, _dp]
+        L.rsh_system_set_values_device.argtypes = [_vp, _vp]
+        L.rsh_system_setup.argtypes = [_vp, ctypes.POINTER(Options)]
+        L.rsh_solve.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.rsh_solve_device.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.rsh_spmv.argtypes = [_vp, _dp, _dp]
+        L.rsh_precond_apply.argtypes = [_vp, _dp, _dp]
+        L.rsh_amg_apply.argtypes = [_vp, _dp, _dp]
+        L.rsh_cpr_weights.argtypes = [_vp, _dp]
+        L.rsh_bilu0_factors.argtypes = [_vp, _dp]
+        L.rsh_amg_levels.argtypes = [_vp, _ip, _ip, _ip, ctypes.c_int]
+        L.rsh_system_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+        L.rsh_device_alloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.rsh_device_free.argtypes = [_vp, _vp]
+        L.rsh_memcpy_h2d.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.rsh_memcpy_d2h.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.rsh_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int, what: str = "") -> int:
+    if rc < 0:
+        raise RshError(rc, f"{what}: {lib().rsh_last_error().decode()}")
+    return rc
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def device_count() -> int:
+    return int(lib().rsh_device_count())
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().rsh_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "precond" and isinstance(v, str):
+            v = PRECONDITIONERS[v]
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One HIP stream on one device (re-entrant use: one context per thread)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().rsh_context_create(device, ctypes.byref(h)), "rsh_context_create")
+        self.h = h
+
+    def synchronize(self):
+        _check(lib().rsh_context_synchronize(self.h), "rsh_context_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rsh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_DEFAULT_CTX = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+@dataclass
+class BSR:
+    """Block CSR matrix: n block rows, nb x nb row-major blocks, sorted columns per row, diagonal present."""
+    n: int
+    nb: int
+    row_ptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @classmethod
+    def of(cls, s) -> "BSR":
+        return cls(int(s.n), int(s.nb), np.ascontiguousarray(s.row_ptr, np.int32), np.ascontiguousarray(s.col, np.int32),
+                   np.ascontiguousarray(s.val, np.float64).reshape(-1))
+
+    @property
+    def nnzb(self) -> int:
+        return int(self.row_ptr[-1])
+
+    def size(self) -> int:
+        return self.n * self.nb
+
+
+class System:
+    """A device-resident block system (pattern + values + preconditioner), the unit the solver keeps across
+    Newton iterations."""
+
+    def __init__(self, A, ctx: Context | None = None):
+        A = A if isinstance(A, BSR) else BSR.of(A)
+        self.ctx = ctx or default_context()
+        self.n, self.nb, self.N = A.n, A.nb, A.n * A.nb
+        self.row_ptr, self.col = A.row_ptr, A.col
+        h = _vp()
+        _check(lib().rsh_system_create(self.ctx.h, A.n, A.nb, _p(A.row_ptr, ctypes.c_int), _p(A.col, ctypes.c_int), ctypes.byref(h)),
+               "rsh_system_create")
+        self.h = h
+        self.set_values(A.val)
+        self.options = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rsh_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_values(self, val):
+        v = np.ascontiguousarray(val, np.float64).reshape(-1)
+        _check(lib().rsh_system_set_values(self.h, _p(v, ctypes.c_double)), "rsh_system_set_values")
+
+    def setup(self, options: Options | None = None, **kw):
+        o = options or default_options(**kw)
+        _check(lib().rsh_system_setup(self.h, ctypes.byref(o)), "rsh_system_setup")
+        self.options = o
+        return self
+
+    def solve(self, b, tol=1e-8, max_it=1000, raise_on_fail=True):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.N)
+        its = ctypes.c_int(0)
+        rel = ctypes.c_double(0)
+        rc = lib().rsh_solve(self.h, _p(b, ctypes.c_double), _p(x, ctypes.c_double), tol, max_it, ctypes.byref(its), ctypes.byref(rel))
+        if rc < 0 and (raise_on_fail or rc not in (ERR_NOT_CONVERGED, ERR_BREAKDOWN)):
+            _check(rc, "rsh_solve")
+        return x, int(its.value), float(rel.value), rc == OK
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.N)
+        _check(lib().rsh_spmv(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double)), "rsh_spmv")
+        return y
+
+    def precond(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(self.N)
+        _check(lib().rsh_precond_apply(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double)), "rsh_precond_apply")
+        return z
+
+    def amg_apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(self.n)
+        _check(lib().rsh_amg_apply(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double)), "rsh_amg_apply")
+        return z
+
+    def cpr_weights(self):
+        w = np.empty(self.N)
+        _check(lib().rsh_cpr_weights(self.h, _p(w, ctypes.c_double)), "rsh_cpr_weights")
+        return w
+
+    def bilu0_factors(self):
+        lu = np.empty(int(self.row_ptr[-1]) * self.nb * self.nb)
+        _check(lib().rsh_bilu0_factors(self.h, _p(lu, ctypes.c_double)), "rsh_bilu0_factors")
+        return lu
+
+    def info(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().rsh_system_info(self.h, buf, len(buf)), "rsh_system_info")
+        return json.loads(buf.value.decode())
+
+    def time_kernel(self, kind: int, reps: int = 20):
+        ms = ctypes.c_double(0)
+        by = ctypes.c_double(0)
+        _check(lib().rsh_time_kernel(self.h, kind, reps, ctypes.byref(ms), ctypes.byref(by)), "rsh_time_kernel")
+        return float(ms.value), float(by.value)
+
+
+class HipBlockSolver:
+    """ressim::GpuBlockSolver on the device (bsr.h:143-151).  Systems are cached by sparsity pattern so a Newton
+    loop only uploads new values; the AMG hierarchy is refreshed (amgReuse semantics) while the pattern holds."""
+
+    def __init__(self, precond: str = "cpr", precision: int = FP64, krylov: int = KRYLOV_BICGSTAB, smoother: int = SMOOTH_GS,
+                 ctx: Context | None = None, reuse_amg: bool = True):
+        self.ctx = ctx or default_context()
+        self.opts = default_options(precond=precond, precision=precision, krylov=krylov, smoother=smoother, reuse_amg=int(reuse_amg))
+        self._sys = {}
+        self.device_solves = 0
+        self.failures = 0
+
+    def name(self) -> str:
+        return "MI355X HIP block solver"
+
+    def solve(self, A, b, tol: float, max_it: int):
+        A = A if isinstance(A, BSR) else BSR.of(A)
+        key = (A.n, A.nb, A.row_ptr.tobytes(), A.col.tobytes())
+        try:
+            s = self._sys.get(key)
+            if s is None:
+                s = System(A, self.ctx)
+                self._sys[key] = s
+            else:
+                s.set_values(A.val)
+            s.setup(self.opts)
+            x, its, rel, ok = s.solve(b, tol, max_it, raise_on_fail=False)
+            self.device_solves += 1
+            if not ok or not np.all(np.isfinite(x)):
+                self.failures += 1
+                return False, x, its
+            return True, x, its
+        except RshError:
+            self.failures += 1
+            return False, np.zeros(A.size()), 0
+
+
+def bicgstab(A, b, precond: str = "bilu0", tol: float = 1e-8, max_it: int = 1000, **kw):
+    s = System(A).setup(precond=precond, krylov=KRYLOV_BICGSTAB, **kw)
+    return s.solve(b, tol, max_it, raise_on_fail=False)
+
+
+def gmres(A, b, precond: str = "bilu0", tol: float = 1e-8, max_it: int = 1000, restart: int = 30, **kw):
+    s = System(A).setup(precond=precond, krylov=KRYLOV_FGMRES, restart=restart, **kw)
+    return s.solve(b, tol, max_it, raise_on_fail=False)

@@ -1,0 +1,220 @@
+// Internal declarations of the MI355X linear-solve library: device buffers, the row-sliced block layouts the
+// kernels stream, level schedules, and the kernel launchers (rsh_kernels.hip) used by the solver drivers
+// (rsh_api.cpp) and the host set-up (rsh_setup.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace rsh {
+
+constexpr int kWave = 64;             // CDNA wavefront
+constexpr int kSliceRows = 64;        // rows per slice: one row per lane of a wavefront
+constexpr int kRedBlocks = 1024;      // partial sums of the deterministic two-pass reductions
+constexpr int kMaxNb = 4;             // block sizes built: 1..4
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+#define RSH_HIP(x)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) throw ::rsh::Error(-2, std::string(#x) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        RSH_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) RSH_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) RSH_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Krylov scalars kept on the device; reductions finish with an epilogue that updates them, so one host read
+// per iteration suffices.
+struct KScalars {
+    double rho, alpha, omega, rhoN, rv, sn, tt, ts, rn, bn, beta, tol;
+    int state, it, pad0, pad1;
+};
+enum KState : int { kRun = 0, kConvS = 1, kConv = 2, kBreak = 3, kOmegaZero = 4 };
+enum Epilogue : int { kEpNone = 0, kEpRho, kEpAlpha, kEpSn, kEpOmega, kEpRn, kEpBn, kEpNorm };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DBuf<double> red;          // kRedBlocks * 4 partial sums
+    DBuf<unsigned> ticket;     // last-block counter of the reductions
+    DBuf<KScalars> ks;
+    DBuf<double> scal;         // general device scalars (dot results, 64 slots)
+    KScalars* hks = nullptr;   // pinned host mirror
+    double* hscal = nullptr;
+};
+
+// ------------------------------------------------------------------ row-sliced block layout
+// Rows are taken 64 at a time (one per lane of a wavefront).  A slice stores, for each of its `width` slots,
+// the column of every lane and the nb*nb block entries element-major: entry e of slot k for lane l lives at
+// val[((off + k) * nb2 + e) * 64 + l], so each load instruction of a wavefront reads 64 consecutive values.
+// Rows shorter than the slice width are padded with zero blocks pointing at their own row; rows longer than
+// the cap are taken out of the slices and handled by one wavefront each (the "long" rows: perforated cells of
+// wells after the Schur fill).
+struct SliceHost {
+    int nslices = 0;
+    long long nslots = 0;
+    std::vector<int> width, off;   // per slice
+    std::vector<int> row, orig;    // per slice*64+lane: working row, original row (-1 = idle lane)
+    std::vector<int> col, src;     // per slot*64+lane: working column, source block index (-1 = padding)
+    // long rows
+    std::vector<int> lrow, lorig, lptr, lcol, lsrc;
+    std::vector<int> dsrc;         // optional per slice*64+lane: diagonal block (ILU backward sweep)
+    std::vector<int> ldsrc;        // per long row
+};
+
+template <typename TA>
+struct SliceDev {
+    int nslices = 0, nlong = 0, nb = 1;
+    DBuf<int2> info;               // (width, slot offset)
+    DBuf<int> row, orig, col, src, dsrc;
+    DBuf<TA> val, dval;
+    DBuf<int> lrow, lorig, lptr, lcol, lsrc, ldsrc;
+    DBuf<TA> lval, ldval;
+    size_t bytes() const {
+        return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
+               lrow.bytes() + lorig.bytes() + lptr.bytes() + lcol.bytes() + lsrc.bytes() + ldsrc.bytes() + lval.bytes() + ldval.bytes();
+    }
+};
+
+// A level schedule over a slice layout: level l owns slices [sStart[l], sStart[l+1]) and long rows
+// [qStart[l], qStart[l+1]).
+struct LevelSched {
+    std::vector<int> sStart, qStart;
+    int levels() const { return int(sStart.size()) - 1; }
+};
+
+// ------------------------------------------------------------------ host set-up (rsh_setup.cpp)
+struct HostBsr {
+    int n = 0, nb = 1;
+    std::vector<int> rp, ci, diag;
+};
+struct HostCsr {
+    int n = 0;
+    std::vector<int> rp, ci, diag;
+    std::vector<double> v;
+};
+
+// levels of the symmetrised lower dependency graph; ord = rows in level order (ascending within a level)
+void levelOrder(const HostBsr& A, std::vector<int>& ord, std::vector<int>& start);
+// levels of the plain lower (forward) or upper (backward) graph of a scalar CSR (Gauss-Seidel sweeps)
+void csrLevels(const HostCsr& A, bool forward, std::vector<int>& ord, std::vector<int>& start);
+// greedy distance-1 colouring, largest degree first (colourBlocks, bsr.cpp:599-630)
+int colourGraph(const HostBsr& A, std::vector<int>& colour, std::vector<int>& perm, std::vector<int>& start);
+// P A P^T pattern with sorted rows and the block map new -> old (permuteBsr, bsr.cpp:632-653)
+void permutePattern(const HostBsr& A, const std::vector<int>& perm, HostBsr& P, std::vector<int>& blockMap);
+int rowCap(const std::vector<int>& lengths);
+// slices over the rows [r0, r1) of working numbering; entries(i) lists (working col, src block) for working row i
+struct RowEntries {
+    std::vector<int> ptr, col, src;
+    std::vector<int> orig;   // working row -> original row
+    std::vector<int> dsrc;   // working row -> diagonal block (or empty)
+};
+void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L);
+
+// AMG set-up on the host (AMG::setup, amg.cpp:137-252): returns the hierarchy
+struct AmgLevelHost {
+    HostCsr A;
+    std::vector<int> agg;         // fine -> aggregate (empty on the coarsest level)
+    int nc = 0;
+    // Galerkin maps to recompute the next level's values from this level's values on the device:
+    //   ap[t] = sum_{u in [apPtr[t], apPtr[t+1])} A.v[apSrc[u]]           (entries of A P, per fine row)
+    //   next.v[q] = sum_{u in [cPtr[q], cPtr[q+1])} ap[cSrc[u]]            (R (A P), fine rows ascending)
+    std::vector<int> apPtr, apSrc, cPtr, cSrc;
+};
+void amgSetup(HostCsr A0, std::vector<AmgLevelHost>& H);
+
+// ------------------------------------------------------------------ kernel launchers (rsh_kernels.hip)
+void launchPackSlices(hipStream_t s, int nb, const double* src, const int* srcIdx, long long nslots, void* dst, bool f32);
+void launchPackLong(hipStream_t s, int nb, const double* src, const int* srcIdx, int count, void* dst, bool f32);
+void launchPackColumn0(hipStream_t s, int nb, const double* src, const int* srcIdx, long long nslots, double* dst);
+void launchGatherBlocks(hipStream_t s, int nb, const double* src, const int* map, int count, double* dst);
+
+// y = A x over a slice layout (all levels at once); TA value type, TV vector type
+void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x,
+                      double* y);
+// ILU sweeps of one level
+void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y);
+void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
+                       const double* xp);
+// BILU0 numeric factorisation of the rows of one level (in place, natural layout)
+void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag,
+                      double* val, int* nreg);
+// CPR kernels
+void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w);
+void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);
+void launchFixDiag(hipStream_t s, int n, const int* diag, double* v);
+void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp);
+void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp,
+                     double* t);
+// scalar AMG level kernels
+void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b,
+                           double* r);
+void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
+                   const double* b, const double* xin, double* xout, bool forward);
+void launchJacobi(hipStream_t s, int n, const int* rp, const int* ci, const int* diag, const double* v, const double* b,
+                  const double* xin, double* xout, double omega, bool fromZero);
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc);
+void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);
+void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y);
+void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y);
+void launchGalerkin(hipStream_t s, int nap, const int* apPtr, const int* apSrc, const double* fine, double* ap, int nnzc,
+                    const int* cPtr, const int* cSrc, double* coarse);
+void launchDenseFromCsr(hipStream_t s, int m, const int* rp, const int* ci, const double* v, double* M);
+void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, double* Mout);
+// vectors and reductions
+void launchFill(hipStream_t s, double* x, double v, long long n);
+void launchCopy(hipStream_t s, double* dst, const double* src, long long n);
+void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1,
+                double* out, int epilogue, KScalars* ks);
+void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p);
+void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv);
+void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* ph, double* x);
+void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv,
+                  const double* t, double* x, double* r);
+void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y);                      // y += a x
+void launchAxpyDev(hipStream_t s, long long n, const double* a, double sign, const double* x, double* y); // y += sign*a[0] x
+void launchScaleTo(hipStream_t s, long long n, const double* num, const double* x, double* y, bool invert);  // y = x / num[0]
+void launchSub(hipStream_t s, long long n, const double* b, const double* ax, double* r);                // r = b - ax
+// out[i] = (V_i, w) for i < k with V_i = V + i * ld; deterministic per-block partials then a fixed-order sum
+void launchMultiDot(hipStream_t s, Ctx& c, long long n, int k, const double* V, long long ld, const double* w, double* out);
+// w -= sum_i h[i] V_i  (h on the device)
+void launchMultiAxpy(hipStream_t s, long long n, int k, const double* V, long long ld, const double* h, double* w);
+// x += sum_i y[i] Z_i  (y on the device)
+void launchMultiAxpyAdd(hipStream_t s, long long n, int k, const double* Z, long long ld, const double* y, double* x);
+void launchCopyKernel(hipStream_t s, const double* src, double* dst, long long n);
+
+}  // namespace rsh

@@ -1,0 +1,1166 @@
+// HIP kernels of the linear-solve path for gfx950 (MI355X): BSR SpMV over row slices, level-scheduled block
+// ILU(0) factorisation and sweeps, the CPR pressure kernels, scalar AMG level kernels, deterministic reductions
+// and the fused Krylov vector updates.  Every kernel is HBM- or latency-bound (a 3x3 FP64 block carries 18 flops
+// per 76 bytes), so the design goal is coalesced streaming: one block row per lane of a 64-wide wavefront with
+// the block entries stored element-major across the 64 rows of a slice (one 512-byte load per entry).
+//
+// Kernels that restate a reference loop keep its order of floating-point operations and disable contraction
+// (#pragma clang fp contract(off)), so that on rows handled by one lane the results equal the reference's
+// (bsr.cpp:14-56 block products, :67-87 inversion, :183-215 row elimination, :437-596 sweeps; amg.cpp:98-131).
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+namespace {
+
+constexpr int kBlock = 256;            // 4 wavefronts per workgroup
+
+__device__ inline double waveSum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T>
+__device__ inline T toT(double v) { return static_cast<T>(v); }
+
+// ---------------------------------------------------------------- dense block helpers (registers)
+template <int NB>
+__device__ inline bool invertBlockDev(double* a) {
+#pragma clang fp contract(off)
+    double m[NB][2 * NB];
+    double scale = 0;
+#pragma unroll
+    for (int i = 0; i < NB * NB; ++i) scale = fmax(scale, fabs(a[i]));
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) { m[i][j] = a[i * NB + j]; m[i][NB + j] = i == j ? 1.0 : 0.0; }
+    if (!(scale > 0) || !isfinite(scale)) return false;
+    for (int k = 0; k < NB; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < NB; ++i) if (fabs(m[i][k]) > fabs(m[piv][k])) piv = i;
+        if (fabs(m[piv][k]) <= 1e-14 * scale) return false;
+        if (piv != k)
+            for (int j = 0; j < 2 * NB; ++j) { const double t = m[k][j]; m[k][j] = m[piv][j]; m[piv][j] = t; }
+        const double inv = 1.0 / m[k][k];
+        for (int j = 0; j < 2 * NB; ++j) m[k][j] *= inv;
+        for (int i = 0; i < NB; ++i) {
+            if (i == k || m[i][k] == 0.0) continue;
+            const double f = m[i][k];
+            for (int j = 0; j < 2 * NB; ++j) m[i][j] -= f * m[k][j];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) a[i * NB + j] = m[i][NB + j];
+    return true;
+}
+
+// invert a pivot block; singular -> diagonal shift by 1e-8 * scale, else identity / scale (bsr.cpp:202-212)
+template <int NB>
+__device__ inline int invertPivotDev(double* d) {
+#pragma clang fp contract(off)
+    double save[NB * NB];
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) save[e] = d[e];
+    if (invertBlockDev<NB>(d)) return 0;
+    double sc = 0;
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) sc = fmax(sc, fabs(save[e]));
+    if (!(sc > 0) || !isfinite(sc)) sc = 1.0;
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) d[e] = save[e];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        const double x = d[e * NB + e];
+        d[e * NB + e] = x + ((x >= 0 ? 1e-8 : -1e-8) * sc + (x == 0 ? sc * 1e-8 : 0));
+    }
+    if (!invertBlockDev<NB>(d)) {
+#pragma unroll
+        for (int e = 0; e < NB * NB; ++e) d[e] = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) d[e * NB + e] = 1.0 / sc;
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------- layout packing
+template <typename TA>
+__global__ void kPackSlices(int nb2, const double* __restrict__ src, const int* __restrict__ srcIdx, long long nslots, TA* __restrict__ dst) {
+    const long long total = nslots * nb2 * kWave;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x) {
+        const int lane = int(idx & 63);
+        const long long q = idx >> 6;
+        const int e = int(q % nb2);
+        const long long slot = q / nb2;
+        const int s = srcIdx[slot * kWave + lane];
+        dst[idx] = s >= 0 ? toT<TA>(src[(long long)s * nb2 + e]) : TA(0);
+    }
+}
+template <typename TA>
+__global__ void kPackLong(int nb2, const double* __restrict__ src, const int* __restrict__ srcIdx, int count, TA* __restrict__ dst) {
+    const long long total = (long long)count * nb2;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x) {
+        const long long k = idx / nb2;
+        const int e = int(idx % nb2);
+        const int s = srcIdx[k];
+        dst[idx] = s >= 0 ? toT<TA>(src[(long long)s * nb2 + e]) : TA(0);
+    }
+}
+__global__ void kPackColumn0(int nb, const double* __restrict__ src, const int* __restrict__ srcIdx, long long nslots, double* __restrict__ dst) {
+    const long long total = nslots * nb * kWave;
+    const int nb2 = nb * nb;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x) {
+        const int lane = int(idx & 63);
+        const long long q = idx >> 6;
+        const int e = int(q % nb);
+        const long long slot = q / nb;
+        const int s = srcIdx[slot * kWave + lane];
+        dst[idx] = s >= 0 ? src[(long long)s * nb2 + e * nb] : 0.0;
+    }
+}
+__global__ void kGatherBlocks(int nb2, const double* __restrict__ src, const int* __restrict__ map, int count, double* __restrict__ dst) {
+    const long long total = (long long)count * nb2;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x)
+        dst[idx] = src[(long long)map[idx / nb2] * nb2 + idx % nb2];
+}
+
+inline int gridFor(long long work, int per = kBlock) {
+    long long g = (work + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > 65535LL * 8) g = 65535LL * 8;
+    return int(g);
+}
+
+// ---------------------------------------------------------------- BSR SpMV over slices (+ long rows)
+// y_i = sum_p A_p x_col(p) (bsr.cpp:109-126); per lane the blocks are visited in column order and each block
+// product is formed as s_e = sum_f a_ef x_f before y_e += s_e, as gemvAdd does.
+template <int NB, typename TA>
+__global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
+                                                      const int* __restrict__ col, const TA* __restrict__ val, int nlong,
+                                                      const int* __restrict__ lrow, const int* __restrict__ lptr,
+                                                      const int* __restrict__ lcol, const TA* __restrict__ lval,
+                                                      const double* __restrict__ x, double* __restrict__ y) {
+#pragma clang fp contract(off)
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid < nslices) {
+        const int2 inf = info[wid];
+        const int r = row[(long long)wid * kWave + lane];
+        double acc[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = 0;
+        for (int k = 0; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
+            const int c = col[slot * kWave + lane];
+            const TA* a = val + slot * (NB * NB * kWave) + lane;
+            double xv[NB];
+#pragma unroll
+            for (int f = 0; f < NB; ++f) xv[f] = x[(long long)c * NB + f];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                double s = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) s += double(a[(e * NB + f) * kWave]) * xv[f];
+                acc[e] += s;
+            }
+        }
+        if (r >= 0) {
+#pragma unroll
+            for (int e = 0; e < NB; ++e) y[(long long)r * NB + e] = acc[e];
+        }
+        return;
+    }
+    const int q = wid - nslices;
+    if (q >= nlong) return;
+    double part[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = 0;
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) {
+        const int c = lcol[p];
+        const TA* a = lval + (long long)p * NB * NB;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            double s = 0;
+#pragma unroll
+            for (int f = 0; f < NB; ++f) s += double(a[e * NB + f]) * x[(long long)c * NB + f];
+            part[e] += s;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = waveSum(part[e]);
+    if (lane == 0) {
+        const int r = lrow[q];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)r * NB + e] = part[e];
+    }
+}
+
+// ---------------------------------------------------------------- BILU0 level sweeps
+// Forward (bsr.cpp:575-583): y_i = r_i - sum_{k<i} L_ik y_k, y in the working (level-ordered) numbering, r read
+// through the original row index.  TA: factor storage, TV: working-vector type (accumulation type).
+template <int NB, typename TA, typename TV>
+__global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, int nq, const int2* __restrict__ info,
+                                                      const int* __restrict__ row, const int* __restrict__ orig,
+                                                      const int* __restrict__ col, const TA* __restrict__ val,
+                                                      const int* __restrict__ lrow, const int* __restrict__ lorig,
+                                                      const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                                      const TA* __restrict__ lval, const double* __restrict__ r, TV* __restrict__ y) {
+#pragma clang fp contract(off)
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid < ns) {
+        const int sl = s0 + wid;
+        const int wr = row[(long long)sl * kWave + lane];
+        if (wr < 0) return;
+        const int o = orig[(long long)sl * kWave + lane];
+        const int2 inf = info[sl];
+        TV acc[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = TV(r[(long long)o * NB + e]);
+        for (int k = 0; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
+            const int c = col[slot * kWave + lane];
+            const TA* a = val + slot * (NB * NB * kWave) + lane;
+            TV yk[NB];
+#pragma unroll
+            for (int f = 0; f < NB; ++f) yk[f] = y[(long long)c * NB + f];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV s = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) s += TV(a[(e * NB + f) * kWave]) * yk[f];
+                acc[e] -= s;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
+        return;
+    }
+    const int qi = wid - ns;
+    if (qi >= nq) return;
+    const int q = q0 + qi;
+    TV part[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = 0;
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) {
+        const int c = lcol[p];
+        const TA* a = lval + (long long)p * NB * NB;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            TV s = 0;
+#pragma unroll
+            for (int f = 0; f < NB; ++f) s += TV(a[e * NB + f]) * y[(long long)c * NB + f];
+            part[e] += s;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = TV(waveSum(double(part[e])));
+    if (lane == 0) {
+        const int wr = lrow[q], o = lorig[q];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = TV(r[(long long)o * NB + e]) - part[e];
+    }
+}
+
+// Backward (bsr.cpp:584-595): z_i = U_ii^-1 (y_i - sum_{j>i} U_ij z_j), in place in y; the result is also
+// scattered to z in the original numbering, with xp added to the pressure unknown when given (the CPR
+// correction z_i0 += x_p,i, bsr.cpp:1083).
+template <int NB, typename TA, typename TV>
+__global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, int nq, const int2* __restrict__ info,
+                                                       const int* __restrict__ row, const int* __restrict__ orig,
+                                                       const int* __restrict__ col, const TA* __restrict__ val,
+                                                       const TA* __restrict__ dval, const int* __restrict__ lrow,
+                                                       const int* __restrict__ lorig, const int* __restrict__ lptr,
+                                                       const int* __restrict__ lcol, const TA* __restrict__ lval,
+                                                       const TA* __restrict__ ldval, TV* __restrict__ y, double* __restrict__ z,
+                                                       const double* __restrict__ xp) {
+#pragma clang fp contract(off)
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    TV acc[NB];
+    int wr, o;
+    const TA* d;
+    int dstride;
+    if (wid < ns) {
+        const int sl = s0 + wid;
+        wr = row[(long long)sl * kWave + lane];
+        if (wr < 0) return;
+        o = orig[(long long)sl * kWave + lane];
+        const int2 inf = info[sl];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = y[(long long)wr * NB + e];
+        for (int k = 0; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
+            const int c = col[slot * kWave + lane];
+            const TA* a = val + slot * (NB * NB * kWave) + lane;
+            TV zj[NB];
+#pragma unroll
+            for (int f = 0; f < NB; ++f) zj[f] = y[(long long)c * NB + f];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV s = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) s += TV(a[(e * NB + f) * kWave]) * zj[f];
+                acc[e] -= s;
+            }
+        }
+        d = dval + (long long)sl * (NB * NB * kWave) + lane;
+        dstride = kWave;
+    } else {
+        const int qi = wid - ns;
+        if (qi >= nq) return;
+        const int q = q0 + qi;
+        TV part[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] = 0;
+        for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) {
+            const int c = lcol[p];
+            const TA* a = lval + (long long)p * NB * NB;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV s = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) s += TV(a[e * NB + f]) * y[(long long)c * NB + f];
+                part[e] += s;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] = TV(waveSum(double(part[e])));
+        if (lane != 0) return;
+        wr = lrow[q];
+        o = lorig[q];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = y[(long long)wr * NB + e] - part[e];
+        d = ldval + (long long)q * NB * NB;
+        dstride = 1;
+    }
+    TV t[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        TV s = 0;
+#pragma unroll
+        for (int f = 0; f < NB; ++f) s += TV(d[(e * NB + f) * dstride]) * acc[f];
+        t[e] = s;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = t[e];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        double v = double(t[e]);
+        if (e == 0 && xp) v += xp[o];
+        z[(long long)o * NB + e] = v;
+    }
+}
+
+// ---------------------------------------------------------------- BILU0 factorisation, one level, in place
+// Row i (bsr.cpp:183-215): for each k < i in row order: L_ik = A_ik U_kk^-1; A_ij -= L_ik U_kj for every j > k
+// present in row i; then the pivot block is inverted (U_ii^-1 stored).  Rows of one level only read rows of
+// earlier levels.
+template <int NB>
+__global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const int* __restrict__ diag, double* __restrict__ val, int* __restrict__ nreg) {
+#pragma clang fp contract(off)
+    const int idx = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= r1) return;
+    const int i = rows ? rows[idx] : idx;
+    const int pe = rp[i + 1], pd = diag[i];
+    constexpr int B2 = NB * NB;
+    for (int p = rp[i]; p < pd; ++p) {
+        const int k = ci[p];
+        double* ap = val + (long long)p * B2;
+        const double* dk = val + (long long)diag[k] * B2;
+        double L[B2];
+#pragma unroll
+        for (int ii = 0; ii < NB; ++ii) {
+            double rr[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) rr[j] = 0;
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) {
+                const double f = ap[ii * NB + kk];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) rr[j] += f * dk[kk * NB + j];
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) L[ii * NB + j] = rr[j];
+        }
+#pragma unroll
+        for (int e = 0; e < B2; ++e) ap[e] = L[e];
+        int s = p + 1;
+        const int qe = rp[k + 1];
+        for (int q = diag[k] + 1; q < qe && s < pe; ++q) {
+            const int j = ci[q];
+            while (s < pe && ci[s] < j) ++s;
+            if (s < pe && ci[s] == j) {
+                const double* u = val + (long long)q * B2;
+                double* as = val + (long long)s * B2;
+#pragma unroll
+                for (int ii = 0; ii < NB; ++ii) {
+                    double rr[NB];
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) rr[jj] = 0;
+#pragma unroll
+                    for (int kk = 0; kk < NB; ++kk) {
+                        const double f = L[ii * NB + kk];
+#pragma unroll
+                        for (int jj = 0; jj < NB; ++jj) rr[jj] += f * u[kk * NB + jj];
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) as[ii * NB + jj] -= rr[jj];
+                }
+            }
+        }
+    }
+    double d[B2];
+    double* dp = val + (long long)pd * B2;
+#pragma unroll
+    for (int e = 0; e < B2; ++e) d[e] = dp[e];
+    const int reg = invertPivotDev<NB>(d);
+#pragma unroll
+    for (int e = 0; e < B2; ++e) dp[e] = d[e];
+    if (reg && nreg) atomicAdd(nreg, 1);
+}
+
+// ---------------------------------------------------------------- CPR (bsr.cpp:993-1084)
+// quasi-IMPES weights: w_i = first row of D_ii^-1; singular -> 1 / sum_e D[e][0]; non-finite or zero -> 1
+template <int NB>
+__global__ void kCprWeights(int n, const int* __restrict__ diag, const double* __restrict__ val, double* __restrict__ w) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* D0 = val + (long long)diag[i] * NB * NB;
+    double D[NB * NB];
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) D[e] = D0[e];
+    double wi[NB];
+    if (invertBlockDev<NB>(D)) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) wi[e] = D[e];
+    } else {
+        double s = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) s += D0[e * NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) wi[e] = fabs(s) > 1e-300 ? 1.0 / s : 1.0;
+    }
+    double wm = 0;
+#pragma unroll
+    for (int e = 0; e < NB; ++e) wm = fmax(wm, fabs(wi[e]));
+    if (!(wm > 0) || !isfinite(wm)) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) wi[e] = 1.0;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) w[(long long)i * NB + e] = wi[e];
+}
+// Ap_ij = sum_e w_ie S_ij[e][0] on the pattern of S
+template <int NB>
+__global__ void kCprPressure(int n, const int* __restrict__ rp, const double* __restrict__ val, const double* __restrict__ w,
+                             double* __restrict__ ap) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double wi[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) wi[e] = w[(long long)i * NB + e];
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const double* a = val + (long long)p * NB * NB;
+        double s = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) s += wi[e] * a[e * NB];
+        ap[p] = s;
+    }
+}
+__global__ void kFixDiag(int n, const int* __restrict__ diag, double* __restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double& d = v[diag[i]];
+    if (fabs(d) < 1e-300) d = 1.0;
+}
+template <int NB>
+__global__ void kCprRestrict(int n, const double* __restrict__ w, const double* __restrict__ r, double* __restrict__ rp) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0;
+#pragma unroll
+    for (int e = 0; e < NB; ++e) s += w[(long long)i * NB + e] * r[(long long)i * NB + e];
+    rp[i] = s;
+}
+// second-stage residual t = r - S(:, 0) x_p over the SpMV slices with the pressure column packed (nb values per
+// slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081)
+template <int NB>
+__global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
+                                                     const int* __restrict__ col, const double* __restrict__ colP, int nlong,
+                                                     const int* __restrict__ lrow, const int* __restrict__ lptr,
+                                                     const int* __restrict__ lcol, const double* __restrict__ lvalFull,
+                                                     const double* __restrict__ r, const double* __restrict__ xp, double* __restrict__ t) {
+#pragma clang fp contract(off)
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid < nslices) {
+        const int rr = row[(long long)wid * kWave + lane];
+        if (rr < 0) return;
+        const int2 inf = info[wid];
+        double acc[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = r[(long long)rr * NB + e];
+        for (int k = 0; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
+            const double xk = xp[col[slot * kWave + lane]];
+            const double* a = colP + slot * (NB * kWave) + lane;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] -= a[e * kWave] * xk;
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = acc[e];
+        return;
+    }
+    const int q = wid - nslices;
+    if (q >= nlong) return;
+    double part[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = 0;
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) {
+        const double xk = xp[lcol[p]];
+        const double* a = lvalFull + (long long)p * NB * NB;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] += a[e * NB] * xk;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) part[e] = waveSum(part[e]);
+    if (lane == 0) {
+        const int rr = lrow[q];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = r[(long long)rr * NB + e] - part[e];
+    }
+}
+
+// ---------------------------------------------------------------- scalar AMG level kernels (amg.cpp:98-131, 303-329)
+__global__ void kCsrResidual(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                             const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * x[ci[p]];
+    r[i] = b ? b[i] - s : s;
+}
+// One level of a Gauss-Seidel sweep.  Forward: columns below i read xout (this sweep), above i read xin (before
+// the sweep, or zero when xin is null); backward the other way round.  s = b_i - sum_{p != diag} a_p x_col in
+// column order, x_i = s / a_ii (amg.cpp:98-113).
+__global__ void kGsLevel(int nrows, const int* __restrict__ rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const int* __restrict__ diag, const double* __restrict__ v, const double* __restrict__ b,
+                         const double* __restrict__ xin, double* __restrict__ xout, int forward) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const int i = rows[t];
+    double s = b[i];
+    for (int p = rp[i]; p < rp[i + 1]; ++p) {
+        const int j = ci[p];
+        if (j == i) continue;
+        const bool useOut = forward ? (j < i) : (j > i);
+        if (useOut) s -= v[p] * xout[j];
+        else if (xin) s -= v[p] * xin[j];
+    }
+    xout[i] = s / v[diag[i]];
+}
+// damped Jacobi: xout = xin + omega D^-1 (b - A xin), or omega D^-1 b from zero (bsr.cpp:1328-1350)
+__global__ void kJacobi(int n, const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ diag,
+                        const double* __restrict__ v, const double* __restrict__ b, const double* __restrict__ xin,
+                        double* __restrict__ xout, double omega, int fromZero) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dinv = 1.0 / v[diag[i]];
+    if (fromZero) { xout[i] = omega * dinv * b[i]; return; }
+    double s = 0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * xin[ci[p]];
+    xout[i] = xin[i] + omega * dinv * (b[i] - s);
+}
+__global__ void kRestrict(int nc, const int* __restrict__ rrp, const int* __restrict__ rci, const double* __restrict__ r,
+                          double* __restrict__ bc) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc) return;
+    double s = 0;
+    for (int q = rrp[a]; q < rrp[a + 1]; ++q) s += r[rci[q]];
+    bc[a] = s;
+}
+__global__ void kProlongAdd(int n, const int* __restrict__ agg, const double* __restrict__ xc, double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + xc[agg[i]];
+}
+// y = M x for a dense m x ld row-major matrix (the explicit coarsest inverse): one wavefront per row
+__global__ __launch_bounds__(kBlock) void kDenseGemv(int m, int ld, const double* __restrict__ M, const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    const int rowi = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (rowi >= m) return;
+    double s = 0;
+    for (int j = lane; j < m; j += kWave) s += M[(long long)rowi * ld + j] * x[j];
+    s = waveSum(s);
+    if (lane == 0) y[rowi] = s;
+}
+__global__ void kGalerkinSum(int count, const int* __restrict__ ptr, const int* __restrict__ srcIdx, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int u0 = ptr[q], u1 = ptr[q + 1];
+    double s = src[srcIdx[u0]];
+    for (int u = u0 + 1; u < u1; ++u) s += src[srcIdx[u]];
+    dst[q] = s;
+}
+// [C | I] (m x 2m) from the coarsest CSR
+__global__ void kDenseFromCsr(int m, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                              double* __restrict__ M) {
+    const long long total = (long long)m * 2 * m;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total; idx += (long long)gridDim.x * blockDim.x) {
+        const int i = int(idx / (2 * m)), j = int(idx % (2 * m));
+        double val = 0;
+        if (j < m) {
+            for (int p = rp[i]; p < rp[i + 1]; ++p) if (ci[p] == j) val += v[p];
+        } else if (j - m == i) val = 1.0;
+        M[idx] = val;
+    }
+}
+// One Gauss-Jordan step with partial pivoting on [C | I], out of place (Min -> Mout).  Every workgroup finds the
+// pivot row of column k itself (first maximum), so the step needs no grid-wide synchronisation.
+__global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const double* __restrict__ Min, double* __restrict__ Mout) {
+    __shared__ double sv[kBlock];
+    __shared__ int si[kBlock];
+    const int ld = 2 * m;
+    double best = -1;
+    int bi = k;
+    for (int i = k + threadIdx.x; i < m; i += kBlock) {
+        const double a = fabs(Min[(long long)i * ld + k]);
+        if (a > best) { best = a; bi = i; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double a = sv[threadIdx.x + o];
+            const int ia = si[threadIdx.x + o];
+            if (a > sv[threadIdx.x] || (a == sv[threadIdx.x] && ia < si[threadIdx.x])) { sv[threadIdx.x] = a; si[threadIdx.x] = ia; }
+        }
+        __syncthreads();
+    }
+    const int pr = si[0];
+    double piv = Min[(long long)pr * ld + k];
+    if (fabs(piv) < 1e-300) piv = piv < 0 ? -1e-300 : 1e-300;
+    const long long total = (long long)m * ld;
+    for (long long idx = blockIdx.x * (long long)kBlock + threadIdx.x; idx < total; idx += (long long)gridDim.x * kBlock) {
+        const int i = int(idx / ld), j = int(idx % ld);
+        const double pk = Min[(long long)pr * ld + j] / piv;    // new row k
+        double out;
+        if (i == k) out = pk;
+        else {
+            const int src = i == pr ? k : i;                     // row pr receives the old row k
+            const double f = Min[(long long)src * ld + k];
+            out = Min[(long long)src * ld + j] - f * pk;
+        }
+        Mout[idx] = out;
+    }
+}
+
+// ---------------------------------------------------------------- vectors and deterministic reductions
+__global__ void kFill(double* __restrict__ x, double v, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) x[i] = v;
+}
+__global__ void kCopy(double* __restrict__ d, const double* __restrict__ s, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+__global__ void kCopyVec(const double2* __restrict__ s, double2* __restrict__ d, long long n2) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__device__ void runEpilogue(int ep, const double* d, KScalars* ks) {
+    if (!ks || ks->state != kRun) return;
+    switch (ep) {
+        case kEpRho:
+            ks->rhoN = d[0];
+            if (fabs(d[0]) < 1e-300) { ks->state = kBreak; break; }
+            ks->beta = (d[0] / ks->rho) * (ks->alpha / ks->omega);
+            ks->rho = d[0];
+            break;
+        case kEpAlpha:
+            ks->rv = d[0];
+            if (fabs(d[0]) < 1e-300) { ks->state = kBreak; break; }
+            ks->alpha = ks->rho / d[0];
+            break;
+        case kEpSn:
+            ks->sn = sqrt(d[0]);
+            if (ks->sn / ks->bn < ks->tol) { ks->state = kConvS; ks->it += 1; }
+            break;
+        case kEpOmega:
+            ks->tt = d[0];
+            ks->ts = d[1];
+            ks->omega = d[0] > 0 ? d[1] / d[0] : 0;
+            break;
+        case kEpRn:
+            ks->rn = sqrt(d[0]);
+            ks->it += 1;
+            if (ks->rn / ks->bn < ks->tol) ks->state = kConv;
+            else if (ks->omega == 0) ks->state = kOmegaZero;
+            break;
+        case kEpBn: ks->bn = sqrt(d[0]); break;
+        default: break;
+    }
+}
+
+// block reduction of K values (fixed tree) -> partials; last block sums the partials in block order
+template <int K>
+__device__ void reduceFinish(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ ticket, double* __restrict__ out,
+                             int ep, KScalars* ks) {
+    __shared__ double sh[K][kBlock / kWave];
+    __shared__ bool last;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = waveSum(v[k]);
+        if (lane == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0;
+            for (int q = 0; q < kBlock / kWave; ++q) s += sh[k][q];
+            partial[(long long)k * gridDim.x + blockIdx.x] = s;
+        }
+        __threadfence();
+        const unsigned prev = atomicAdd(ticket, 1u);
+        last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        acc[k] = 0;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) acc[k] += partial[(long long)k * gridDim.x + b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = waveSum(acc[k]);
+        if (lane == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double res[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0;
+            for (int q = 0; q < kBlock / kWave; ++q) s += sh[k][q];
+            res[k] = s;
+            if (out) out[k] = s;
+        }
+        runEpilogue(ep, res, ks);
+        *ticket = 0;
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void kDots(long long n, const double* __restrict__ a0, const double* __restrict__ b0,
+                                                const double* __restrict__ a1, const double* __restrict__ b1, double* partial,
+                                                unsigned* ticket, double* out, int ep, KScalars* ks) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        v[0] += a0[i] * b0[i];
+        if constexpr (K > 1) v[1] += a1[i] * b1[i];
+    }
+    reduceFinish<K>(v, partial, ticket, out, ep, ks);
+}
+
+// p = r + beta (p - omega v)
+__global__ void kBicgP(long long n, const KScalars* __restrict__ ks, const double* __restrict__ r, const double* __restrict__ v,
+                       double* __restrict__ p) {
+#pragma clang fp contract(off)
+    if (ks->state != kRun) return;
+    const double beta = ks->beta, omega = ks->omega;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+// s = r - alpha v, with (s, s)
+__global__ __launch_bounds__(kBlock) void kBicgS(long long n, KScalars* ks, const double* __restrict__ r, const double* __restrict__ v,
+                                                 double* __restrict__ sv, double* partial, unsigned* ticket) {
+#pragma clang fp contract(off)
+    double acc[1] = {0};
+    if (ks->state == kRun) {
+        const double alpha = ks->alpha;
+        for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+            const double s = r[i] - alpha * v[i];
+            sv[i] = s;
+            acc[0] += s * s;
+        }
+    }
+    reduceFinish<1>(acc, partial, ticket, nullptr, kEpSn, ks);
+}
+// early exit on ||s||: x += alpha ph
+__global__ void kBicgXs(long long n, const KScalars* __restrict__ ks, const double* __restrict__ ph, double* __restrict__ x) {
+#pragma clang fp contract(off)
+    if (ks->state != kConvS) return;
+    const double alpha = ks->alpha;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] += alpha * ph[i];
+}
+// x += alpha ph + omega sh; r = s - omega t; with (r, r)
+__global__ __launch_bounds__(kBlock) void kBicgXR(long long n, KScalars* ks, const double* __restrict__ ph, const double* __restrict__ sh,
+                                                  const double* __restrict__ sv, const double* __restrict__ t, double* __restrict__ x,
+                                                  double* __restrict__ r, double* partial, unsigned* ticket) {
+#pragma clang fp contract(off)
+    double acc[1] = {0};
+    if (ks->state == kRun) {
+        const double alpha = ks->alpha, omega = ks->omega;
+        for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+            x[i] += alpha * ph[i] + omega * sh[i];
+            const double ri = sv[i] - omega * t[i];
+            r[i] = ri;
+            acc[0] += ri * ri;
+        }
+    }
+    reduceFinish<1>(acc, partial, ticket, nullptr, kEpRn, ks);
+}
+__global__ void kAxpy(long long n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+__global__ void kAxpyDev(long long n, const double* __restrict__ a, double sign, const double* __restrict__ x, double* __restrict__ y) {
+    const double aa = sign * a[0];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] += aa * x[i];
+}
+__global__ void kScaleTo(long long n, const double* __restrict__ num, const double* __restrict__ x, double* __restrict__ y, int invert) {
+    const double a = num[0];
+    const double f = invert ? (a > 0 ? 1.0 / a : 0.0) : a;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] = x[i] * f;
+}
+__global__ void kSub(long long n, const double* __restrict__ b, const double* __restrict__ ax, double* __restrict__ r) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) r[i] = b[i] - ax[i];
+}
+// out[i] = (V_i, w), i < k <= 32
+__global__ __launch_bounds__(kBlock) void kMultiDot(long long n, int k, const double* __restrict__ V, long long ld,
+                                                    const double* __restrict__ w, double* partial, unsigned* ticket, double* out) {
+    __shared__ double sh[32][kBlock / kWave];
+    __shared__ bool last;
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0;
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < 32; ++q) if (q < k) acc[q] += V[q * ld + i] * wi;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < k; ++q) {
+        const double t = waveSum(acc[q]);
+        if (lane == 0) sh[q][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double s = 0;
+        for (int u = 0; u < kBlock / kWave; ++u) s += sh[threadIdx.x][u];
+        partial[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < k) {
+        double s = 0;
+        for (unsigned b = 0; b < gridDim.x; ++b) s += partial[(long long)threadIdx.x * gridDim.x + b];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *ticket = 0;
+}
+__global__ void kMultiAxpy(long long n, int k, const double* __restrict__ V, long long ld, const double* __restrict__ h,
+                           double* __restrict__ w, double sign) {
+    __shared__ double hs[32];
+    if (threadIdx.x < k) hs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int q = 0; q < k; ++q) s += hs[q] * V[q * ld + i];
+        w[i] += sign * s;
+    }
+}
+
+template <int NB>
+void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y) {
+    if (f32) {
+        const SliceDev<float>& d = *d32;
+        const int waves = d.nslices + d.nlong;
+        if (!waves) return;
+        hipLaunchKernelGGL((kSpmvSliced<NB, float>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
+                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y);
+    } else {
+        const SliceDev<double>& d = *d64;
+        const int waves = d.nslices + d.nlong;
+        if (!waves) return;
+        hipLaunchKernelGGL((kSpmvSliced<NB, double>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
+                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y);
+    }
+}
+
+template <int NB, typename TA, typename TV>
+void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, const double* r, TV* y) {
+    const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
+    if (!waves) return;
+    hipLaunchKernelGGL((kIluForward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p, d.orig.p,
+                       d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y);
+}
+template <int NB, typename TA, typename TV>
+void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp) {
+    const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
+    if (!waves) return;
+    hipLaunchKernelGGL((kIluBackward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p,
+                       d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, y, z, xp);
+}
+
+template <int NB>
+void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+    const int cnt = r1 - r0;
+    hipLaunchKernelGGL(kBilu0Level<NB>, dim3((cnt + 127) / 128), dim3(128), 0, s, rows, r0, r1, rp, ci, diag, val, nreg);
+}
+template <int NB>
+void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w) {
+    hipLaunchKernelGGL(kCprWeights<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, val, w);
+}
+template <int NB>
+void cprPressureT(hipStream_t s, int n, const int* rp, const double* val, const double* w, double* ap) {
+    hipLaunchKernelGGL(kCprPressure<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, val, w, ap);
+}
+template <int NB>
+void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double* rp) {
+    hipLaunchKernelGGL(kCprRestrict<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp);
+}
+template <int NB>
+void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp, double* t) {
+    const int waves = sl->nslices + sl->nlong;
+    hipLaunchKernelGGL(kCprSecond<NB>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, colP,
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+}
+
+inline void checkLaunch() {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(-2, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+#define RSH_NB_DISPATCH(nb, CALL)                                   \
+    switch (nb) {                                                   \
+        case 1: { constexpr int NB = 1; CALL; } break;              \
+        case 2: { constexpr int NB = 2; CALL; } break;              \
+        case 3: { constexpr int NB = 3; CALL; } break;              \
+        case 4: { constexpr int NB = 4; CALL; } break;              \
+        default: throw Error(-6, "block size not built (1..4)");    \
+    }
+
+}  // namespace
+
+// ================================================================ launchers
+void launchPackSlices(hipStream_t s, int nb, const double* src, const int* srcIdx, long long nslots, void* dst, bool f32) {
+    const long long total = nslots * nb * nb * kWave;
+    if (!total) return;
+    if (f32) hipLaunchKernelGGL(kPackSlices<float>, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, srcIdx, nslots, (float*)dst);
+    else hipLaunchKernelGGL(kPackSlices<double>, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, srcIdx, nslots, (double*)dst);
+    checkLaunch();
+}
+void launchPackLong(hipStream_t s, int nb, const double* src, const int* srcIdx, int count, void* dst, bool f32) {
+    const long long total = (long long)count * nb * nb;
+    if (!total) return;
+    if (f32) hipLaunchKernelGGL(kPackLong<float>, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, srcIdx, count, (float*)dst);
+    else hipLaunchKernelGGL(kPackLong<double>, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, srcIdx, count, (double*)dst);
+    checkLaunch();
+}
+void launchPackColumn0(hipStream_t s, int nb, const double* src, const int* srcIdx, long long nslots, double* dst) {
+    const long long total = nslots * nb * kWave;
+    if (!total) return;
+    hipLaunchKernelGGL(kPackColumn0, dim3(gridFor(total)), dim3(kBlock), 0, s, nb, src, srcIdx, nslots, dst);
+    checkLaunch();
+}
+void launchGatherBlocks(hipStream_t s, int nb, const double* src, const int* map, int count, double* dst) {
+    const long long total = (long long)count * nb * nb;
+    if (!total) return;
+    hipLaunchKernelGGL(kGatherBlocks, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, map, count, dst);
+    checkLaunch();
+}
+void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y) {
+    RSH_NB_DISPATCH(nb, (spmvT<NB>(s, f32, d64, d32, x, y)));
+    checkLaunch();
+}
+void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y) {
+    if (f32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y)))
+    else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y)))
+    checkLaunch();
+}
+void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp) {
+    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp)))
+    else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp)))
+    checkLaunch();
+}
+void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+    const int cnt = r1 - r0;
+    if (cnt <= 0) return;
+    RSH_NB_DISPATCH(nb, (bilu0LevelT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg)));
+    checkLaunch();
+}
+void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w) {
+    RSH_NB_DISPATCH(nb, (cprWeightsT<NB>(s, n, diag, val, w)));
+    checkLaunch();
+}
+void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap) {
+    RSH_NB_DISPATCH(nb, (cprPressureT<NB>(s, n, rp, val, w, ap)));
+    checkLaunch();
+}
+void launchFixDiag(hipStream_t s, int n, const int* diag, double* v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kFixDiag, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, v);
+    checkLaunch();
+}
+void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp) {
+    RSH_NB_DISPATCH(nb, (cprRestrictT<NB>(s, n, w, r, rp)));
+    checkLaunch();
+}
+void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp, double* t) {
+    const int waves = sl->nslices + sl->nlong;
+    if (!waves) return;
+    RSH_NB_DISPATCH(nb, (cprSecondT<NB>(s, sl, colP, r, xp, t)));
+    checkLaunch();
+}
+void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b, double* r) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kCsrResidual, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, x, b, r);
+    checkLaunch();
+}
+void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, const int* ci, const int* diag, const double* v, const double* b,
+                   const double* xin, double* xout, bool forward) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(kGsLevel, dim3((nrows + 127) / 128), dim3(128), 0, s, nrows, rows, rp, ci, diag, v, b, xin, xout, forward ? 1 : 0);
+    checkLaunch();
+}
+void launchJacobi(hipStream_t s, int n, const int* rp, const int* ci, const int* diag, const double* v, const double* b, const double* xin,
+                  double* xout, double omega, bool fromZero) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kJacobi, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, diag, v, b, xin, xout, omega, fromZero ? 1 : 0);
+    checkLaunch();
+}
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc) {
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(kRestrict, dim3((nc + 255) / 256), dim3(256), 0, s, nc, rrp, rci, r, bc);
+    checkLaunch();
+}
+void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kProlongAdd, dim3((n + 255) / 256), dim3(256), 0, s, n, agg, xc, x);
+    checkLaunch();
+}
+void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y) { launchDenseGemv2(s, m, m, M, x, y); }
+void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(kDenseGemv, dim3((m + 3) / 4), dim3(kBlock), 0, s, m, ld, M, x, y);
+    checkLaunch();
+}
+void launchGalerkin(hipStream_t s, int nap, const int* apPtr, const int* apSrc, const double* fine, double* ap, int nnzc, const int* cPtr,
+                    const int* cSrc, double* coarse) {
+    if (nap > 0) hipLaunchKernelGGL(kGalerkinSum, dim3((nap + 255) / 256), dim3(256), 0, s, nap, apPtr, apSrc, fine, ap);
+    if (nnzc > 0) hipLaunchKernelGGL(kGalerkinSum, dim3((nnzc + 255) / 256), dim3(256), 0, s, nnzc, cPtr, cSrc, ap, coarse);
+    checkLaunch();
+}
+void launchDenseFromCsr(hipStream_t s, int m, const int* rp, const int* ci, const double* v, double* M) {
+    const long long total = (long long)m * 2 * m;
+    if (!total) return;
+    hipLaunchKernelGGL(kDenseFromCsr, dim3(gridFor(total)), dim3(kBlock), 0, s, m, rp, ci, v, M);
+    checkLaunch();
+}
+void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, double* Mout) {
+    const long long total = (long long)m * 2 * m;
+    int g = int(std::min<long long>((total + kBlock * 4 - 1) / (kBlock * 4), 2048));
+    hipLaunchKernelGGL(kGaussJordanStep, dim3(g), dim3(kBlock), 0, s, m, k, Min, Mout);
+    checkLaunch();
+}
+void launchFill(hipStream_t s, double* x, double v, long long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kFill, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, x, v, n);
+    checkLaunch();
+}
+void launchCopy(hipStream_t s, double* dst, const double* src, long long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kCopy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, dst, src, n);
+    checkLaunch();
+}
+static int redGrid(long long n) {
+    long long g = (n + kBlock * 8 - 1) / (kBlock * 8);
+    if (g < 1) g = 1;
+    if (g > kRedBlocks) g = kRedBlocks;
+    return int(g);
+}
+void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
+                int ep, KScalars* ks) {
+    const int g = redGrid(n);
+    if (k == 1) hipLaunchKernelGGL(kDots<1>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p, c.ticket.p, out, ep, ks);
+    else hipLaunchKernelGGL(kDots<2>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p, c.ticket.p, out, ep, ks);
+    checkLaunch();
+}
+void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p) {
+    hipLaunchKernelGGL(kBicgP, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, ks, r, v, p);
+    checkLaunch();
+}
+void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv) {
+    hipLaunchKernelGGL(kBicgS, dim3(redGrid(n)), dim3(kBlock), 0, s, n, ks, r, v, sv, c.red.p, c.ticket.p);
+    checkLaunch();
+}
+void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* ph, double* x) {
+    hipLaunchKernelGGL(kBicgXs, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, ks, ph, x);
+    checkLaunch();
+}
+void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv, const double* t,
+                  double* x, double* r) {
+    hipLaunchKernelGGL(kBicgXR, dim3(redGrid(n)), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, x, r, c.red.p, c.ticket.p);
+    checkLaunch();
+}
+void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y) {
+    hipLaunchKernelGGL(kAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, a, x, y);
+    checkLaunch();
+}
+void launchAxpyDev(hipStream_t s, long long n, const double* a, double sign, const double* x, double* y) {
+    hipLaunchKernelGGL(kAxpyDev, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, a, sign, x, y);
+    checkLaunch();
+}
+void launchScaleTo(hipStream_t s, long long n, const double* num, const double* x, double* y, bool invert) {
+    hipLaunchKernelGGL(kScaleTo, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, num, x, y, invert ? 1 : 0);
+    checkLaunch();
+}
+void launchSub(hipStream_t s, long long n, const double* b, const double* ax, double* r) {
+    hipLaunchKernelGGL(kSub, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, b, ax, r);
+    checkLaunch();
+}
+void launchMultiDot(hipStream_t s, Ctx& c, long long n, int k, const double* V, long long ld, const double* w, double* out) {
+    if (k < 1 || k > 32) throw Error(-1, "multi-dot width out of range");
+    hipLaunchKernelGGL(kMultiDot, dim3(redGrid(n)), dim3(kBlock), 0, s, n, k, V, ld, w, c.red.p, c.ticket.p, out);
+    checkLaunch();
+}
+void launchMultiAxpy(hipStream_t s, long long n, int k, const double* V, long long ld, const double* h, double* w) {
+    hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, k, V, ld, h, w, -1.0);
+    checkLaunch();
+}
+void launchMultiAxpyAdd(hipStream_t s, long long n, int k, const double* Z, long long ld, const double* y, double* x) {
+    hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, k, Z, ld, y, x, 1.0);
+    checkLaunch();
+}
+void launchCopyKernel(hipStream_t s, const double* src, double* dst, long long n) {
+    const long long n2 = n / 2;
+    hipLaunchKernelGGL(kCopyVec, dim3(gridFor(n2, kBlock * 4)), dim3(kBlock), 0, s, (const double2*)src, (double2*)dst, n2);
+    checkLaunch();
+}
+
+}  // namespace rsh

@@ -1,0 +1,358 @@
+// Host-side symbolic set-up of the device solver: dependency levels, the multicolour ordering, the row-sliced
+// layouts streamed by the kernels, and the aggregation AMG hierarchy.  These steps are sequential and
+// order-dependent in the reference (greedy aggregation amg.cpp:165-194, greedy colouring bsr.cpp:612-622), so they
+// run once per sparsity pattern on the host and their results stay resident on the device; the numeric work
+// (factorisation, Galerkin products, sweeps) runs on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+void levelOrder(const HostBsr& A, std::vector<int>& ord, std::vector<int>& start) {
+    // level(i) = 1 + max level(j) over j < i coupled to i in either direction (the symmetrised lower graph): a
+    // topological order of the L dependencies that also keeps every coupled pair in its relative order, so the
+    // reversed levels are valid levels of the U sweep
+    const int n = A.n;
+    std::vector<int> lev(static_cast<size_t>(n), 0);
+    for (int j = 0; j < n; ++j) {
+        int l = lev[size_t(j)];
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c < j) l = std::max(l, lev[size_t(c)] + 1);
+        }
+        lev[size_t(j)] = l;
+        // push j's level to its later neighbours (transposed lower entries)
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c > j) lev[size_t(c)] = std::max(lev[size_t(c)], l + 1);
+        }
+    }
+    int nl = 0;
+    for (int i = 0; i < n; ++i) nl = std::max(nl, lev[size_t(i)] + 1);
+    start.assign(size_t(nl) + 1, 0);
+    for (int i = 0; i < n; ++i) start[size_t(lev[size_t(i)]) + 1]++;
+    for (int l = 0; l < nl; ++l) start[size_t(l) + 1] += start[size_t(l)];
+    ord.assign(size_t(n), 0);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; ++i) ord[size_t(fill[size_t(lev[size_t(i)])]++)] = i;
+}
+
+void csrLevels(const HostCsr& A, bool forward, std::vector<int>& ord, std::vector<int>& start) {
+    const int n = A.n;
+    std::vector<int> lev(static_cast<size_t>(n), 0);
+    int nl = 0;
+    for (int t = 0; t < n; ++t) {
+        const int i = forward ? t : n - 1 - t;
+        int l = 0;
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+            const int j = A.ci[size_t(p)];
+            if (forward ? j < i : j > i) l = std::max(l, lev[size_t(j)] + 1);
+        }
+        lev[size_t(i)] = l;
+        nl = std::max(nl, l + 1);
+    }
+    start.assign(size_t(nl) + 1, 0);
+    for (int i = 0; i < n; ++i) start[size_t(lev[size_t(i)]) + 1]++;
+    for (int l = 0; l < nl; ++l) start[size_t(l) + 1] += start[size_t(l)];
+    ord.assign(size_t(n), 0);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; ++i) ord[size_t(fill[size_t(lev[size_t(i)])]++)] = i;
+}
+
+int colourGraph(const HostBsr& A, std::vector<int>& colour, std::vector<int>& perm, std::vector<int>& start) {
+    // the reference's greedy: symmetrised adjacency, cells visited by decreasing degree (stable), smallest colour
+    // not used by a neighbour (bsr.cpp:599-630)
+    const int n = A.n;
+    std::vector<int> deg(static_cast<size_t>(n), 0);
+    std::vector<std::vector<int>> adj(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+            const int j = A.ci[size_t(p)];
+            if (j != i) { adj[size_t(i)].push_back(j); adj[size_t(j)].push_back(i); }
+        }
+    for (auto& a : adj) { std::sort(a.begin(), a.end()); a.erase(std::unique(a.begin(), a.end()), a.end()); }
+    std::vector<int> order(static_cast<size_t>(n));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return adj[size_t(x)].size() > adj[size_t(y)].size(); });
+    colour.assign(size_t(n), -1);
+    int colours = 0;
+    std::vector<char> used;
+    for (int i : order) {
+        used.assign(adj[size_t(i)].size() + 2, 0);
+        for (int j : adj[size_t(i)]) {
+            const int c = colour[size_t(j)];
+            if (c >= 0 && c < int(used.size())) used[size_t(c)] = 1;
+        }
+        int c = 0;
+        while (used[size_t(c)]) ++c;
+        colour[size_t(i)] = c;
+        colours = std::max(colours, c + 1);
+    }
+    start.assign(size_t(colours) + 1, 0);
+    for (int i = 0; i < n; ++i) start[size_t(colour[size_t(i)]) + 1]++;
+    for (int c = 0; c < colours; ++c) start[size_t(c) + 1] += start[size_t(c)];
+    perm.assign(size_t(n), 0);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < n; ++i) perm[size_t(fill[size_t(colour[size_t(i)])]++)] = i;
+    return colours;
+}
+
+void permutePattern(const HostBsr& A, const std::vector<int>& perm, HostBsr& P, std::vector<int>& blockMap) {
+    const int n = A.n;
+    std::vector<int> iperm(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) iperm[size_t(perm[size_t(k)])] = k;
+    P = HostBsr{};
+    P.n = n; P.nb = A.nb;
+    P.rp.assign(size_t(n) + 1, 0);
+    P.ci.reserve(A.ci.size());
+    blockMap.clear();
+    blockMap.reserve(A.ci.size());
+    std::vector<std::pair<int, int>> row;
+    for (int k = 0; k < n; ++k) {
+        const int i = perm[size_t(k)];
+        row.clear();
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) row.push_back({iperm[size_t(A.ci[size_t(p)])], p});
+        std::sort(row.begin(), row.end());
+        for (auto& e : row) { P.ci.push_back(e.first); blockMap.push_back(e.second); }
+        P.rp[size_t(k) + 1] = int(P.ci.size());
+    }
+    P.diag.assign(size_t(n), -1);
+    for (int k = 0; k < n; ++k)
+        for (int p = P.rp[size_t(k)]; p < P.rp[size_t(k) + 1]; ++p) if (P.ci[size_t(p)] == k) { P.diag[size_t(k)] = p; break; }
+}
+
+int rowCap(const std::vector<int>& lengths) {
+    if (lengths.empty()) return 1;
+    std::vector<int> s(lengths);
+    std::sort(s.begin(), s.end());
+    const int mx = s.back();
+    const int p = s[size_t(std::min<double>(double(s.size() - 1), std::floor(0.998 * double(s.size()))))];
+    const int cap = std::max(p, 1);
+    return mx <= cap + cap / 2 + 1 ? mx : cap;   // no long rows unless some rows are well beyond the bulk
+}
+
+void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L) {
+    const int nl = int(levelStart.size()) - 1;
+    const int n = int(E.ptr.size()) - 1;
+    const bool withDiag = !E.dsrc.empty();
+    S = SliceHost{};
+    L.sStart.assign(size_t(nl) + 1, 0);
+    L.qStart.assign(size_t(nl) + 1, 0);
+    S.lptr.push_back(0);
+    std::vector<int> shortRows;
+    for (int l = 0; l < nl; ++l) {
+        shortRows.clear();
+        for (int w = levelStart[size_t(l)]; w < levelStart[size_t(l) + 1]; ++w) {
+            const int len = E.ptr[size_t(w) + 1] - E.ptr[size_t(w)];
+            if (len > cap) {
+                S.lrow.push_back(w);
+                S.lorig.push_back(E.orig[size_t(w)]);
+                for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) { S.lcol.push_back(E.col[size_t(p)]); S.lsrc.push_back(E.src[size_t(p)]); }
+                S.lptr.push_back(int(S.lcol.size()));
+                if (withDiag) S.ldsrc.push_back(E.dsrc[size_t(w)]);
+            } else shortRows.push_back(w);
+        }
+        for (size_t c0 = 0; c0 < shortRows.size(); c0 += kSliceRows) {
+            const size_t c1 = std::min(shortRows.size(), c0 + kSliceRows);
+            int width = 0;
+            for (size_t t = c0; t < c1; ++t) { const int w = shortRows[t]; width = std::max(width, E.ptr[size_t(w) + 1] - E.ptr[size_t(w)]); }
+            S.width.push_back(width);
+            S.off.push_back(int(S.nslots));
+            const size_t base = S.row.size();
+            S.row.resize(base + kSliceRows, -1);
+            S.orig.resize(base + kSliceRows, -1);
+            if (withDiag) S.dsrc.resize(base + kSliceRows, -1);
+            const size_t cbase = S.col.size();
+            S.col.resize(cbase + size_t(width) * kSliceRows, n);     // padding: the dummy row n (zero in the sweeps)
+            S.src.resize(cbase + size_t(width) * kSliceRows, -1);
+            for (size_t t = c0; t < c1; ++t) {
+                const int lane = int(t - c0), w = shortRows[t];
+                S.row[base + size_t(lane)] = w;
+                S.orig[base + size_t(lane)] = E.orig[size_t(w)];
+                if (withDiag) S.dsrc[base + size_t(lane)] = E.dsrc[size_t(w)];
+                const int p0 = E.ptr[size_t(w)], len = E.ptr[size_t(w) + 1] - p0;
+                for (int k = 0; k < width; ++k) {
+                    const size_t at = cbase + size_t(k) * kSliceRows + size_t(lane);
+                    if (k < len) { S.col[at] = E.col[size_t(p0 + k)]; S.src[at] = E.src[size_t(p0 + k)]; }
+                }
+            }
+            S.nslots += width;
+            S.nslices++;
+        }
+        L.sStart[size_t(l) + 1] = S.nslices;
+        L.qStart[size_t(l) + 1] = int(S.lrow.size());
+    }
+}
+
+// ------------------------------------------------------------------ aggregation AMG set-up (amg.cpp:137-252)
+namespace {
+void csrDiag(HostCsr& A) {
+    A.diag.assign(size_t(A.n), -1);
+    for (int i = 0; i < A.n; ++i)
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) if (A.ci[size_t(p)] == i) { A.diag[size_t(i)] = p; break; }
+}
+
+// strength of connection: |a_ij| >= 0.08 sqrt|a_ii a_jj| or >= 0.25 max_k |a_ik| (amg.cpp:150-164)
+void strength(const HostCsr& A, std::vector<int>& sp, std::vector<int>& sc) {
+    const int n = A.n;
+    sp.assign(size_t(n) + 1, 0);
+    sc.clear();
+    sc.reserve(A.ci.size());
+    for (int i = 0; i < n; ++i) {
+        double rowMax = 0;
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) if (A.ci[size_t(p)] != i) rowMax = std::max(rowMax, std::fabs(A.v[size_t(p)]));
+        const double di = A.v[size_t(A.diag[size_t(i)])];
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+            const int j = A.ci[size_t(p)];
+            if (j == i) continue;
+            const double a = std::fabs(A.v[size_t(p)]);
+            if (a >= 0.08 * std::sqrt(std::fabs(di * A.v[size_t(A.diag[size_t(j)])])) || a >= 0.25 * rowMax) sc.push_back(j);
+        }
+        sp[size_t(i) + 1] = int(sc.size());
+    }
+}
+
+// greedy aggregation in three passes (amg.cpp:165-193); returns the number of aggregates
+int aggregate(const HostCsr& A, const std::vector<int>& sp, const std::vector<int>& sc, std::vector<int>& agg) {
+    const int n = A.n;
+    agg.assign(size_t(n), -1);
+    int na = 0;
+    for (int i = 0; i < n; ++i) {   // roots whose strong neighbourhood is still free
+        if (agg[size_t(i)] >= 0 || sp[size_t(i)] == sp[size_t(i) + 1]) continue;
+        bool freeNb = true;
+        for (int p = sp[size_t(i)]; p < sp[size_t(i) + 1] && freeNb; ++p) freeNb = agg[size_t(sc[size_t(p)])] < 0;
+        if (!freeNb) continue;
+        agg[size_t(i)] = na;
+        for (int p = sp[size_t(i)]; p < sp[size_t(i) + 1]; ++p) agg[size_t(sc[size_t(p)])] = na;
+        ++na;
+    }
+    const std::vector<int> first = agg;
+    for (int i = 0; i < n; ++i) {   // join the neighbouring first-pass aggregate with the largest |a_ij|
+        if (agg[size_t(i)] >= 0 || sp[size_t(i)] == sp[size_t(i) + 1]) continue;
+        double best = 0;
+        int ba = -1;
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+            const int j = A.ci[size_t(p)];
+            if (j != i && first[size_t(j)] >= 0 && std::fabs(A.v[size_t(p)]) > best) { best = std::fabs(A.v[size_t(p)]); ba = first[size_t(j)]; }
+        }
+        if (ba >= 0) agg[size_t(i)] = ba;
+    }
+    for (int i = 0; i < n; ++i) {   // leftovers start new aggregates with their free strong neighbours
+        if (agg[size_t(i)] >= 0) continue;
+        agg[size_t(i)] = na;
+        for (int p = sp[size_t(i)]; p < sp[size_t(i) + 1]; ++p) if (agg[size_t(sc[size_t(p)])] < 0) agg[size_t(sc[size_t(p)])] = na;
+        ++na;
+    }
+    return na;
+}
+
+// Galerkin operator of plain aggregation with the summation order of R (A P) formed as two sparse products
+// (amg.cpp:36-68, 232): entries of A P summed per fine row in column order, then per aggregate over its fine rows
+// in ascending order; rows sorted by column.
+void galerkin(AmgLevelHost& L, HostCsr& C) {
+    const HostCsr& A = L.A;
+    const int n = A.n, nc = L.nc;
+    // A P: per fine row, the distinct aggregates of its columns, sorted; entry lists of source positions
+    std::vector<int> apRp(size_t(n) + 1, 0), apCol;
+    L.apPtr.assign(1, 0);
+    L.apSrc.clear();
+    std::vector<int> mark(size_t(nc), -1), slot(size_t(nc), -1);
+    std::vector<std::vector<int>> lists;
+    std::vector<int> cols;
+    for (int i = 0; i < n; ++i) {
+        cols.clear();
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+            const int b = L.agg[size_t(A.ci[size_t(p)])];
+            if (mark[size_t(b)] != i) { mark[size_t(b)] = i; slot[size_t(b)] = int(cols.size()); cols.push_back(b); if (lists.size() < cols.size()) lists.emplace_back(); lists[cols.size() - 1].clear(); }
+            lists[size_t(slot[size_t(b)])].push_back(p);
+        }
+        std::vector<int> idx(cols.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](int x, int y) { return cols[size_t(x)] < cols[size_t(y)]; });
+        for (int t : idx) {
+            apCol.push_back(cols[size_t(t)]);
+            L.apSrc.insert(L.apSrc.end(), lists[size_t(t)].begin(), lists[size_t(t)].end());
+            L.apPtr.push_back(int(L.apSrc.size()));
+        }
+        apRp[size_t(i) + 1] = int(apCol.size());
+    }
+    std::vector<double> ap(apCol.size());
+    for (size_t t = 0; t < apCol.size(); ++t) {
+        double s = A.v[size_t(L.apSrc[size_t(L.apPtr[t])])];
+        for (int u = L.apPtr[t] + 1; u < L.apPtr[t + 1]; ++u) s += A.v[size_t(L.apSrc[size_t(u)])];
+        ap[t] = s;
+    }
+    // R (A P): aggregates' fine rows ascending
+    std::vector<int> rrp(size_t(nc) + 1, 0), rci(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) rrp[size_t(L.agg[size_t(i)]) + 1]++;
+    for (int a = 0; a < nc; ++a) rrp[size_t(a) + 1] += rrp[size_t(a)];
+    {
+        std::vector<int> fill(rrp.begin(), rrp.end() - 1);
+        for (int i = 0; i < n; ++i) rci[size_t(fill[size_t(L.agg[size_t(i)])]++)] = i;
+    }
+    C = HostCsr{};
+    C.n = nc;
+    C.rp.assign(size_t(nc) + 1, 0);
+    L.cPtr.assign(1, 0);
+    L.cSrc.clear();
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int a = 0; a < nc; ++a) {
+        cols.clear();
+        for (int q = rrp[size_t(a)]; q < rrp[size_t(a) + 1]; ++q) {
+            const int i = rci[size_t(q)];
+            for (int t = apRp[size_t(i)]; t < apRp[size_t(i) + 1]; ++t) {
+                const int b = apCol[size_t(t)];
+                if (mark[size_t(b)] != a) { mark[size_t(b)] = a; slot[size_t(b)] = int(cols.size()); cols.push_back(b); if (lists.size() < cols.size()) lists.emplace_back(); lists[cols.size() - 1].clear(); }
+                lists[size_t(slot[size_t(b)])].push_back(t);
+            }
+        }
+        std::vector<int> idx(cols.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::sort(idx.begin(), idx.end(), [&](int x, int y) { return cols[size_t(x)] < cols[size_t(y)]; });
+        for (int t : idx) {
+            C.ci.push_back(cols[size_t(t)]);
+            L.cSrc.insert(L.cSrc.end(), lists[size_t(t)].begin(), lists[size_t(t)].end());
+            L.cPtr.push_back(int(L.cSrc.size()));
+            double s = ap[size_t(lists[size_t(t)][0])];
+            for (size_t u = 1; u < lists[size_t(t)].size(); ++u) s += ap[size_t(lists[size_t(t)][u])];
+            C.v.push_back(s);
+        }
+        C.rp[size_t(a) + 1] = int(C.ci.size());
+    }
+    csrDiag(C);
+    for (int a = 0; a < nc; ++a) {
+        if (C.diag[size_t(a)] < 0) throw Error(-6, "coarse operator without a diagonal entry");
+        if (std::fabs(C.v[size_t(C.diag[size_t(a)])]) < 1e-300) C.v[size_t(C.diag[size_t(a)])] = 1.0;
+    }
+}
+}  // namespace
+
+void amgSetup(HostCsr A0, std::vector<AmgLevelHost>& H) {
+    H.clear();
+    csrDiag(A0);
+    for (int i = 0; i < A0.n; ++i) {
+        if (A0.diag[size_t(i)] < 0) throw Error(-6, "pressure matrix row without a diagonal entry");
+        if (std::fabs(A0.v[size_t(A0.diag[size_t(i)])]) < 1e-300) A0.v[size_t(A0.diag[size_t(i)])] = 1.0;
+    }
+    H.emplace_back();
+    H[0].A = std::move(A0);
+    std::vector<int> sp, sc, agg;
+    while (true) {
+        AmgLevelHost& L = H.back();
+        const int n = L.A.n;
+        if (n <= 400 || H.size() >= 14) break;
+        strength(L.A, sp, sc);
+        const int na = aggregate(L.A, sp, sc, agg);
+        if (na >= int(0.85 * n) || na < 1) break;
+        L.agg = agg;
+        L.nc = na;
+        AmgLevelHost next;
+        galerkin(L, next.A);
+        H.push_back(std::move(next));
+    }
+}
+
+}  // namespace rsh

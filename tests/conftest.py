@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+    return oracle.oracle()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    import oracle
+    r = oracle.reference()
+    if r is None:
+        pytest.skip("compiled reference (oracle/_ref) not built here")
+    return r

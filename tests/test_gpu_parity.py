@@ -1,0 +1,173 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle restatement, on seeded systems small enough for the
+oracle to finish in seconds.  Kernel-level outputs that keep the reference's order of operations are compared
+bit for bit; Krylov solves by iteration count and by solution error against the FP64 oracle solution
+(<= 1e-6 relative, the path's tolerance), with the true relative residual <= the requested 1e-8."""
+import numpy as np
+import pytest
+
+import ressimstudio_amd as rs
+from ressimstudio_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def systems():
+    return {
+        "small": gen.small(),
+        "laplace": gen.block_laplace(40, 30, 3, 11),
+        "chain": gen.chain(),
+        "nb2": gen.reservoir(12, 10, 3, 2, model=0, sigma=1.5, wells=2, seed=5),
+        "nb4": gen.reservoir(10, 9, 3, 4, model=0, sigma=0.8, wells=1, seed=6),
+        "nb1": gen.reservoir(16, 12, 4, 1, model=0, sigma=1.0, wells=2, seed=8),
+        "layered": gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7),
+    }
+
+
+@pytest.fixture(scope="module")
+def sysd():
+    return systems()
+
+
+def _true_rel(s, x, b):
+    import oracle
+    r = b - oracle.oracle().spmv(s, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def test_device_present():
+    assert rs.device_count() >= 1
+
+
+@pytest.mark.parametrize("name", ["small", "laplace", "chain", "nb2", "nb4", "nb1", "layered"])
+def test_spmv_matches_oracle(sysd, orc, name):
+    s = sysd[name]
+    d = rs.System(s)
+    x = np.random.default_rng(1).standard_normal(s.n * s.nb)
+    y = d.spmv(x)
+    yo = orc.spmv(s, x)
+    info = d.info()
+    if info["spmv_long_rows"] == 0:
+        assert np.array_equal(y, yo), "rows handled by one lane keep the reference's order: bit-identical"
+    assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
+
+
+@pytest.mark.parametrize("name", ["small", "laplace", "chain", "nb2", "nb4", "nb1", "layered"])
+def test_bilu0_factors_and_apply(sysd, orc, name):
+    s = sysd[name]
+    d = rs.System(s).setup(precond="bilu0")
+    o = orc.bilu0(s, 1)
+    lu, luo = d.bilu0_factors(), o.factors()
+    assert np.array_equal(lu, luo), "level-scheduled factorisation reproduces the natural-order factors bit for bit"
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    z, zo = d.precond(r), o.apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("name", ["small", "laplace", "layered", "nb2"])
+def test_mcbilu0_apply(sysd, orc, name):
+    s = sysd[name]
+    d = rs.System(s).setup(precond="mcbilu0")
+    o = orc.mcbilu0(s)
+    r = np.sin(0.37 * np.arange(s.n * s.nb)) + 0.5
+    z, zo = d.precond(r), o.apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("name", ["small", "laplace", "layered", "nb2", "nb4", "chain"])
+def test_cpr_pieces(sysd, orc, name):
+    s = sysd[name]
+    d = rs.System(s).setup(precond="cpr", smoother=rs.SMOOTH_GS)
+    o = orc.cpr(s)
+    assert np.array_equal(d.cpr_weights(), o.weights())
+    levels = d.info()["amg_levels"]
+    assert levels == [n for n, _ in o.levels()]
+    rp = np.cos(0.11 * np.arange(s.n))
+    xa, xo = d.amg_apply(rp), o.amg_apply(rp)
+    assert np.max(np.abs(xa - xo)) <= 1e-10 * np.max(np.abs(xo))
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    z, zo = d.precond(r), o.apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-10 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("prec", ["bilu0", "mcbilu0", "cpr"])
+@pytest.mark.parametrize("name", ["small", "layered", "nb2", "chain"])
+def test_bicgstab_matches_oracle(sysd, orc, name, prec):
+    s = sysd[name]
+    d = rs.System(s).setup(precond=prec, krylov=rs.KRYLOV_BICGSTAB)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+    xo, its_o, rel_o, ok_o = orc.krylov(s, s.rhs, prec, "bicgstab", TOL, 2000)
+    assert ok and ok_o
+    assert abs(its - its_o) <= max(1, its_o // 20)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert _true_rel(s, x, s.rhs) <= 10 * TOL
+
+
+@pytest.mark.parametrize("prec", ["bilu0", "cpr"])
+@pytest.mark.parametrize("name", ["small", "layered"])
+def test_fgmres_matches_oracle_solution(sysd, orc, name, prec):
+    s = sysd[name]
+    d = rs.System(s).setup(precond=prec, krylov=rs.KRYLOV_FGMRES, restart=30)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+    xo, its_o, _, ok_o = orc.krylov(s, s.rhs, prec, "gmres", TOL, 2000, restart=30)
+    assert ok and ok_o
+    assert abs(its - its_o) <= max(2, its_o // 10)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("prec", ["bilu0", "cpr", "cpr_mc"])
+@pytest.mark.parametrize("name", ["small", "layered", "nb4"])
+def test_mixed_precision_error_vs_fp64(sysd, orc, name, prec):
+    s = sysd[name]
+    d = rs.System(s).setup(precond=prec, precision=rs.MIXED)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok and ok_o
+    assert _true_rel(s, x, s.rhs) <= TOL
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"mixed {name}/{prec}: {its} inner its, {d.info()['last_outer']} refinements, error vs FP64 {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_jacobi_smoother_converges(sysd, orc):
+    s = sysd["layered"]
+    d = rs.System(s).setup(precond="cpr", smoother=rs.SMOOTH_JACOBI)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+    xo, its_o, _, _ = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 2000)
+    assert ok
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_refresh_equals_fresh_setup(sysd, orc):
+    s = sysd["layered"]
+    d = rs.System(s).setup(precond="cpr", reuse_amg=1)
+    v2 = s.val * 1.5
+    d.set_values(v2)
+    d.setup(precond="cpr", reuse_amg=1)
+    fresh = rs.System(s)
+    fresh.set_values(v2)
+    fresh.setup(precond="cpr", reuse_amg=0)
+    r = np.cos(0.3 * np.arange(s.n * s.nb))
+    z1, z2 = d.precond(r), fresh.precond(r)
+    assert np.max(np.abs(z1 - z2)) <= 1e-10 * np.max(np.abs(z2))
+
+
+def test_zero_rhs_and_errors(sysd):
+    s = sysd["small"]
+    d = rs.System(s).setup(precond="bilu0")
+    x, its, rel, ok = d.solve(np.zeros(s.n * s.nb), TOL, 10)
+    assert ok and its == 0 and np.all(x == 0)
+    with pytest.raises(rs.RshError):
+        rs.System(rs.BSR(2, 3, np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.zeros(18)))
+
+
+def test_hook_adapter(sysd, orc):
+    s = sysd["small"]
+    hook = rs.HipBlockSolver("cpr")
+    ok, x, its = hook.solve(s, s.rhs, TOL, 500)
+    xo, its_o, _, _ = orc.krylov(s, s.rhs, "cpr", "bicgstab", TOL, 500)
+    assert ok and abs(its - its_o) <= 1
+    ok2, x2, _ = hook.solve(s, s.rhs, TOL, 500)     # cached system: values re-uploaded, hierarchy refreshed
+    assert ok2 and np.array_equal(x, x2)
+    assert hook.device_solves == 2 and hook.failures == 0
