@@ -1,0 +1,299 @@
+"""Pins the oracle restatement (oracle/oracle.cpp) before it is trusted as the checker of the HIP path:
+  * against the committed fixtures written by the reference's own compiled code (tests/golden/),
+  * against the compiled reference itself where it is built (oracle/_ref),
+  * against what the mathematics fixes: closed forms, textbook special cases, dense linear algebra on tiny inputs,
+    and the reference tests' own known-answer checks (run_tests.cpp:1531-1655, c18_tests.inc:142-218).
+CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from ressimstudio_amd import gen
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def _dense(s):
+    return sp.bsr_matrix((np.asarray(s.val).reshape(-1, s.nb, s.nb), s.col, s.row_ptr), shape=(s.n * s.nb, s.n * s.nb)).toarray()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLD, "ref_vectors.json")) as f:
+        meta = json.load(f)
+    arrs = np.load(os.path.join(GOLD, "ref_vectors.npz"))
+    return meta, arrs
+
+
+def golden_cases():
+    return {
+        "small": gen.small(),
+        "chain": gen.chain(),
+        "laplace": gen.block_laplace(40, 30, 3, 11),
+        "nb2": gen.reservoir(12, 10, 3, 2, model=0, sigma=1.5, wells=2, seed=5),
+        "layered": gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7),
+    }
+
+
+@pytest.fixture(scope="module")
+def cases():
+    return golden_cases()
+
+
+# ------------------------------------------------------------------ fixtures from the compiled reference
+def test_generator_reproduces_fixture_inputs(golden, cases):
+    meta, _ = golden
+    for name, s in cases.items():
+        c = meta["cases"][name]
+        assert (s.n, s.nb, s.nnzb) == (c["n"], c["nb"], c["nnzb"])
+        assert float(np.sum(s.val)) == c["val_sum"] and float(np.sum(s.rhs)) == c["rhs_sum"]
+
+
+@pytest.mark.parametrize("name", ["small", "chain", "laplace", "nb2", "layered"])
+def test_oracle_solves_equal_reference_fixtures(golden, cases, orc, name):
+    meta, arrs = golden
+    s = cases[name]
+    for key, g in meta["cases"][name]["solves"].items():
+        prec, meth = key.split("/")
+        x, its, rel, conv = orc.krylov(s, s.rhs, prec, meth, 1e-8, 2000, 40)
+        assert (its, conv) == (g["its"], g["conv"]), key
+        assert rel == g["rel"] and float(np.linalg.norm(x)) == g["x_norm"], key
+        xk = f"{name}__{prec}__{meth}__x"
+        if xk in arrs:
+            assert np.array_equal(x, arrs[xk])
+
+
+@pytest.mark.parametrize("name", ["small", "chain", "laplace", "nb2", "layered"])
+def test_oracle_preconditioners_equal_reference_fixtures(golden, cases, orc, name):
+    meta, arrs = golden
+    s = cases[name]
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    cp = orc.cpr(s)
+    vecs = {"bilu0_apply": orc.bilu0(s, 1).apply(r), "mcbilu0_apply": orc.mcbilu0(s).apply(r), "cpr_weights": cp.weights(),
+            "cpr_apply": cp.apply(r)}
+    for k, v in vecs.items():
+        if f"{name}__{k}" in arrs:
+            assert np.array_equal(v, arrs[f"{name}__{k}"]), k
+        assert [float(np.sum(v)), float(np.linalg.norm(v))] == meta["cases"][name]["checksums"][k], k
+    k, colour, _ = orc.mcbilu0(s).colouring()
+    assert k == meta["cases"][name]["colours"]
+    assert np.array_equal(colour, arrs[f"{name}__colour"].astype(np.int32))
+    assert [list(t) for t in cp.levels()] == meta["cases"][name]["amg_levels"]
+    if len(cp.levels()) > 1:
+        assert np.array_equal(cp.aggregates(0)[1], arrs[f"{name}__agg0"])
+
+
+# ------------------------------------------------------------------ the compiled reference, live (where built)
+@pytest.mark.parametrize("seed", [3, 4])
+def test_oracle_equals_compiled_reference(orc, ref, seed):
+    for s in [gen.reservoir(9, 8, 3, 3, model=0, sigma=1.2, wells=2, seed=seed), gen.reservoir(8, 14, 6, 4, model=1, sigma=2.0, kvkh=0.1, wells=2, seed=seed),
+              gen.reservoir(11, 7, 5, 1, model=0, sigma=1.0, wells=1, seed=seed)]:
+        x = np.random.default_rng(seed).standard_normal(s.n * s.nb)
+        assert np.array_equal(orc.spmv(s, x), ref.spmv(s, x))
+        for mode in (1, 3):
+            assert np.array_equal(orc.bilu0(s, mode).apply(x), ref.bilu0(s, mode).apply(x))
+        assert np.array_equal(orc.mcbilu0(s).apply(x), ref.mcbilu0(s).apply(x))
+        assert np.array_equal(orc.cpr(s).apply(x), ref.cpr(s).apply(x))
+        for prec in ["bilu0", "mcbilu0", "cpr", "bilu0f"]:
+            a = orc.krylov(s, s.rhs, prec, "bicgstab", 1e-9, 500)
+            b = ref.krylov(s, s.rhs, prec, "bicgstab", 1e-9, 500)
+            assert a[1:] == b[1:] and np.array_equal(a[0], b[0])
+
+
+def test_float_factor_mode_equals_reference(orc, ref):
+    # float32 factors after a double factorisation (C38) exist in the reference only in its level-ordered layout,
+    # forced on for a 40x30x8 grid; the sweeps' arithmetic per row is the same as in natural order
+    s = gen.reservoir(40, 30, 8, 3, model=0, sigma=1.0, wells=0, seed=9)
+    r = np.sin(0.3 * np.arange(s.n * s.nb))
+    a, b = orc.bilu0(s, 2).apply(r), ref.bilu0(s, 2).apply(r)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+def test_cpr_refresh_equals_reference(orc, ref):
+    s = gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7)
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    co, cr = orc.cpr(s), ref.cpr(s)
+    assert cr.refresh(s.val * 1.5) == 1
+    co.refresh(s.val * 1.5)
+    assert np.array_equal(co.apply(r), cr.apply(r))
+
+
+# ------------------------------------------------------------------ what the mathematics fixes
+def test_spmv_equals_dense_product(orc):
+    s = gen.reservoir(7, 6, 3, 3, model=0, wells=2, seed=2)
+    x = np.random.default_rng(0).standard_normal(s.n * s.nb)
+    assert np.allclose(orc.spmv(s, x), _dense(s) @ x, rtol=1e-13, atol=1e-13 * np.abs(x).max())
+
+
+def test_bilu0_exact_on_block_tridiagonal(orc):
+    # run_tests.cpp:1596-1607: ILU(0) has no dropped fill on a block-tridiagonal pattern, so M^-1 A x = x
+    rng = np.random.default_rng(5)
+    n, nb = 50, 3
+    rp, ci, vals = [0], [], []
+    for i in range(n):
+        for j in range(max(0, i - 1), min(n - 1, i + 1) + 1):
+            ci.append(j)
+            blk = np.full((nb, nb), -0.5) + 0.1 * rng.uniform(-1, 1, (nb, nb))
+            if j == i:
+                blk = np.ones((nb, nb)) + 5.0 * np.eye(nb) + 0.1 * rng.uniform(-1, 1, (nb, nb))
+            vals.append(blk)
+        rp.append(len(ci))
+    s = gen.BlockSystem(n=n, nb=nb, row_ptr=np.array(rp, np.int32), col=np.array(ci, np.int32), val=np.array(vals), rhs=np.zeros(n * nb))
+    x = 1.0 + np.arange(n * nb) % 7
+    z = orc.bilu0(s, 1).apply(orc.spmv(s, x))
+    assert np.max(np.abs(z - x)) < 1e-10
+
+
+def test_scalar_bilu0_is_textbook_ilu0(orc):
+    # nb = 1: block ILU(0) is scalar ILU(0): (L U)_ij = A_ij on the pattern of A, L unit lower, U upper
+    s = gen.reservoir(9, 7, 4, 1, model=0, sigma=1.0, wells=2, seed=11)
+    lu = orc.bilu0(s, 1).factors()
+    n = s.n
+    L = sp.lil_matrix((n, n))
+    U = sp.lil_matrix((n, n))
+    for i in range(n):
+        L[i, i] = 1.0
+        for p in range(s.row_ptr[i], s.row_ptr[i + 1]):
+            j = s.col[p]
+            if j < i:
+                L[i, j] = lu[p]
+            elif j == i:
+                U[i, i] = 1.0 / lu[p]           # the diagonal stores U_ii^-1
+            else:
+                U[i, j] = lu[p]
+    P = (L.tocsr() @ U.tocsr()).toarray()
+    A = _dense(s)
+    mask = A != 0
+    assert np.allclose(P[mask], A[mask], rtol=1e-12, atol=1e-12 * np.abs(A).max())
+
+
+def test_multicolour_known_answers(orc):
+    # c18_tests.inc:205-215: 5-point -> red-black; MC-BILU0 = BILU0 of P A P^T; colouring valid with wells
+    lap = gen.block_laplace(120, 100, 3, 11)
+    k, colour, perm = orc.mcbilu0(lap).colouring()
+    assert k == 2
+    s = gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7)
+    m = orc.mcbilu0(s)
+    k, colour, perm = m.colouring()
+    for i in range(s.n):
+        for p in range(s.row_ptr[i], s.row_ptr[i + 1]):
+            j = s.col[p]
+            assert j == i or colour[i] != colour[j]
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(s.n)
+    A = sp.bsr_matrix((s.val, s.col, s.row_ptr), shape=(s.n * 3, s.n * 3)).tobsr(blocksize=(3, 3))
+    Pb = sp.csr_matrix((np.ones(s.n), (np.arange(s.n), perm)), shape=(s.n, s.n))
+    Pm = sp.kron(Pb, sp.eye(3)).tocsr()
+    PA = (Pm @ A @ Pm.T).tobsr(blocksize=(3, 3))
+    PA.sort_indices()
+    ps = gen.BlockSystem(n=s.n, nb=3, row_ptr=PA.indptr.astype(np.int32), col=PA.indices.astype(np.int32), val=PA.data, rhs=np.zeros(s.n * 3))
+    r = np.cos(0.7 * np.arange(s.n * 3))
+    z1 = m.apply(r)
+    zp = orc.bilu0(ps, 1).apply(Pm @ r)
+    assert np.max(np.abs(Pm.T @ zp - z1)) <= 1e-12 * np.max(np.abs(z1))
+
+
+@pytest.mark.parametrize("prec,meth", [("bilu0", "bicgstab"), ("cpr", "bicgstab"), ("mcbilu0", "bicgstab"), ("bilu0", "gmres"), ("cpr", "gmres")])
+def test_krylov_solution_against_direct_solve(orc, prec, meth):
+    s = gen.small()
+    x, its, rel, conv = orc.krylov(s, s.rhs, prec, meth, 1e-10, 2000)
+    A = sp.csr_matrix(_dense(s))
+    xd = spla.spsolve(A.tocsc(), s.rhs)
+    assert conv
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    assert np.linalg.norm(s.rhs - A @ x) <= 1e-9 * np.linalg.norm(s.rhs)
+
+
+def test_cpr_weights_and_pressure_matrix_closed_form(orc):
+    s = gen.reservoir(8, 9, 4, 3, model=0, wells=2, seed=3)
+    cp = orc.cpr(s)
+    w = cp.weights().reshape(s.n, 3)
+    diag = [next(p for p in range(s.row_ptr[i], s.row_ptr[i + 1]) if s.col[p] == i) for i in range(s.n)]
+    for i in range(s.n):
+        assert np.allclose(w[i], np.linalg.inv(s.val[diag[i]])[0], rtol=1e-12, atol=1e-14)
+    rp0, ci0, v0 = cp.level_matrix(0)
+    rows = np.repeat(np.arange(s.n), np.diff(s.row_ptr))
+    expect = np.einsum("pe,pe->p", w[rows], s.val[:, :, 0])
+    assert np.array_equal(ci0, s.col) and np.allclose(v0, expect, rtol=1e-14, atol=0)
+
+
+def test_amg_coarse_operator_is_galerkin(orc):
+    s = gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7)
+    cp = orc.cpr(s)
+    levels = cp.levels()
+    assert len(levels) >= 2
+    for l in range(len(levels) - 1):
+        rp, ci, v = cp.level_matrix(l)
+        A = sp.csr_matrix((v, ci, rp), shape=(levels[l][0],) * 2)
+        nc, agg = cp.aggregates(l)
+        P = sp.csr_matrix((np.ones(levels[l][0]), (np.arange(levels[l][0]), agg)), shape=(levels[l][0], nc))
+        rp1, ci1, v1 = cp.level_matrix(l + 1)
+        C = sp.csr_matrix((v1, ci1, rp1), shape=(nc, nc)).toarray()
+        G = (P.T @ A @ P).toarray()
+        assert np.allclose(C, G, rtol=1e-12, atol=1e-12 * np.abs(G).max())
+        # aggregation covers every row and coarsens
+        assert agg.min() == 0 and agg.max() == nc - 1 and nc < 0.85 * levels[l][0]
+
+
+def test_amg_refresh_equals_fresh_setup(orc):
+    # run_tests.cpp:1588-1593: same pattern, new values -> refresh matches a fresh setup
+    s = gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7)
+    a = orc.cpr(s)
+    a.refresh(s.val * 1.5)
+    s2 = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=s.val * 1.5, rhs=s.rhs)
+    b = orc.cpr(s2)
+    r = np.cos(0.3 * np.arange(s.n * 3))
+    za, zb = a.apply(r), b.apply(r)
+    assert np.max(np.abs(za - zb)) <= 1e-10 * np.max(np.abs(zb))
+
+
+def test_well_schur_recovers_full_solution(orc):
+    # run_tests.cpp:1570-1578: Schur-reduced solve + BHP recovery = the exact full solution
+    c = gen.chain()
+    x, its, rel, conv = orc.krylov(c, c.rhs, "bilu0", "gmres", 1e-12, 200, 40)
+    assert conv
+    full = c.recover(c.full["rhs"], x)
+    xt = np.sin(0.1 * np.arange(c.full["n"])) + 2.0
+    assert np.max(np.abs(full - xt)) < 1e-8
+
+
+def test_well_schur_is_dense_schur_complement():
+    s = gen.reservoir(6, 5, 3, 3, model=0, wells=2, seed=4)
+    f = s.full
+    J = sp.csr_matrix((f["val"], f["col"], f["row_ptr"]), shape=(f["n"], f["n"])).toarray()
+    nc = f["na"] * f["nv"]
+    Acc, Acw, Awc, Aww = J[:nc, :nc], J[:nc, nc:], J[nc:, :nc], J[nc:, nc:]
+    assert np.count_nonzero(Aww - np.diag(np.diag(Aww))) == 0
+    S = Acc - Acw @ np.diag(1.0 / np.diag(Aww)) @ Awc
+    assert np.allclose(_dense(s), S, rtol=1e-13, atol=1e-13 * np.abs(S).max())
+    bc = f["rhs"][:nc] - Acw @ (f["rhs"][nc:] / np.diag(Aww))
+    assert np.allclose(s.rhs, bc, rtol=1e-13, atol=1e-14)
+    assert s.fill_blocks > 0
+
+
+def test_generator_named_workloads():
+    s = gen.small()
+    assert (s.n, s.nb) == (1600, 3) and s.full["nw"] == 2
+    # the 7-point cell pattern: interior rows have 7 blocks; perforated cells gain the Schur fill
+    lens = np.diff(s.row_ptr)
+    assert lens.min() == 4 and np.median(lens) >= 6
+    big = gen.large(scale=0.25)
+    assert big.full["nw"] == 5 and big.nb == 3
+
+
+def test_planted_mistake_is_caught(orc):
+    # the checks above are sharp: a restatement with a dropped term (here: the diagonal of one block flipped)
+    # changes the preconditioner output well above the tolerances used against the HIP path
+    s = gen.small()
+    r = np.cos(0.7 * np.arange(s.n * 3))
+    z = orc.bilu0(s, 1).apply(r)
+    bad = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=s.val.copy(), rhs=s.rhs)
+    bad.val[5, 1, 1] *= -1
+    zb = orc.bilu0(bad, 1).apply(r)
+    assert np.max(np.abs(z - zb)) > 1e-6 * np.max(np.abs(z))
