@@ -7,9 +7,12 @@ input already resident in HBM: new matrix values handed over on the device (D2D)
 BILU0 factorisation) and the Krylov solve.  The symbolic set-up (levels, slices, aggregation) happens once per
 pattern, before the timed region, as in the Newton loop.
 
-Printed: one JSON line.  value = CPR-BiCGSTAB FP64 solves/s (the drop-in); beside it the plain FP64
-ILU0-BiCGSTAB and the mixed-precision CPR solves/s, the roofline of the dominant kernel and the CPU baseline (the
-reference's own solver code compiled from its sources, OpenMP, timed on a bounded sample on this host).
+Printed: one JSON line.  value = solves/s of the FP64 CPR-BiCGSTAB drop-in (variant `cpr_mc`: quasi-IMPES
+weights, aggregation AMG V(1,1) with damped-Jacobi smoothing, multicolour BILU0 second stage); beside it every other
+variant (natural-order BILU0 second stage, plain FP64 ILU0-BiCGSTAB, mixed precision; --extra adds the exact
+Gauss-Seidel AMG, which reproduces the reference CPU arithmetic and is slow), the roofline of the dominant kernel and
+the CPU baseline (the reference's own solver code compiled from its sources, OpenMP, timed on a bounded sample on
+this host).
 """
 from __future__ import annotations
 
@@ -89,11 +92,10 @@ def time_config(rs, sysd, dev, opts, tol, max_it, steps, warmup):
     return dt, its, rel, info
 
 
-def cpu_baseline(s, its_full):
-    """The reference's BlockCPR + BiCGSTAB compiled from its sources (OpenMP, -O3 -mavx2 -mfma), on this host's
-    cores.  Bounded sample: a solve capped at 1 and at 6 iterations on the full system gives the set-up time and
-    the time per iteration; one full solve = set-up + its_full iterations (its_full from the device run of the
-    same algorithm)."""
+def cpu_baseline(s, tol):
+    """The reference's BlockCPR + BiCGSTAB (BlockCPR::setup + bicgstab, bsr.cpp:993-1084, linsolve.cpp:159-216)
+    compiled from its sources with its release flags (OpenMP, -O3 -mavx2 -mfma), on this host's cores: one complete
+    set-up + solve of the same system to the same tolerance (about 20 s of CPU work)."""
     import oracle
     ref = oracle.reference(fast=True)
     kind = "reference"
@@ -102,17 +104,12 @@ def cpu_baseline(s, its_full):
         kind = "port"
     threads = ref.set_threads(0)
     t0 = time.perf_counter()
-    ref.krylov(s, s.rhs, "cpr", "bicgstab", 1e-8, 1)
-    t1 = time.perf_counter()
-    ref.krylov(s, s.rhs, "cpr", "bicgstab", 1e-8, 6)
-    t2 = time.perf_counter()
-    per_it = max((t2 - t1) - (t1 - t0), 1e-9) / 5.0
-    setup = max((t1 - t0) - per_it, 0.0)
-    solve_s = setup + its_full * per_it
-    return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": int(threads), "kind": kind,
-            "sample": f"full {s.n}-cell system, CPR-BiCGSTAB capped at 1 and 6 iterations: set-up {setup:.2f} s + "
-                      f"{per_it * 1e3:.1f} ms/iteration x {its_full} iterations (device iteration count)",
-            "setup_s": setup, "ms_per_iteration": per_it * 1e3}
+    _, its, rel, conv = ref.krylov(s, s.rhs, "cpr", "bicgstab", tol, 3000)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": int(threads), "kind": kind,
+            "sample": f"one full CPR-BiCGSTAB set-up + solve of the {s.n}-cell system to {tol:g}: {dt:.1f} s, {its} iterations, "
+                      f"rel.res {rel:.2e}, converged {conv}",
+            "iterations": its}
 
 
 def main():
@@ -124,7 +121,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-it", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also time the other preconditioner variants")
+    ap.add_argument("--extra", action="store_true", help="also time the exact Gauss-Seidel CPR (slow)")
+    ap.add_argument("--configs", default="", help="comma list of variants to run (default: all but the --extra ones)")
+    ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timing loops (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus})", file=sys.stderr)
@@ -140,24 +139,35 @@ def main():
     sysd = rs.System(s, ctx)
     dev = DeviceInputs(rs, ctx, s)
 
-    cfgs = {
-        "cpr_fp64": rs.default_options(precond="cpr", precision=rs.FP64, krylov=rs.KRYLOV_BICGSTAB, smoother=rs.SMOOTH_GS),
-        "ilu0_fp64": rs.default_options(precond="bilu0", precision=rs.FP64, krylov=rs.KRYLOV_BICGSTAB),
-        "cpr_mixed": rs.default_options(precond="cpr", precision=rs.MIXED, smoother=rs.SMOOTH_GS),
+    J, GS, FP64, MIXED, BI = rs.SMOOTH_JACOBI, rs.SMOOTH_GS, rs.FP64, rs.MIXED, rs.KRYLOV_BICGSTAB
+    all_cfgs = {
+        "cpr_mc": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J),
+        "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J),
+        "ilu0": rs.default_options(precond="bilu0", precision=FP64, krylov=BI),
+        "mcilu0": rs.default_options(precond="mcbilu0", precision=FP64, krylov=BI),
+        "cpr_mc_mixed": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=BI, smoother=J),
+        "cpr_mixed": rs.default_options(precond="cpr", precision=MIXED, krylov=BI, smoother=J),
+        "cpr_exact_gs": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=GS),
     }
-    if args.extra:
-        cfgs["cpr_mc_jacobi_mixed"] = rs.default_options(precond="cpr_mc", precision=rs.MIXED, smoother=rs.SMOOTH_JACOBI)
-        cfgs["cpr_jacobi_fp64"] = rs.default_options(precond="cpr", precision=rs.FP64, smoother=rs.SMOOTH_JACOBI)
+    names = [c for c in args.configs.split(",") if c] or [n for n in all_cfgs if n != "cpr_exact_gs" or args.extra]
+    cfgs = {n: all_cfgs[n] for n in names}
+    if args.kernels_only:
+        step(rs, sysd, dev, cfgs[names[0]], args.tol, 1)
+        for kind, name in [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps")]:
+            ms, by = sysd.time_kernel(kind, 20)
+            log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {by / ms / 1e6:.0f} GB/s" if by > 0 else ""))
+        return
     results = {}
     x_fp64 = None
     for name, o in cfgs.items():
         # symbolic set-up (once per pattern) outside the timed region
         step(rs, sysd, dev, o, args.tol, args.max_it)
-        dt, its, rel, info = time_config(rs, sysd, dev, o, args.tol, args.max_it, args.steps, args.warmup)
+        steps = args.steps if name == names[0] else 1
+        dt, its, rel, info = time_config(rs, sysd, dev, o, args.tol, args.max_it, steps, args.warmup if name == names[0] else 0)
         x = dev.x_host()
-        results[name] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iterations": its, "rel_res": rel,
+        results[name] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iterations": its, "rel_res": rel, "steps": steps,
                          "setup_ms": info["last_setup_ms"], "solve_ms": info["last_solve_ms"], "outer": info["last_outer"]}
-        if name == "cpr_fp64":
+        if x_fp64 is None and o.precision == FP64:
             x_fp64 = x
         elif x_fp64 is not None:
             results[name]["err_vs_fp64"] = float(np.linalg.norm(x - x_fp64) / np.linalg.norm(x_fp64))
@@ -165,28 +175,29 @@ def main():
             f"{its} its, rel {rel:.2e}")
 
     # kernel timing (HIP events on the context's stream) for the roofline of the dominant kernel
-    sysd.setup(cfgs["cpr_fp64"])
+    head = names[0]
+    sysd.setup(cfgs[head])
     k = {}
     for kind, name in [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps"), (3, "amg_vcycle"), (1, "cpr_apply")]:
         ms, by = sysd.time_kernel(kind, 10 if kind in (1, 2, 3) else 50)
         k[name] = {"ms": ms, "bytes": by, "GBs": (by / (ms * 1e-3) / 1e9) if by > 0 else None}
         log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {k[name]['GBs']:.0f} GB/s" if by > 0 else ""))
-    its_cpr = results["cpr_fp64"]["iterations"]
+    its_cpr = results[head]["iterations"]
     # the preconditioner dominates (two applies per iteration); within it the BILU0 sweeps are the largest part
     dom = "ilu_sweeps"
-    roof = {"bound": "hbm", "kernel": "BILU0 level-scheduled sweeps (forward+backward, one launch per level)",
+    roof = {"bound": "hbm", "kernel": f"BILU0 sweeps of the {head} preconditioner (forward + backward, all launches of one apply)",
             "achieved": k[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
             "spmv_GBs": k["spmv"]["GBs"], "spmv_frac": k["spmv"]["GBs"] / HBM_PEAK_GBS, "copy_GBs": k["copy"]["GBs"]}
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(s, its_cpr)
+            cpu = cpu_baseline(s, args.tol)
             log(f"cpu baseline: {cpu['value']:.4f} solves/s on {cpu['cores']} threads ({cpu['sample']})")
         except Exception as e:  # noqa: BLE001
             log(f"cpu baseline unavailable: {e}")
 
-    main_r = results["cpr_fp64"]
+    main_r = results[head]
     out = {
         "metric": "CPR/ILU0 solves/s @1.1M cells, rel.res 1e-8; HBM GB/s; p50/p99 batched latency",
         "value": main_r["solves_per_s"],
@@ -200,8 +211,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator, ressimstudio_amd/csrc/gen)",
-        "config": {"workload": f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells, CPR(quasi-IMPES, aggregation AMG "
-                               f"V(1,1) Gauss-Seidel) + BILU0 BiCGSTAB to 1e-8 (setup + solve per step)",
+        "config": {"workload": f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells; variant {head}: BiCGSTAB to 1e-8 "
+                               f"preconditioned by CPR (quasi-IMPES, aggregation AMG V(1,1)) + BILU0; numeric setup + solve per step",
                    "cells": s.n, "block": s.nb, "blocks": s.nnzb, "tol": args.tol, "scale": args.scale},
         "iterations": main_r["iterations"],
         "variants": results,

@@ -8,9 +8,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <sstream>
+#include <tuple>
 
 #include "../../../include/ressim_hip.h"
 #include "rsh_internal.hpp"
@@ -93,8 +96,15 @@ struct IluDev {
     DBuf<float> y32;
     DBuf<int> nreg;
     int regularised = 0;
-
+    int generation = 0;
+    // factorisation schedule: per level the rows eliminated one per thread, then the long rows one per wavefront
+    DBuf<int> fShort, fLong;
+    std::vector<int> fShortStart, fLongStart;
+    // sweep schedule: runs of levels (fused small levels or single launches) and device copies of level starts
+    std::vector<LevelRun> runsL, runsU;
+    DBuf<int> dLs, dLq, dUs, dUq;
     void build(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, bool multicolour, hipStream_t s) {
+        ++generation;
         mc = multicolour;
         n = S.n;
         nb = S.nb;
@@ -141,6 +151,36 @@ struct IluDev {
         buildSlices(EU, lstart, rowCap(lenU), hU, sU);
         uploadSlices(hL, L64, nb, s);
         uploadSlices(hU, U64, nb, s);
+        {
+            const int nl = int(lstart.size()) - 1;
+            std::vector<int> wL(static_cast<size_t>(nl)), wU(static_cast<size_t>(nl));
+            for (int l = 0; l < nl; ++l) {
+                wL[size_t(l)] = (sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)]) + (sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)]);
+                wU[size_t(l)] = (sU.sStart[size_t(l) + 1] - sU.sStart[size_t(l)]) + (sU.qStart[size_t(l) + 1] - sU.qStart[size_t(l)]);
+            }
+            runsL = levelRuns(wL, 16);
+            runsU = levelRuns(wU, 16);
+            dLs.upload(sL.sStart, s);
+            dLq.upload(sL.qStart, s);
+            dUs.upload(sU.sStart, s);
+            dUq.upload(sU.qStart, s);
+            // factor rows (F numbering) per level, long rows apart
+            const HostBsr& F = mc ? fpat : S;
+            std::vector<int> sh, lo;
+            fShortStart.assign(1, 0);
+            fLongStart.assign(1, 0);
+            for (int l = 0; l < nl; ++l) {
+                for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) {
+                    const int i = mc ? w : ord[size_t(w)];
+                    const int lower = F.diag[size_t(i)] - F.rp[size_t(i)];
+                    (lower > 24 ? lo : sh).push_back(i);
+                }
+                fShortStart.push_back(int(sh.size()));
+                fLongStart.push_back(int(lo.size()));
+            }
+            fShort.upload(sh, s);
+            fLong.upload(lo, s);
+        }
         lu.alloc(size_t(S.rp[size_t(n)]) * nb * nb);
         y64.alloc(size_t(n + 1) * nb);
         y64.zero(s);
@@ -161,8 +201,10 @@ struct IluDev {
         else RSH_HIP(hipMemcpyAsync(lu.p, val, nvals * sizeof(double), hipMemcpyDeviceToDevice, s));
         nreg.zero(s);
         const int nl = int(lstart.size()) - 1;
-        for (int l = 0; l < nl; ++l)
-            launchBilu0Level(s, nb, mc ? nullptr : perm.p, lstart[size_t(l)], lstart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+        for (int l = 0; l < nl; ++l) {
+            launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+            launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+        }
         packSlices(s, nb, lu.p, L64, hL.nslots, false);
         packSlices(s, nb, lu.p, U64, hU.nslots, false);
         if (f32) {
@@ -173,17 +215,19 @@ struct IluDev {
         RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     void apply(const double* r, double* z, const double* xp, bool f32, hipStream_t s) const {
-        const int nl = sL.levels();
-        if (f32) {
-            for (int l = 0; l < nl; ++l)
-                launchIluForward(s, nb, true, &L32, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y32.p);
-            for (int l = nl - 1; l >= 0; --l)
-                launchIluBackward(s, nb, true, &U32, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y32.p, z, xp);
-        } else {
-            for (int l = 0; l < nl; ++l)
-                launchIluForward(s, nb, false, &L64, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y64.p);
-            for (int l = nl - 1; l >= 0; --l)
-                launchIluBackward(s, nb, false, &U64, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y64.p, z, xp);
+        const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
+        const void* Up = f32 ? static_cast<const void*>(&U32) : static_cast<const void*>(&U64);
+        void* y = f32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
+        for (const LevelRun& run : runsL) {
+            if (run.fused) { launchIluFused(s, nb, f32, Lp, true, run.l0, run.l1, dLs.p, dLq.p, r, y, z, xp); continue; }
+            const int l = run.l0;
+            launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y);
+        }
+        for (auto it = runsU.rbegin(); it != runsU.rend(); ++it) {
+            const LevelRun& run = *it;
+            if (run.fused) { launchIluFused(s, nb, f32, Up, false, run.l0, run.l1, dUs.p, dUq.p, r, y, z, xp); continue; }
+            const int l = run.l0;
+            launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z, xp);
         }
     }
     size_t bytes() const { return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes(); }
@@ -200,6 +244,8 @@ struct AmgLevelDev {
     DBuf<double> ap;
     DBuf<int> gsF, gsB;                  // rows in forward / backward level order
     std::vector<int> gsFs, gsBs;         // level starts
+    DBuf<int> dgsFs, dgsBs;
+    std::vector<LevelRun> runsF, runsB;
     DBuf<double> x1, x2, b, r;
     const double* result = nullptr;
 };
@@ -217,7 +263,9 @@ struct CprDev {
     bool built = false;
     std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
 
+    int generation = 0;
     void buildHierarchy(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* apHost, hipStream_t s) {
+        ++generation;
         HostCsr A0;
         A0.n = S.n;
         A0.rp = S.rp;
@@ -261,6 +309,15 @@ struct CprDev {
             L->gsF.upload(ord, s);
             csrLevels(h.A, false, ord, L->gsBs);
             L->gsB.upload(ord, s);
+            L->dgsFs.upload(L->gsFs, s);
+            L->dgsBs.upload(L->gsBs, s);
+            auto work = [](const std::vector<int>& st) {
+                std::vector<int> w(st.size() - 1);
+                for (size_t q = 0; q + 1 < st.size(); ++q) w[q] = st[q + 1] - st[q];
+                return w;
+            };
+            L->runsF = levelRuns(work(L->gsFs), 2048);
+            L->runsB = levelRuns(work(L->gsBs), 2048);
             L->x1.alloc(size_t(L->n));
             L->x2.alloc(size_t(L->n));
             L->b.alloc(size_t(L->n));
@@ -319,6 +376,18 @@ struct CprDev {
         }
         factorCoarsest(s);
     }
+    // one Gauss-Seidel sweep of level L over its level schedule (fused runs of small levels)
+    static void gsSweep(hipStream_t s, AmgLevelDev& L, const double* xin, double* xout, bool fwd) {
+        const std::vector<LevelRun>& runs = fwd ? L.runsF : L.runsB;
+        const std::vector<int>& st = fwd ? L.gsFs : L.gsBs;
+        const int* rows = fwd ? L.gsF.p : L.gsB.p;
+        const int* dst = fwd ? L.dgsFs.p : L.dgsBs.p;
+        for (const LevelRun& run : runs) {
+            if (run.fused) { launchGsFused(s, run.l0, run.l1, dst, rows, L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd); continue; }
+            const int q = run.l0;
+            launchGsLevel(s, st[size_t(q) + 1] - st[size_t(q)], rows + st[size_t(q)], L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd);
+        }
+    }
     void cycle(size_t l, hipStream_t s) {
         AmgLevelDev& L = *lv[l];
         if (l + 1 == lv.size()) {
@@ -331,11 +400,7 @@ struct CprDev {
                 double* a = L.x1.p;
                 double* b = L.x2.p;
                 for (int sw = 0; sw < 40; ++sw) {
-                    const bool fwd = (sw % 2) == 0;
-                    const std::vector<int>& st = fwd ? L.gsFs : L.gsBs;
-                    const int* rows = fwd ? L.gsF.p : L.gsB.p;
-                    for (size_t q = 0; q + 1 < st.size(); ++q)
-                        launchGsLevel(s, st[q + 1] - st[q], rows + st[q], L.rp, L.ci, L.diag, L.v.p, L.b.p, a, b, fwd);
+                    gsSweep(s, L, a, b, (sw % 2) == 0);
                     std::swap(a, b);
                 }
                 L.result = a;
@@ -344,8 +409,7 @@ struct CprDev {
         }
         AmgLevelDev& C = *lv[l + 1];
         if (smoother == RSH_SMOOTH_GS) {
-            for (size_t q = 0; q + 1 < L.gsFs.size(); ++q)
-                launchGsLevel(s, L.gsFs[q + 1] - L.gsFs[q], L.gsF.p + L.gsFs[q], L.rp, L.ci, L.diag, L.v.p, L.b.p, nullptr, L.x1.p, true);
+            gsSweep(s, L, nullptr, L.x1.p, true);
         } else {
             launchJacobi(s, L.n, L.rp, L.ci, L.diag, L.v.p, L.b.p, nullptr, L.x1.p, 0.6667, true);
         }
@@ -354,8 +418,7 @@ struct CprDev {
         cycle(l + 1, s);
         launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
         if (smoother == RSH_SMOOTH_GS) {
-            for (size_t q = 0; q + 1 < L.gsBs.size(); ++q)
-                launchGsLevel(s, L.gsBs[q + 1] - L.gsBs[q], L.gsB.p + L.gsBs[q], L.rp, L.ci, L.diag, L.v.p, L.b.p, L.x1.p, L.x2.p, false);
+            gsSweep(s, L, L.x1.p, L.x2.p, false);
         } else {
             launchJacobi(s, L.n, L.rp, L.ci, L.diag, L.v.p, L.b.p, L.x1.p, L.x2.p, 0.6667, false);
         }
@@ -409,17 +472,53 @@ struct rsh_system {
     std::unique_ptr<IluDev> ilu;
     std::unique_ptr<CprDev> cpr;
     // Krylov workspace
-    DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w;
+    DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w, irR, irD;
     DBuf<double> V, Z, hcol, yv;
     int restartAlloc = 0;
     // last solve
     int lastIts = 0, lastOuter = 0;
-    double lastRel = 0, lastSetupMs = 0, lastSolveMs = 0;
+    double lastRel = 0, lastSetupMs = 0, lastSolveMs = 0, lastIluMs = 0, lastCprMs = 0;
     std::string lastStop;
 
     hipStream_t s() const { return ctx->c.stream; }
 
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
+
+    // Preconditioner applies are captured once per (input, output, precision) into a HIP graph and replayed: a
+    // level-scheduled apply is hundreds of small dependent launches, which the graph issues without host work.
+    std::map<std::tuple<const void*, const void*, int>, hipGraphExec_t> graphs;
+    std::pair<int, int> graphGen{-1, -1};
+    bool useGraphs = [] { const char* e = std::getenv("RSH_GRAPHS"); return !(e && e[0] == '0'); }();
+    void clearGraphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    ~rsh_system() { clearGraphs(); }
+    void precondG(const double* rr, double* z, bool f32) {
+        if (!useGraphs) { precond(rr, z, f32); return; }
+        const std::pair<int, int> gen{ilu ? ilu->generation : 0, cpr ? cpr->generation : 0};
+        if (gen != graphGen) { clearGraphs(); graphGen = gen; }
+        const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z), (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5));
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            hipGraph_t g = nullptr;
+            RSH_HIP(hipStreamBeginCapture(s(), hipStreamCaptureModeThreadLocal));
+            try {
+                precond(rr, z, f32);
+            } catch (...) {
+                (void)hipStreamEndCapture(s(), &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            RSH_HIP(hipStreamEndCapture(s(), &g));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) throw Error(RSH_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            it = graphs.emplace(key, ex).first;
+        }
+        RSH_HIP(hipGraphLaunch(it->second, s()));
+    }
 
     void spmv(const double* x, double* y, bool f32) {
         launchSpmvSliced(s(), nb, f32, &spmv64, &spmv32, x, y);
@@ -454,7 +553,7 @@ void syncScalars(rsh_system& S) {
 
 // Right-preconditioned BiCGSTAB from x0 = 0 (linsolve.cpp:159-216); the loop structure, breakdown tests and
 // the two convergence tests (on ||s|| and on the recursive ||r||) are the reference's.
-int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel) {
+int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool f32 = false) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
     const long long N = S.N;
@@ -476,15 +575,15 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     for (int it = 1; it <= maxIt; ++it) {
         launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
         launchBicgP(st, N, c.ks.p, S.r.p, S.v.p, S.p.p);
-        S.precond(S.p.p, S.ph.p, false);
-        S.spmv(S.ph.p, S.v.p, false);
+        S.precondG(S.p.p, S.ph.p, f32);
+        S.spmv(S.ph.p, S.v.p, f32);
         launchDots(st, c, N, 1, S.rh.p, S.v.p, nullptr, nullptr, c.scal.p, kEpAlpha, c.ks.p);
         launchBicgS(st, c, N, c.ks.p, S.r.p, S.v.p, S.sv.p);
         launchBicgXs(st, N, c.ks.p, S.ph.p, x);
         syncScalars(S);
         if (c.hks->state != kRun) break;
-        S.precond(S.sv.p, S.sh.p, false);
-        S.spmv(S.sh.p, S.t.p, false);
+        S.precondG(S.sv.p, S.sh.p, f32);
+        S.spmv(S.sh.p, S.t.p, f32);
         launchDots(st, c, N, 2, S.t.p, S.t.p, S.t.p, S.sv.p, c.scal.p, kEpOmega, c.ks.p);
         launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, x, S.r.p);
         syncScalars(S);
@@ -548,7 +647,7 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         while (j < m && total < maxIt) {
             double* Vj = S.V.p + size_t(j) * N;
             double* Zj = S.Z.p + size_t(j) * N;
-            S.precond(Vj, Zj, f32);
+            S.precondG(Vj, Zj, f32);
             S.spmv(Zj, S.w.p, f32);
             // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2
             launchMultiDot(st, c, N, j + 1, S.V.p, N, S.w.p, S.hcol.p);
@@ -597,15 +696,16 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
 }
 
 // Mixed precision: FP64 iterative refinement x += d, r = b - A x (FP64 SpMV, as refineWithDevice bsr.cpp:1373-1396)
-// around inner FGMRES solves that use the FP32 matrix, FP32 ILU factors and FP32 sweeps.
+// around inner solves (BiCGSTAB as the reference's device solve, or FGMRES) that use the FP32 matrix, the FP32
+// ILU factors and sweeps; their dot products and Krylov vectors stay FP64.
 int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, int& outer) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
     const long long N = S.N;
-    S.ensureVec(S.sv);   // residual
-    S.ensureVec(S.t);    // correction
+    S.ensureVec(S.irR);  // residual
+    S.ensureVec(S.irD);  // correction
     launchFill(st, x, 0.0, N);
-    launchCopy(st, S.sv.p, b, N);
+    launchCopy(st, S.irR.p, b, N);
     launchDots(st, c, N, 1, b, b, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
     RSH_HIP(hipMemcpyAsync(c.hscal, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
     RSH_HIP(hipStreamSynchronize(st));
@@ -622,14 +722,17 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         if (rel <= tol) return RSH_OK;
         int inIts = 0;
         double inRel = 0;
-        const int rc = fgmres(S, S.sv.p, S.t.p, std::max(innerTol, tol * bn / rn * 0.5), std::max(1, maxIt - its), S.opt.restart > 0 ? S.opt.restart : 30,
-                              true, false, inIts, inRel);
+        const double itol = std::max(innerTol, tol * bn / rn * 0.5);
+        const int left = std::max(1, maxIt - its);
+        const int rc = S.opt.krylov == RSH_KRYLOV_FGMRES
+                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, false, inIts, inRel)
+                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true);
         its += inIts;
-        if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED) return rc;
-        launchAxpy(st, N, 1.0, S.t.p, x);
-        S.spmv(x, S.sv.p, false);
-        launchSub(st, N, b, S.sv.p, S.sv.p);
-        launchDots(st, c, N, 1, S.sv.p, S.sv.p, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
+        if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
+        launchAxpy(st, N, 1.0, S.irD.p, x);
+        S.spmv(x, S.irR.p, false);
+        launchSub(st, N, b, S.irR.p, S.irR.p);
+        launchDots(st, c, N, 1, S.irR.p, S.irR.p, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
         RSH_HIP(hipMemcpyAsync(c.hscal, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
         RSH_HIP(hipStreamSynchronize(st));
         rn = std::sqrt(c.hscal[0]);
@@ -873,8 +976,12 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             launchPackColumn0(st, S.nb, S.val.p, S.spmv64.src.p, S.spmvHost.nslots, S.colP.p);
             S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st);
         }
+        RSH_HIP(hipStreamSynchronize(st));
+        const double t1 = nowMs();
         S.ilu->factor(S.val.p, f32, st);
         RSH_HIP(hipStreamSynchronize(st));
+        S.lastIluMs = nowMs() - t1;
+        S.lastCprMs = t1 - t0;
         S.opt = o;
         S.ready = true;
         S.lastSetupMs = nowMs() - t0;
@@ -934,7 +1041,7 @@ int rsh_precond_apply(rsh_system* sys, const double* r, double* z) {
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
         RSH_HIP(hipMemcpyAsync(S.vb.p, r, size_t(S.N) * sizeof(double), hipMemcpyHostToDevice, S.s()));
-        S.precond(S.vb.p, S.vx.p, S.opt.precision == RSH_MIXED);
+        S.precondG(S.vb.p, S.vx.p, S.opt.precision == RSH_MIXED);
         RSH_HIP(hipMemcpyAsync(z, S.vx.p, size_t(S.N) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
         RSH_HIP(hipStreamSynchronize(S.s()));
         return RSH_OK;
@@ -1016,7 +1123,7 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             o << "],\"coarse_dense\":" << (S.cpr->dense ? 1 : 0) << ",\"cpr_bytes\":" << S.cpr->bytes();
         }
         o << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
-          << ",\"last_setup_ms\":" << S.lastSetupMs << ",\"last_solve_ms\":" << S.lastSolveMs << ",\"last_stop\":\"" << S.lastStop << "\"}";
+ << ",\"last_setup_ms\":" << S.lastSetupMs << ",\"last_setup_ilu_ms\":" << S.lastIluMs << ",\"last_setup_other_ms\":" << S.lastCprMs << ",\"last_solve_ms\":" << S.lastSolveMs << ",\"last_stop\":\"" << S.lastStop << "\"}";
         const std::string str = o.str();
         if (int(str.size()) + 1 > cap) return fail(RSH_ERR_ARG, "info buffer too small");
         std::memcpy(json, str.c_str(), str.size() + 1);
@@ -1074,7 +1181,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         auto once = [&]() {
             switch (kind) {
                 case 0: S.spmv(S.vb.p, S.vx.p, f32); break;
-                case 1: S.precond(S.vb.p, S.vx.p, f32); break;
+                case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
                 case 2: S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st); break;
                 case 3: S.cpr->vcycle(st); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;

@@ -116,6 +116,10 @@ struct LevelSched {
     std::vector<int> sStart, qStart;
     int levels() const { return int(sStart.size()) - 1; }
 };
+// consecutive levels grouped for launching: a run of small levels is swept by one workgroup (fused), a large
+// level gets its own launch
+struct LevelRun { int l0, l1; bool fused; };
+std::vector<LevelRun> levelRuns(const std::vector<int>& workPerLevel, int smallMax);
 
 // ------------------------------------------------------------------ host set-up (rsh_setup.cpp)
 struct HostBsr {
@@ -170,6 +174,13 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
 void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y);
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
                        const double* xp);
+// a run of small levels [l0, l1) swept by one workgroup (level starts on the device)
+void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
+                    const double* r, void* y, double* z, const double* xp);
+void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
+                   const double* b, const double* xin, double* xout, bool forward);
+// BILU0 elimination of long rows rows[r0..r1): one wavefront per row
+void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg);
 // BILU0 numeric factorisation of the rows of one level (in place, natural layout)
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag,
                       double* val, int* nreg);

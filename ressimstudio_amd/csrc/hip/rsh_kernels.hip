@@ -355,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
     }
 }
 
+
 // ---------------------------------------------------------------- BILU0 factorisation, one level, in place
 // Row i (bsr.cpp:183-215): for each k < i in row order: L_ik = A_ik U_kk^-1; A_ij -= L_ik U_kj for every j > k
 // present in row i; then the pivot block is inverted (U_ii^-1 stored).  Rows of one level only read rows of
@@ -422,6 +423,219 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
 #pragma unroll
     for (int e = 0; e < B2; ++e) dp[e] = d[e];
     if (reg && nreg) atomicAdd(nreg, 1);
+}
+
+
+// ---------------------------------------------------------------- fused runs of small levels
+// Levels with few rows (the colour classes of the well cliques, the coarse AMG levels) are swept by one
+// workgroup of 1024 threads that walks the run of levels in order with a workgroup barrier between levels:
+// one launch replaces one launch per level, and the barrier costs far less than a dispatch.
+template <typename TA>
+struct SliceView {
+    const int2* info; const int* row; const int* orig; const int* col; const TA* val; const TA* dval;
+    const int* lrow; const int* lorig; const int* lptr; const int* lcol; const TA* lval; const TA* ldval;
+};
+
+// one slice (code >= 0) or long row (code = -q-1) of an ILU sweep, computed by the calling wavefront
+template <int NB, typename TA, typename TV, bool FWD>
+__device__ inline void iluTask(const SliceView<TA>& A, int code, int lane, const double* __restrict__ r, TV* y, double* __restrict__ z,
+                               const double* __restrict__ xp) {
+#pragma clang fp contract(off)
+    TV acc[NB];
+    int wr = -1, o = -1;
+    const TA* d = nullptr;
+    int dstride = 1;
+    if (code >= 0) {
+        wr = A.row[(long long)code * kWave + lane];
+        if (wr >= 0) {
+            o = A.orig[(long long)code * kWave + lane];
+            const int2 inf = A.info[code];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] = FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e];
+            for (int k = 0; k < inf.x; ++k) {
+                const long long slot = (long long)inf.y + k;
+                const int c = A.col[slot * kWave + lane];
+                const TA* a = A.val + slot * (NB * NB * kWave) + lane;
+                TV yk[NB];
+#pragma unroll
+                for (int f = 0; f < NB; ++f) yk[f] = y[(long long)c * NB + f];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) {
+                    TV sacc = 0;
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) sacc += TV(a[(e * NB + f) * kWave]) * yk[f];
+                    acc[e] -= sacc;
+                }
+            }
+            if (!FWD) { d = A.dval + (long long)code * (NB * NB * kWave) + lane; dstride = kWave; }
+        }
+    } else {
+        const int q = -code - 1;
+        TV part[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] = 0;
+        for (int p = A.lptr[q] + lane; p < A.lptr[q + 1]; p += kWave) {
+            const int c = A.lcol[p];
+            const TA* a = A.lval + (long long)p * NB * NB;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV sacc = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) sacc += TV(a[e * NB + f]) * y[(long long)c * NB + f];
+                part[e] += sacc;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] = TV(waveSum(double(part[e])));
+        if (lane == 0) {
+            wr = A.lrow[q];
+            o = A.lorig[q];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] = (FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e]) - part[e];
+            if (!FWD) { d = A.ldval + (long long)q * NB * NB; dstride = 1; }
+        }
+    }
+    if (wr < 0) return;
+    if (FWD) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
+        return;
+    }
+    TV tv[NB];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        TV sacc = 0;
+#pragma unroll
+        for (int f = 0; f < NB; ++f) sacc += TV(d[(e * NB + f) * dstride]) * acc[f];
+        tv[e] = sacc;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = tv[e];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        double v = double(tv[e]);
+        if (e == 0 && xp) v += xp[o];
+        z[(long long)o * NB + e] = v;
+    }
+}
+
+constexpr int kFuseThreads = 1024;
+
+template <int NB, typename TA, typename TV, bool FWD>
+__global__ __launch_bounds__(kFuseThreads) void kIluFused(int l0, int l1, const int* __restrict__ sStart, const int* __restrict__ qStart,
+                                                          SliceView<TA> A, const double* __restrict__ r, TV* y, double* __restrict__ z,
+                                                          const double* __restrict__ xp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int waves = kFuseThreads / kWave;
+    for (int li = 0; li < l1 - l0; ++li) {
+        const int l = FWD ? l0 + li : l1 - 1 - li;
+        const int s0 = sStart[l], ns = sStart[l + 1] - s0, q0 = qStart[l], nq = qStart[l + 1] - q0;
+        for (int t = wave; t < ns + nq; t += waves) iluTask<NB, TA, TV, FWD>(A, t < ns ? s0 + t : -(q0 + t - ns) - 1, lane, r, y, z, xp);
+        __syncthreads();
+    }
+}
+
+// Gauss-Seidel over a run of small levels (rows listed level by level, starts in lstart)
+__global__ __launch_bounds__(kFuseThreads) void kGsFused(int l0, int l1, const int* __restrict__ lstart, const int* __restrict__ rows,
+                                                         const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ diag,
+                                                         const double* __restrict__ v, const double* __restrict__ b,
+                                                         const double* __restrict__ xin, double* xout, int forward) {
+#pragma clang fp contract(off)
+    for (int l = l0; l < l1; ++l) {
+        for (int t = lstart[l] + threadIdx.x; t < lstart[l + 1]; t += kFuseThreads) {
+            const int i = rows[t];
+            double s = b[i];
+            for (int p = rp[i]; p < rp[i + 1]; ++p) {
+                const int j = ci[p];
+                if (j == i) continue;
+                const bool useOut = forward ? (j < i) : (j > i);
+                if (useOut) s -= v[p] * xout[j];
+                else if (xin) s -= v[p] * xin[j];
+            }
+            xout[i] = s / v[diag[i]];
+        }
+        __syncthreads();
+    }
+}
+
+// BILU0 elimination of long rows (the perforated cells after the Schur fill): one wavefront per row; the L entries
+// are taken in order, and for each of them the lanes share the U entries of row k (each finds its column in row
+// i by binary search and updates a distinct block), so the updates are the same as in the one-thread loop.
+template <int NB>
+__global__ __launch_bounds__(kWave) void kBilu0LongRows(const int* __restrict__ rows, int r0, const int* __restrict__ rp,
+                                                        const int* __restrict__ ci, const int* __restrict__ diag, double* val,
+                                                        int* __restrict__ nreg) {
+#pragma clang fp contract(off)
+    constexpr int B2 = NB * NB;
+    const int i = rows[r0 + blockIdx.x];
+    const int lane = threadIdx.x;
+    const int pe = rp[i + 1], pd = diag[i];
+    for (int p = rp[i]; p < pd; ++p) {
+        const int k = ci[p];
+        const double* dk = val + (long long)diag[k] * B2;
+        double L[B2];
+        {
+            const double* ap = val + (long long)p * B2;
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) rr[j] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = ap[ii * NB + kk];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) rr[j] += f * dk[kk * NB + j];
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) L[ii * NB + j] = rr[j];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < B2; ++e) val[(long long)p * B2 + e] = L[e];
+        }
+        const int qe = rp[k + 1];
+        for (int q = diag[k] + 1 + lane; q < qe; q += kWave) {
+            const int j = ci[q];
+            int lo = p + 1, hi = pe - 1, s = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int cm = ci[mid];
+                if (cm == j) { s = mid; break; }
+                if (cm < j) lo = mid + 1; else hi = mid - 1;
+            }
+            if (s < 0) continue;
+            const double* u = val + (long long)q * B2;
+            double* as = val + (long long)s * B2;
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) rr[jj] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = L[ii * NB + kk];
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) rr[jj] += f * u[kk * NB + jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) as[ii * NB + jj] -= rr[jj];
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        double d[B2];
+        double* dp = val + (long long)pd * B2;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) d[e] = dp[e];
+        const int reg = invertPivotDev<NB>(d);
+#pragma unroll
+        for (int e = 0; e < B2; ++e) dp[e] = d[e];
+        if (reg && nreg) atomicAdd(nreg, 1);
+    }
 }
 
 // ---------------------------------------------------------------- CPR (bsr.cpp:993-1084)
@@ -951,6 +1165,20 @@ void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const double* colP, c
                        sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
 }
 
+
+
+template <int NB, typename TA, typename TV>
+void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, const int* sStart, const int* qStart, const double* r, TV* y,
+               double* z, const double* xp) {
+    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p};
+    if (fwd) hipLaunchKernelGGL((kIluFused<NB, TA, TV, true>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, r, y, z, xp);
+    else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, r, y, z, xp);
+}
+template <int NB>
+void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+    hipLaunchKernelGGL(kBilu0LongRows<NB>, dim3(r1 - r0), dim3(kWave), 0, s, rows, r0, rp, ci, diag, val, nreg);
+}
+
 inline void checkLaunch() {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(-2, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -1006,6 +1234,26 @@ void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp) {
     if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp)))
     else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp)))
+    checkLaunch();
+}
+
+
+void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
+                    const double* r, void* y, double* z, const double* xp) {
+    if (l1 <= l0) return;
+    if (f32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, r, (float*)y, z, xp)))
+    else RSH_NB_DISPATCH(nb, (iluFusedT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, l0, l1, sStart, qStart, r, (double*)y, z, xp)))
+    checkLaunch();
+}
+void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
+                   const double* b, const double* xin, double* xout, bool forward) {
+    if (l1 <= l0) return;
+    hipLaunchKernelGGL(kGsFused, dim3(1), dim3(kFuseThreads), 0, s, l0, l1, lstart, rows, rp, ci, diag, v, b, xin, xout, forward ? 1 : 0);
+    checkLaunch();
+}
+void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+    if (r1 <= r0) return;
+    RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg)));
     checkLaunch();
 }
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
