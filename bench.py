@@ -152,7 +152,9 @@ def main():
     names = [c for c in args.configs.split(",") if c] or [n for n in all_cfgs if n != "cpr_exact_gs" or args.extra]
     cfgs = {n: all_cfgs[n] for n in names}
     if args.kernels_only:
-        step(rs, sysd, dev, cfgs[names[0]], args.tol, 1)
+        L = rs.lib()
+        rs._check(L.rsh_system_set_values_device(sysd.h, dev.val), "set_values_device")
+        rs._check(L.rsh_system_setup(sysd.h, ctypes.byref(cfgs[names[0]])), "setup")
         for kind, name in [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps")]:
             ms, by = sysd.time_kernel(kind, 20)
             log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {by / ms / 1e6:.0f} GB/s" if by > 0 else ""))
