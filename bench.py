@@ -139,13 +139,16 @@ def main():
     sysd = rs.System(s, ctx)
     dev = DeviceInputs(rs, ctx, s)
 
-    J, GS, FP64, MIXED, BI = rs.SMOOTH_JACOBI, rs.SMOOTH_GS, rs.FP64, rs.MIXED, rs.KRYLOV_BICGSTAB
+    J, GS, FP64, MIXED, BI, FG = rs.SMOOTH_JACOBI, rs.SMOOTH_GS, rs.FP64, rs.MIXED, rs.KRYLOV_BICGSTAB, rs.KRYLOV_FGMRES
     all_cfgs = {
         "cpr_mc": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J),
         "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J),
         "ilu0": rs.default_options(precond="bilu0", precision=FP64, krylov=BI),
         "mcilu0": rs.default_options(precond="mcbilu0", precision=FP64, krylov=BI),
+        # FP32 preconditioner inside FP64 BiCGSTAB / FGMRES(30); FP64 refinement around FP32-operator inner solves
         "cpr_mc_mixed": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=BI, smoother=J),
+        "cpr_mc_mixed_fgmres": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=FG, smoother=J, restart=30),
+        "cpr_mc_mixed_ir": rs.default_options(precond="cpr_mc", precision=rs.MIXED_IR, krylov=BI, smoother=J),
         "cpr_mixed": rs.default_options(precond="cpr", precision=MIXED, krylov=BI, smoother=J),
         "cpr_exact_gs": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=GS),
     }
@@ -189,7 +192,16 @@ def main():
     dom = "ilu_sweeps"
     roof = {"bound": "hbm", "kernel": f"BILU0 sweeps of the {head} preconditioner (forward + backward, all launches of one apply)",
             "achieved": k[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": k[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
-            "spmv_GBs": k["spmv"]["GBs"], "spmv_frac": k["spmv"]["GBs"] / HBM_PEAK_GBS, "copy_GBs": k["copy"]["GBs"]}
+            "algorithmic_bytes": k[dom]["bytes"], "spmv_GBs": k["spmv"]["GBs"], "spmv_frac": k["spmv"]["GBs"] / HBM_PEAK_GBS,
+            "copy_GBs": k["copy"]["GBs"]}
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this workload (profiles/)
+    tfile = os.path.join(ROOT, "profiles", "traffic_cpr_mc_1.1M.json")
+    if args.scale == 1.0 and head == "cpr_mc" and os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        roof["traffic"] = tr["mc_bilu0_apply"]["traffic_bytes"]
+        roof["spmv_traffic"] = tr["spmv"]["traffic_bytes"]
+        roof["traffic_source"] = "profiles/cpr_mc_fp64_1.1M_hbm_traffic.md"
 
     cpu = None
     if not args.no_cpu:

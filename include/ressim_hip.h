@@ -45,9 +45,12 @@ extern "C" {
 #define RSH_KRYLOV_FGMRES 1       /* flexible restarted GMRES (linsolve.cpp:218-273) */
 /* precision */
 #define RSH_FP64 0                /* every kernel in double: the drop-in for setDistributedSolver */
-#define RSH_MIXED 1               /* FP32 matrix / ILU factors / AMG hierarchy / smoothers inside an FP64 outer
-                                     iterative-refinement loop over FGMRES; FP64 true residual, dots,
-                                     orthogonalisation and coarsest-level solve */
+#define RSH_MIXED 1               /* FP32 preconditioner (ILU factors and sweeps, AMG hierarchy and smoothers) inside
+                                     the FP64 Krylov method (BiCGSTAB, or FGMRES whose restarts recompute the FP64
+                                     true residual); FP64 matrix, vectors, dots and coarsest-level solve */
+#define RSH_MIXED_IR 2            /* FP64 iterative refinement (refineWithDevice, bsr.cpp:1373-1396) around inner
+                                     solves on the FP32 matrix and FP32 preconditioner (the float device solve,
+                                     bsr.cpp:1130-1215); FP64 true residual, dots and Krylov vectors */
 /* AMG smoothers */
 #define RSH_SMOOTH_GS 0           /* level-scheduled Gauss-Seidel: forward pre-, backward post-sweep, the exact
                                      arithmetic of AMG::cycle (amg.cpp:303-329) */
@@ -59,12 +62,12 @@ typedef struct rsh_system rsh_system;
 
 typedef struct rsh_options {
     int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
-    int krylov;         /* RSH_KRYLOV_* (default BiCGSTAB; RSH_MIXED always uses FGMRES inside refinement) */
-    int precision;      /* RSH_FP64 or RSH_MIXED */
+    int krylov;         /* RSH_KRYLOV_* (default BiCGSTAB; with RSH_MIXED_IR the method of the inner solves) */
+    int precision;      /* RSH_FP64, RSH_MIXED or RSH_MIXED_IR */
     int smoother;       /* RSH_SMOOTH_* */
     int restart;        /* FGMRES restart length (default 30) */
-    int max_outer;      /* RSH_MIXED: iterative-refinement steps (default 10) */
-    double inner_tol;   /* RSH_MIXED: relative tolerance of each inner FGMRES solve (default 1e-5) */
+    int max_outer;      /* RSH_MIXED_IR: iterative-refinement steps (default 10) */
+    double inner_tol;   /* RSH_MIXED_IR: relative tolerance of each inner solve (default 1e-5) */
     int reuse_amg;      /* 1: keep the AMG aggregates of the last setup while the pattern is unchanged and only
                            recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
     int reserved[7];
@@ -87,8 +90,8 @@ int rsh_system_set_values(rsh_system* sys, const double* val);
 int rsh_system_set_values_device(rsh_system* sys, const double* val_dev);
 /* build (first call) or refresh the preconditioner for the current values */
 int rsh_system_setup(rsh_system* sys, const rsh_options* opt);
-/* solve A x = b to relative residual tol (FP64 true residual for RSH_MIXED, the recursive residual of BiCGSTAB
- * for RSH_FP64 as linsolve.cpp:210-212).  Host pointers (n*nb doubles) or device pointers. */
+/* solve A x = b to relative residual tol (the recursive residual of BiCGSTAB as linsolve.cpp:210-212, the FGMRES
+ * estimate checked against the true residual at restarts, the FP64 true residual for RSH_MIXED_IR).  Host pointers (n*nb doubles) or device pointers. */
 int rsh_solve(rsh_system* sys, const double* b, double* x, double tol, int max_it, int* iterations, double* rel_res);
 int rsh_solve_device(rsh_system* sys, const double* b_dev, double* x_dev, double tol, int max_it, int* iterations,
                      double* rel_res);

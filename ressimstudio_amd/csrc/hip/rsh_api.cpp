@@ -75,6 +75,108 @@ void packSlices(hipStream_t s, int nb, const double* src, SliceDev<TA>& d, long 
     if (d.ldsrc.n) launchPackLong(s, nb, src, d.ldsrc.p, int(d.ldsrc.n), d.ldval.p, f32);
 }
 
+// ------------------------------------------------------------------ dense groups of fused runs
+// Per sweep direction: the runs of small levels whose inside couplings split into groups of at most kDenseMaxRows
+// rows, with the dense inverse of each group (recomputed after every factorisation).
+struct DenseSide {
+    std::vector<int> runTask0, runTask1;   // per run: its range in tasks, or -1 when swept level by level
+    int ngroups = 0, nmap = 0;
+    long long elems = 0;
+    DBuf<int> start, rows;
+    DBuf<long long> off;
+    DBuf<int4> map;
+    DBuf<int2> tasks;
+    DBuf<double> raw, inv;
+    DBuf<float> inv32;
+
+    // E: the sweep's entries per working row (src = factor block; dsrc = pivot inverse for the upper sweep)
+    void build(const std::vector<LevelRun>& runs, const RowEntries& E, const std::vector<int>& lstart, int nb, bool upper, hipStream_t s) {
+        std::vector<int> hstart(1, 0), hrows;
+        std::vector<long long> hoff;
+        std::vector<int4> hmap;
+        std::vector<int2> htasks;
+        ngroups = 0;
+        elems = 0;
+        runTask0.assign(runs.size(), -1);
+        runTask1.assign(runs.size(), -1);
+        constexpr long long kBudget = 1ll << 26;   // dense elements per direction (512 MiB in FP64)
+        for (size_t ri = 0; ri < runs.size(); ++ri) {
+            const LevelRun& run = runs[ri];
+            if (!run.fused) continue;
+            const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)], m = we - wb;
+            std::vector<int> par(static_cast<size_t>(m));
+            for (int a = 0; a < m; ++a) par[size_t(a)] = a;
+            auto find = [&](int a) {
+                while (par[size_t(a)] != a) { par[size_t(a)] = par[size_t(par[size_t(a)])]; a = par[size_t(a)]; }
+                return a;
+            };
+            for (int w = wb; w < we; ++w)
+                for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) {
+                    const int c = E.col[size_t(p)];
+                    if (c < wb || c >= we) continue;
+                    const int a = find(w - wb), b = find(c - wb);
+                    if (a != b) par[size_t(std::max(a, b))] = std::min(a, b);
+                }
+            std::vector<int> gid(static_cast<size_t>(m), -1), loc(static_cast<size_t>(m));
+            std::vector<std::vector<int>> groups;
+            for (int a = 0; a < m; ++a) {
+                const int r = find(a);
+                if (gid[size_t(r)] < 0) { gid[size_t(r)] = int(groups.size()); groups.emplace_back(); }
+                loc[size_t(a)] = int(groups[size_t(gid[size_t(r)])].size());
+                groups[size_t(gid[size_t(r)])].push_back(wb + a);
+            }
+            size_t maxRows = 0;
+            long long add = 0;
+            for (const auto& g : groups) {
+                maxRows = std::max(maxRows, g.size());
+                add += (long long)g.size() * nb * (long long)g.size() * nb;
+            }
+            // one dense product per group pays off against a dependent step per level once the run is a few
+            // levels deep (a level step costs about 2.5 us, the dense product about 4 us + 10 ns per scalar row)
+            const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.01 * double(maxRows) * nb;
+            if (maxRows > size_t(kDenseMaxRows) || elems + add > kBudget || denseCost >= levelCost) continue;
+            runTask0[ri] = int(htasks.size());
+            for (const auto& g : groups) {
+                const int gi = ngroups++;
+                const int K = int(g.size()) * nb;
+                hoff.push_back(elems);
+                elems += (long long)K * K;
+                for (size_t i = 0; i < g.size(); ++i) {
+                    const int w = g[i];
+                    hrows.push_back(w);
+                    for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) {
+                        const int c = E.col[size_t(p)];
+                        if (c < wb || c >= we) continue;
+                        hmap.push_back(make_int4(gi, int(i), loc[size_t(c - wb)], E.src[size_t(p)]));
+                    }
+                    if (upper) hmap.push_back(make_int4(gi, int(i), int(i), E.dsrc[size_t(w)]));
+                }
+                hstart.push_back(int(hrows.size()));
+                for (int a0 = 0; a0 < K; a0 += kDenseThreads) htasks.push_back(make_int2(gi, a0));
+            }
+            runTask1[ri] = int(htasks.size());
+        }
+        nmap = int(hmap.size());
+        start.upload(hstart, s);
+        rows.upload(hrows, s);
+        off.upload(hoff, s);
+        map.upload(hmap, s);
+        tasks.upload(htasks, s);
+        raw.alloc(size_t(elems));
+        inv.alloc(size_t(elems));
+        inv32.release();
+    }
+    void factor(int nb, bool upper, const double* lu, bool f32, hipStream_t s) {
+        if (!ngroups) return;
+        launchDenseGroups(s, nb, upper, ngroups, start.p, off.p, nmap, map.p, lu, raw.p, elems, inv.p);
+        if (f32) {
+            inv32.alloc(size_t(elems));
+            launchToFloat(s, elems, inv.p, inv32.p);
+        }
+    }
+    size_t bytes() const { return raw.bytes() + inv.bytes() + inv32.bytes(); }
+};
+
 // ------------------------------------------------------------------ level-scheduled block ILU(0)
 struct IluDev {
     bool mc = false;
@@ -103,6 +205,9 @@ struct IluDev {
     // sweep schedule: runs of levels (fused small levels or single launches) and device copies of level starts
     std::vector<LevelRun> runsL, runsU;
     DBuf<int> dLs, dLq, dUs, dUq;
+    DenseSide denseL, denseU;
+    DBuf<double> rt64;                  // outside contributions of the run being swept
+    DBuf<float> rt32;
     void build(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, bool multicolour, hipStream_t s) {
         ++generation;
         mc = multicolour;
@@ -158,8 +263,12 @@ struct IluDev {
                 wL[size_t(l)] = (sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)]) + (sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)]);
                 wU[size_t(l)] = (sU.sStart[size_t(l) + 1] - sU.sStart[size_t(l)]) + (sU.qStart[size_t(l) + 1] - sU.qStart[size_t(l)]);
             }
-            runsL = levelRuns(wL, 16);
-            runsU = levelRuns(wU, 16);
+            std::vector<int> rowsL(static_cast<size_t>(nl));
+            for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
+            runsL = levelRuns(wL, 16, &rowsL, kRunRowsMax);
+            runsU = levelRuns(wU, 16, &rowsL, kRunRowsMax);
+            denseL.build(runsL, EL, lstart, nb, false, s);
+            denseU.build(runsU, EU, lstart, nb, true, s);
             dLs.upload(sL.sStart, s);
             dLq.upload(sL.qStart, s);
             dUs.upload(sU.sStart, s);
@@ -184,6 +293,7 @@ struct IluDev {
         lu.alloc(size_t(S.rp[size_t(n)]) * nb * nb);
         y64.alloc(size_t(n + 1) * nb);
         y64.zero(s);
+        rt64.alloc(size_t(kRunRowsMax) * nb);
         nreg.alloc(1);
     }
     void ensure32(hipStream_t s) {
@@ -192,6 +302,7 @@ struct IluDev {
         uploadSlices(hU, U32, nb, s);
         y32.alloc(size_t(n + 1) * nb);
         y32.zero(s);
+        rt32.alloc(size_t(kRunRowsMax) * nb);
         have32 = true;
     }
     // numeric factorisation from the system's values (natural layout), then the sweep layouts
@@ -212,25 +323,46 @@ struct IluDev {
             packSlices(s, nb, lu.p, L32, hL.nslots, true);
             packSlices(s, nb, lu.p, U32, hU.nslots, true);
         }
+        denseL.factor(nb, false, lu.p, f32, s);
+        denseU.factor(nb, true, lu.p, f32, s);
         RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     void apply(const double* r, double* z, const double* xp, bool f32, hipStream_t s) const {
         const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
         const void* Up = f32 ? static_cast<const void*>(&U32) : static_cast<const void*>(&U64);
         void* y = f32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
-        for (const LevelRun& run : runsL) {
-            if (run.fused) { launchIluFused(s, nb, f32, Lp, true, run.l0, run.l1, dLs.p, dLq.p, r, y, z, xp); continue; }
+        void* rt = f32 ? static_cast<void*>(rt32.p) : static_cast<void*>(rt64.p);
+        auto sweepRun = [&](const LevelRun& run, size_t ri, bool fwd) {
+            const DenseSide& D = fwd ? denseL : denseU;
+            const LevelSched& sc = fwd ? sL : sU;
+            const void* sl = fwd ? Lp : Up;
+            const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)];
+            if (D.runTask0[ri] >= 0) {
+                const int t0 = sc.sStart[size_t(run.l0)], u0 = sc.qStart[size_t(run.l0)];
+                const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
+                launchRunDense(s, nb, f32, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
+                               D.runTask1[ri] - D.runTask0[ri], D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r,
+                               rt, y, z, xp);
+                return;
+            }
+            launchIluFused(s, nb, f32, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
+        };
+        for (size_t ri = 0; ri < runsL.size(); ++ri) {
+            const LevelRun& run = runsL[ri];
+            if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
             launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y);
         }
-        for (auto it = runsU.rbegin(); it != runsU.rend(); ++it) {
-            const LevelRun& run = *it;
-            if (run.fused) { launchIluFused(s, nb, f32, Up, false, run.l0, run.l1, dUs.p, dUq.p, r, y, z, xp); continue; }
+        for (size_t k = runsU.size(); k-- > 0;) {
+            const LevelRun& run = runsU[k];
+            if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
             launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z, xp);
         }
     }
-    size_t bytes() const { return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes(); }
+    size_t bytes() const {
+        return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes() + denseL.bytes() + denseU.bytes();
+    }
 };
 
 // ------------------------------------------------------------------ AMG hierarchy on the device
@@ -247,10 +379,12 @@ struct AmgLevelDev {
     DBuf<int> dgsFs, dgsBs;
     std::vector<LevelRun> runsF, runsB;
     DBuf<double> x1, x2, b, r;
+    DBuf<double> dinv;                   // 1 / a_ii (Jacobi)
     const double* result = nullptr;
 };
 
 struct CprDev {
+    static constexpr double kOmega = 0.6667;   // damped-Jacobi weight of the reference's device V-cycle (bsr.h:173)
     int n = 0, nb = 1;
     DBuf<double> w, xp, t;
     std::vector<std::unique_ptr<AmgLevelDev>> lv;
@@ -261,6 +395,9 @@ struct CprDev {
     double* Minv = nullptr;
     int smoother = RSH_SMOOTH_GS;
     bool built = false;
+    // level 0 values in the SpMV row-slice layout of S (same pattern) for the streamed Jacobi steps
+    const SliceDev<double>* plan = nullptr;
+    DBuf<double> apS, apL;
     std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
 
     int generation = 0;
@@ -320,6 +457,7 @@ struct CprDev {
             L->runsB = levelRuns(work(L->gsBs), 2048);
             L->x1.alloc(size_t(L->n));
             L->x2.alloc(size_t(L->n));
+            L->dinv.alloc(size_t(L->n));
             L->b.alloc(size_t(L->n));
             L->r.alloc(size_t(L->n));
             lv.push_back(std::move(L));
@@ -375,6 +513,13 @@ struct CprDev {
             }
         }
         factorCoarsest(s);
+        for (auto& L : lv) launchDinv(s, L->n, L->diag, L->v.p, L->dinv.p);
+        if (plan) {
+            apS.alloc(size_t(plan->val.n / (size_t(nb) * nb)));
+            apL.alloc(plan->lsrc.n);
+            launchPackSlices(s, 1, lv[0]->v.p, plan->src.p, (long long)(apS.n / kWave), apS.p, false);
+            if (plan->lsrc.n) launchPackLong(s, 1, lv[0]->v.p, plan->lsrc.p, int(plan->lsrc.n), apL.p, false);
+        }
     }
     // one Gauss-Seidel sweep of level L over its level schedule (fused runs of small levels)
     static void gsSweep(hipStream_t s, AmgLevelDev& L, const double* xin, double* xout, bool fwd) {
@@ -408,20 +553,22 @@ struct CprDev {
             return;
         }
         AmgLevelDev& C = *lv[l + 1];
-        if (smoother == RSH_SMOOTH_GS) {
-            gsSweep(s, L, nullptr, L.x1.p, true);
-        } else {
-            launchJacobi(s, L.n, L.rp, L.ci, L.diag, L.v.p, L.b.p, nullptr, L.x1.p, 0.6667, true);
+        if (smoother == RSH_SMOOTH_JACOBI) {
+            // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing
+            const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
+            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p);
+            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p);
+            cycle(l + 1, s);
+            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, L.x1.p, L.agg.p, C.result, kOmega, L.x2.p, nullptr);
+            L.result = L.x2.p;
+            return;
         }
+        gsSweep(s, L, nullptr, L.x1.p, true);
         launchCsrSpmvResidual(s, L.n, L.rp, L.ci, L.v.p, L.x1.p, L.b.p, L.r.p);
         launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p);
         cycle(l + 1, s);
         launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
-        if (smoother == RSH_SMOOTH_GS) {
-            gsSweep(s, L, L.x1.p, L.x2.p, false);
-        } else {
-            launchJacobi(s, L.n, L.rp, L.ci, L.diag, L.v.p, L.b.p, L.x1.p, L.x2.p, 0.6667, false);
-        }
+        gsSweep(s, L, L.x1.p, L.x2.p, false);
         L.result = L.x2.p;
     }
     void launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out);
@@ -553,7 +700,10 @@ void syncScalars(rsh_system& S) {
 
 // Right-preconditioned BiCGSTAB from x0 = 0 (linsolve.cpp:159-216); the loop structure, breakdown tests and
 // the two convergence tests (on ||s|| and on the recursive ||r||) are the reference's.
-int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool f32 = false) {
+// precF32: the preconditioner's factors, hierarchy and sweeps in FP32; opF32: the operator too (inner solves of the
+// refinement loop).  Vectors, dot products and the Krylov recurrences stay FP64.
+int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool precF32 = false,
+             bool opF32 = false) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
     const long long N = S.N;
@@ -575,15 +725,15 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     for (int it = 1; it <= maxIt; ++it) {
         launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
         launchBicgP(st, N, c.ks.p, S.r.p, S.v.p, S.p.p);
-        S.precondG(S.p.p, S.ph.p, f32);
-        S.spmv(S.ph.p, S.v.p, f32);
+        S.precondG(S.p.p, S.ph.p, precF32);
+        S.spmv(S.ph.p, S.v.p, opF32);
         launchDots(st, c, N, 1, S.rh.p, S.v.p, nullptr, nullptr, c.scal.p, kEpAlpha, c.ks.p);
         launchBicgS(st, c, N, c.ks.p, S.r.p, S.v.p, S.sv.p);
         launchBicgXs(st, N, c.ks.p, S.ph.p, x);
         syncScalars(S);
         if (c.hks->state != kRun) break;
-        S.precondG(S.sv.p, S.sh.p, f32);
-        S.spmv(S.sh.p, S.t.p, f32);
+        S.precondG(S.sv.p, S.sh.p, precF32);
+        S.spmv(S.sh.p, S.t.p, opF32);
         launchDots(st, c, N, 2, S.t.p, S.t.p, S.t.p, S.sv.p, c.scal.p, kEpOmega, c.ks.p);
         launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, x, S.r.p);
         syncScalars(S);
@@ -602,7 +752,8 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
 // Flexible restarted GMRES (linsolve.cpp:218-273): x0 = 0 (or the given x when warm), Z_j = M V_j stored, the
 // true residual recomputed at each restart, the estimate |g_{j+1}| / ||b|| tested each step.  Orthogonalisation:
 // classical Gram-Schmidt applied twice (CGS2, one multi-dot and one multi-axpy per pass) in FP64.
-int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int m, bool f32, bool warm, int& its, double& rel) {
+int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int m, bool precF32, bool opF32, bool warm, int& its,
+           double& rel) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
     const long long N = S.N;
@@ -647,8 +798,8 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         while (j < m && total < maxIt) {
             double* Vj = S.V.p + size_t(j) * N;
             double* Zj = S.Z.p + size_t(j) * N;
-            S.precondG(Vj, Zj, f32);
-            S.spmv(Zj, S.w.p, f32);
+            S.precondG(Vj, Zj, precF32);
+            S.spmv(Zj, S.w.p, opF32);
             // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2
             launchMultiDot(st, c, N, j + 1, S.V.p, N, S.w.p, S.hcol.p);
             launchMultiAxpy(st, N, j + 1, S.V.p, N, S.hcol.p, S.w.p);
@@ -695,9 +846,10 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     return RSH_ERR_NOT_CONVERGED;
 }
 
-// Mixed precision: FP64 iterative refinement x += d, r = b - A x (FP64 SpMV, as refineWithDevice bsr.cpp:1373-1396)
-// around inner solves (BiCGSTAB as the reference's device solve, or FGMRES) that use the FP32 matrix, the FP32
-// ILU factors and sweeps; their dot products and Krylov vectors stay FP64.
+// RSH_MIXED_IR: FP64 iterative refinement x += d, r = b - A x (FP64 SpMV, as refineWithDevice bsr.cpp:1373-1396)
+// around inner solves (BiCGSTAB, as the reference's float device solve bsr.cpp:1130-1215, or FGMRES) on the FP32
+// matrix and the FP32 preconditioner; their dot products and Krylov vectors stay FP64.  A correction that does not
+// reduce the FP64 residual (a diverged inner solve) is taken back and the loop stops.
 int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, int& outer) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
@@ -725,8 +877,8 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         const double itol = std::max(innerTol, tol * bn / rn * 0.5);
         const int left = std::max(1, maxIt - its);
         const int rc = S.opt.krylov == RSH_KRYLOV_FGMRES
-                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, false, inIts, inRel)
-                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true);
+                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, true, false, inIts, inRel)
+                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true, true);
         its += inIts;
         if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
         launchAxpy(st, N, 1.0, S.irD.p, x);
@@ -735,8 +887,18 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         launchDots(st, c, N, 1, S.irR.p, S.irR.p, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
         RSH_HIP(hipMemcpyAsync(c.hscal, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
         RSH_HIP(hipStreamSynchronize(st));
-        rn = std::sqrt(c.hscal[0]);
-        if (!std::isfinite(rn)) return RSH_ERR_NONFINITE;
+        const double rnew = std::sqrt(c.hscal[0]);
+        if (!(rnew < rn)) {
+            // no progress (or a non-finite correction): take the step back, keep the last residual
+            launchAxpy(st, N, -1.0, S.irD.p, x);
+            if (!std::isfinite(rnew)) {
+                S.spmv(x, S.irR.p, false);
+                launchSub(st, N, b, S.irR.p, S.irR.p);
+            }
+            S.lastStop = "refinement stalled";
+            break;
+        }
+        rn = rnew;
         if (its >= maxIt) break;
     }
     rel = rn / bn;
@@ -748,9 +910,11 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     const double t0 = nowMs();
     int its = 0, outer = 0, rc;
     double rel = 0;
-    if (S.opt.precision == RSH_MIXED) rc = refine(S, b, x, tol, maxIt, its, rel, outer);
-    else if (S.opt.krylov == RSH_KRYLOV_FGMRES) rc = fgmres(S, b, x, tol, maxIt, S.opt.restart > 0 ? S.opt.restart : 30, false, false, its, rel);
-    else rc = bicgstab(S, b, x, tol, maxIt, its, rel);
+    const bool precF32 = S.opt.precision == RSH_MIXED;
+    if (S.opt.precision == RSH_MIXED_IR) rc = refine(S, b, x, tol, maxIt, its, rel, outer);
+    else if (S.opt.krylov == RSH_KRYLOV_FGMRES)
+        rc = fgmres(S, b, x, tol, maxIt, S.opt.restart > 0 ? S.opt.restart : 30, precF32, false, false, its, rel);
+    else rc = bicgstab(S, b, x, tol, maxIt, its, rel, precF32, false);
     RSH_HIP(hipStreamSynchronize(S.s()));
     S.lastSolveMs = nowMs() - t0;
     S.lastIts = its;
@@ -951,13 +1115,13 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         rsh_default_options(&o);
         if (opt) o = *opt;
         if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_MC) return fail(RSH_ERR_ARG, "unknown preconditioner");
-        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED) return fail(RSH_ERR_ARG, "unknown precision");
+        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR) return fail(RSH_ERR_ARG, "unknown precision");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
         rsh_system& S = *sys;
         hipStream_t st = S.s();
         const double t0 = nowMs();
-        const bool f32 = o.precision == RSH_MIXED;
-        if (f32 && !S.have32) {
+        const bool f32 = o.precision != RSH_FP64;             // FP32 preconditioner
+        if (o.precision == RSH_MIXED_IR && !S.have32) {       // FP32 operator of the inner solves
             uploadSlices(S.spmvHost, S.spmv32, S.nb, st);
             packSlices(st, S.nb, S.val.p, S.spmv32, S.spmvHost.nslots, true);
             S.have32 = true;
@@ -970,6 +1134,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC) {
             if (!S.cpr) S.cpr = std::make_unique<CprDev>();
             S.cpr->smoother = o.smoother;
+            S.cpr->plan = &S.spmv64;
             if (S.colP.n == 0) {
                 S.colP.alloc(size_t(S.spmvHost.nslots) * S.nb * kWave);
             }
@@ -1041,7 +1206,7 @@ int rsh_precond_apply(rsh_system* sys, const double* r, double* z) {
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
         RSH_HIP(hipMemcpyAsync(S.vb.p, r, size_t(S.N) * sizeof(double), hipMemcpyHostToDevice, S.s()));
-        S.precondG(S.vb.p, S.vx.p, S.opt.precision == RSH_MIXED);
+        S.precondG(S.vb.p, S.vx.p, S.opt.precision != RSH_FP64);
         RSH_HIP(hipMemcpyAsync(z, S.vx.p, size_t(S.N) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
         RSH_HIP(hipStreamSynchronize(S.s()));
         return RSH_OK;
@@ -1112,6 +1277,11 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
         if (S.ilu) {
             o << ",\"ilu_levels\":" << S.ilu->sL.levels() << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
               << ",\"ilu_long_rows_L\":" << S.ilu->hL.lrow.size() << ",\"ilu_long_rows_U\":" << S.ilu->hU.lrow.size() << ",\"ilu_bytes\":" << S.ilu->bytes();
+            int fusedL = 0, fusedU = 0, denseRunsL = 0, denseRunsU = 0;
+            for (size_t i = 0; i < S.ilu->runsL.size(); ++i) { fusedL += S.ilu->runsL[i].fused; denseRunsL += S.ilu->denseL.runTask0[i] >= 0; }
+            for (size_t i = 0; i < S.ilu->runsU.size(); ++i) { fusedU += S.ilu->runsU[i].fused; denseRunsU += S.ilu->denseU.runTask0[i] >= 0; }
+            o << ",\"ilu_fused_runs\":[" << fusedL << "," << fusedU << "],\"ilu_dense_runs\":[" << denseRunsL << "," << denseRunsU
+              << "],\"ilu_dense_groups\":[" << S.ilu->denseL.ngroups << "," << S.ilu->denseU.ngroups << "]";
         }
         if (S.cpr) {
             o << ",\"amg_levels\":[";
@@ -1174,13 +1344,14 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
         launchFill(st, S.vb.p, 1.0, S.N);
-        const bool f32 = S.ready && S.opt.precision == RSH_MIXED;
+        const bool f32 = S.ready && S.opt.precision != RSH_FP64;          // preconditioner precision
+        const bool op32 = S.ready && S.opt.precision == RSH_MIXED_IR;     // operator precision
         const double nb2 = double(S.nb) * S.nb;
         const double pa = f32 ? 4.0 : 8.0;
         double bytes = 0;
         auto once = [&]() {
             switch (kind) {
-                case 0: S.spmv(S.vb.p, S.vx.p, f32); break;
+                case 0: S.spmv(S.vb.p, S.vx.p, op32); break;
                 case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
                 case 2: S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st); break;
                 case 3: S.cpr->vcycle(st); break;
@@ -1188,8 +1359,8 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                 default: throw Error(RSH_ERR_ARG, "unknown kernel kind");
             }
         };
-        // algorithmic bytes (values + indices + vectors), see DESIGN.md for the formulas
-        if (kind == 0) bytes = S.nnzb * (nb2 * pa + 4.0) + (S.n + 1) * 4.0 + 2.0 * S.N * 8.0;
+        // algorithmic bytes: values + indices + vectors
+        if (kind == 0) bytes = S.nnzb * (nb2 * (op32 ? 4.0 : 8.0) + 4.0) + (S.n + 1) * 4.0 + 2.0 * S.N * 8.0;
         else if (kind == 2) bytes = S.nnzb * (nb2 * pa + 4.0) + 2.0 * (S.n + 1) * 4.0 + 4.0 * S.N * (f32 ? 4.0 : 8.0);
         else if (kind == 4) bytes = 2.0 * S.N * 8.0;
         once();   // warm-up

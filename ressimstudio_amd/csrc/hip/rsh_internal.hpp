@@ -119,7 +119,9 @@ struct LevelSched {
 // consecutive levels grouped for launching: a run of small levels is swept by one workgroup (fused), a large
 // level gets its own launch
 struct LevelRun { int l0, l1; bool fused; };
-std::vector<LevelRun> levelRuns(const std::vector<int>& workPerLevel, int smallMax);
+// rowsPerLevel (optional) caps the rows of a fused run at maxRunRows
+std::vector<LevelRun> levelRuns(const std::vector<int>& workPerLevel, int smallMax, const std::vector<int>* rowsPerLevel = nullptr,
+                                int maxRunRows = 0);
 
 // ------------------------------------------------------------------ host set-up (rsh_setup.cpp)
 struct HostBsr {
@@ -174,9 +176,23 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
 void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y);
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
                        const double* xp);
-// a run of small levels [l0, l1) swept by one workgroup (level starts on the device)
+// a run of small levels [l0, l1) swept by one workgroup (level starts on the device); its rows are the working rows
+// [wbase, wbase + wcount), wcount <= kRunRowsMax, perm maps working rows to original rows
+constexpr int kRunRowsMax = 2048;
 void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
-                    const double* r, void* y, double* z, const double* xp);
+                    int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp);
+// dense groups of a run: largest group (block rows) and workgroup size of the dense product
+constexpr int kDenseMaxRows = 256;
+constexpr int kDenseThreads = 256;
+// a run swept through its dense groups: outside contributions of the tasks [t0, t0+ns) / long rows [u0, u0+nq) into
+// rt, then per task {group, first scalar row} the dense product into y (and z after the backward sweep)
+void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
+                    int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
+                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp);
+// numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
+void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
+                       const double* lu, double* raw, long long rawElems, double* inv);
+void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
 // BILU0 elimination of long rows rows[r0..r1): one wavefront per row
@@ -196,9 +212,13 @@ void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, c
                            double* r);
 void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
-void launchJacobi(hipStream_t s, int n, const int* rp, const int* ci, const int* diag, const double* v, const double* b,
-                  const double* xin, double* xout, double omega, bool fromZero);
 void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc);
+// fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc.
+// sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v)
+void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
+                       const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
+                       const double* xc, double omega, double* out0, double* out1);
+void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv);
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);
 void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y);
 void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y);

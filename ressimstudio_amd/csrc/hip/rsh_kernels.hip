@@ -436,103 +436,300 @@ struct SliceView {
     const int* lrow; const int* lorig; const int* lptr; const int* lcol; const TA* lval; const TA* ldval;
 };
 
-// one slice (code >= 0) or long row (code = -q-1) of an ILU sweep, computed by the calling wavefront
-template <int NB, typename TA, typename TV, bool FWD>
-__device__ inline void iluTask(const SliceView<TA>& A, int code, int lane, const double* __restrict__ r, TV* y, double* __restrict__ z,
-                               const double* __restrict__ xp) {
+constexpr int kFuseThreads = 1024;
+
+// A run of small levels [l0, l1) whose rows are the contiguous working rows [wbase, wbase + wcount), swept by one
+// workgroup.  Phase A (all rows at once): the contributions of entries whose column lies outside the run - already
+// final before the run starts - and the right-hand side, into LDS.  Phase B (level by level, a workgroup barrier
+// between levels): the entries inside the run, read from and written to LDS.  Phase C: the run's values to HBM.
+// The sequential part thus touches only the run's own couplings (the well cliques) and LDS.
+template <int NB, typename TA, typename TV, bool FWD, bool INSIDE>
+__device__ inline void runTask(const SliceView<TA>& A, int code, int lane, int wbase, int wend, TV* ys, const double* __restrict__ r,
+                               const TV* y) {
 #pragma clang fp contract(off)
-    TV acc[NB];
-    int wr = -1, o = -1;
-    const TA* d = nullptr;
-    int dstride = 1;
+    auto use = [&](int c) { return INSIDE ? (c >= wbase && c < wend) : !(c >= wbase && c < wend); };
+    auto yval = [&](int c, int f) -> TV { return INSIDE ? ys[(c - wbase) * NB + f] : y[(long long)c * NB + f]; };
     if (code >= 0) {
-        wr = A.row[(long long)code * kWave + lane];
-        if (wr >= 0) {
-            o = A.orig[(long long)code * kWave + lane];
-            const int2 inf = A.info[code];
+        const int wr = A.row[(long long)code * kWave + lane];
+        if (wr < 0) return;
+        const int2 inf = A.info[code];
+        TV acc[NB];
+        if (INSIDE) {
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] = ys[(wr - wbase) * NB + e];
+        } else {
+            const int o = A.orig[(long long)code * kWave + lane];
 #pragma unroll
             for (int e = 0; e < NB; ++e) acc[e] = FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e];
-            for (int k = 0; k < inf.x; ++k) {
-                const long long slot = (long long)inf.y + k;
-                const int c = A.col[slot * kWave + lane];
-                const TA* a = A.val + slot * (NB * NB * kWave) + lane;
-                TV yk[NB];
-#pragma unroll
-                for (int f = 0; f < NB; ++f) yk[f] = y[(long long)c * NB + f];
-#pragma unroll
-                for (int e = 0; e < NB; ++e) {
-                    TV sacc = 0;
-#pragma unroll
-                    for (int f = 0; f < NB; ++f) sacc += TV(a[(e * NB + f) * kWave]) * yk[f];
-                    acc[e] -= sacc;
-                }
-            }
-            if (!FWD) { d = A.dval + (long long)code * (NB * NB * kWave) + lane; dstride = kWave; }
         }
-    } else {
-        const int q = -code - 1;
-        TV part[NB];
+        for (int k = 0; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
+            const int c = A.col[slot * kWave + lane];
+            if (!use(c)) continue;
+            const TA* a = A.val + slot * (NB * NB * kWave) + lane;
+            TV yk[NB];
 #pragma unroll
-        for (int e = 0; e < NB; ++e) part[e] = 0;
-        for (int p = A.lptr[q] + lane; p < A.lptr[q + 1]; p += kWave) {
-            const int c = A.lcol[p];
-            const TA* a = A.lval + (long long)p * NB * NB;
+            for (int f = 0; f < NB; ++f) yk[f] = yval(c, f);
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 TV sacc = 0;
 #pragma unroll
-                for (int f = 0; f < NB; ++f) sacc += TV(a[e * NB + f]) * y[(long long)c * NB + f];
-                part[e] += sacc;
+                for (int f = 0; f < NB; ++f) sacc += TV(a[(e * NB + f) * kWave]) * yk[f];
+                acc[e] -= sacc;
             }
         }
+        if (INSIDE && !FWD) {
+            const TA* d = A.dval + (long long)code * (NB * NB * kWave) + lane;
+            TV tv[NB];
 #pragma unroll
-        for (int e = 0; e < NB; ++e) part[e] = TV(waveSum(double(part[e])));
-        if (lane == 0) {
-            wr = A.lrow[q];
-            o = A.lorig[q];
+            for (int e = 0; e < NB; ++e) {
+                TV sacc = 0;
 #pragma unroll
-            for (int e = 0; e < NB; ++e) acc[e] = (FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e]) - part[e];
-            if (!FWD) { d = A.ldval + (long long)q * NB * NB; dstride = 1; }
+                for (int f = 0; f < NB; ++f) sacc += TV(d[(e * NB + f) * kWave]) * acc[f];
+                tv[e] = sacc;
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] = tv[e];
         }
-    }
-    if (wr < 0) return;
-    if (FWD) {
 #pragma unroll
-        for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
+        for (int e = 0; e < NB; ++e) ys[(wr - wbase) * NB + e] = acc[e];
         return;
     }
-    TV tv[NB];
+    const int q = -code - 1;
+    TV part[NB];
 #pragma unroll
-    for (int e = 0; e < NB; ++e) {
-        TV sacc = 0;
+    for (int e = 0; e < NB; ++e) part[e] = 0;
+    for (int p = A.lptr[q] + lane; p < A.lptr[q + 1]; p += kWave) {
+        const int c = A.lcol[p];
+        if (!use(c)) continue;
+        const TA* a = A.lval + (long long)p * NB * NB;
 #pragma unroll
-        for (int f = 0; f < NB; ++f) sacc += TV(d[(e * NB + f) * dstride]) * acc[f];
-        tv[e] = sacc;
+        for (int e = 0; e < NB; ++e) {
+            TV sacc = 0;
+#pragma unroll
+            for (int f = 0; f < NB; ++f) sacc += TV(a[e * NB + f]) * yval(c, f);
+            part[e] += sacc;
+        }
     }
 #pragma unroll
-    for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = tv[e];
+    for (int e = 0; e < NB; ++e) part[e] = TV(waveSum(double(part[e])));
+    if (lane != 0) return;
+    const int wr = A.lrow[q];
+    TV acc[NB];
+    if (INSIDE) {
 #pragma unroll
-    for (int e = 0; e < NB; ++e) {
-        double v = double(tv[e]);
-        if (e == 0 && xp) v += xp[o];
-        z[(long long)o * NB + e] = v;
+        for (int e = 0; e < NB; ++e) acc[e] = ys[(wr - wbase) * NB + e] - part[e];
+        if (!FWD) {
+            const TA* d = A.ldval + (long long)q * NB * NB;
+            TV tv[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV sacc = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) sacc += TV(d[e * NB + f]) * acc[f];
+                tv[e] = sacc;
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) acc[e] = tv[e];
+        }
+    } else {
+        const int o = A.lorig[q];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[e] = (FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e]) - part[e];
     }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) ys[(wr - wbase) * NB + e] = acc[e];
 }
-
-constexpr int kFuseThreads = 1024;
 
 template <int NB, typename TA, typename TV, bool FWD>
 __global__ __launch_bounds__(kFuseThreads) void kIluFused(int l0, int l1, const int* __restrict__ sStart, const int* __restrict__ qStart,
-                                                          SliceView<TA> A, const double* __restrict__ r, TV* y, double* __restrict__ z,
+                                                          SliceView<TA> A, int wbase, int wcount, const int* __restrict__ perm,
+                                                          const double* __restrict__ r, TV* y, double* __restrict__ z,
                                                           const double* __restrict__ xp) {
+    __shared__ TV ys[kRunRowsMax * NB];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int waves = kFuseThreads / kWave;
+    const int wend = wbase + wcount;
+    // phase A: every task of the run, contributions from outside the run
+    {
+        const int t0 = sStart[l0], t1 = sStart[l1], u0 = qStart[l0], u1 = qStart[l1];
+        const int ns = t1 - t0, nq = u1 - u0;
+        for (int t = wave; t < ns + nq; t += waves)
+            runTask<NB, TA, TV, FWD, false>(A, t < ns ? t0 + t : -(u0 + t - ns) - 1, lane, wbase, wend, ys, r, y);
+    }
+    __syncthreads();
+    // phase B: level by level, the couplings inside the run
     for (int li = 0; li < l1 - l0; ++li) {
         const int l = FWD ? l0 + li : l1 - 1 - li;
         const int s0 = sStart[l], ns = sStart[l + 1] - s0, q0 = qStart[l], nq = qStart[l + 1] - q0;
-        for (int t = wave; t < ns + nq; t += waves) iluTask<NB, TA, TV, FWD>(A, t < ns ? s0 + t : -(q0 + t - ns) - 1, lane, r, y, z, xp);
+        for (int t = wave; t < ns + nq; t += waves)
+            runTask<NB, TA, TV, FWD, true>(A, t < ns ? s0 + t : -(q0 + t - ns) - 1, lane, wbase, wend, ys, r, y);
         __syncthreads();
     }
+    // phase C: results to HBM (the working vector, and z in the original numbering after the backward sweep)
+    for (int k = threadIdx.x; k < wcount; k += kFuseThreads) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)(wbase + k) * NB + e] = ys[k * NB + e];
+        if (!FWD) {
+            const int o = perm[wbase + k];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                double v = double(ys[k * NB + e]);
+                if (e == 0 && xp) v += xp[o];
+                z[(long long)o * NB + e] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- dense groups inside a run of small levels
+// Rows of a run that are coupled to each other inside the run form groups (a well's perforated cells, coupled
+// all-to-all by the Schur-reduced bottom-hole-pressure row, make one group per well).  Per group the unit
+// block-lower factor L_GG (forward sweep) and (I + D^-1 U_GG)^-1 D^-1 (backward sweep, D^-1 the inverted pivots)
+// are inverted once per factorisation into dense column-major matrices, so that a sweep over the run is one
+// parallel pass (the contributions from rows outside the run) and one dense product per group, instead of one
+// dependent step per level.
+
+// outside contributions of every task of the levels [l0, l1): rt[(row - wbase) * NB + e], one wavefront per task
+template <int NB, typename TA, typename TV, bool FWD>
+__global__ __launch_bounds__(256) void kRunOutside(int t0, int ns, int u0, int nq, SliceView<TA> A, int wbase, int wend,
+                                                   const double* __restrict__ r, const TV* y, TV* rt) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= ns + nq) return;
+    runTask<NB, TA, TV, FWD, false>(A, t < ns ? t0 + t : -(u0 + t - ns) - 1, lane, wbase, wend, rt, r, y);
+}
+
+// group g: rows rows[start[g] .. start[g+1]) (working numbering, ascending), dense K x K at off[g], K = k NB
+template <int NB, typename TM, typename TV, bool FWD>
+__global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restrict__ tasks, const int* __restrict__ start,
+                                                          const int* __restrict__ rows, const long long* __restrict__ off,
+                                                          const TM* __restrict__ inv, int wbase, const TV* __restrict__ rt, TV* y,
+                                                          double* __restrict__ z, const int* __restrict__ perm,
+                                                          const double* __restrict__ xp) {
+    __shared__ TV v[kDenseMaxRows * NB];
+    const int2 tk = tasks[blockIdx.x];   // x group, y first scalar row of this workgroup
+    const int r0 = start[tk.x], K = (start[tk.x + 1] - r0) * NB;
+    for (int t = threadIdx.x; t < K; t += kDenseThreads) v[t] = rt[(long long)(rows[r0 + t / NB] - wbase) * NB + t % NB];
+    __syncthreads();
+    const int a = tk.y + threadIdx.x;
+    if (a >= K) return;
+    const TM* M = inv + off[tk.x] + a;
+    const int blk = a / NB;
+    const int b0 = FWD ? 0 : blk * NB, b1 = FWD ? (blk + 1) * NB : K;
+    TV acc0 = 0, acc1 = 0;
+    int b = b0;
+    for (; b + 1 < b1; b += 2) {
+        acc0 += TV(M[(long long)b * K]) * v[b];
+        acc1 += TV(M[(long long)(b + 1) * K]) * v[b + 1];
+    }
+    if (b < b1) acc0 += TV(M[(long long)b * K]) * v[b];
+    const TV out = acc0 + acc1;
+    const int w = rows[r0 + blk], e = a - blk * NB;
+    y[(long long)w * NB + e] = out;
+    if (!FWD) {
+        const int o = perm[w];
+        double val = double(out);
+        if (e == 0 && xp) val += xp[o];
+        z[(long long)o * NB + e] = val;
+    }
+}
+
+// gather: raw[group, local row i, local col j] = lu block src (map = {group, i, j, src}); raw zeroed beforehand
+__global__ void kDenseGather(int count, int nb, const int4* __restrict__ map, const int* __restrict__ start,
+                             const long long* __restrict__ off, const double* __restrict__ lu, double* __restrict__ raw) {
+    const int nb2 = nb * nb;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)count * nb2) return;
+    const int q = int(t / nb2), ef = int(t % nb2), e = ef / nb, f = ef % nb;
+    const int4 m = map[q];
+    const long long K = (long long)(start[m.x + 1] - start[m.x]) * nb;
+    raw[off[m.x] + (m.z * nb + f) * K + m.y * nb + e] = lu[(long long)m.w * nb2 + ef];
+}
+
+// one workgroup per group: X = L^-1 (lower) or X = (I + D^-1 U)^-1 D^-1 (upper; D^-1 on raw's diagonal blocks)
+template <int NB, bool UPPER>
+__global__ __launch_bounds__(256) void kDenseInvert(const int* __restrict__ start, const long long* __restrict__ off, double* raw,
+                                                    double* inv) {
+    const int g = blockIdx.x;
+    const int k = start[g + 1] - start[g];
+    const long long K = (long long)k * NB;
+    double* M = raw + off[g];
+    double* X = inv + off[g];
+    auto blk = [&](double* B, int i, int j, double (&o)[NB][NB]) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+#pragma unroll
+            for (int f = 0; f < NB; ++f) o[e][f] = B[(j * NB + f) * K + i * NB + e];
+    };
+    auto put = [&](double* B, int i, int j, const double (&o)[NB][NB]) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+#pragma unroll
+            for (int f = 0; f < NB; ++f) B[(j * NB + f) * K + i * NB + e] = o[e][f];
+    };
+    auto mul = [](const double (&a)[NB][NB], const double (&b)[NB][NB], double (&o)[NB][NB]) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+#pragma unroll
+            for (int f = 0; f < NB; ++f) {
+                double s = 0;
+#pragma unroll
+                for (int h = 0; h < NB; ++h) s += a[e][h] * b[h][f];
+                o[e][f] = s;
+            }
+    };
+    for (long long t = threadIdx.x; t < K * K; t += blockDim.x) X[t] = (t / K == t % K) ? 1.0 : 0.0;
+    if (UPPER) {
+        // W_im = D^-1_i U_im (i < m), in place
+        for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
+            const int i = t % k, m = t / k;
+            if (i >= m) continue;
+            double d[NB][NB], u[NB][NB], w[NB][NB];
+            blk(M, i, i, d);
+            blk(M, i, m, u);
+            mul(d, u, w);
+            put(M, i, m, w);
+        }
+    }
+    __syncthreads();
+    // right-looking elimination: step m finalises block row m of X, then removes its coupling from the rows after
+    // it (lower: rows i > m, columns j <= m; upper: rows i < m, columns j >= m)
+    for (int s = 0; s < k; ++s) {
+        const int m = UPPER ? k - 1 - s : s;
+        const int ni = UPPER ? m : k - 1 - m, nj = UPPER ? k - m : m + 1;
+        for (int t = threadIdx.x; t < ni * nj; t += blockDim.x) {
+            const int i = UPPER ? t / nj : m + 1 + t / nj;
+            const int j = UPPER ? m + t % nj : t % nj;
+            double a[NB][NB], x[NB][NB], p[NB][NB], c[NB][NB];
+            blk(M, i, m, a);
+            blk(X, m, j, x);
+            mul(a, x, p);
+            blk(X, i, j, c);
+#pragma unroll
+            for (int e = 0; e < NB; ++e)
+#pragma unroll
+                for (int f = 0; f < NB; ++f) c[e][f] -= p[e][f];
+            put(X, i, j, c);
+        }
+        __syncthreads();
+    }
+    if (UPPER) {
+        // X_ij <- X_ij D^-1_j (i <= j)
+        for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
+            const int i = t % k, j = t / k;
+            if (i > j) continue;
+            double x[NB][NB], d[NB][NB], o[NB][NB];
+            blk(X, i, j, x);
+            blk(M, j, j, d);
+            mul(x, d, o);
+            put(X, i, j, o);
+        }
+    }
+}
+
+__global__ void kToFloat(long long n, const double* __restrict__ a, float* __restrict__ b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = float(a[i]);
 }
 
 // Gauss-Seidel over a run of small levels (rows listed level by level, starts in lstart)
@@ -783,18 +980,6 @@ __global__ void kGsLevel(int nrows, const int* __restrict__ rows, const int* __r
     }
     xout[i] = s / v[diag[i]];
 }
-// damped Jacobi: xout = xin + omega D^-1 (b - A xin), or omega D^-1 b from zero (bsr.cpp:1328-1350)
-__global__ void kJacobi(int n, const int* __restrict__ rp, const int* __restrict__ ci, const int* __restrict__ diag,
-                        const double* __restrict__ v, const double* __restrict__ b, const double* __restrict__ xin,
-                        double* __restrict__ xout, double omega, int fromZero) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double dinv = 1.0 / v[diag[i]];
-    if (fromZero) { xout[i] = omega * dinv * b[i]; return; }
-    double s = 0;
-    for (int p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * xin[ci[p]];
-    xout[i] = xin[i] + omega * dinv * (b[i] - s);
-}
 __global__ void kRestrict(int nc, const int* __restrict__ rrp, const int* __restrict__ rci, const double* __restrict__ r,
                           double* __restrict__ bc) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -882,6 +1067,76 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
     }
 }
 
+
+// ---------------------------------------------------------------- fused damped-Jacobi V-cycle steps (DeviceCprAmg smoothing)
+// MODE 0 (pre-smoothing from zero + residual): x1 = w D^-1 b, r = b - A x1, with x1_j formed on the fly.
+// MODE 1 (prolongation + post-smoothing): x = x1 + P xc, x2 = x + w D^-1 (b - A x), with x_j formed on the fly.
+template <int MODE>
+__device__ inline double jacX(int j, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
+                              const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
+    return MODE == 0 ? omega * dinv[j] * b[j] : x1[j] + xc[agg[j]];
+}
+template <int MODE>
+__device__ inline void jacFinish(int i, double acc, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
+                                 const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc,
+                                 double* __restrict__ out0, double* __restrict__ out1) {
+    if (MODE == 0) {
+        out0[i] = omega * dinv[i] * b[i];
+        out1[i] = b[i] - acc;
+    } else {
+        const double xi = x1[i] + xc[agg[i]];
+        out0[i] = xi + omega * dinv[i] * (b[i] - acc);
+    }
+}
+// level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
+                                                        const int* __restrict__ col, const double* __restrict__ val, int nlong,
+                                                        const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                                        const double* __restrict__ lval, const double* __restrict__ dinv,
+                                                        const double* __restrict__ b, const double* __restrict__ x1, const int* __restrict__ agg,
+                                                        const double* __restrict__ xc, double omega, double* __restrict__ out0,
+                                                        double* __restrict__ out1) {
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid < nslices) {
+        const int i = row[(long long)wid * kWave + lane];
+        if (i < 0) return;
+        const int2 inf = info[wid];
+        double acc = 0;
+        for (int k = 0; k < inf.x; ++k) {
+            const long long at = ((long long)inf.y + k) * kWave + lane;
+            acc += val[at] * jacX<MODE>(col[at], omega, dinv, b, x1, agg, xc);
+        }
+        jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
+        return;
+    }
+    const int q = wid - nslices;
+    if (q >= nlong) return;
+    double acc = 0;
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += lval[p] * jacX<MODE>(lcol[p], omega, dinv, b, x1, agg, xc);
+    acc = waveSum(acc);
+    if (lane == 0) jacFinish<MODE>(lrow[q], acc, omega, dinv, b, x1, agg, xc, out0, out1);
+}
+// coarse levels (CSR, one thread per row)
+template <int MODE>
+__global__ void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                           const double* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
+                           const int* __restrict__ agg, const double* __restrict__ xc, double omega, double* __restrict__ out0,
+                           double* __restrict__ out1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) acc += v[p] * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
+    jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
+}
+__global__ void kDinv(int n, const int* __restrict__ diag, const double* __restrict__ v, double* __restrict__ dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = v[diag[i]];
+    dinv[i] = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+}
+
 // ---------------------------------------------------------------- vectors and deterministic reductions
 __global__ void kFill(double* __restrict__ x, double v, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) x[i] = v;
@@ -927,12 +1182,12 @@ __device__ void runEpilogue(int ep, const double* d, KScalars* ks) {
     }
 }
 
-// block reduction of K values (fixed tree) -> partials; last block sums the partials in block order
+// Deterministic two-launch reductions: every block writes its K partial sums (fixed tree inside the block), then
+// one block sums the partials in block order and runs the epilogue.  The kernel boundary publishes the partials,
+// so no block needs a device-wide fence (on this multi-XCD part such a fence writes back the whole L2).
 template <int K>
-__device__ void reduceFinish(double (&v)[K], double* __restrict__ partial, unsigned* __restrict__ ticket, double* __restrict__ out,
-                             int ep, KScalars* ks) {
+__device__ void reducePartial(double (&v)[K], double* __restrict__ partial) {
     __shared__ double sh[K][kBlock / kWave];
-    __shared__ bool last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -940,36 +1195,24 @@ __device__ void reduceFinish(double (&v)[K], double* __restrict__ partial, unsig
         if (lane == 0) sh[k][w] = t;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double s = 0;
-            for (int q = 0; q < kBlock / kWave; ++q) s += sh[k][q];
-            partial[(long long)k * gridDim.x + blockIdx.x] = s;
-        }
-        __threadfence();
-        const unsigned prev = atomicAdd(ticket, 1u);
-        last = prev == gridDim.x - 1;
+    if (threadIdx.x < K) {
+        double s = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) s += sh[threadIdx.x][q];
+        partial[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[K];
-#pragma unroll
+}
+__global__ __launch_bounds__(kBlock) void kReduceFinal(int K, int G, const double* __restrict__ partial, double* out, int ep, KScalars* ks) {
+    __shared__ double sh[32][kBlock / kWave];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = 0; k < K; ++k) {
-        acc[k] = 0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) acc[k] += partial[(long long)k * gridDim.x + b];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double t = waveSum(acc[k]);
-        if (lane == 0) sh[k][w] = t;
+        double acc = 0;
+        for (int b = threadIdx.x; b < G; b += kBlock) acc += partial[(long long)k * G + b];
+        acc = waveSum(acc);
+        if (lane == 0) sh[k][w] = acc;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double res[K];
-#pragma unroll
+        double res[32];
         for (int k = 0; k < K; ++k) {
             double s = 0;
             for (int q = 0; q < kBlock / kWave; ++q) s += sh[k][q];
@@ -977,14 +1220,12 @@ __device__ void reduceFinish(double (&v)[K], double* __restrict__ partial, unsig
             if (out) out[k] = s;
         }
         runEpilogue(ep, res, ks);
-        *ticket = 0;
     }
 }
 
 template <int K>
 __global__ __launch_bounds__(kBlock) void kDots(long long n, const double* __restrict__ a0, const double* __restrict__ b0,
-                                                const double* __restrict__ a1, const double* __restrict__ b1, double* partial,
-                                                unsigned* ticket, double* out, int ep, KScalars* ks) {
+                                                const double* __restrict__ a1, const double* __restrict__ b1, double* partial) {
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0;
@@ -992,7 +1233,7 @@ __global__ __launch_bounds__(kBlock) void kDots(long long n, const double* __res
         v[0] += a0[i] * b0[i];
         if constexpr (K > 1) v[1] += a1[i] * b1[i];
     }
-    reduceFinish<K>(v, partial, ticket, out, ep, ks);
+    reducePartial<K>(v, partial);
 }
 
 // p = r + beta (p - omega v)
@@ -1005,8 +1246,8 @@ __global__ void kBicgP(long long n, const KScalars* __restrict__ ks, const doubl
         p[i] = r[i] + beta * (p[i] - omega * v[i]);
 }
 // s = r - alpha v, with (s, s)
-__global__ __launch_bounds__(kBlock) void kBicgS(long long n, KScalars* ks, const double* __restrict__ r, const double* __restrict__ v,
-                                                 double* __restrict__ sv, double* partial, unsigned* ticket) {
+__global__ __launch_bounds__(kBlock) void kBicgS(long long n, const KScalars* ks, const double* __restrict__ r, const double* __restrict__ v,
+                                                 double* __restrict__ sv, double* partial) {
 #pragma clang fp contract(off)
     double acc[1] = {0};
     if (ks->state == kRun) {
@@ -1017,7 +1258,7 @@ __global__ __launch_bounds__(kBlock) void kBicgS(long long n, KScalars* ks, cons
             acc[0] += s * s;
         }
     }
-    reduceFinish<1>(acc, partial, ticket, nullptr, kEpSn, ks);
+    reducePartial<1>(acc, partial);
 }
 // early exit on ||s||: x += alpha ph
 __global__ void kBicgXs(long long n, const KScalars* __restrict__ ks, const double* __restrict__ ph, double* __restrict__ x) {
@@ -1028,9 +1269,9 @@ __global__ void kBicgXs(long long n, const KScalars* __restrict__ ks, const doub
         x[i] += alpha * ph[i];
 }
 // x += alpha ph + omega sh; r = s - omega t; with (r, r)
-__global__ __launch_bounds__(kBlock) void kBicgXR(long long n, KScalars* ks, const double* __restrict__ ph, const double* __restrict__ sh,
+__global__ __launch_bounds__(kBlock) void kBicgXR(long long n, const KScalars* ks, const double* __restrict__ ph, const double* __restrict__ sh,
                                                   const double* __restrict__ sv, const double* __restrict__ t, double* __restrict__ x,
-                                                  double* __restrict__ r, double* partial, unsigned* ticket) {
+                                                  double* __restrict__ r, double* partial) {
 #pragma clang fp contract(off)
     double acc[1] = {0};
     if (ks->state == kRun) {
@@ -1042,7 +1283,7 @@ __global__ __launch_bounds__(kBlock) void kBicgXR(long long n, KScalars* ks, con
             acc[0] += ri * ri;
         }
     }
-    reduceFinish<1>(acc, partial, ticket, nullptr, kEpRn, ks);
+    reducePartial<1>(acc, partial);
 }
 __global__ void kAxpy(long long n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] += a * x[i];
@@ -1059,11 +1300,10 @@ __global__ void kScaleTo(long long n, const double* __restrict__ num, const doub
 __global__ void kSub(long long n, const double* __restrict__ b, const double* __restrict__ ax, double* __restrict__ r) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) r[i] = b[i] - ax[i];
 }
-// out[i] = (V_i, w), i < k <= 32
+// partial sums of (V_i, w), i < k <= 32
 __global__ __launch_bounds__(kBlock) void kMultiDot(long long n, int k, const double* __restrict__ V, long long ld,
-                                                    const double* __restrict__ w, double* partial, unsigned* ticket, double* out) {
+                                                    const double* __restrict__ w, double* partial) {
     __shared__ double sh[32][kBlock / kWave];
-    __shared__ bool last;
     double acc[32];
 #pragma unroll
     for (int q = 0; q < 32; ++q) acc[q] = 0;
@@ -1083,19 +1323,6 @@ __global__ __launch_bounds__(kBlock) void kMultiDot(long long n, int k, const do
         for (int u = 0; u < kBlock / kWave; ++u) s += sh[threadIdx.x][u];
         partial[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (threadIdx.x < k) {
-        double s = 0;
-        for (unsigned b = 0; b < gridDim.x; ++b) s += partial[(long long)threadIdx.x * gridDim.x + b];
-        out[threadIdx.x] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *ticket = 0;
 }
 __global__ void kMultiAxpy(long long n, int k, const double* __restrict__ V, long long ld, const double* __restrict__ h,
                            double* __restrict__ w, double sign) {
@@ -1168,11 +1395,11 @@ void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const double* colP, c
 
 
 template <int NB, typename TA, typename TV>
-void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, const int* sStart, const int* qStart, const double* r, TV* y,
-               double* z, const double* xp) {
+void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, const int* sStart, const int* qStart, int wbase, int wcount,
+               const int* perm, const double* r, TV* y, double* z, const double* xp) {
     const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p};
-    if (fwd) hipLaunchKernelGGL((kIluFused<NB, TA, TV, true>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, r, y, z, xp);
-    else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, r, y, z, xp);
+    if (fwd) hipLaunchKernelGGL((kIluFused<NB, TA, TV, true>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
+    else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
 }
 template <int NB>
 void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
@@ -1238,11 +1465,61 @@ void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int 
 }
 
 
+template <int NB, typename TA, typename TV>
+void runOutsideT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend, const double* r,
+                 const TV* y, TV* rt) {
+    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p};
+    const int blocks = (ns + nq + 3) / 4;
+    if (fwd) hipLaunchKernelGGL((kRunOutside<NB, TA, TV, true>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt);
+    else hipLaunchKernelGGL((kRunOutside<NB, TA, TV, false>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt);
+}
+template <int NB, typename TM, typename TV>
+void runDenseT(hipStream_t s, bool fwd, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const TM* inv,
+               int wbase, const TV* rt, TV* y, double* z, const int* perm, const double* xp) {
+    if (fwd) hipLaunchKernelGGL((kRunDense<NB, TM, TV, true>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp);
+    else hipLaunchKernelGGL((kRunDense<NB, TM, TV, false>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp);
+}
+void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
+                    int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
+                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp) {
+    if (ns + nq > 0) {
+        if (f32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt)))
+        else RSH_NB_DISPATCH(nb, (runOutsideT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt)))
+        checkLaunch();
+    }
+    if (f32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp)))
+    else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp)))
+    checkLaunch();
+}
+template <int NB>
+void denseInvertT(hipStream_t s, bool upper, int ngroups, const int* start, const long long* off, double* raw, double* inv) {
+    if (upper) hipLaunchKernelGGL((kDenseInvert<NB, true>), dim3(ngroups), dim3(256), 0, s, start, off, raw, inv);
+    else hipLaunchKernelGGL((kDenseInvert<NB, false>), dim3(ngroups), dim3(256), 0, s, start, off, raw, inv);
+}
+void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
+                       const double* lu, double* raw, long long rawElems, double* inv) {
+    if (ngroups == 0) return;
+    launchFill(s, raw, 0.0, rawElems);
+    if (nmap > 0) {
+        const long long t = (long long)nmap * nb * nb;
+        hipLaunchKernelGGL(kDenseGather, dim3(unsigned((t + 255) / 256)), dim3(256), 0, s, nmap, nb, map, start, off, lu, raw);
+        checkLaunch();
+    }
+    RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, start, off, raw, inv)))
+    checkLaunch();
+}
+void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kToFloat, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, a, b);
+    checkLaunch();
+}
+
 void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
-                    const double* r, void* y, double* z, const double* xp) {
+                    int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp) {
     if (l1 <= l0) return;
-    if (f32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, r, (float*)y, z, xp)))
-    else RSH_NB_DISPATCH(nb, (iluFusedT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, l0, l1, sStart, qStart, r, (double*)y, z, xp)))
+    if (wcount > kRunRowsMax) throw Error(-1, "fused level run longer than the LDS buffer");
+    if (f32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (float*)y, z, xp)))
+    else RSH_NB_DISPATCH(nb, (iluFusedT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (double*)y, z, xp)))
     checkLaunch();
 }
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
@@ -1296,12 +1573,6 @@ void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, con
     hipLaunchKernelGGL(kGsLevel, dim3((nrows + 127) / 128), dim3(128), 0, s, nrows, rows, rp, ci, diag, v, b, xin, xout, forward ? 1 : 0);
     checkLaunch();
 }
-void launchJacobi(hipStream_t s, int n, const int* rp, const int* ci, const int* diag, const double* v, const double* b, const double* xin,
-                  double* xout, double omega, bool fromZero) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(kJacobi, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, diag, v, b, xin, xout, omega, fromZero ? 1 : 0);
-    checkLaunch();
-}
 void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc) {
     if (nc <= 0) return;
     hipLaunchKernelGGL(kRestrict, dim3((nc + 255) / 256), dim3(256), 0, s, nc, rrp, rci, r, bc);
@@ -1336,6 +1607,33 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
     hipLaunchKernelGGL(kGaussJordanStep, dim3(g), dim3(kBlock), 0, s, m, k, Min, Mout);
     checkLaunch();
 }
+
+void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
+                       const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
+                       const double* xc, double omega, double* out0, double* out1) {
+    if (sl) {
+        const int waves = sl->nslices + sl->nlong;
+        if (!waves) return;
+        if (mode == 0)
+            hipLaunchKernelGGL(kJacobiSliced<0>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+                               sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
+        else
+            hipLaunchKernelGGL(kJacobiSliced<1>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+                               sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
+    } else {
+        if (n <= 0) return;
+        if (mode == 0)
+            hipLaunchKernelGGL(kJacobiCsr<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+        else
+            hipLaunchKernelGGL(kJacobiCsr<1>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+    }
+    checkLaunch();
+}
+void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kDinv, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, v, dinv);
+    checkLaunch();
+}
 void launchFill(hipStream_t s, double* x, double v, long long n) {
     if (n <= 0) return;
     hipLaunchKernelGGL(kFill, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, x, v, n);
@@ -1355,8 +1653,9 @@ static int redGrid(long long n) {
 void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 int ep, KScalars* ks) {
     const int g = redGrid(n);
-    if (k == 1) hipLaunchKernelGGL(kDots<1>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p, c.ticket.p, out, ep, ks);
-    else hipLaunchKernelGGL(kDots<2>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p, c.ticket.p, out, ep, ks);
+    if (k == 1) hipLaunchKernelGGL(kDots<1>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p);
+    else hipLaunchKernelGGL(kDots<2>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, k, g, c.red.p, out, ep, ks);
     checkLaunch();
 }
 void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p) {
@@ -1364,7 +1663,9 @@ void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r
     checkLaunch();
 }
 void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv) {
-    hipLaunchKernelGGL(kBicgS, dim3(redGrid(n)), dim3(kBlock), 0, s, n, ks, r, v, sv, c.red.p, c.ticket.p);
+    const int g = redGrid(n);
+    hipLaunchKernelGGL(kBicgS, dim3(g), dim3(kBlock), 0, s, n, ks, r, v, sv, c.red.p);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, c.red.p, nullptr, int(kEpSn), ks);
     checkLaunch();
 }
 void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* ph, double* x) {
@@ -1373,7 +1674,9 @@ void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* 
 }
 void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv, const double* t,
                   double* x, double* r) {
-    hipLaunchKernelGGL(kBicgXR, dim3(redGrid(n)), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, x, r, c.red.p, c.ticket.p);
+    const int g = redGrid(n);
+    hipLaunchKernelGGL(kBicgXR, dim3(g), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, x, r, c.red.p);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, c.red.p, nullptr, int(kEpRn), ks);
     checkLaunch();
 }
 void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y) {
@@ -1394,7 +1697,9 @@ void launchSub(hipStream_t s, long long n, const double* b, const double* ax, do
 }
 void launchMultiDot(hipStream_t s, Ctx& c, long long n, int k, const double* V, long long ld, const double* w, double* out) {
     if (k < 1 || k > 32) throw Error(-1, "multi-dot width out of range");
-    hipLaunchKernelGGL(kMultiDot, dim3(redGrid(n)), dim3(kBlock), 0, s, n, k, V, ld, w, c.red.p, c.ticket.p, out);
+    const int g = redGrid(n);
+    hipLaunchKernelGGL(kMultiDot, dim3(g), dim3(kBlock), 0, s, n, k, V, ld, w, c.red.p);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, k, g, c.red.p, out, int(kEpNone), nullptr);
     checkLaunch();
 }
 void launchMultiAxpy(hipStream_t s, long long n, int k, const double* V, long long ld, const double* h, double* w) {

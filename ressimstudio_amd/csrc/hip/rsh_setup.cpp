@@ -124,14 +124,24 @@ void permutePattern(const HostBsr& A, const std::vector<int>& perm, HostBsr& P, 
         for (int p = P.rp[size_t(k)]; p < P.rp[size_t(k) + 1]; ++p) if (P.ci[size_t(p)] == k) { P.diag[size_t(k)] = p; break; }
 }
 
-std::vector<LevelRun> levelRuns(const std::vector<int>& workPerLevel, int smallMax) {
+std::vector<LevelRun> levelRuns(const std::vector<int>& workPerLevel, int smallMax, const std::vector<int>* rowsPerLevel, int maxRunRows) {
     std::vector<LevelRun> runs;
     const int nl = int(workPerLevel.size());
     int l = 0;
     while (l < nl) {
-        if (workPerLevel[size_t(l)] > smallMax) { runs.push_back({l, l + 1, false}); ++l; continue; }
-        int e = l;
-        while (e < nl && workPerLevel[size_t(e)] <= smallMax) ++e;
+        if (workPerLevel[size_t(l)] > smallMax || (rowsPerLevel && (*rowsPerLevel)[size_t(l)] > maxRunRows)) {
+            runs.push_back({l, l + 1, false});
+            ++l;
+            continue;
+        }
+        int e = l, rows = 0;
+        while (e < nl && workPerLevel[size_t(e)] <= smallMax) {
+            if (rowsPerLevel) {
+                if (rows + (*rowsPerLevel)[size_t(e)] > maxRunRows) break;
+                rows += (*rowsPerLevel)[size_t(e)];
+            }
+            ++e;
+        }
         // a single small level is cheaper as a plain launch with its own grid
         runs.push_back({l, e, e - l > 1});
         l = e;
