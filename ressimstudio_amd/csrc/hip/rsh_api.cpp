@@ -133,7 +133,7 @@ struct DenseSide {
             }
             // one dense product per group pays off against a dependent step per level once the run is a few
             // levels deep (a level step costs about 2.5 us, the dense product about 4 us + 10 ns per scalar row)
-            const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.01 * double(maxRows) * nb;
+            const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.005 * double(maxRows) * nb;
             if (maxRows > size_t(kDenseMaxRows) || elems + add > kBudget || denseCost >= levelCost) continue;
             runTask0[ri] = int(htasks.size());
             for (const auto& g : groups) {
@@ -152,7 +152,7 @@ struct DenseSide {
                     if (upper) hmap.push_back(make_int4(gi, int(i), int(i), E.dsrc[size_t(w)]));
                 }
                 hstart.push_back(int(hrows.size()));
-                for (int a0 = 0; a0 < K; a0 += kDenseThreads) htasks.push_back(make_int2(gi, a0));
+                for (int a0 = 0; a0 < K; a0 += kWave) htasks.push_back(make_int2(gi, a0));
             }
             runTask1[ri] = int(htasks.size());
         }
@@ -556,16 +556,18 @@ struct CprDev {
         if (smoother == RSH_SMOOTH_JACOBI) {
             // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing
             const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
-            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p);
-            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p);
+            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p,
+                              double(L.nnz) / std::max(L.n, 1));
+            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
             cycle(l + 1, s);
-            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, L.x1.p, L.agg.p, C.result, kOmega, L.x2.p, nullptr);
+            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, L.x1.p, L.agg.p, C.result, kOmega, L.x2.p, nullptr,
+                              double(L.nnz) / std::max(L.n, 1));
             L.result = L.x2.p;
             return;
         }
         gsSweep(s, L, nullptr, L.x1.p, true);
         launchCsrSpmvResidual(s, L.n, L.rp, L.ci, L.v.p, L.x1.p, L.b.p, L.r.p);
-        launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p);
+        launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
         cycle(l + 1, s);
         launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
         gsSweep(s, L, L.x1.p, L.x2.p, false);

@@ -181,9 +181,10 @@ void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int 
 constexpr int kRunRowsMax = 2048;
 void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
                     int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp);
-// dense groups of a run: largest group (block rows) and workgroup size of the dense product
+// dense groups of a run: largest group (block rows); the dense product runs one workgroup of kDenseThreads per
+// tile of kWave scalar rows of a group
 constexpr int kDenseMaxRows = 256;
-constexpr int kDenseThreads = 256;
+constexpr int kDenseThreads = 1024;
 // a run swept through its dense groups: outside contributions of the tasks [t0, t0+ns) / long rows [u0, u0+nq) into
 // rt, then per task {group, first scalar row} the dense product into y (and z after the backward sweep)
 void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
@@ -212,12 +213,13 @@ void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, c
                            double* r);
 void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
-void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc);
+// meanAgg: mean aggregate size (sets the lanes per aggregate)
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg);
 // fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc.
-// sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v)
+// sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
 void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
                        const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double* out1);
+                       const double* xc, double omega, double* out0, double* out1, double meanRow);
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv);
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);
 void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y);

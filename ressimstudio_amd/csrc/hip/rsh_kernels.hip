@@ -20,6 +20,13 @@ __device__ inline double waveSum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// sum over aligned groups of G lanes (G a power of two <= 64)
+template <int G>
+__device__ inline double groupSum(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
 
 template <typename T>
 __device__ inline T toT(double v) { return static_cast<T>(v); }
@@ -599,32 +606,45 @@ __global__ __launch_bounds__(256) void kRunOutside(int t0, int ns, int u0, int n
     runTask<NB, TA, TV, FWD, false>(A, t < ns ? t0 + t : -(u0 + t - ns) - 1, lane, wbase, wend, rt, r, y);
 }
 
-// group g: rows rows[start[g] .. start[g+1]) (working numbering, ascending), dense K x K at off[g], K = k NB
+// group g: rows rows[start[g] .. start[g+1]) (working numbering, ascending), dense K x K at off[g], K = k NB, column-major
+// with exact zeros outside the block triangle.  One workgroup per 64-row tile of a group: its 16 wavefronts split the
+// tile's non-zero columns, each lane one row; the partial sums are added in a fixed order.
 template <int NB, typename TM, typename TV, bool FWD>
 __global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restrict__ tasks, const int* __restrict__ start,
                                                           const int* __restrict__ rows, const long long* __restrict__ off,
                                                           const TM* __restrict__ inv, int wbase, const TV* __restrict__ rt, TV* y,
                                                           double* __restrict__ z, const int* __restrict__ perm,
                                                           const double* __restrict__ xp) {
+    constexpr int W = kDenseThreads / kWave;
     __shared__ TV v[kDenseMaxRows * NB];
-    const int2 tk = tasks[blockIdx.x];   // x group, y first scalar row of this workgroup
+    __shared__ TV part[W][kWave];
+    const int2 tk = tasks[blockIdx.x];   // x group, y first scalar row of the tile
     const int r0 = start[tk.x], K = (start[tk.x + 1] - r0) * NB;
     for (int t = threadIdx.x; t < K; t += kDenseThreads) v[t] = rt[(long long)(rows[r0 + t / NB] - wbase) * NB + t % NB];
     __syncthreads();
-    const int a = tk.y + threadIdx.x;
-    if (a >= K) return;
-    const TM* M = inv + off[tk.x] + a;
-    const int blk = a / NB;
-    const int b0 = FWD ? 0 : blk * NB, b1 = FWD ? (blk + 1) * NB : K;
-    TV acc0 = 0, acc1 = 0;
-    int b = b0;
-    for (; b + 1 < b1; b += 2) {
-        acc0 += TV(M[(long long)b * K]) * v[b];
-        acc1 += TV(M[(long long)(b + 1) * K]) * v[b + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a = tk.y + lane;
+    const int aLast = min(K, tk.y + kWave) - 1;
+    const int c0 = FWD ? 0 : (tk.y / NB) * NB, c1 = FWD ? (aLast / NB + 1) * NB : K;
+    TV s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (a < K) {
+        const TM* M = inv + off[tk.x] + a;
+        int b = c0 + wave;
+        for (; b + 3 * W < c1; b += 4 * W) {
+            s0 += TV(M[(long long)b * K]) * v[b];
+            s1 += TV(M[(long long)(b + W) * K]) * v[b + W];
+            s2 += TV(M[(long long)(b + 2 * W) * K]) * v[b + 2 * W];
+            s3 += TV(M[(long long)(b + 3 * W) * K]) * v[b + 3 * W];
+        }
+        for (; b < c1; b += W) s0 += TV(M[(long long)b * K]) * v[b];
     }
-    if (b < b1) acc0 += TV(M[(long long)b * K]) * v[b];
-    const TV out = acc0 + acc1;
-    const int w = rows[r0 + blk], e = a - blk * NB;
+    part[wave][lane] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (wave != 0 || a >= K) return;
+    TV out = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) out += part[w][lane];
+    const int blk = a / NB, w = rows[r0 + blk], e = a - blk * NB;
     y[(long long)w * NB + e] = out;
     if (!FWD) {
         const int o = perm[w];
@@ -980,13 +1000,17 @@ __global__ void kGsLevel(int nrows, const int* __restrict__ rows, const int* __r
     }
     xout[i] = s / v[diag[i]];
 }
-__global__ void kRestrict(int nc, const int* __restrict__ rrp, const int* __restrict__ rci, const double* __restrict__ r,
-                          double* __restrict__ bc) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+// restriction: the sum of the fine residuals of each aggregate, G lanes per aggregate
+template <int G>
+__global__ __launch_bounds__(256) void kRestrict(int nc, const int* __restrict__ rrp, const int* __restrict__ rci, const double* __restrict__ r,
+                                                 double* __restrict__ bc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = int(t / G), sub = int(t % G);
     if (a >= nc) return;
     double s = 0;
-    for (int q = rrp[a]; q < rrp[a + 1]; ++q) s += r[rci[q]];
-    bc[a] = s;
+    for (int q = rrp[a] + sub; q < rrp[a + 1]; q += G) s += r[rci[q]];
+    s = groupSum<G>(s);
+    if (sub == 0) bc[a] = s;
 }
 __global__ void kProlongAdd(int n, const int* __restrict__ agg, const double* __restrict__ xc, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1118,17 +1142,20 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
     acc = waveSum(acc);
     if (lane == 0) jacFinish<MODE>(lrow[q], acc, omega, dinv, b, x1, agg, xc, out0, out1);
 }
-// coarse levels (CSR, one thread per row)
-template <int MODE>
-__global__ void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
-                           const double* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
-                           const int* __restrict__ agg, const double* __restrict__ xc, double omega, double* __restrict__ out0,
-                           double* __restrict__ out1) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// coarse levels (CSR): G lanes per row (G a power of two near the mean row length), the row's entries strided over
+// the group so their loads are issued together, then a shuffle reduction in a fixed order
+template <int MODE, int G>
+__global__ __launch_bounds__(256) void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                                                  const double* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
+                                                  const int* __restrict__ agg, const double* __restrict__ xc, double omega,
+                                                  double* __restrict__ out0, double* __restrict__ out1) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = int(t / G), sub = int(t % G);
+    if (i >= n) return;                       // whole groups leave together (n * G is a multiple of G)
     double acc = 0;
-    for (int p = rp[i]; p < rp[i + 1]; ++p) acc += v[p] * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
-    jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
+    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += v[p] * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
+    acc = groupSum<G>(acc);
+    if (sub == 0) jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
 }
 __global__ void kDinv(int n, const int* __restrict__ diag, const double* __restrict__ v, double* __restrict__ dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1573,9 +1600,34 @@ void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, con
     hipLaunchKernelGGL(kGsLevel, dim3((nrows + 127) / 128), dim3(128), 0, s, nrows, rows, rp, ci, diag, v, b, xin, xout, forward ? 1 : 0);
     checkLaunch();
 }
-void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc) {
+template <int G>
+void restrictT(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc) {
+    const long long th = (long long)nc * G;
+    hipLaunchKernelGGL(kRestrict<G>, dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, nc, rrp, rci, r, bc);
+}
+template <int MODE, int G>
+void jacobiCsrT(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* dinv, const double* b, const double* x1,
+                const int* agg, const double* xc, double omega, double* out0, double* out1) {
+    const long long th = (long long)n * G;
+    hipLaunchKernelGGL((kJacobiCsr<MODE, G>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+}
+// lanes per row for a mean row length
+int groupLanes(double mean) {
+    int g = 4;
+    while (g < 64 && g < mean) g *= 2;
+    return g;
+}
+#define RSH_G_DISPATCH(g, CALL)                                  \
+    switch (g) {                                                 \
+        case 4: { constexpr int G = 4; CALL; } break;            \
+        case 8: { constexpr int G = 8; CALL; } break;            \
+        case 16: { constexpr int G = 16; CALL; } break;          \
+        case 32: { constexpr int G = 32; CALL; } break;          \
+        default: { constexpr int G = 64; CALL; } break;          \
+    }
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg) {
     if (nc <= 0) return;
-    hipLaunchKernelGGL(kRestrict, dim3((nc + 255) / 256), dim3(256), 0, s, nc, rrp, rci, r, bc);
+    RSH_G_DISPATCH(groupLanes(meanAgg), (restrictT<G>(s, nc, rrp, rci, r, bc)))
     checkLaunch();
 }
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
@@ -1610,7 +1662,7 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
 
 void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
                        const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double* out1) {
+                       const double* xc, double omega, double* out0, double* out1, double meanRow) {
     if (sl) {
         const int waves = sl->nslices + sl->nlong;
         if (!waves) return;
@@ -1622,10 +1674,9 @@ void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, cons
                                sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
     } else {
         if (n <= 0) return;
-        if (mode == 0)
-            hipLaunchKernelGGL(kJacobiCsr<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
-        else
-            hipLaunchKernelGGL(kJacobiCsr<1>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+        const int g = groupLanes(meanRow);
+        if (mode == 0) RSH_G_DISPATCH(g, (jacobiCsrT<0, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1)))
+        else RSH_G_DISPATCH(g, (jacobiCsrT<1, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1)))
     }
     checkLaunch();
 }
