@@ -46,8 +46,9 @@ extern "C" {
 /* precision */
 #define RSH_FP64 0                /* every kernel in double: the drop-in for setDistributedSolver */
 #define RSH_MIXED 1               /* FP32 preconditioner (ILU factors and sweeps, AMG hierarchy and smoothers) inside
-                                     the FP64 Krylov method (BiCGSTAB, or FGMRES whose restarts recompute the FP64
-                                     true residual); FP64 matrix, vectors, dots and coarsest-level solve */
+                                     FP64 FGMRES (whose restarts recompute the FP64 true residual) or FP64 BiCGSTAB
+                                     restarted by iterative refinement on the FP64 true residual; FP64 matrix,
+                                     vectors, dots and coarsest-level solve */
 #define RSH_MIXED_IR 2            /* FP64 iterative refinement (refineWithDevice, bsr.cpp:1373-1396) around inner
                                      solves on the FP32 matrix and FP32 preconditioner (the float device solve,
                                      bsr.cpp:1130-1215); FP64 true residual, dots and Krylov vectors */
@@ -59,6 +60,7 @@ extern "C" {
 
 typedef struct rsh_context rsh_context;
 typedef struct rsh_system rsh_system;
+typedef struct rsh_batch rsh_batch;
 
 typedef struct rsh_options {
     int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
@@ -66,8 +68,9 @@ typedef struct rsh_options {
     int precision;      /* RSH_FP64, RSH_MIXED or RSH_MIXED_IR */
     int smoother;       /* RSH_SMOOTH_* */
     int restart;        /* FGMRES restart length (default 30) */
-    int max_outer;      /* RSH_MIXED_IR: iterative-refinement steps (default 10) */
-    double inner_tol;   /* RSH_MIXED_IR: relative tolerance of each inner solve (default 1e-5) */
+    int max_outer;      /* refinement steps of RSH_MIXED (BiCGSTAB) and RSH_MIXED_IR (default 10) */
+    double inner_tol;   /* their inner solves' relative tolerance (default 1e-4, refineWithDevice's, bsr.cpp:1383);
+                           each inner solve runs at most max_it / max_outer (>= 50) iterations */
     int reuse_amg;      /* 1: keep the AMG aggregates of the last setup while the pattern is unchanged and only
                            recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
     int reserved[7];
@@ -115,6 +118,27 @@ int rsh_memcpy_d2h(rsh_context* ctx, void* dst, const void* src, size_t bytes);
  * stream, bracketed by HIP events.  kind: 0 BSR SpMV (precision of the system setup), 1 preconditioner apply,
  * 2 ILU forward+backward sweeps, 3 AMG V-cycle, 4 copy kernel of `bytes` (bandwidth calibration). */
 int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double* bytes_per_launch);
+
+/* ---- batched solves of many independent systems (no counterpart in the reference, whose engine solves one
+ * system per call; this serves its ensemble callers, runBatch study.cpp:326-364, in one launch stream).
+ * Systems of one block size are concatenated into one block-diagonal resident system; each system keeps its own
+ * BiCGSTAB scalars and convergence, and its preconditioner is the one it gets alone (BILU0 or MC-BILU0, FP64 or
+ * FP32 factors with RSH_MIXED).  One BiCGSTAB iteration over all systems is one captured HIP graph.  The level
+ * chunking of the sweeps and the entries per workgroup of the vector kernels are tuned at the first setup. */
+int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, const int* const* row_ptr, const int* const* col,
+                     rsh_batch** out);
+int rsh_batch_destroy(rsh_batch* batch);
+int rsh_batch_set_values(rsh_batch* batch, const double* const* val);           /* host: val[i] = nnzb_i*nb_i^2 */
+int rsh_batch_set_values_device(rsh_batch* batch, const double* val_dev);       /* device: input-order concatenation */
+int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt);
+/* per-system tol[i]; iterations / rel_res / status per system (may be null).  Returns RSH_OK when every system
+ * converged, else the first failing status (the converged systems' x are valid either way). */
+int rsh_batch_solve(rsh_batch* batch, const double* const* b, double* const* x, const double* tol, int max_it, int* iterations,
+                    double* rel_res, int* status);
+int rsh_batch_solve_device(rsh_batch* batch, const double* b_dev, double* x_dev, const double* tol, int max_it, int* iterations,
+                           double* rel_res, int* status);
+int rsh_batch_latencies(rsh_batch* batch, double* ms);   /* per system: ms from the solve's start to its convergence */
+int rsh_batch_info(rsh_batch* batch, char* json, int cap);
 
 #ifdef __cplusplus
 }

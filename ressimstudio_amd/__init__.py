@@ -38,7 +38,9 @@ EXPORTED = [
     "rsh_context_synchronize", "rsh_system_create", "rsh_system_destroy", "rsh_system_set_values", "rsh_system_set_values_device",
     "rsh_system_setup", "rsh_solve", "rsh_solve_device", "rsh_spmv", "rsh_precond_apply", "rsh_amg_apply", "rsh_cpr_weights",
     "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
-    "rsh_memcpy_d2h", "rsh_time_kernel",
+    "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
+    "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
+    "rsh_batch_info",
 ]
 
 
@@ -93,6 +95,16 @@ def lib():
         L.rsh_memcpy_h2d.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
         L.rsh_memcpy_d2h.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
         L.rsh_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _pp = ctypes.POINTER(_vp)
+        L.rsh_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _pp, _pp, ctypes.POINTER(_vp)]
+        L.rsh_batch_destroy.argtypes = [_vp]
+        L.rsh_batch_set_values.argtypes = [_vp, _pp]
+        L.rsh_batch_set_values_device.argtypes = [_vp, _vp]
+        L.rsh_batch_setup.argtypes = [_vp, ctypes.POINTER(Options)]
+        L.rsh_batch_solve.argtypes = [_vp, _pp, _pp, _dp, ctypes.c_int, _ip, _dp, _ip]
+        L.rsh_batch_solve_device.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_int, _ip, _dp, _ip]
+        L.rsh_batch_latencies.argtypes = [_vp, _dp]
+        L.rsh_batch_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -255,6 +267,73 @@ class System:
         by = ctypes.c_double(0)
         _check(lib().rsh_time_kernel(self.h, kind, reps, ctypes.byref(ms), ctypes.byref(by)), "rsh_time_kernel")
         return float(ms.value), float(by.value)
+
+
+def _ptrs(arrays):
+    return (_vp * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+class Batch:
+    """Many independent block systems solved together (rsh_batch_*): one block-diagonal resident system per block
+    size, per-system BiCGSTAB scalars and convergence, one captured graph per BiCGSTAB iteration."""
+
+    def __init__(self, systems, ctx: Context | None = None):
+        mats = [A if isinstance(A, BSR) else BSR.of(A) for A in systems]
+        self.ctx = ctx or default_context()
+        self.mats = mats
+        self.sizes = [A.n * A.nb for A in mats]
+        n = np.array([A.n for A in mats], np.int32)
+        nb = np.array([A.nb for A in mats], np.int32)
+        self._rp = [A.row_ptr for A in mats]
+        self._ci = [A.col for A in mats]
+        h = _vp()
+        _check(lib().rsh_batch_create(self.ctx.h, len(mats), _p(n, ctypes.c_int), _p(nb, ctypes.c_int), _ptrs(self._rp), _ptrs(self._ci),
+                                      ctypes.byref(h)), "rsh_batch_create")
+        self.h = h
+        self.set_values([A.val for A in mats])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rsh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_values(self, vals):
+        vs = [np.ascontiguousarray(v, np.float64).reshape(-1) for v in vals]
+        _check(lib().rsh_batch_set_values(self.h, _ptrs(vs)), "rsh_batch_set_values")
+
+    def setup(self, options: Options | None = None, **kw):
+        o = options or default_options(**{"precond": "mcbilu0", **kw})
+        _check(lib().rsh_batch_setup(self.h, ctypes.byref(o)), "rsh_batch_setup")
+        self.options = o
+        return self
+
+    def solve(self, bs, tol=1e-8, max_it=1000, raise_on_fail=True):
+        """Returns (xs, iterations, rel_res, status) per system."""
+        bs = [np.ascontiguousarray(b, np.float64) for b in bs]
+        xs = [np.zeros(k) for k in self.sizes]
+        m = len(bs)
+        tols = np.full(m, tol) if np.isscalar(tol) else np.ascontiguousarray(tol, np.float64)
+        its = np.zeros(m, np.int32)
+        rel = np.zeros(m)
+        st = np.zeros(m, np.int32)
+        rc = lib().rsh_batch_solve(self.h, _ptrs(bs), _ptrs(xs), _p(tols, ctypes.c_double), max_it, _p(its, ctypes.c_int), _p(rel, ctypes.c_double),
+                                   _p(st, ctypes.c_int))
+        if rc < 0 and (raise_on_fail or rc not in (ERR_NOT_CONVERGED, ERR_BREAKDOWN)):
+            _check(rc, "rsh_batch_solve")
+        return xs, its, rel, st
+
+    def latencies(self):
+        ms = np.zeros(len(self.sizes))
+        _check(lib().rsh_batch_latencies(self.h, _p(ms, ctypes.c_double)), "rsh_batch_latencies")
+        return ms
+
+    def info(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().rsh_batch_info(self.h, buf, len(buf)), "rsh_batch_info")
+        return json.loads(buf.value.decode())
 
 
 class HipBlockSolver:

@@ -4,695 +4,11 @@
 // a Newton iteration only moves new values (set_values), refreshes the preconditioner numerically (setup) and
 // runs the Krylov loop (solve).  All Krylov scalars live on the device; BiCGSTAB reads them back once per
 // iteration, FGMRES once per Arnoldi step.
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <sstream>
-#include <tuple>
-
-#include "../../../include/ressim_hip.h"
-#include "rsh_internal.hpp"
+#include "rsh_solver.hpp"
 
 using namespace rsh;
 
 namespace {
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-template <typename F>
-int guarded(F&& f) {
-    try {
-        return f();
-    } catch (const Error& e) {
-        return fail(e.code, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(RSH_ERR_DEVICE, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(RSH_ERR_DEVICE, e.what());
-    }
-}
-
-template <typename TA>
-void uploadSlices(const SliceHost& h, SliceDev<TA>& d, int nb, hipStream_t s) {
-    d.nslices = h.nslices;
-    d.nlong = int(h.lrow.size());
-    d.nb = nb;
-    std::vector<int2> info(size_t(h.nslices));
-    for (int i = 0; i < h.nslices; ++i) info[size_t(i)] = make_int2(h.width[size_t(i)], h.off[size_t(i)]);
-    d.info.upload(info, s);
-    d.row.upload(h.row, s);
-    d.orig.upload(h.orig, s);
-    d.col.upload(h.col, s);
-    d.src.upload(h.src, s);
-    d.dsrc.upload(h.dsrc, s);
-    d.lrow.upload(h.lrow, s);
-    d.lorig.upload(h.lorig, s);
-    d.lptr.upload(h.lptr, s);
-    d.lcol.upload(h.lcol, s);
-    d.lsrc.upload(h.lsrc, s);
-    d.ldsrc.upload(h.ldsrc, s);
-    const size_t b2 = size_t(nb) * nb;
-    d.val.alloc(size_t(h.nslots) * b2 * kWave);
-    d.lval.alloc(h.lcol.size() * b2);
-    if (!h.dsrc.empty()) d.dval.alloc(size_t(h.nslices) * b2 * kWave);
-    if (!h.ldsrc.empty()) d.ldval.alloc(h.ldsrc.size() * b2);
-}
-
-template <typename TA>
-void packSlices(hipStream_t s, int nb, const double* src, SliceDev<TA>& d, long long nslots, bool f32) {
-    launchPackSlices(s, nb, src, d.src.p, nslots, d.val.p, f32);
-    if (d.lsrc.n) launchPackLong(s, nb, src, d.lsrc.p, int(d.lsrc.n), d.lval.p, f32);
-    if (d.dsrc.n) launchPackSlices(s, nb, src, d.dsrc.p, d.nslices, d.dval.p, f32);
-    if (d.ldsrc.n) launchPackLong(s, nb, src, d.ldsrc.p, int(d.ldsrc.n), d.ldval.p, f32);
-}
-
-// ------------------------------------------------------------------ dense groups of fused runs
-// Per sweep direction: the runs of small levels whose inside couplings split into groups of at most kDenseMaxRows
-// rows, with the dense inverse of each group (recomputed after every factorisation).
-struct DenseSide {
-    std::vector<int> runTask0, runTask1;   // per run: its range in tasks, or -1 when swept level by level
-    int ngroups = 0, nmap = 0;
-    long long elems = 0;
-    DBuf<int> start, rows;
-    DBuf<long long> off;
-    DBuf<int4> map;
-    DBuf<int2> tasks;
-    DBuf<double> raw, inv;
-    DBuf<float> inv32;
-
-    // E: the sweep's entries per working row (src = factor block; dsrc = pivot inverse for the upper sweep)
-    void build(const std::vector<LevelRun>& runs, const RowEntries& E, const std::vector<int>& lstart, int nb, bool upper, hipStream_t s) {
-        std::vector<int> hstart(1, 0), hrows;
-        std::vector<long long> hoff;
-        std::vector<int4> hmap;
-        std::vector<int2> htasks;
-        ngroups = 0;
-        elems = 0;
-        runTask0.assign(runs.size(), -1);
-        runTask1.assign(runs.size(), -1);
-        constexpr long long kBudget = 1ll << 26;   // dense elements per direction (512 MiB in FP64)
-        for (size_t ri = 0; ri < runs.size(); ++ri) {
-            const LevelRun& run = runs[ri];
-            if (!run.fused) continue;
-            const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)], m = we - wb;
-            std::vector<int> par(static_cast<size_t>(m));
-            for (int a = 0; a < m; ++a) par[size_t(a)] = a;
-            auto find = [&](int a) {
-                while (par[size_t(a)] != a) { par[size_t(a)] = par[size_t(par[size_t(a)])]; a = par[size_t(a)]; }
-                return a;
-            };
-            for (int w = wb; w < we; ++w)
-                for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) {
-                    const int c = E.col[size_t(p)];
-                    if (c < wb || c >= we) continue;
-                    const int a = find(w - wb), b = find(c - wb);
-                    if (a != b) par[size_t(std::max(a, b))] = std::min(a, b);
-                }
-            std::vector<int> gid(static_cast<size_t>(m), -1), loc(static_cast<size_t>(m));
-            std::vector<std::vector<int>> groups;
-            for (int a = 0; a < m; ++a) {
-                const int r = find(a);
-                if (gid[size_t(r)] < 0) { gid[size_t(r)] = int(groups.size()); groups.emplace_back(); }
-                loc[size_t(a)] = int(groups[size_t(gid[size_t(r)])].size());
-                groups[size_t(gid[size_t(r)])].push_back(wb + a);
-            }
-            size_t maxRows = 0;
-            long long add = 0;
-            for (const auto& g : groups) {
-                maxRows = std::max(maxRows, g.size());
-                add += (long long)g.size() * nb * (long long)g.size() * nb;
-            }
-            // one dense product per group pays off against a dependent step per level once the run is a few
-            // levels deep (a level step costs about 2.5 us, the dense product about 4 us + 10 ns per scalar row)
-            const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.005 * double(maxRows) * nb;
-            if (maxRows > size_t(kDenseMaxRows) || elems + add > kBudget || denseCost >= levelCost) continue;
-            runTask0[ri] = int(htasks.size());
-            for (const auto& g : groups) {
-                const int gi = ngroups++;
-                const int K = int(g.size()) * nb;
-                hoff.push_back(elems);
-                elems += (long long)K * K;
-                for (size_t i = 0; i < g.size(); ++i) {
-                    const int w = g[i];
-                    hrows.push_back(w);
-                    for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) {
-                        const int c = E.col[size_t(p)];
-                        if (c < wb || c >= we) continue;
-                        hmap.push_back(make_int4(gi, int(i), loc[size_t(c - wb)], E.src[size_t(p)]));
-                    }
-                    if (upper) hmap.push_back(make_int4(gi, int(i), int(i), E.dsrc[size_t(w)]));
-                }
-                hstart.push_back(int(hrows.size()));
-                for (int a0 = 0; a0 < K; a0 += kWave) htasks.push_back(make_int2(gi, a0));
-            }
-            runTask1[ri] = int(htasks.size());
-        }
-        nmap = int(hmap.size());
-        start.upload(hstart, s);
-        rows.upload(hrows, s);
-        off.upload(hoff, s);
-        map.upload(hmap, s);
-        tasks.upload(htasks, s);
-        raw.alloc(size_t(elems));
-        inv.alloc(size_t(elems));
-        inv32.release();
-    }
-    void factor(int nb, bool upper, const double* lu, bool f32, hipStream_t s) {
-        if (!ngroups) return;
-        launchDenseGroups(s, nb, upper, ngroups, start.p, off.p, nmap, map.p, lu, raw.p, elems, inv.p);
-        if (f32) {
-            inv32.alloc(size_t(elems));
-            launchToFloat(s, elems, inv.p, inv32.p);
-        }
-    }
-    size_t bytes() const { return raw.bytes() + inv.bytes() + inv32.bytes(); }
-};
-
-// ------------------------------------------------------------------ level-scheduled block ILU(0)
-struct IluDev {
-    bool mc = false;
-    int n = 0, nb = 1, colours = 0;
-    std::vector<int> lstart;            // level (or colour) starts in working numbering
-    DBuf<int> perm;                     // working row -> original row
-    // factorisation matrix: natural mode = the system's pattern (rows visited in level order),
-    // multicolour mode = the colour-permuted pattern P A P^T
-    HostBsr fpat;
-    DBuf<int> frp, fci, fdiag, fmap;
-    const int *rp = nullptr, *ci = nullptr, *diag = nullptr;
-    DBuf<double> lu;
-    SliceHost hL, hU;
-    LevelSched sL, sU;
-    SliceDev<double> L64, U64;
-    SliceDev<float> L32, U32;
-    bool have32 = false;
-    DBuf<double> y64;
-    DBuf<float> y32;
-    DBuf<int> nreg;
-    int regularised = 0;
-    int generation = 0;
-    // factorisation schedule: per level the rows eliminated one per thread, then the long rows one per wavefront
-    DBuf<int> fShort, fLong;
-    std::vector<int> fShortStart, fLongStart;
-    // sweep schedule: runs of levels (fused small levels or single launches) and device copies of level starts
-    std::vector<LevelRun> runsL, runsU;
-    DBuf<int> dLs, dLq, dUs, dUq;
-    DenseSide denseL, denseU;
-    DBuf<double> rt64;                  // outside contributions of the run being swept
-    DBuf<float> rt32;
-    void build(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, bool multicolour, hipStream_t s) {
-        ++generation;
-        mc = multicolour;
-        n = S.n;
-        nb = S.nb;
-        std::vector<int> ord, colour;
-        RowEntries EL, EU;
-        EL.ptr.assign(1, 0);
-        EU.ptr.assign(1, 0);
-        if (!mc) {
-            levelOrder(S, ord, lstart);
-            std::vector<int> iperm(static_cast<size_t>(n));
-            for (int w = 0; w < n; ++w) iperm[size_t(ord[size_t(w)])] = w;
-            rp = drp; ci = dci; diag = ddiag;
-            for (int w = 0; w < n; ++w) {
-                const int i = ord[size_t(w)];
-                for (int p = S.rp[size_t(i)]; p < S.diag[size_t(i)]; ++p) { EL.col.push_back(iperm[size_t(S.ci[size_t(p)])]); EL.src.push_back(p); }
-                for (int p = S.diag[size_t(i)] + 1; p < S.rp[size_t(i) + 1]; ++p) { EU.col.push_back(iperm[size_t(S.ci[size_t(p)])]); EU.src.push_back(p); }
-                EL.ptr.push_back(int(EL.col.size()));
-                EU.ptr.push_back(int(EU.col.size()));
-                EU.dsrc.push_back(S.diag[size_t(i)]);
-            }
-        } else {
-            colours = colourGraph(S, colour, ord, lstart);
-            std::vector<int> bmap;
-            permutePattern(S, ord, fpat, bmap);
-            frp.upload(fpat.rp, s);
-            fci.upload(fpat.ci, s);
-            fdiag.upload(fpat.diag, s);
-            fmap.upload(bmap, s);
-            rp = frp.p; ci = fci.p; diag = fdiag.p;
-            for (int w = 0; w < n; ++w) {
-                for (int p = fpat.rp[size_t(w)]; p < fpat.diag[size_t(w)]; ++p) { EL.col.push_back(fpat.ci[size_t(p)]); EL.src.push_back(p); }
-                for (int p = fpat.diag[size_t(w)] + 1; p < fpat.rp[size_t(w) + 1]; ++p) { EU.col.push_back(fpat.ci[size_t(p)]); EU.src.push_back(p); }
-                EL.ptr.push_back(int(EL.col.size()));
-                EU.ptr.push_back(int(EU.col.size()));
-                EU.dsrc.push_back(fpat.diag[size_t(w)]);
-            }
-        }
-        EL.orig = ord;
-        EU.orig = ord;
-        perm.upload(ord, s);
-        std::vector<int> lenL(static_cast<size_t>(n)), lenU(static_cast<size_t>(n));
-        for (int w = 0; w < n; ++w) { lenL[size_t(w)] = EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)]; lenU[size_t(w)] = EU.ptr[size_t(w) + 1] - EU.ptr[size_t(w)]; }
-        buildSlices(EL, lstart, rowCap(lenL), hL, sL);
-        buildSlices(EU, lstart, rowCap(lenU), hU, sU);
-        uploadSlices(hL, L64, nb, s);
-        uploadSlices(hU, U64, nb, s);
-        {
-            const int nl = int(lstart.size()) - 1;
-            std::vector<int> wL(static_cast<size_t>(nl)), wU(static_cast<size_t>(nl));
-            for (int l = 0; l < nl; ++l) {
-                wL[size_t(l)] = (sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)]) + (sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)]);
-                wU[size_t(l)] = (sU.sStart[size_t(l) + 1] - sU.sStart[size_t(l)]) + (sU.qStart[size_t(l) + 1] - sU.qStart[size_t(l)]);
-            }
-            std::vector<int> rowsL(static_cast<size_t>(nl));
-            for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
-            runsL = levelRuns(wL, 16, &rowsL, kRunRowsMax);
-            runsU = levelRuns(wU, 16, &rowsL, kRunRowsMax);
-            denseL.build(runsL, EL, lstart, nb, false, s);
-            denseU.build(runsU, EU, lstart, nb, true, s);
-            dLs.upload(sL.sStart, s);
-            dLq.upload(sL.qStart, s);
-            dUs.upload(sU.sStart, s);
-            dUq.upload(sU.qStart, s);
-            // factor rows (F numbering) per level, long rows apart
-            const HostBsr& F = mc ? fpat : S;
-            std::vector<int> sh, lo;
-            fShortStart.assign(1, 0);
-            fLongStart.assign(1, 0);
-            for (int l = 0; l < nl; ++l) {
-                for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) {
-                    const int i = mc ? w : ord[size_t(w)];
-                    const int lower = F.diag[size_t(i)] - F.rp[size_t(i)];
-                    (lower > 24 ? lo : sh).push_back(i);
-                }
-                fShortStart.push_back(int(sh.size()));
-                fLongStart.push_back(int(lo.size()));
-            }
-            fShort.upload(sh, s);
-            fLong.upload(lo, s);
-        }
-        lu.alloc(size_t(S.rp[size_t(n)]) * nb * nb);
-        y64.alloc(size_t(n + 1) * nb);
-        y64.zero(s);
-        rt64.alloc(size_t(kRunRowsMax) * nb);
-        nreg.alloc(1);
-    }
-    void ensure32(hipStream_t s) {
-        if (have32) return;
-        uploadSlices(hL, L32, nb, s);
-        uploadSlices(hU, U32, nb, s);
-        y32.alloc(size_t(n + 1) * nb);
-        y32.zero(s);
-        rt32.alloc(size_t(kRunRowsMax) * nb);
-        have32 = true;
-    }
-    // numeric factorisation from the system's values (natural layout), then the sweep layouts
-    void factor(const double* val, bool f32, hipStream_t s) {
-        const size_t nvals = lu.n;
-        if (mc) launchGatherBlocks(s, nb, val, fmap.p, int(nvals / (size_t(nb) * nb)), lu.p);
-        else RSH_HIP(hipMemcpyAsync(lu.p, val, nvals * sizeof(double), hipMemcpyDeviceToDevice, s));
-        nreg.zero(s);
-        const int nl = int(lstart.size()) - 1;
-        for (int l = 0; l < nl; ++l) {
-            launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
-            launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
-        }
-        packSlices(s, nb, lu.p, L64, hL.nslots, false);
-        packSlices(s, nb, lu.p, U64, hU.nslots, false);
-        if (f32) {
-            ensure32(s);
-            packSlices(s, nb, lu.p, L32, hL.nslots, true);
-            packSlices(s, nb, lu.p, U32, hU.nslots, true);
-        }
-        denseL.factor(nb, false, lu.p, f32, s);
-        denseU.factor(nb, true, lu.p, f32, s);
-        RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    }
-    void apply(const double* r, double* z, const double* xp, bool f32, hipStream_t s) const {
-        const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
-        const void* Up = f32 ? static_cast<const void*>(&U32) : static_cast<const void*>(&U64);
-        void* y = f32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
-        void* rt = f32 ? static_cast<void*>(rt32.p) : static_cast<void*>(rt64.p);
-        auto sweepRun = [&](const LevelRun& run, size_t ri, bool fwd) {
-            const DenseSide& D = fwd ? denseL : denseU;
-            const LevelSched& sc = fwd ? sL : sU;
-            const void* sl = fwd ? Lp : Up;
-            const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)];
-            if (D.runTask0[ri] >= 0) {
-                const int t0 = sc.sStart[size_t(run.l0)], u0 = sc.qStart[size_t(run.l0)];
-                const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
-                launchRunDense(s, nb, f32, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
-                               D.runTask1[ri] - D.runTask0[ri], D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r,
-                               rt, y, z, xp);
-                return;
-            }
-            launchIluFused(s, nb, f32, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
-        };
-        for (size_t ri = 0; ri < runsL.size(); ++ri) {
-            const LevelRun& run = runsL[ri];
-            if (run.fused) { sweepRun(run, ri, true); continue; }
-            const int l = run.l0;
-            launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y);
-        }
-        for (size_t k = runsU.size(); k-- > 0;) {
-            const LevelRun& run = runsU[k];
-            if (run.fused) { sweepRun(run, k, false); continue; }
-            const int l = run.l0;
-            launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z, xp);
-        }
-    }
-    size_t bytes() const {
-        return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes() + denseL.bytes() + denseU.bytes();
-    }
-};
-
-// ------------------------------------------------------------------ AMG hierarchy on the device
-struct AmgLevelDev {
-    int n = 0, nnz = 0, nc = 0;
-    const int *rp = nullptr, *ci = nullptr, *diag = nullptr;   // level 0: the system's pattern
-    DBuf<int> rpB, ciB, diagB;
-    DBuf<double> v;
-    DBuf<int> agg, rrp, rci;
-    DBuf<int> apPtr, apSrc, cPtr, cSrc;
-    DBuf<double> ap;
-    DBuf<int> gsF, gsB;                  // rows in forward / backward level order
-    std::vector<int> gsFs, gsBs;         // level starts
-    DBuf<int> dgsFs, dgsBs;
-    std::vector<LevelRun> runsF, runsB;
-    DBuf<double> x1, x2, b, r;
-    DBuf<double> dinv;                   // 1 / a_ii (Jacobi)
-    const double* result = nullptr;
-};
-
-struct CprDev {
-    static constexpr double kOmega = 0.6667;   // damped-Jacobi weight of the reference's device V-cycle (bsr.h:173)
-    int n = 0, nb = 1;
-    DBuf<double> w, xp, t;
-    std::vector<std::unique_ptr<AmgLevelDev>> lv;
-    std::vector<int> hostAggSig;          // sizes per level, for reporting
-    int m = 0;                            // coarsest size
-    bool dense = false;
-    DBuf<double> M0, M1;                  // [C | I] ping-pong; the inverse ends in Minv
-    double* Minv = nullptr;
-    int smoother = RSH_SMOOTH_GS;
-    bool built = false;
-    // level 0 values in the SpMV row-slice layout of S (same pattern) for the streamed Jacobi steps
-    const SliceDev<double>* plan = nullptr;
-    DBuf<double> apS, apL;
-    std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
-
-    int generation = 0;
-    void buildHierarchy(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* apHost, hipStream_t s) {
-        ++generation;
-        HostCsr A0;
-        A0.n = S.n;
-        A0.rp = S.rp;
-        A0.ci = S.ci;
-        A0.v.assign(apHost, apHost + S.ci.size());
-        std::vector<AmgLevelHost> H;
-        amgSetup(std::move(A0), H);
-        lv.clear();
-        for (size_t l = 0; l < H.size(); ++l) {
-            auto L = std::make_unique<AmgLevelDev>();
-            AmgLevelHost& h = H[l];
-            L->n = h.A.n;
-            L->nnz = int(h.A.ci.size());
-            L->nc = h.nc;
-            if (l == 0) {
-                L->rp = drp; L->ci = dci; L->diag = ddiag;
-            } else {
-                L->rpB.upload(h.A.rp, s);
-                L->ciB.upload(h.A.ci, s);
-                L->diagB.upload(h.A.diag, s);
-                L->rp = L->rpB.p; L->ci = L->ciB.p; L->diag = L->diagB.p;
-            }
-            L->v.upload(h.A.v, s);
-            if (!h.agg.empty()) {
-                L->agg.upload(h.agg, s);
-                std::vector<int> rrp(size_t(h.nc) + 1, 0), rci(h.agg.size());
-                for (int a : h.agg) rrp[size_t(a) + 1]++;
-                for (int a = 0; a < h.nc; ++a) rrp[size_t(a) + 1] += rrp[size_t(a)];
-                std::vector<int> fill(rrp.begin(), rrp.end() - 1);
-                for (size_t i = 0; i < h.agg.size(); ++i) rci[size_t(fill[size_t(h.agg[i])]++)] = int(i);
-                L->rrp.upload(rrp, s);
-                L->rci.upload(rci, s);
-                L->apPtr.upload(h.apPtr, s);
-                L->apSrc.upload(h.apSrc, s);
-                L->cPtr.upload(h.cPtr, s);
-                L->cSrc.upload(h.cSrc, s);
-                L->ap.alloc(h.apPtr.size() - 1);
-            }
-            std::vector<int> ord;
-            csrLevels(h.A, true, ord, L->gsFs);
-            L->gsF.upload(ord, s);
-            csrLevels(h.A, false, ord, L->gsBs);
-            L->gsB.upload(ord, s);
-            L->dgsFs.upload(L->gsFs, s);
-            L->dgsBs.upload(L->gsBs, s);
-            auto work = [](const std::vector<int>& st) {
-                std::vector<int> w(st.size() - 1);
-                for (size_t q = 0; q + 1 < st.size(); ++q) w[q] = st[q + 1] - st[q];
-                return w;
-            };
-            L->runsF = levelRuns(work(L->gsFs), 2048);
-            L->runsB = levelRuns(work(L->gsBs), 2048);
-            L->x1.alloc(size_t(L->n));
-            L->x2.alloc(size_t(L->n));
-            L->dinv.alloc(size_t(L->n));
-            L->b.alloc(size_t(L->n));
-            L->r.alloc(size_t(L->n));
-            lv.push_back(std::move(L));
-        }
-        m = lv.back()->n;
-        dense = m <= 800;
-        if (dense) {
-            M0.alloc(size_t(m) * 2 * m);
-            M1.alloc(size_t(m) * 2 * m);
-        }
-        patRp = S.rp;
-        patCi = S.ci;
-        built = true;
-    }
-    void factorCoarsest(hipStream_t s) {
-        if (!dense) return;
-        AmgLevelDev& C = *lv.back();
-        launchDenseFromCsr(s, m, C.rp, C.ci, C.v.p, M0.p);
-        double* a = M0.p;
-        double* b = M1.p;
-        for (int k = 0; k < m; ++k) {
-            launchGaussJordanStep(s, m, k, a, b);
-            std::swap(a, b);
-        }
-        Minv = a;
-    }
-    // numeric set-up on the device values of S: weights, pressure matrix, hierarchy (fresh or refreshed)
-    void setup(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* val, bool reuse, hipStream_t s) {
-        n = S.n;
-        nb = S.nb;
-        w.alloc(size_t(n) * nb);
-        xp.alloc(size_t(n));
-        t.alloc(size_t(n) * nb);
-        launchCprWeights(s, nb, n, ddiag, val, w.p);
-        const bool refresh = reuse && built && patRp == S.rp && patCi == S.ci;
-        DBuf<double> apTmp;
-        double* ap0;
-        if (refresh) ap0 = lv[0]->v.p;
-        else { apTmp.alloc(S.ci.size()); ap0 = apTmp.p; }
-        launchCprPressure(s, nb, n, drp, val, w.p, ap0);
-        launchFixDiag(s, n, ddiag, ap0);
-        if (!refresh) {
-            std::vector<double> h(S.ci.size());
-            RSH_HIP(hipMemcpyAsync(h.data(), ap0, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-            RSH_HIP(hipStreamSynchronize(s));
-            buildHierarchy(S, drp, dci, ddiag, h.data(), s);
-        } else {
-            for (size_t l = 0; l + 1 < lv.size(); ++l) {
-                AmgLevelDev& L = *lv[l];
-                AmgLevelDev& Cn = *lv[l + 1];
-                launchGalerkin(s, int(L.ap.n), L.apPtr.p, L.apSrc.p, L.v.p, L.ap.p, Cn.nnz, L.cPtr.p, L.cSrc.p, Cn.v.p);
-                launchFixDiag(s, Cn.n, Cn.diag, Cn.v.p);
-            }
-        }
-        factorCoarsest(s);
-        for (auto& L : lv) launchDinv(s, L->n, L->diag, L->v.p, L->dinv.p);
-        if (plan) {
-            apS.alloc(size_t(plan->val.n / (size_t(nb) * nb)));
-            apL.alloc(plan->lsrc.n);
-            launchPackSlices(s, 1, lv[0]->v.p, plan->src.p, (long long)(apS.n / kWave), apS.p, false);
-            if (plan->lsrc.n) launchPackLong(s, 1, lv[0]->v.p, plan->lsrc.p, int(plan->lsrc.n), apL.p, false);
-        }
-    }
-    // one Gauss-Seidel sweep of level L over its level schedule (fused runs of small levels)
-    static void gsSweep(hipStream_t s, AmgLevelDev& L, const double* xin, double* xout, bool fwd) {
-        const std::vector<LevelRun>& runs = fwd ? L.runsF : L.runsB;
-        const std::vector<int>& st = fwd ? L.gsFs : L.gsBs;
-        const int* rows = fwd ? L.gsF.p : L.gsB.p;
-        const int* dst = fwd ? L.dgsFs.p : L.dgsBs.p;
-        for (const LevelRun& run : runs) {
-            if (run.fused) { launchGsFused(s, run.l0, run.l1, dst, rows, L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd); continue; }
-            const int q = run.l0;
-            launchGsLevel(s, st[size_t(q) + 1] - st[size_t(q)], rows + st[size_t(q)], L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd);
-        }
-    }
-    void cycle(size_t l, hipStream_t s) {
-        AmgLevelDev& L = *lv[l];
-        if (l + 1 == lv.size()) {
-            if (dense) {
-                launchDenseGemvLd(s, L, L.b.p, L.x1.p);
-                L.result = L.x1.p;
-            } else {
-                // 20 symmetric Gauss-Seidel sweeps from zero (amg.cpp:315-316)
-                launchFill(s, L.x1.p, 0.0, L.n);
-                double* a = L.x1.p;
-                double* b = L.x2.p;
-                for (int sw = 0; sw < 40; ++sw) {
-                    gsSweep(s, L, a, b, (sw % 2) == 0);
-                    std::swap(a, b);
-                }
-                L.result = a;
-            }
-            return;
-        }
-        AmgLevelDev& C = *lv[l + 1];
-        if (smoother == RSH_SMOOTH_JACOBI) {
-            // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing
-            const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
-            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p,
-                              double(L.nnz) / std::max(L.n, 1));
-            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
-            cycle(l + 1, s);
-            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, L.x1.p, L.agg.p, C.result, kOmega, L.x2.p, nullptr,
-                              double(L.nnz) / std::max(L.n, 1));
-            L.result = L.x2.p;
-            return;
-        }
-        gsSweep(s, L, nullptr, L.x1.p, true);
-        launchCsrSpmvResidual(s, L.n, L.rp, L.ci, L.v.p, L.x1.p, L.b.p, L.r.p);
-        launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
-        cycle(l + 1, s);
-        launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
-        gsSweep(s, L, L.x1.p, L.x2.p, false);
-        L.result = L.x2.p;
-    }
-    void launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out);
-    // one V-cycle on r_p (in lv[0].b) -> result pointer
-    const double* vcycle(hipStream_t s) {
-        cycle(0, s);
-        return lv[0]->result;
-    }
-    size_t bytes() const {
-        size_t b = w.bytes() + xp.bytes() + t.bytes() + M0.bytes() + M1.bytes();
-        for (auto& L : lv)
-            b += L->rpB.bytes() + L->ciB.bytes() + L->diagB.bytes() + L->v.bytes() + L->agg.bytes() + L->rrp.bytes() + L->rci.bytes() +
-                 L->apPtr.bytes() + L->apSrc.bytes() + L->cPtr.bytes() + L->cSrc.bytes() + L->ap.bytes() + L->gsF.bytes() + L->gsB.bytes() +
-                 L->x1.bytes() + L->x2.bytes() + L->b.bytes() + L->r.bytes();
-        return b;
-    }
-};
-
-}  // namespace
-
-// dense y = Minv b with the inverse in the right half of [I | C^-1] (leading dimension 2m)
-void CprDev::launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out) {
-    launchDenseGemv2(s, m, 2 * m, Minv + m, bv, out);
-    (void)L;
-}
-
-// ------------------------------------------------------------------ context and system
-struct rsh_context {
-    Ctx c;
-    std::mutex mu;
-};
-
-struct rsh_system {
-    rsh_context* ctx = nullptr;
-    int n = 0, nb = 1, nnzb = 0;
-    long long N = 0;
-    HostBsr pat;
-    DBuf<int> rp, ci, diag;
-    DBuf<double> val;
-    SliceHost spmvHost;
-    SliceDev<double> spmv64;
-    SliceDev<float> spmv32;
-    bool have32 = false;
-    DBuf<double> colP;                  // pressure column of S in the SpMV slice layout (CPR second stage)
-    rsh_options opt{};
-    bool ready = false;
-    bool valuesSet = false;
-    std::unique_ptr<IluDev> ilu;
-    std::unique_ptr<CprDev> cpr;
-    // Krylov workspace
-    DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w, irR, irD;
-    DBuf<double> V, Z, hcol, yv;
-    int restartAlloc = 0;
-    // last solve
-    int lastIts = 0, lastOuter = 0;
-    double lastRel = 0, lastSetupMs = 0, lastSolveMs = 0, lastIluMs = 0, lastCprMs = 0;
-    std::string lastStop;
-
-    hipStream_t s() const { return ctx->c.stream; }
-
-    void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
-
-    // Preconditioner applies are captured once per (input, output, precision) into a HIP graph and replayed: a
-    // level-scheduled apply is hundreds of small dependent launches, which the graph issues without host work.
-    std::map<std::tuple<const void*, const void*, int>, hipGraphExec_t> graphs;
-    std::pair<int, int> graphGen{-1, -1};
-    bool useGraphs = [] { const char* e = std::getenv("RSH_GRAPHS"); return !(e && e[0] == '0'); }();
-    void clearGraphs() {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
-    }
-    ~rsh_system() { clearGraphs(); }
-    void precondG(const double* rr, double* z, bool f32) {
-        if (!useGraphs) { precond(rr, z, f32); return; }
-        const std::pair<int, int> gen{ilu ? ilu->generation : 0, cpr ? cpr->generation : 0};
-        if (gen != graphGen) { clearGraphs(); graphGen = gen; }
-        const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z), (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5));
-        auto it = graphs.find(key);
-        if (it == graphs.end()) {
-            hipGraph_t g = nullptr;
-            RSH_HIP(hipStreamBeginCapture(s(), hipStreamCaptureModeThreadLocal));
-            try {
-                precond(rr, z, f32);
-            } catch (...) {
-                (void)hipStreamEndCapture(s(), &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            RSH_HIP(hipStreamEndCapture(s(), &g));
-            hipGraphExec_t ex = nullptr;
-            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) throw Error(RSH_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-            it = graphs.emplace(key, ex).first;
-        }
-        RSH_HIP(hipGraphLaunch(it->second, s()));
-    }
-
-    void spmv(const double* x, double* y, bool f32) {
-        launchSpmvSliced(s(), nb, f32, &spmv64, &spmv32, x, y);
-    }
-    // preconditioner z = M^-1 r (device vectors)
-    void precond(const double* rr, double* z, bool f32) {
-        const int pc = opt.precond;
-        if (pc == RSH_PREC_BILU0 || pc == RSH_PREC_MCBILU0) {
-            ilu->apply(rr, z, nullptr, f32, s());
-            return;
-        }
-        // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p
-        CprDev& C = *cpr;
-        launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p);
-        const double* xp = C.vcycle(s());
-        launchCprSecond(s(), nb, &spmv64, colP.p, rr, xp, C.t.p);
-        ilu->apply(C.t.p, z, xp, f32, s());
-    }
-};
-
-namespace {
-
-double nowMs() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
 
 void syncScalars(rsh_system& S) {
     Ctx& c = S.ctx->c;
@@ -848,11 +164,11 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     return RSH_ERR_NOT_CONVERGED;
 }
 
-// RSH_MIXED_IR: FP64 iterative refinement x += d, r = b - A x (FP64 SpMV, as refineWithDevice bsr.cpp:1373-1396)
-// around inner solves (BiCGSTAB, as the reference's float device solve bsr.cpp:1130-1215, or FGMRES) on the FP32
-// matrix and the FP32 preconditioner; their dot products and Krylov vectors stay FP64.  A correction that does not
+// FP64 iterative refinement x += d, r = b - A x (FP64 SpMV, as refineWithDevice bsr.cpp:1373-1396) around inner
+// solves (BiCGSTAB, as the reference's float device solve bsr.cpp:1130-1215, or FGMRES) with the FP32
+// preconditioner and, when opF32, the FP32 matrix; their dot products and Krylov vectors stay FP64.  A correction that does not
 // reduce the FP64 residual (a diverged inner solve) is taken back and the loop stops.
-int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, int& outer) {
+int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, int& outer, bool opF32) {
     Ctx& c = S.ctx->c;
     hipStream_t st = c.stream;
     const long long N = S.N;
@@ -868,7 +184,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     outer = 0;
     rel = 0;
     if (bn == 0) return RSH_OK;
-    const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-5;
+    const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-4;
     const int maxOuter = S.opt.max_outer > 0 ? S.opt.max_outer : 10;
     double rn = bn;
     for (outer = 0; outer < maxOuter; ++outer) {
@@ -877,10 +193,11 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         int inIts = 0;
         double inRel = 0;
         const double itol = std::max(innerTol, tol * bn / rn * 0.5);
-        const int left = std::max(1, maxIt - its);
+        // an inner solve that stalls is restarted by the next refinement step: at most maxIt / maxOuter iterations
+        const int left = std::max(1, std::min(maxIt - its, std::max(50, maxIt / maxOuter)));
         const int rc = S.opt.krylov == RSH_KRYLOV_FGMRES
-                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, true, false, inIts, inRel)
-                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true, true);
+                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, opF32, false, inIts, inRel)
+                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true, opF32);
         its += inIts;
         if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
         launchAxpy(st, N, 1.0, S.irD.p, x);
@@ -912,11 +229,15 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     const double t0 = nowMs();
     int its = 0, outer = 0, rc;
     double rel = 0;
-    const bool precF32 = S.opt.precision == RSH_MIXED;
-    if (S.opt.precision == RSH_MIXED_IR) rc = refine(S, b, x, tol, maxIt, its, rel, outer);
+    // RSH_MIXED: FGMRES with the FP32 preconditioner (its restarts recompute the FP64 true residual), or BiCGSTAB
+    // with the FP32 preconditioner restarted on the FP64 true residual (refinement with the FP64 operator), which
+    // keeps it from stalling on the preconditioner's FP32 rounding; RSH_MIXED_IR: refinement around FP32-operator
+    // inner solves
+    if (S.opt.precision == RSH_MIXED_IR) rc = refine(S, b, x, tol, maxIt, its, rel, outer, true);
+    else if (S.opt.precision == RSH_MIXED && S.opt.krylov == RSH_KRYLOV_BICGSTAB) rc = refine(S, b, x, tol, maxIt, its, rel, outer, false);
     else if (S.opt.krylov == RSH_KRYLOV_FGMRES)
-        rc = fgmres(S, b, x, tol, maxIt, S.opt.restart > 0 ? S.opt.restart : 30, precF32, false, false, its, rel);
-    else rc = bicgstab(S, b, x, tol, maxIt, its, rel, precF32, false);
+        rc = fgmres(S, b, x, tol, maxIt, S.opt.restart > 0 ? S.opt.restart : 30, S.opt.precision == RSH_MIXED, false, false, its, rel);
+    else rc = bicgstab(S, b, x, tol, maxIt, its, rel, false, false);
     RSH_HIP(hipStreamSynchronize(S.s()));
     S.lastSolveMs = nowMs() - t0;
     S.lastIts = its;
@@ -958,7 +279,7 @@ void rsh_default_options(rsh_options* o) {
     o->smoother = RSH_SMOOTH_GS;
     o->restart = 30;
     o->max_outer = 10;
-    o->inner_tol = 1e-5;
+    o->inner_tol = 1e-4;
     o->reuse_amg = 1;
 }
 
@@ -1083,11 +404,7 @@ int rsh_system_destroy(rsh_system* sys) {
 }
 
 static int afterValues(rsh_system& S) {
-    hipStream_t st = S.s();
-    packSlices(st, S.nb, S.val.p, S.spmv64, S.spmvHost.nslots, false);
-    if (S.have32) packSlices(st, S.nb, S.val.p, S.spmv32, S.spmvHost.nslots, true);
-    S.valuesSet = true;
-    S.ready = false;
+    S.valuesChanged();
     return RSH_OK;
 }
 
@@ -1131,6 +448,8 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         const bool wantMc = o.precond == RSH_PREC_MCBILU0 || o.precond == RSH_PREC_CPR_MC;
         if (!S.ilu || S.ilu->mc != wantMc) {
             S.ilu = std::make_unique<IluDev>();
+            S.ilu->keepEntries = S.keepIluEntries;
+            S.ilu->fuseMax = S.iluFuseMax;
             S.ilu->build(S.pat, S.rp.p, S.ci.p, S.diag.p, wantMc, st);
         }
         if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC) {

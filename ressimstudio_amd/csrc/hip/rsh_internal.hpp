@@ -63,7 +63,8 @@ struct KScalars {
     double rho, alpha, omega, rhoN, rv, sn, tt, ts, rn, bn, beta, tol;
     int state, it, pad0, pad1;
 };
-enum KState : int { kRun = 0, kConvS = 1, kConv = 2, kBreak = 3, kOmegaZero = 4 };
+// kDoneS: converged on ||s|| with the final update applied (batched solves, where other systems keep iterating)
+enum KState : int { kRun = 0, kConvS = 1, kConv = 2, kBreak = 3, kOmegaZero = 4, kDoneS = 5 };
 enum Epilogue : int { kEpNone = 0, kEpRho, kEpAlpha, kEpSn, kEpOmega, kEpRn, kEpBn, kEpNorm };
 
 struct Ctx {
@@ -249,5 +250,20 @@ void launchMultiAxpy(hipStream_t s, long long n, int k, const double* V, long lo
 // x += sum_i y[i] Z_i  (y on the device)
 void launchMultiAxpyAdd(hipStream_t s, long long n, int k, const double* Z, long long ld, const double* y, double* x);
 void launchCopyKernel(hipStream_t s, const double* src, double* dst, long long n);
+
+// batched BiCGSTAB over independent systems: task = {system, first entry, end entry} of the flat vectors,
+// tStart[s] = first task of system s; partial holds K * T sums
+void launchBInit(hipStream_t s, int ns, const double* tol, KScalars* ks);
+// per-system sums (into out[sys * K + k] when out is set) and the epilogue (when ks is set)
+void launchBFinal(hipStream_t s, int ns, int K, const int* tStart, const double* partial, int T, int ep, KScalars* ks, double* out = nullptr);
+void launchBAdd(hipStream_t s, int T, const int4* task, const double* d, double* x);   // x += d
+void launchBMask(hipStream_t s, int T, const int4* task, const int* keep, double* v);  // v = 0 where keep[system] == 0
+void launchBDots(hipStream_t s, int T, const int4* task, int K, const double* a0, const double* b0, const double* a1, const double* b1,
+                 double* partial);
+void launchBBicgP(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* r, const double* v, double* p);
+void launchBBicgS(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* r, const double* v, double* sv, double* partial);
+void launchBBicgXs(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* ph, double* x);
+void launchBBicgXR(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* ph, const double* sh, const double* sv,
+                   const double* t, double* x, double* r, double* partial);
 
 }  // namespace rsh

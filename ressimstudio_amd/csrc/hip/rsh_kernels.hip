@@ -1312,6 +1312,133 @@ __global__ __launch_bounds__(kBlock) void kBicgXR(long long n, const KScalars* k
     }
     reducePartial<1>(acc, partial);
 }
+// ---------------------------------------------------------------- batched BiCGSTAB (independent systems, per-system scalars)
+// The flat vectors hold every system; task t = {system, first entry, end entry} is one workgroup of the vector
+// kernels, so each workgroup reads its system's scalars once.  Reductions: one partial per task (fixed tree), then
+// one thread per system adds its tasks' partials in task order and runs the system's epilogue.
+__device__ inline void batchEpilogue(int ep, const double* d, KScalars* ks) {
+    if (ks->state == kConvS) {            // the x += alpha ph of the early exit has run: freeze the system
+        if (ep == kEpOmega) ks->state = kDoneS;
+        return;
+    }
+    if (ep == kEpBn) {
+        ks->bn = sqrt(d[0]);
+        if (ks->bn == 0) ks->state = kConv;   // zero right-hand side: x = 0
+        return;
+    }
+    runEpilogue(ep, d, ks);
+}
+template <int K>
+__device__ void taskPartial(double (&v)[K], double* __restrict__ partial, int T) {
+    __shared__ double sh[K][kBlock / kWave];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double t = waveSum(v[k]);
+        if (lane == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) s += sh[threadIdx.x][q];
+        partial[(long long)threadIdx.x * T + blockIdx.x] = s;
+    }
+}
+__global__ void kBFinal(int ns, int K, const int* __restrict__ tStart, const double* __restrict__ partial, int T, int ep, KScalars* ks,
+                        double* __restrict__ out) {
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= ns) return;
+    double d[2] = {0, 0};
+    for (int k = 0; k < K; ++k)
+        for (int t = tStart[sys]; t < tStart[sys + 1]; ++t) d[k] += partial[(long long)k * T + t];
+    if (out)
+        for (int k = 0; k < K; ++k) out[(long long)sys * K + k] = d[k];
+    if (ks) batchEpilogue(ep, d, ks + sys);
+}
+// zero the entries of the systems whose flag is 0
+__global__ __launch_bounds__(kBlock) void kBMask(const int4* __restrict__ task, const int* __restrict__ keep, double* __restrict__ v) {
+    const int4 tk = task[blockIdx.x];
+    if (keep[tk.x]) return;
+    for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) v[i] = 0.0;
+}
+// x += d over every task (refinement step)
+__global__ __launch_bounds__(kBlock) void kBAdd(const int4* __restrict__ task, const double* __restrict__ d, double* __restrict__ x) {
+    const int4 tk = task[blockIdx.x];
+    for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) x[i] += d[i];
+}
+template <int K>
+__global__ __launch_bounds__(kBlock) void kBDots(const int4* __restrict__ task, int T, const double* __restrict__ a0, const double* __restrict__ b0,
+                                                 const double* __restrict__ a1, const double* __restrict__ b1, double* partial) {
+    const int4 tk = task[blockIdx.x];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0;
+    for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) {
+        v[0] += a0[i] * b0[i];
+        if constexpr (K > 1) v[1] += a1[i] * b1[i];
+    }
+    taskPartial<K>(v, partial, T);
+}
+__global__ __launch_bounds__(kBlock) void kBBicgP(const int4* __restrict__ task, const KScalars* __restrict__ ks, const double* __restrict__ r,
+                                                  const double* __restrict__ v, double* __restrict__ p) {
+#pragma clang fp contract(off)
+    const int4 tk = task[blockIdx.x];
+    const KScalars& k = ks[tk.x];
+    if (k.state != kRun) return;
+    const double beta = k.beta, omega = k.omega;
+    for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+__global__ __launch_bounds__(kBlock) void kBBicgS(const int4* __restrict__ task, int T, const KScalars* __restrict__ ks, const double* __restrict__ r,
+                                                  const double* __restrict__ v, double* __restrict__ sv, double* partial) {
+#pragma clang fp contract(off)
+    const int4 tk = task[blockIdx.x];
+    const KScalars& k = ks[tk.x];
+    double acc[1] = {0};
+    if (k.state == kRun) {
+        const double alpha = k.alpha;
+        for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) {
+            const double s = r[i] - alpha * v[i];
+            sv[i] = s;
+            acc[0] += s * s;
+        }
+    }
+    taskPartial<1>(acc, partial, T);
+}
+__global__ __launch_bounds__(kBlock) void kBBicgXs(const int4* __restrict__ task, const KScalars* __restrict__ ks, const double* __restrict__ ph,
+                                                   double* __restrict__ x) {
+#pragma clang fp contract(off)
+    const int4 tk = task[blockIdx.x];
+    const KScalars& k = ks[tk.x];
+    if (k.state != kConvS) return;
+    const double alpha = k.alpha;
+    for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) x[i] += alpha * ph[i];
+}
+__global__ __launch_bounds__(kBlock) void kBBicgXR(const int4* __restrict__ task, int T, const KScalars* __restrict__ ks, const double* __restrict__ ph,
+                                                   const double* __restrict__ sh, const double* __restrict__ sv, const double* __restrict__ t,
+                                                   double* __restrict__ x, double* __restrict__ r, double* partial) {
+#pragma clang fp contract(off)
+    const int4 tk = task[blockIdx.x];
+    const KScalars& k = ks[tk.x];
+    double acc[1] = {0};
+    if (k.state == kRun) {
+        const double alpha = k.alpha, omega = k.omega;
+        for (int i = tk.y + threadIdx.x; i < tk.z; i += kBlock) {
+            x[i] += alpha * ph[i] + omega * sh[i];
+            const double ri = sv[i] - omega * t[i];
+            r[i] = ri;
+            acc[0] += ri * ri;
+        }
+    }
+    taskPartial<1>(acc, partial, T);
+}
+__global__ void kBInit(int ns, const double* __restrict__ tol, KScalars* ks) {
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= ns) return;
+    KScalars k{};
+    k.rho = 1; k.alpha = 1; k.omega = 1; k.tol = tol[sys]; k.state = kRun; k.it = 0;
+    ks[sys] = k;
+}
+
 __global__ void kAxpy(long long n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) y[i] += a * x[i];
 }
@@ -1764,6 +1891,47 @@ void launchMultiAxpyAdd(hipStream_t s, long long n, int k, const double* Z, long
 void launchCopyKernel(hipStream_t s, const double* src, double* dst, long long n) {
     const long long n2 = n / 2;
     hipLaunchKernelGGL(kCopyVec, dim3(gridFor(n2, kBlock * 4)), dim3(kBlock), 0, s, (const double2*)src, (double2*)dst, n2);
+    checkLaunch();
+}
+
+// ---------------------------------------------------------------- batched BiCGSTAB launchers
+void launchBInit(hipStream_t s, int ns, const double* tol, KScalars* ks) {
+    hipLaunchKernelGGL(kBInit, dim3((ns + 255) / 256), dim3(256), 0, s, ns, tol, ks);
+    checkLaunch();
+}
+void launchBFinal(hipStream_t s, int ns, int K, const int* tStart, const double* partial, int T, int ep, KScalars* ks, double* out) {
+    hipLaunchKernelGGL(kBFinal, dim3((ns + 255) / 256), dim3(256), 0, s, ns, K, tStart, partial, T, ep, ks, out);
+    checkLaunch();
+}
+void launchBMask(hipStream_t s, int T, const int4* task, const int* keep, double* v) {
+    hipLaunchKernelGGL(kBMask, dim3(T), dim3(kBlock), 0, s, task, keep, v);
+    checkLaunch();
+}
+void launchBAdd(hipStream_t s, int T, const int4* task, const double* d, double* x) {
+    hipLaunchKernelGGL(kBAdd, dim3(T), dim3(kBlock), 0, s, task, d, x);
+    checkLaunch();
+}
+void launchBDots(hipStream_t s, int T, const int4* task, int K, const double* a0, const double* b0, const double* a1, const double* b1,
+                 double* partial) {
+    if (K == 1) hipLaunchKernelGGL(kBDots<1>, dim3(T), dim3(kBlock), 0, s, task, T, a0, b0, a1, b1, partial);
+    else hipLaunchKernelGGL(kBDots<2>, dim3(T), dim3(kBlock), 0, s, task, T, a0, b0, a1, b1, partial);
+    checkLaunch();
+}
+void launchBBicgP(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* r, const double* v, double* p) {
+    hipLaunchKernelGGL(kBBicgP, dim3(T), dim3(kBlock), 0, s, task, ks, r, v, p);
+    checkLaunch();
+}
+void launchBBicgS(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* r, const double* v, double* sv, double* partial) {
+    hipLaunchKernelGGL(kBBicgS, dim3(T), dim3(kBlock), 0, s, task, T, ks, r, v, sv, partial);
+    checkLaunch();
+}
+void launchBBicgXs(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* ph, double* x) {
+    hipLaunchKernelGGL(kBBicgXs, dim3(T), dim3(kBlock), 0, s, task, ks, ph, x);
+    checkLaunch();
+}
+void launchBBicgXR(hipStream_t s, int T, const int4* task, const KScalars* ks, const double* ph, const double* sh, const double* sv,
+                   const double* t, double* x, double* r, double* partial) {
+    hipLaunchKernelGGL(kBBicgXR, dim3(T), dim3(kBlock), 0, s, task, T, ks, ph, sh, sv, t, x, r, partial);
     checkLaunch();
 }
 
