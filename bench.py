@@ -112,6 +112,112 @@ def cpu_baseline(s, tol):
             "iterations": its}
 
 
+def _upload(rs, ctx, a, ptrs):
+    a = np.ascontiguousarray(a)
+    p = ctypes.c_void_p()
+    rs._check(rs.lib().rsh_device_alloc(ctx.h, max(a.nbytes, 8), ctypes.byref(p)), "alloc")
+    rs._check(rs.lib().rsh_memcpy_h2d(ctx.h, p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes), "h2d")
+    ptrs.append(p)
+    return p
+
+
+def batched_leg(rs, gen, args):
+    """256 independent systems (block sizes 2/3/4, 432..45k cells) in one batch: one step = new values (D2D) +
+    numeric setup + the batched solve of every system to 1e-8, inputs resident in HBM; against the single-system
+    path looped over the same systems (same preconditioner, resident inputs)."""
+    L = rs.lib()
+    t = time.perf_counter()
+    systems = gen.batch_workload(args.batch_count)
+    cells = int(sum(s.n for s in systems))
+    log(f"batched workload: {len(systems)} systems, {cells} cells, generated in {time.perf_counter() - t:.1f}s")
+    ctx = rs.default_context()
+    ptrs = []
+    val = _upload(rs, ctx, np.concatenate([np.asarray(s.val, np.float64).reshape(-1) for s in systems]), ptrs)
+    rhs = _upload(rs, ctx, np.concatenate([s.rhs for s in systems]), ptrs)
+    N = int(sum(s.n * s.nb for s in systems))
+    xd = _upload(rs, ctx, np.zeros(N), ptrs)
+    m = len(systems)
+    tol = np.full(m, args.tol)
+    its = np.zeros(m, np.int32)
+    rel = np.zeros(m)
+    st = np.zeros(m, np.int32)
+    t = time.perf_counter()
+    B = rs.Batch(systems, ctx)
+    log(f"batch created in {time.perf_counter() - t:.1f}s")
+    out = {"workload": f"{m} systems, grid dims log-uniform in [10x10x3, 64x64x16], block size 2/3/4 in ratio 2:5:1, "
+                       f"sigma ln k in [0.5, 2.5], 1-2 wells each; {cells} cells in total", "systems": m, "cells": cells}
+
+    def bstep(o):
+        rs._check(L.rsh_batch_set_values_device(B.h, val), "batch set_values")
+        rs._check(L.rsh_batch_setup(B.h, ctypes.byref(o)), "batch setup")
+        rc = L.rsh_batch_solve_device(B.h, rhs, xd, rs._p(tol, ctypes.c_double), args.max_it, rs._p(its, ctypes.c_int),
+                                      rs._p(rel, ctypes.c_double), rs._p(st, ctypes.c_int))
+        rs._check(rc, "batch solve")
+
+    xs = {}
+    for name, prec in (("batched_mixed", rs.MIXED), ("batched_fp64", rs.FP64)):
+        o = rs.default_options(precond="mcbilu0", precision=prec)
+        t = time.perf_counter()
+        bstep(o)                      # symbolic set-up and tuning (once per pattern) outside the timed region
+        log(f"{name}: first step (symbolic + tuning) {time.perf_counter() - t:.1f}s")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            bstep(o)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / args.batch_steps
+        lat = B.latencies()
+        info = B.info()
+        x = np.empty(N)
+        rs._check(L.rsh_memcpy_d2h(ctx.h, x.ctypes.data_as(ctypes.c_void_p), xd, x.nbytes), "d2h")
+        xs[name] = x
+        out[name] = {"solves_per_s": m / dt, "ms_per_batch": dt * 1e3, "setup_ms": info["last_setup_ms"], "solve_ms": info["last_solve_ms"],
+                     "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
+                     "iterations_mean": float(its.mean()), "iterations_max": int(its.max()), "sweeps": info["last_sweeps"],
+                     "outer": info["last_outer"], "max_rel_res": float(rel.max()), "all_converged": bool((st == 0).all()),
+                     "chunk": info["chunk"], "fuse_max": [g.get("fuse_max") for g in info["groups"]]}
+        log(f"{name}: {m / dt:.1f} solves/s ({dt * 1e3:.1f} ms per batch: setup {info['last_setup_ms']:.1f} + solve {info['last_solve_ms']:.1f}), "
+            f"latency p50 {out[name]['latency_p50_ms']:.1f} ms p99 {out[name]['latency_p99_ms']:.1f} ms, its mean {its.mean():.0f} max {its.max()}")
+    B.close()
+    # the single-system path looped over the same systems (resident inputs, same preconditioner, FP64)
+    o = rs.default_options(precond="mcbilu0", precision=rs.FP64)
+    sysd, devs = [], []
+    t = time.perf_counter()
+    for s in systems:
+        d = rs.System(s, ctx)
+        dv = DeviceInputs(rs, ctx, s)
+        step(rs, d, dv, o, args.tol, args.max_it)
+        sysd.append(d)
+        devs.append(dv)
+    log(f"loop: symbolic set-up of {m} systems {time.perf_counter() - t:.1f}s")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lits = []
+    for d, dv in zip(sysd, devs):
+        k, _ = step(rs, d, dv, o, args.tol, args.max_it)
+        lits.append(k)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    err = 0.0
+    off = 0
+    for s, dv in zip(systems, devs):
+        xl = dv.x_host()
+        n = s.n * s.nb
+        err = max(err, float(np.linalg.norm(xs["batched_fp64"][off:off + n] - xl) / max(np.linalg.norm(xl), 1e-300)))
+        off += n
+    out["loop_fp64"] = {"solves_per_s": m / dt, "ms_total": dt * 1e3, "iterations_mean": float(np.mean(lits)),
+                        "max_rel_diff_batched_fp64_vs_loop": err}
+    out["speedup_mixed_vs_loop"] = out["batched_mixed"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
+    out["speedup_fp64_vs_loop"] = out["batched_fp64"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
+    log(f"loop_fp64: {m / dt:.1f} solves/s ({dt * 1e3:.0f} ms for {m} systems), batched vs loop max rel diff {err:.2e}")
+    for d, dv in zip(sysd, devs):
+        dv.free()
+        d.close()
+    for p in ptrs:
+        L.rsh_device_free(ctx.h, p)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +230,10 @@ def main():
     ap.add_argument("--extra", action="store_true", help="also time the exact Gauss-Seidel CPR (slow)")
     ap.add_argument("--configs", default="", help="comma list of variants to run (default: all but the --extra ones)")
     ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timing loops (for counter runs)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-systems leg")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched-systems leg")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus})", file=sys.stderr)
@@ -132,6 +242,9 @@ def main():
     import ressimstudio_amd as rs
     from ressimstudio_amd import gen
 
+    if args.batched_only:
+        print(json.dumps(batched_leg(rs, gen, args)))
+        return
     t = time.perf_counter()
     s = gen.large(scale=args.scale)
     log(f"workload: {s.n} cells x {s.nb}, {s.nnzb} blocks ({s.fill_blocks} well fill), generated in {time.perf_counter() - t:.1f}s")
@@ -203,6 +316,14 @@ def main():
         roof["spmv_traffic"] = tr["spmv"]["traffic_bytes"]
         roof["traffic_source"] = "profiles/cpr_mc_fp64_1.1M_hbm_traffic.md"
 
+    batched = None
+    if not args.no_batched and args.scale == 1.0:
+        try:
+            batched = batched_leg(rs, gen, args)
+        except Exception as e:  # noqa: BLE001
+            log(f"batched leg failed: {e}")
+            batched = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -233,6 +354,7 @@ def main():
         "kernels": k,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
     dev.free()

@@ -450,6 +450,8 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             S.ilu = std::make_unique<IluDev>();
             S.ilu->keepEntries = S.keepIluEntries;
             S.ilu->fuseMax = S.iluFuseMax;
+            S.ilu->rowSys = S.rowSys.empty() ? nullptr : &S.rowSys;
+            S.ilu->gate = S.gate;
             S.ilu->build(S.pat, S.rp.p, S.ci.p, S.diag.p, wantMc, st);
         }
         if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC) {

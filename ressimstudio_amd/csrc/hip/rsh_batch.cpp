@@ -85,8 +85,8 @@ struct rsh_batch {
     void precondAll(const double* in, double* out) {
         for (auto& g : groups) g.S->precond(in + g.off, out + g.off, f32());
     }
-    void spmvAll(const double* in, double* out) {
-        for (auto& g : groups) g.S->spmv(in + g.off, out + g.off, false);
+    void spmvAll(const double* in, double* out, bool gated = true) {
+        for (auto& g : groups) g.S->spmv(in + g.off, out + g.off, false, gated);
     }
     // one BiCGSTAB iteration of every running system (linsolve.cpp:174-212 per system), then the states to the host
     void iteration() {
@@ -247,7 +247,7 @@ struct rsh_batch {
                 if (hks[i].state == kBreak) lastStatus[size_t(i)] = RSH_ERR_BREAKDOWN;
             }
             launchBAdd(st, T, task.p, x.p, xo.p);
-            spmvAll(xo.p, ax.p);
+            spmvAll(xo.p, ax.p, false);    // true residual of every system (no gating by the inner solve's states)
             launchSub(st, N, b.p, ax.p, rt.p);
             std::vector<double> rnew;
             norms(rt.p, rnew);
@@ -315,6 +315,10 @@ int log2Bucket(long long v) {
 void rsh_batch::tune() {
     hipStream_t st = s();
     std::lock_guard<std::mutex> lk(g_tuneMu);
+    // every system running while the candidates are timed
+    std::vector<double> tol(sys.size(), 1e-8);
+    RSH_HIP(hipMemcpyAsync(tolD.p, tol.data(), sizeof(double) * tol.size(), hipMemcpyHostToDevice, st));
+    launchBInit(st, int(sys.size()), tolD.p, ks.p);
     fuseChosen.assign(groups.size(), 16);
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         Group& g = groups[gi];
@@ -346,8 +350,6 @@ void rsh_batch::tune() {
         int best = chunk;
         launchFill(st, r.p, 1.0, N);
         launchFill(st, v.p, 1.0, N);
-        std::vector<double> tol(sys.size(), 1e-8);
-        RSH_HIP(hipMemcpyAsync(tolD.p, tol.data(), sizeof(double) * tol.size(), hipMemcpyHostToDevice, st));
         for (int cand : {2048, 8192, 32768}) {
             buildTasks(cand);
             launchBInit(st, int(sys.size()), tolD.p, ks.p);
@@ -429,6 +431,15 @@ int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, c
             if (rc != RSH_OK) return rc;
             S->keepIluEntries = true;
             g.S = S;
+            // block row -> system of the batch; slices and long rows of stopped systems are skipped
+            S->rowSys.resize(size_t(rows));
+            for (int i : g.members) {
+                const auto& q = B->sys[size_t(i)];
+                std::fill(S->rowSys.begin() + q.rowBase, S->rowSys.begin() + q.rowBase + q.n, i);
+            }
+            sliceOwners(S->spmvHost, S->rowSys);
+            S->spmv64.owner.upload(S->spmvHost.owner, ctx->c.stream);
+            S->spmv64.lowner.upload(S->spmvHost.lowner, ctx->c.stream);
             off += g.N;
             B->groups.push_back(std::move(g));
         }
@@ -441,6 +452,7 @@ int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, c
         B->ks.alloc(size_t(count));
         RSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->hks), sizeof(KScalars) * size_t(count), hipHostMallocDefault));
         B->buildTasks(8192);
+        for (auto& g : B->groups) g.S->gate = B->ks.p;
         RSH_HIP(hipStreamSynchronize(B->s()));
         *out = B.release();
         return RSH_OK;

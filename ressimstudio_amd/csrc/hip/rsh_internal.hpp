@@ -95,6 +95,8 @@ struct SliceHost {
     std::vector<int> lrow, lorig, lptr, lcol, lsrc;
     std::vector<int> dsrc;         // optional per slice*64+lane: diagonal block (ILU backward sweep)
     std::vector<int> ldsrc;        // per long row
+    // optional (batched systems): the system owning each slice / long row, -1 when a slice mixes systems
+    std::vector<int> owner, lowner;
 };
 
 template <typename TA>
@@ -105,6 +107,7 @@ struct SliceDev {
     DBuf<TA> val, dval;
     DBuf<int> lrow, lorig, lptr, lcol, lsrc, ldsrc;
     DBuf<TA> lval, ldval;
+    DBuf<int> owner, lowner;
     size_t bytes() const {
         return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
                lrow.bytes() + lorig.bytes() + lptr.bytes() + lcol.bytes() + lsrc.bytes() + ldsrc.bytes() + lval.bytes() + ldval.bytes();
@@ -170,13 +173,15 @@ void launchPackLong(hipStream_t s, int nb, const double* src, const int* srcIdx,
 void launchPackColumn0(hipStream_t s, int nb, const double* src, const int* srcIdx, long long nslots, double* dst);
 void launchGatherBlocks(hipStream_t s, int nb, const double* src, const int* map, int count, double* dst);
 
+// gate (batched systems): with ks set, a slice / long row whose owner system is no longer iterating is skipped
 // y = A x over a slice layout (all levels at once); TA value type, TV vector type
 void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x,
-                      double* y);
+                      double* y, const KScalars* gate = nullptr);
 // ILU sweeps of one level
-void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y);
+void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
+                      const KScalars* gate = nullptr);
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
-                       const double* xp);
+                       const double* xp, const KScalars* gate = nullptr);
 // a run of small levels [l0, l1) swept by one workgroup (level starts on the device); its rows are the working rows
 // [wbase, wbase + wcount), wcount <= kRunRowsMax, perm maps working rows to original rows
 constexpr int kRunRowsMax = 2048;
@@ -190,7 +195,8 @@ constexpr int kDenseThreads = 1024;
 // rt, then per task {group, first scalar row} the dense product into y (and z after the backward sweep)
 void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
-                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp);
+                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
+                    const KScalars* gate = nullptr);
 // numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);

@@ -20,6 +20,12 @@ __device__ inline double waveSum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// a slice / long row i of a batched system that stopped iterating (owner -1: the slice mixes systems, always run)
+__device__ inline bool gatedOff(const int* __restrict__ owner, int i, const KScalars* __restrict__ ks) {
+    if (!ks || !owner) return false;
+    const int o = owner[i];
+    return o >= 0 && ks[o].state != kRun;
+}
 // sum over aligned groups of G lanes (G a power of two <= 64)
 template <int G>
 __device__ inline double groupSum(double v) {
@@ -149,11 +155,13 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
                                                       const int* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                       const int* __restrict__ lrow, const int* __restrict__ lptr,
                                                       const int* __restrict__ lcol, const TA* __restrict__ lval,
-                                                      const double* __restrict__ x, double* __restrict__ y) {
+                                                      const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ owner,
+                                                      const int* __restrict__ lowner, const KScalars* __restrict__ gate) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wid < nslices) {
+        if (gatedOff(owner, wid, gate)) return;
         const int2 inf = info[wid];
         const int r = row[(long long)wid * kWave + lane];
         double acc[NB];
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
         return;
     }
     const int q = wid - nslices;
-    if (q >= nlong) return;
+    if (q >= nlong || gatedOff(lowner, q, gate)) return;
     double part[NB];
 #pragma unroll
     for (int e = 0; e < NB; ++e) part[e] = 0;
@@ -214,12 +222,15 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
                                                       const int* __restrict__ col, const TA* __restrict__ val,
                                                       const int* __restrict__ lrow, const int* __restrict__ lorig,
                                                       const int* __restrict__ lptr, const int* __restrict__ lcol,
-                                                      const TA* __restrict__ lval, const double* __restrict__ r, TV* __restrict__ y) {
+                                                      const TA* __restrict__ lval, const double* __restrict__ r, TV* __restrict__ y,
+                                                      const int* __restrict__ owner, const int* __restrict__ lowner,
+                                                      const KScalars* __restrict__ gate) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wid < ns) {
         const int sl = s0 + wid;
+        if (gatedOff(owner, sl, gate)) return;
         const int wr = row[(long long)sl * kWave + lane];
         if (wr < 0) return;
         const int o = orig[(long long)sl * kWave + lane];
@@ -249,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
     const int qi = wid - ns;
     if (qi >= nq) return;
     const int q = q0 + qi;
+    if (gatedOff(lowner, q, gate)) return;
     TV part[NB];
 #pragma unroll
     for (int e = 0; e < NB; ++e) part[e] = 0;
@@ -283,7 +295,8 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
                                                        const int* __restrict__ lorig, const int* __restrict__ lptr,
                                                        const int* __restrict__ lcol, const TA* __restrict__ lval,
                                                        const TA* __restrict__ ldval, TV* __restrict__ y, double* __restrict__ z,
-                                                       const double* __restrict__ xp) {
+                                                       const double* __restrict__ xp, const int* __restrict__ owner,
+                                                       const int* __restrict__ lowner, const KScalars* __restrict__ gate) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -293,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
     int dstride;
     if (wid < ns) {
         const int sl = s0 + wid;
+        if (gatedOff(owner, sl, gate)) return;
         wr = row[(long long)sl * kWave + lane];
         if (wr < 0) return;
         o = orig[(long long)sl * kWave + lane];
@@ -320,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
         const int qi = wid - ns;
         if (qi >= nq) return;
         const int q = q0 + qi;
+        if (gatedOff(lowner, q, gate)) return;
         TV part[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) part[e] = 0;
@@ -441,6 +456,7 @@ template <typename TA>
 struct SliceView {
     const int2* info; const int* row; const int* orig; const int* col; const TA* val; const TA* dval;
     const int* lrow; const int* lorig; const int* lptr; const int* lcol; const TA* lval; const TA* ldval;
+    const int* owner; const int* lowner;
 };
 
 constexpr int kFuseThreads = 1024;
@@ -600,9 +616,10 @@ __global__ __launch_bounds__(kFuseThreads) void kIluFused(int l0, int l1, const 
 // outside contributions of every task of the levels [l0, l1): rt[(row - wbase) * NB + e], one wavefront per task
 template <int NB, typename TA, typename TV, bool FWD>
 __global__ __launch_bounds__(256) void kRunOutside(int t0, int ns, int u0, int nq, SliceView<TA> A, int wbase, int wend,
-                                                   const double* __restrict__ r, const TV* y, TV* rt) {
+                                                   const double* __restrict__ r, const TV* y, TV* rt, const KScalars* __restrict__ gate) {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= ns + nq) return;
+    if (t < ns ? gatedOff(A.owner, t0 + t, gate) : gatedOff(A.lowner, u0 + t - ns, gate)) return;
     runTask<NB, TA, TV, FWD, false>(A, t < ns ? t0 + t : -(u0 + t - ns) - 1, lane, wbase, wend, rt, r, y);
 }
 
@@ -614,11 +631,13 @@ __global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restric
                                                           const int* __restrict__ rows, const long long* __restrict__ off,
                                                           const TM* __restrict__ inv, int wbase, const TV* __restrict__ rt, TV* y,
                                                           double* __restrict__ z, const int* __restrict__ perm,
-                                                          const double* __restrict__ xp) {
+                                                          const double* __restrict__ xp, const int* __restrict__ gowner,
+                                                          const KScalars* __restrict__ gate) {
     constexpr int W = kDenseThreads / kWave;
     __shared__ TV v[kDenseMaxRows * NB];
     __shared__ TV part[W][kWave];
     const int2 tk = tasks[blockIdx.x];   // x group, y first scalar row of the tile
+    if (gatedOff(gowner, tk.x, gate)) return;
     const int r0 = start[tk.x], K = (start[tk.x + 1] - r0) * NB;
     for (int t = threadIdx.x; t < K; t += kDenseThreads) v[t] = rt[(long long)(rows[r0 + t / NB] - wbase) * NB + t % NB];
     __syncthreads();
@@ -1491,35 +1510,36 @@ __global__ void kMultiAxpy(long long n, int k, const double* __restrict__ V, lon
 }
 
 template <int NB>
-void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y) {
+void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y, const KScalars* gate) {
     if (f32) {
         const SliceDev<float>& d = *d32;
         const int waves = d.nslices + d.nlong;
         if (!waves) return;
         hipLaunchKernelGGL((kSpmvSliced<NB, float>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
-                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y);
+                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
     } else {
         const SliceDev<double>& d = *d64;
         const int waves = d.nslices + d.nlong;
         if (!waves) return;
         hipLaunchKernelGGL((kSpmvSliced<NB, double>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
-                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y);
+                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
     }
 }
 
 template <int NB, typename TA, typename TV>
-void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, const double* r, TV* y) {
+void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, const double* r, TV* y, const KScalars* gate) {
     const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
     if (!waves) return;
     hipLaunchKernelGGL((kIluForward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p, d.orig.p,
-                       d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y);
+                       d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y, d.owner.p, d.lowner.p, gate);
 }
 template <int NB, typename TA, typename TV>
-void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp) {
+void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp, const KScalars* gate) {
     const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
     if (!waves) return;
     hipLaunchKernelGGL((kIluBackward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p,
-                       d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, y, z, xp);
+                       d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, y, z, xp, d.owner.p,
+                       d.lowner.p, gate);
 }
 
 template <int NB>
@@ -1551,7 +1571,7 @@ void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const double* colP, c
 template <int NB, typename TA, typename TV>
 void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, const int* sStart, const int* qStart, int wbase, int wcount,
                const int* perm, const double* r, TV* y, double* z, const double* xp) {
-    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p};
+    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, d.owner.p, d.lowner.p};
     if (fwd) hipLaunchKernelGGL((kIluFused<NB, TA, TV, true>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
     else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
 }
@@ -1603,46 +1623,51 @@ void launchGatherBlocks(hipStream_t s, int nb, const double* src, const int* map
     hipLaunchKernelGGL(kGatherBlocks, dim3(gridFor(total)), dim3(kBlock), 0, s, nb * nb, src, map, count, dst);
     checkLaunch();
 }
-void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y) {
-    RSH_NB_DISPATCH(nb, (spmvT<NB>(s, f32, d64, d32, x, y)));
+void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y,
+                      const KScalars* gate) {
+    RSH_NB_DISPATCH(nb, (spmvT<NB>(s, f32, d64, d32, x, y, gate)));
     checkLaunch();
 }
-void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y) {
-    if (f32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y)))
-    else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y)))
+void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
+                      const KScalars* gate) {
+    if (f32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y, gate)))
+    else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y, gate)))
     checkLaunch();
 }
-void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp) {
-    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp)))
-    else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp)))
+void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp,
+                       const KScalars* gate) {
+    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp, gate)))
+    else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp, gate)))
     checkLaunch();
 }
 
 
 template <int NB, typename TA, typename TV>
 void runOutsideT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend, const double* r,
-                 const TV* y, TV* rt) {
-    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p};
+                 const TV* y, TV* rt, const KScalars* gate) {
+    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, d.owner.p, d.lowner.p};
     const int blocks = (ns + nq + 3) / 4;
-    if (fwd) hipLaunchKernelGGL((kRunOutside<NB, TA, TV, true>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt);
-    else hipLaunchKernelGGL((kRunOutside<NB, TA, TV, false>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt);
+    if (fwd) hipLaunchKernelGGL((kRunOutside<NB, TA, TV, true>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt, gate);
+    else hipLaunchKernelGGL((kRunOutside<NB, TA, TV, false>), dim3(blocks), dim3(256), 0, s, t0, ns, u0, nq, A, wbase, wend, r, y, rt, gate);
 }
 template <int NB, typename TM, typename TV>
 void runDenseT(hipStream_t s, bool fwd, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const TM* inv,
-               int wbase, const TV* rt, TV* y, double* z, const int* perm, const double* xp) {
-    if (fwd) hipLaunchKernelGGL((kRunDense<NB, TM, TV, true>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp);
-    else hipLaunchKernelGGL((kRunDense<NB, TM, TV, false>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp);
+               int wbase, const TV* rt, TV* y, double* z, const int* perm, const double* xp, const int* gowner, const KScalars* gate) {
+    if (fwd) hipLaunchKernelGGL((kRunDense<NB, TM, TV, true>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp,
+                                gowner, gate);
+    else hipLaunchKernelGGL((kRunDense<NB, TM, TV, false>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp,
+                            gowner, gate);
 }
 void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
-                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp) {
+                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner, const KScalars* gate) {
     if (ns + nq > 0) {
-        if (f32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt)))
-        else RSH_NB_DISPATCH(nb, (runOutsideT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt)))
+        if (f32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt, gate)))
+        else RSH_NB_DISPATCH(nb, (runOutsideT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
         checkLaunch();
     }
-    if (f32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp)))
-    else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp)))
+    if (f32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate)))
+    else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate)))
     checkLaunch();
 }
 template <int NB>

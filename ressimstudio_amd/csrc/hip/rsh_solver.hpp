@@ -38,6 +38,23 @@ int guarded(F&& f) {
     }
 }
 
+// owner system of every slice / long row (-1 when a slice mixes systems); rowSys maps original rows to systems
+inline void sliceOwners(SliceHost& h, const std::vector<int>& rowSys) {
+    h.owner.assign(size_t(h.nslices), -1);
+    for (int sl = 0; sl < h.nslices; ++sl) {
+        int o = -2;
+        for (int lane = 0; lane < kSliceRows; ++lane) {
+            const int r = h.orig.empty() ? h.row[size_t(sl) * kSliceRows + size_t(lane)] : h.orig[size_t(sl) * kSliceRows + size_t(lane)];
+            if (r < 0) continue;
+            const int q = rowSys[size_t(r)];
+            o = (o == -2 || o == q) ? q : -1;
+        }
+        h.owner[size_t(sl)] = o < 0 ? -1 : o;
+    }
+    h.lowner.assign(h.lrow.size(), -1);
+    for (size_t q = 0; q < h.lrow.size(); ++q) h.lowner[q] = rowSys[size_t(h.lorig.empty() ? h.lrow[q] : h.lorig[q])];
+}
+
 template <typename TA>
 void uploadSlices(const SliceHost& h, SliceDev<TA>& d, int nb, hipStream_t s) {
     d.nslices = h.nslices;
@@ -57,6 +74,8 @@ void uploadSlices(const SliceHost& h, SliceDev<TA>& d, int nb, hipStream_t s) {
     d.lcol.upload(h.lcol, s);
     d.lsrc.upload(h.lsrc, s);
     d.ldsrc.upload(h.ldsrc, s);
+    d.owner.upload(h.owner, s);
+    d.lowner.upload(h.lowner, s);
     const size_t b2 = size_t(nb) * nb;
     d.val.alloc(size_t(h.nslots) * b2 * kWave);
     d.lval.alloc(h.lcol.size() * b2);
@@ -85,10 +104,13 @@ struct DenseSide {
     DBuf<int2> tasks;
     DBuf<double> raw, inv;
     DBuf<float> inv32;
+    DBuf<int> gowner;                      // batched systems: the system of each group
 
-    // E: the sweep's entries per working row (src = factor block; dsrc = pivot inverse for the upper sweep)
-    void build(const std::vector<LevelRun>& runs, const RowEntries& E, const std::vector<int>& lstart, int nb, bool upper, hipStream_t s) {
-        std::vector<int> hstart(1, 0), hrows;
+    // E: the sweep's entries per working row (src = factor block; dsrc = pivot inverse for the upper sweep);
+    // rowSys (optional): original row -> system
+    void build(const std::vector<LevelRun>& runs, const RowEntries& E, const std::vector<int>& lstart, int nb, bool upper, hipStream_t s,
+               const std::vector<int>* rowSys = nullptr) {
+        std::vector<int> hstart(1, 0), hrows, hown;
         std::vector<long long> hoff;
         std::vector<int4> hmap;
         std::vector<int2> htasks;
@@ -149,6 +171,7 @@ struct DenseSide {
                     if (upper) hmap.push_back(make_int4(gi, int(i), int(i), E.dsrc[size_t(w)]));
                 }
                 hstart.push_back(int(hrows.size()));
+                if (rowSys) hown.push_back((*rowSys)[size_t(E.orig[size_t(g[0])])]);
                 for (int a0 = 0; a0 < K; a0 += kWave) htasks.push_back(make_int2(gi, a0));
             }
             runTask1[ri] = int(htasks.size());
@@ -159,6 +182,7 @@ struct DenseSide {
         off.upload(hoff, s);
         map.upload(hmap, s);
         tasks.upload(htasks, s);
+        gowner.upload(hown, s);
         raw.alloc(size_t(elems));
         inv.alloc(size_t(elems));
         inv32.release();
@@ -207,6 +231,9 @@ struct IluDev {
     DBuf<float> rt32;
     // level chunking: levels with at most fuseMax tasks are swept in runs (one workgroup or dense groups)
     int fuseMax = 16;
+    // batched systems: original row -> system, and the systems' BiCGSTAB states (slices of stopped systems skipped)
+    const std::vector<int>* rowSys = nullptr;
+    const KScalars* gate = nullptr;
     // kept when keepEntries is set (batched systems re-chunk the levels when tuning)
     bool keepEntries = false;
     RowEntries keptL, keptU;
@@ -257,6 +284,10 @@ struct IluDev {
         for (int w = 0; w < n; ++w) { lenL[size_t(w)] = EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)]; lenU[size_t(w)] = EU.ptr[size_t(w) + 1] - EU.ptr[size_t(w)]; }
         buildSlices(EL, lstart, rowCap(lenL), hL, sL);
         buildSlices(EU, lstart, rowCap(lenU), hU, sU);
+        if (rowSys) {
+            sliceOwners(hL, *rowSys);
+            sliceOwners(hU, *rowSys);
+        }
         uploadSlices(hL, L64, nb, s);
         uploadSlices(hU, U64, nb, s);
         {
@@ -270,8 +301,8 @@ struct IluDev {
             for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
             runsL = levelRuns(wL, fuseMax, &rowsL, kRunRowsMax);
             runsU = levelRuns(wU, fuseMax, &rowsL, kRunRowsMax);
-            denseL.build(runsL, EL, lstart, nb, false, s);
-            denseU.build(runsU, EU, lstart, nb, true, s);
+            denseL.build(runsL, EL, lstart, nb, false, s, rowSys);
+            denseU.build(runsU, EU, lstart, nb, true, s, rowSys);
             if (keepEntries) {
                 keptL = EL;
                 keptU = EU;
@@ -322,8 +353,8 @@ struct IluDev {
         fuseMax = newFuseMax;
         runsL = levelRuns(workL, fuseMax, &rowsPerLevel, kRunRowsMax);
         runsU = levelRuns(workU, fuseMax, &rowsPerLevel, kRunRowsMax);
-        denseL.build(runsL, keptL, lstart, nb, false, s);
-        denseU.build(runsU, keptU, lstart, nb, true, s);
+        denseL.build(runsL, keptL, lstart, nb, false, s, rowSys);
+        denseU.build(runsU, keptU, lstart, nb, true, s, rowSys);
         denseL.factor(nb, false, lu.p, have32, s);
         denseU.factor(nb, true, lu.p, have32, s);
         ++generation;
@@ -365,7 +396,7 @@ struct IluDev {
                 const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
                 launchRunDense(s, nb, f32, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
                                D.runTask1[ri] - D.runTask0[ri], D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r,
-                               rt, y, z, xp);
+                               rt, y, z, xp, D.gowner.p, gate);
                 return;
             }
             launchIluFused(s, nb, f32, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
@@ -374,13 +405,15 @@ struct IluDev {
             const LevelRun& run = runsL[ri];
             if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
-            launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y);
+            launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
+                             gate);
         }
         for (size_t k = runsU.size(); k-- > 0;) {
             const LevelRun& run = runsU[k];
             if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
-            launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z, xp);
+            launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
+                              xp, gate);
         }
     }
     size_t bytes() const {
@@ -651,6 +684,8 @@ struct rsh_system {
     std::unique_ptr<CprDev> cpr;
     bool keepIluEntries = false;        // member of a batch: the ILU level chunking can be retuned
     int iluFuseMax = 16;
+    std::vector<int> rowSys;            // member of a batch: block row -> system of the batch
+    const KScalars* gate = nullptr;     // member of a batch: the systems' states (stopped systems are skipped)
     // Krylov workspace
     DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w, irR, irD;
     DBuf<double> V, Z, hcol, yv;
@@ -708,8 +743,9 @@ struct rsh_system {
         RSH_HIP(hipGraphLaunch(it->second, s()));
     }
 
-    void spmv(const double* x, double* y, bool f32) {
-        launchSpmvSliced(s(), nb, f32, &spmv64, &spmv32, x, y);
+    // gated: skip the rows of stopped batched systems (their outputs are then stale)
+    void spmv(const double* x, double* y, bool f32, bool gated = true) {
+        launchSpmvSliced(s(), nb, f32, &spmv64, &spmv32, x, y, gated ? gate : nullptr);
     }
     // preconditioner z = M^-1 r (device vectors)
     void precond(const double* rr, double* z, bool f32) {

@@ -142,6 +142,22 @@ def large(seed: int = 20240602, scale: float = 1.0) -> BlockSystem:
     return reservoir(nx, ny, nz, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=seed)
 
 
+def batch_workload(count: int = 256, seed: int = 20240604) -> list:
+    """Independent systems of uneven size and block size: grid dimensions log-uniform between 10x10x3 and 64x64x16
+    (each dimension drawn on its own), block size 2, 3 or 4 in ratio 2:5:1, sigma ln k uniform in [0.5, 2.5], one or
+    two wells with BHP rows; system i uses generator seed ``seed + i``.  The draws come from numpy's PCG64 seeded
+    with ``seed``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(10), np.log(64))))) for _ in range(2))
+        nz = int(round(np.exp(rng.uniform(np.log(3), np.log(16)))))
+        nb = int(rng.choice([2, 3, 4], p=[0.25, 0.625, 0.125]))
+        sigma = float(rng.uniform(0.5, 2.5))
+        out.append(reservoir(nx, ny, nz, nb, model=0, sigma=sigma, wells=1 + i % 2, seed=seed + i))
+    return out
+
+
 def block_laplace(nx: int, ny: int, nb: int, seed: int) -> BlockSystem:
     """The reference tests' 2-D block Laplacian (run_tests.cpp:3694-3724), rhs = 1."""
     return _wrap(_lib().rsg_block_laplace(nx, ny, nb, seed))
