@@ -218,6 +218,59 @@ def batched_leg(rs, gen, args):
     return out
 
 
+def serving_leg(rs, gen, args):
+    """The serving loop under Poisson arrivals: 64 resident models (5k-500k cells, log-uniform), each request one
+    Newton iteration of a random model (values perturbed +-5 % on the device from the model's base values, new
+    numeric setup with the resident AMG aggregates refreshed, CPR-BiCGSTAB to 1e-8); latency = submit -> done."""
+    t = time.perf_counter()
+    models = gen.serving_models(args.serve_models)
+    cells = int(sum(m.n for m in models))
+    log(f"serving workload: {len(models)} models, {cells} cells, generated in {time.perf_counter() - t:.1f}s")
+    rng = np.random.default_rng(20240606)
+    lambdas = [float(x) for x in args.serve_lambdas.split(",") if x]
+    out = {"workload": f"{len(models)} resident models, 5k-500k cells log-uniform ({cells} cells), nb 3, CPR(Jacobi AMG)+MC-BILU0 "
+                       f"BiCGSTAB to 1e-8, values perturbed +-5 % per request", "models": len(models), "cells": cells,
+           "slots": args.serve_slots, "requests_per_rate": args.serve_requests, "hbm_budget_GB": 200}
+    for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED)):
+        srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9)
+        ids = [srv.add_model(m, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, precision=prec) for m in models]
+        t = time.perf_counter()
+        warm = [srv.submit(i, m.rhs, perturb=0.05, seed=1, tol=args.tol, max_it=args.max_it, want_x=False) for i, m in zip(ids, models)]
+        for tk in warm:
+            srv.wait(tk)
+        log(f"serving {name}: models loaded (symbolic set-up + first iteration) in {time.perf_counter() - t:.1f}s; {srv.stats()}")
+        newton = [2] * len(models)
+        rows = []
+        for lam in lambdas:
+            gaps = rng.exponential(1.0 / lam, args.serve_requests)
+            picks = rng.integers(0, len(models), args.serve_requests)
+            tickets = []
+            t0 = time.perf_counter()
+            due = t0
+            for g, k in zip(gaps, picks):
+                due += g
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets.append(srv.submit(ids[k], models[k].rhs, perturb=0.05, seed=1000 * int(k) + newton[k], tol=args.tol,
+                                          max_it=args.max_it, want_x=False))
+                newton[k] = 2 + (newton[k] - 1) % 11      # 4-12 Newton iterations per step: cycle through the seeds
+            res = [srv.wait(tk) for tk in tickets]
+            t1 = time.perf_counter()
+            lat = np.array([r["latency_ms"] for r in res])
+            ok = sum(r["status"] == 0 for r in res)
+            row = {"lambda": lam, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "mean_service_ms": float(np.mean([r["service_ms"] for r in res])), "throughput_req_s": len(res) / (t1 - t0),
+                   "converged": ok, "requests": len(res), "iterations_mean": float(np.mean([r["iterations"] for r in res]))}
+            rows.append(row)
+            log(f"serving {name} lambda {lam:g}/s: p50 {row['p50_ms']:.1f} ms, p99 {row['p99_ms']:.1f} ms, throughput "
+                f"{row['throughput_req_s']:.1f} req/s, {ok}/{len(res)} converged")
+        st = srv.stats()
+        srv.close()
+        out[name] = {"rates": rows, "peak_running": st["peak_running"], "hbm_used_GB": st["hbm_used"] / 1e9, "evictions": st["evictions"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +287,12 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="only the batched-systems leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving-loop leg")
+    ap.add_argument("--serve-models", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-lambdas", default="10,25,50,200,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus})", file=sys.stderr)
@@ -244,6 +303,9 @@ def main():
 
     if args.batched_only:
         print(json.dumps(batched_leg(rs, gen, args)))
+        return
+    if args.serving_only:
+        print(json.dumps(serving_leg(rs, gen, args)))
         return
     t = time.perf_counter()
     s = gen.large(scale=args.scale)
@@ -324,6 +386,14 @@ def main():
             log(f"batched leg failed: {e}")
             batched = {"error": str(e)}
 
+    serving = None
+    if not args.no_serving and args.scale == 1.0:
+        try:
+            serving = serving_leg(rs, gen, args)
+        except Exception as e:  # noqa: BLE001
+            log(f"serving leg failed: {e}")
+            serving = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -355,6 +425,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
     dev.free()

@@ -61,6 +61,7 @@ extern "C" {
 typedef struct rsh_context rsh_context;
 typedef struct rsh_system rsh_system;
 typedef struct rsh_batch rsh_batch;
+typedef struct rsh_server rsh_server;
 
 typedef struct rsh_options {
     int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
@@ -139,6 +140,39 @@ int rsh_batch_solve_device(rsh_batch* batch, const double* b_dev, double* x_dev,
                            double* rel_res, int* status);
 int rsh_batch_latencies(rsh_batch* batch, double* ms);   /* per system: ms from the solve's start to its convergence */
 int rsh_batch_info(rsh_batch* batch, char* json, int cap);
+
+/* ---- serving loop for ensemble callers (no counterpart in the reference; its ensemble drivers runBatch / runESMDA,
+ * study.cpp:326-364, 1254, call the global hook from many Simulator threads, one Newton iteration per call).
+ * Models (one pattern each) stay resident with their slice layouts, ILU schedule and AMG hierarchy; requests queue
+ * in arrival order and are admitted as soon as a worker slot (one HIP stream each) is free, the model is idle and
+ * the model fits the HBM budget (idle models are evicted least-recently-used).  Pointers in a request must stay
+ * valid until rsh_server_wait returns for it. */
+typedef struct rsh_request {
+    int model;
+    const double* values;          /* nnzb*nb*nb new values, or NULL for the model's base values */
+    int values_on_device;
+    double perturb;                /* > 0: values *= 1 + perturb*u, u uniform in [-1,1) hashed from perturb_seed */
+    unsigned long long perturb_seed;
+    const double* rhs;             /* n*nb */
+    int rhs_on_device;
+    double* x;                     /* n*nb, or NULL */
+    int x_on_device;
+    double tol;
+    int max_it;
+} rsh_request;
+typedef struct rsh_result {
+    int status;                    /* RSH_OK or the solve's error code */
+    int iterations;
+    double rel_res;
+    double queue_ms, service_ms, latency_ms;   /* submit -> start, start -> end, submit -> end */
+} rsh_result;
+int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server** out);
+int rsh_server_destroy(rsh_server* srv);   /* finishes the queued requests first */
+int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, const int* col, const double* values, const rsh_options* opt,
+                         int* model);
+int rsh_server_submit(rsh_server* srv, const rsh_request* req, long long* ticket);
+int rsh_server_wait(rsh_server* srv, long long ticket, rsh_result* res);
+int rsh_server_stats(rsh_server* srv, char* json, int cap);
 
 #ifdef __cplusplus
 }

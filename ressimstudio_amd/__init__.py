@@ -40,7 +40,8 @@ EXPORTED = [
     "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
     "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
-    "rsh_batch_info",
+    "rsh_batch_info", "rsh_server_create", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
+    "rsh_server_stats",
 ]
 
 
@@ -54,6 +55,17 @@ class Options(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int), ("krylov", ctypes.c_int), ("precision", ctypes.c_int), ("smoother", ctypes.c_int),
                 ("restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("inner_tol", ctypes.c_double), ("reuse_amg", ctypes.c_int),
                 ("reserved", ctypes.c_int * 7)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int), ("values", ctypes.c_void_p), ("values_on_device", ctypes.c_int), ("perturb", ctypes.c_double),
+                ("perturb_seed", ctypes.c_ulonglong), ("rhs", ctypes.c_void_p), ("rhs_on_device", ctypes.c_int), ("x", ctypes.c_void_p),
+                ("x_on_device", ctypes.c_int), ("tol", ctypes.c_double), ("max_it", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("rel_res", ctypes.c_double), ("queue_ms", ctypes.c_double),
+                ("service_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
 
 
 _LIB = None
@@ -105,6 +117,12 @@ def lib():
         L.rsh_batch_solve_device.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_int, _ip, _dp, _ip]
         L.rsh_batch_latencies.argtypes = [_vp, _dp]
         L.rsh_batch_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+        L.rsh_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]
+        L.rsh_server_destroy.argtypes = [_vp]
+        L.rsh_server_add_model.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(Options), _ip]
+        L.rsh_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
+        L.rsh_server_wait.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(Result)]
+        L.rsh_server_stats.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -334,6 +352,72 @@ class Batch:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(lib().rsh_batch_info(self.h, buf, len(buf)), "rsh_batch_info")
         return json.loads(buf.value.decode())
+
+
+def perturbed_values(base, amp: float, seed: int) -> np.ndarray:
+    """The values a served request with ``perturb = amp`` and ``perturb_seed = seed`` solves with:
+    base * (1 + amp * u), u uniform in [-1, 1) from a SplitMix64 hash of (seed, index) (the server's kPerturb)."""
+    b = np.ascontiguousarray(base, np.float64).reshape(-1)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.arange(b.size, dtype=np.uint64) + np.uint64(1)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+    return b * (1.0 + amp * u)
+
+
+class Server:
+    """Serving loop (rsh_server_*): resident models, FIFO admission by worker slot, model availability and HBM
+    budget, one HIP stream per slot."""
+
+    def __init__(self, slots: int = 4, hbm_budget: float = 200e9, device: int = 0):
+        h = _vp()
+        _check(lib().rsh_server_create(device, slots, float(hbm_budget), ctypes.byref(h)), "rsh_server_create")
+        self.h = h
+        self._keep = {}
+        self._models = []
+
+    def add_model(self, A, options: Options | None = None, **kw) -> int:
+        A = A if isinstance(A, BSR) else BSR.of(A)
+        o = options or default_options(**kw)
+        mid = ctypes.c_int(0)
+        _check(lib().rsh_server_add_model(self.h, A.n, A.nb, _p(A.row_ptr, ctypes.c_int), _p(A.col, ctypes.c_int), _p(A.val, ctypes.c_double),
+                                          ctypes.byref(o), ctypes.byref(mid)), "rsh_server_add_model")
+        self._models.append(A)
+        return int(mid.value)
+
+    def submit(self, model: int, rhs, *, values=None, perturb: float = 0.0, seed: int = 0, tol: float = 1e-8, max_it: int = 1000,
+               want_x: bool = True) -> int:
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.zeros(rhs.size) if want_x else None
+        vals = None if values is None else np.ascontiguousarray(values, np.float64).reshape(-1)
+        r = Request(model=model, values=None if vals is None else vals.ctypes.data, values_on_device=0, perturb=perturb, perturb_seed=seed,
+                    rhs=rhs.ctypes.data, rhs_on_device=0, x=None if x is None else x.ctypes.data, x_on_device=0, tol=tol, max_it=max_it)
+        t = ctypes.c_longlong(0)
+        _check(lib().rsh_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)), "rsh_server_submit")
+        self._keep[int(t.value)] = (rhs, x, vals)
+        return int(t.value)
+
+    def wait(self, ticket: int) -> dict:
+        res = Result()
+        _check(lib().rsh_server_wait(self.h, ticket, ctypes.byref(res)), "rsh_server_wait")
+        rhs, x, vals = self._keep.pop(ticket)
+        return {"status": res.status, "iterations": res.iterations, "rel_res": res.rel_res, "queue_ms": res.queue_ms,
+                "service_ms": res.service_ms, "latency_ms": res.latency_ms, "x": x}
+
+    def stats(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().rsh_server_stats(self.h, buf, len(buf)), "rsh_server_stats")
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rsh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class HipBlockSolver:

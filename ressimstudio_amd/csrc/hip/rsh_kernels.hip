@@ -766,6 +766,20 @@ __global__ __launch_bounds__(256) void kDenseInvert(const int* __restrict__ star
     }
 }
 
+// synthetic Newton iterations of a served model: out = base * (1 + amp * u), u uniform in [-1, 1) from a
+// SplitMix64 hash of (seed, i)
+__global__ void kPerturb(long long n, const double* __restrict__ base, double* __restrict__ out, double amp, unsigned long long seed) {
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)i + 1ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = double(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    out[i] = base[i] * (1.0 + amp * u);
+}
+
 __global__ void kToFloat(long long n, const double* __restrict__ a, float* __restrict__ b) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = float(a[i]);
@@ -1685,6 +1699,11 @@ void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int
         checkLaunch();
     }
     RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, start, off, raw, inv)))
+    checkLaunch();
+}
+void launchPerturb(hipStream_t s, long long n, const double* base, double* out, double amp, unsigned long long seed) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kPerturb, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, base, out, amp, seed);
     checkLaunch();
 }
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {

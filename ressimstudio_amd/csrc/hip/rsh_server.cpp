@@ -1,0 +1,331 @@
+// Serving loop for ensemble solves (include/ressim_hip.h, rsh_server_*).
+//
+// The reference's ensemble drivers (runBatch / runESMDA, study.cpp:326-364, 1254) run many Simulators in threads,
+// each calling the global solver hook once per Newton iteration.  The server is that hook's device side for many
+// callers at once: models (one sparsity pattern each) stay resident with their slice layouts, ILU schedule and AMG
+// hierarchy, so a request is only new values + numeric setup + solve; requests queue in arrival order and are
+// admitted as soon as a worker slot is free, the request's model is idle and the model fits the HBM budget (idle
+// models are evicted least-recently-used to make room); each worker owns a HIP stream, so admitted requests run
+// concurrently on the device.  Every kernel is deterministic, so a request's result does not depend on what runs
+// beside it.
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <thread>
+#include <unordered_map>
+
+#include "rsh_solver.hpp"
+
+using namespace rsh;
+
+
+struct rsh_server {
+    struct Model {
+        int n = 0, nb = 1, nnzb = 0;
+        std::vector<int> rp, ci;
+        std::vector<double> baseHost;        // kept on the host so an evicted model can come back
+        rsh_options opt{};
+        rsh_system* S = nullptr;             // resident system (null when evicted)
+        DBuf<double> base, x, b;             // resident base values, right-hand side and solution
+        size_t footprint = 0;                // HBM bytes while resident (estimated until first residency)
+        bool busy = false;
+        double lastUse = 0;
+        int loads = 0;
+    };
+    struct Req {
+        long long id = 0;
+        rsh_request r{};
+        double tSubmit = 0, tStart = 0, tEnd = 0;
+        rsh_result res{};
+        bool done = false;
+    };
+    int device = 0;
+    size_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<Model>> models;
+    std::vector<rsh_context*> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::deque<long long> queue;
+    std::unordered_map<long long, std::unique_ptr<Req>> reqs;
+    long long nextId = 1;
+    bool stop = false;
+    int running = 0, peakRunning = 0;
+    long long completed = 0, evictions = 0;
+
+    static size_t estimate(const Model& m) {
+        const size_t vals = size_t(m.nnzb) * m.nb * m.nb * 8;
+        const size_t vec = size_t(m.n) * m.nb * 8;
+        // natural values, SpMV slices, ILU factors + sweep slices, pressure column, base values; ~20 vectors
+        return vals * 6 + vec * 20 + size_t(m.nnzb) * 16;
+    }
+    size_t measure(const Model& m) const {
+        const rsh_system& S = *m.S;
+        size_t b = S.val.bytes() + S.spmv64.bytes() + S.spmv32.bytes() + S.colP.bytes() + S.rp.bytes() + S.ci.bytes() + S.diag.bytes();
+        if (S.ilu) b += S.ilu->bytes();
+        if (S.cpr) b += S.cpr->bytes();
+        for (const DBuf<double>* v : {&S.vb, &S.vx, &S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh, &S.w, &S.irR, &S.irD, &S.V, &S.Z})
+            b += v->bytes();
+        return b + m.base.bytes() + m.x.bytes() + m.b.bytes();
+    }
+    // make room for `need` bytes by evicting idle resident models, least recently used first (lock held)
+    bool makeRoom(size_t need, int except) {
+        while (used + need > budget) {
+            int victim = -1;
+            for (size_t i = 0; i < models.size(); ++i) {
+                const Model& m = *models[i];
+                if (int(i) == except || !m.S || m.busy) continue;
+                if (victim < 0 || m.lastUse < models[size_t(victim)]->lastUse) victim = int(i);
+            }
+            if (victim < 0) return false;
+            Model& v = *models[size_t(victim)];
+            rsh_system_destroy(v.S);
+            v.S = nullptr;
+            v.base.release();
+            v.x.release();
+            v.b.release();
+            used -= std::min(used, v.footprint);
+            ++evictions;
+        }
+        return true;
+    }
+    // next admissible request in arrival order (lock held): its model idle and resident or fitting the budget
+    long long pick() {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            Req& q = *reqs[*it];
+            Model& m = *models[size_t(q.r.model)];
+            if (m.busy) continue;
+            if (!m.S && !makeRoom(m.footprint, q.r.model)) continue;
+            const long long id = *it;
+            queue.erase(it);
+            return id;
+        }
+        return 0;
+    }
+    void serve(Req& q, rsh_context* ctx) {
+        Model& m = *models[size_t(q.r.model)];
+        hipStream_t st = ctx->c.stream;
+        RSH_HIP(hipSetDevice(device));
+        if (!m.S) {
+            // (re)load: pattern, base values, symbolic set-up on the first setup below
+            rsh_system* S = nullptr;
+            const int rc = rsh_system_create(ctx, m.n, m.nb, m.rp.data(), m.ci.data(), &S);
+            if (rc != RSH_OK) throw Error(rc, g_err);
+            m.S = S;
+            m.base.upload(m.baseHost, st);
+            m.x.alloc(size_t(m.n) * m.nb);
+            m.b.alloc(size_t(m.n) * m.nb);
+            ++m.loads;
+        }
+        rsh_system& S = *m.S;
+        S.ctx = ctx;                          // this request runs on the worker's stream
+        const long long nv = (long long)m.nnzb * m.nb * m.nb;
+        const double* src = q.r.values ? q.r.values : m.base.p;
+        if (q.r.values && !q.r.values_on_device) {
+            RSH_HIP(hipMemcpyAsync(S.val.p, q.r.values, size_t(nv) * sizeof(double), hipMemcpyHostToDevice, st));
+            src = S.val.p;
+        }
+        if (q.r.perturb > 0) {
+            launchPerturb(st, nv, src, S.val.p, q.r.perturb, q.r.perturb_seed);
+        } else if (src != S.val.p) {
+            RSH_HIP(hipMemcpyAsync(S.val.p, src, size_t(nv) * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        S.valuesChanged();
+        int rc = rsh_system_setup(&S, &m.opt);
+        if (rc != RSH_OK) throw Error(rc, g_err);
+        const size_t N = size_t(m.n) * m.nb;
+        const double* bd = m.b.p;
+        if (q.r.rhs_on_device) bd = q.r.rhs;
+        else RSH_HIP(hipMemcpyAsync(m.b.p, q.r.rhs, N * sizeof(double), hipMemcpyHostToDevice, st));
+        double* xd = q.r.x_on_device ? q.r.x : m.x.p;
+        int its = 0;
+        double rel = 0;
+        rc = rsh_solve_device(&S, bd, xd, q.r.tol, q.r.max_it, &its, &rel);
+        if (!q.r.x_on_device && q.r.x) RSH_HIP(hipMemcpyAsync(q.r.x, m.x.p, N * sizeof(double), hipMemcpyDeviceToHost, st));
+        RSH_HIP(hipStreamSynchronize(st));
+        q.res.status = rc;
+        q.res.iterations = its;
+        q.res.rel_res = rel;
+    }
+    void workerLoop(int w) {
+        rsh_context* ctx = ctxs[size_t(w)];
+        for (;;) {
+            long long id = 0;
+            Req* qp = nullptr;
+            Model* mp = nullptr;
+            bool wasResident = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cvWork.wait(lk, [&] { return stop || (id = pick()) != 0; });
+                if (!id) return;                  // stop with nothing admissible
+                qp = reqs[id].get();
+                mp = models[size_t(qp->r.model)].get();
+                mp->busy = true;
+                wasResident = mp->S != nullptr;
+                if (!wasResident) used += mp->footprint;    // reserved now, corrected after the load
+                qp->tStart = nowMs();
+                peakRunning = std::max(peakRunning, ++running);
+            }
+            Req& q = *qp;
+            Model& m = *mp;
+            try {
+                serve(q, ctx);
+            } catch (const Error& e) {
+                q.res.status = e.code;
+            } catch (const std::exception&) {
+                q.res.status = RSH_ERR_DEVICE;
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (m.S) {
+                    const size_t real = measure(m);
+                    if (!wasResident || real != m.footprint) {
+                        used -= std::min(used, m.footprint);
+                        m.footprint = real;
+                        used += real;
+                    }
+                } else if (!wasResident) {
+                    used -= std::min(used, m.footprint);   // the load failed: release the reservation
+                }
+                q.tEnd = nowMs();
+                q.res.queue_ms = q.tStart - q.tSubmit;
+                q.res.service_ms = q.tEnd - q.tStart;
+                q.res.latency_ms = q.tEnd - q.tSubmit;
+                q.done = true;
+                m.busy = false;
+                m.lastUse = q.tEnd;
+                --running;
+                ++completed;
+            }
+            cvDone.notify_all();
+            cvWork.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server** out) {
+    return guarded([&] {
+        if (!out) return fail(RSH_ERR_ARG, "null output pointer");
+        *out = nullptr;
+        if (slots < 1 || slots > 16) return fail(RSH_ERR_ARG, "slots must be 1..16");
+        if (!(hbm_budget_bytes > 0)) return fail(RSH_ERR_ARG, "the HBM budget must be positive");
+        auto S = std::make_unique<rsh_server>();
+        S->device = device;
+        S->budget = size_t(hbm_budget_bytes);
+        for (int w = 0; w < slots; ++w) {
+            rsh_context* c = nullptr;
+            const int rc = rsh_context_create(device, &c);
+            if (rc != RSH_OK) {
+                for (auto* x : S->ctxs) rsh_context_destroy(x);
+                return rc;
+            }
+            S->ctxs.push_back(c);
+        }
+        rsh_server* raw = S.get();
+        for (int w = 0; w < slots; ++w) S->workers.emplace_back([raw, w] { raw->workerLoop(w); });
+        *out = S.release();
+        return RSH_OK;
+    });
+}
+
+int rsh_server_destroy(rsh_server* srv) {
+    if (!srv) return RSH_OK;
+    return guarded([&] {
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cvDone.wait(lk, [&] { return srv->queue.empty() && srv->running == 0; });
+            srv->stop = true;
+        }
+        srv->cvWork.notify_all();
+        for (auto& t : srv->workers) t.join();
+        for (auto& m : srv->models)
+            if (m->S) rsh_system_destroy(m->S);
+        srv->models.clear();
+        for (auto* c : srv->ctxs) rsh_context_destroy(c);
+        delete srv;
+        return RSH_OK;
+    });
+}
+
+int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, const int* col, const double* values, const rsh_options* opt,
+                         int* model) {
+    return guarded([&] {
+        if (!srv || !row_ptr || !col || !values || !opt || !model) return fail(RSH_ERR_ARG, "null argument");
+        if (n <= 0 || nb < 1 || nb > kMaxNb) return fail(RSH_ERR_ARG, "bad system size");
+        auto m = std::make_unique<rsh_server::Model>();
+        m->n = n;
+        m->nb = nb;
+        m->nnzb = row_ptr[n];
+        m->rp.assign(row_ptr, row_ptr + n + 1);
+        m->ci.assign(col, col + m->nnzb);
+        m->baseHost.assign(values, values + size_t(m->nnzb) * nb * nb);
+        m->opt = *opt;
+        m->footprint = rsh_server::estimate(*m);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (m->footprint > srv->budget) return fail(RSH_ERR_ARG, "the model alone exceeds the HBM budget");
+        *model = int(srv->models.size());
+        srv->models.push_back(std::move(m));
+        return RSH_OK;
+    });
+}
+
+int rsh_server_submit(rsh_server* srv, const rsh_request* req, long long* ticket) {
+    return guarded([&] {
+        if (!srv || !req || !ticket) return fail(RSH_ERR_ARG, "null argument");
+        if (!req->rhs) return fail(RSH_ERR_ARG, "null right-hand side");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (req->model < 0 || req->model >= int(srv->models.size())) return fail(RSH_ERR_ARG, "unknown model");
+        auto q = std::make_unique<rsh_server::Req>();
+        q->id = srv->nextId++;
+        q->r = *req;
+        q->tSubmit = nowMs();
+        *ticket = q->id;
+        srv->queue.push_back(q->id);
+        srv->reqs.emplace(q->id, std::move(q));
+        srv->cvWork.notify_one();
+        return RSH_OK;
+    });
+}
+
+// blocks until the request is done; the request's record is released
+int rsh_server_wait(rsh_server* srv, long long ticket, rsh_result* res) {
+    return guarded([&] {
+        if (!srv || !res) return fail(RSH_ERR_ARG, "null argument");
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->reqs.find(ticket);
+        if (it == srv->reqs.end()) return fail(RSH_ERR_ARG, "unknown ticket");
+        rsh_server::Req* q = it->second.get();
+        srv->cvDone.wait(lk, [&] { return q->done; });
+        *res = q->res;
+        srv->reqs.erase(it);
+        return RSH_OK;
+    });
+}
+
+int rsh_server_stats(rsh_server* srv, char* json, int cap) {
+    return guarded([&] {
+        if (!srv || !json || cap <= 0) return fail(RSH_ERR_ARG, "null argument");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        int resident = 0, loads = 0;
+        for (auto& m : srv->models) {
+            resident += m->S != nullptr;
+            loads += m->loads;
+        }
+        std::ostringstream o;
+        o << "{\"models\":" << srv->models.size() << ",\"resident\":" << resident << ",\"loads\":" << loads << ",\"evictions\":" << srv->evictions
+          << ",\"hbm_used\":" << srv->used << ",\"hbm_budget\":" << srv->budget << ",\"slots\":" << srv->ctxs.size() << ",\"queued\":"
+          << srv->queue.size() << ",\"running\":" << srv->running << ",\"peak_running\":" << srv->peakRunning << ",\"completed\":" << srv->completed
+          << ",\"footprints\":[";
+        for (size_t i = 0; i < srv->models.size(); ++i) o << (i ? "," : "") << srv->models[i]->footprint;
+        o << "]}";
+        const std::string s = o.str();
+        if (int(s.size()) + 1 > cap) return fail(RSH_ERR_ARG, "buffer too small");
+        std::memcpy(json, s.c_str(), s.size() + 1);
+        return RSH_OK;
+    });
+}
+
+}  // extern "C"

@@ -158,6 +158,22 @@ def batch_workload(count: int = 256, seed: int = 20240604) -> list:
     return out
 
 
+def serving_models(count: int = 64, seed: int = 20240605) -> list:
+    """Resident ensemble models for the serving loop: cells log-uniform in [5,000, 500,000] (nz uniform in 4..20,
+    nx = ny from the cell count), nb 3, lognormal k with sigma ln k uniform in [0.5, 1.5], two to four wells; model i
+    uses generator seed ``seed + i``; the draws come from numpy's PCG64 seeded with ``seed``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        cells = float(np.exp(rng.uniform(np.log(5e3), np.log(5e5))))
+        nz = int(rng.integers(4, 21))
+        nx = max(4, int(round(np.sqrt(cells / nz))))
+        sigma = float(rng.uniform(0.5, 1.5))
+        wells = int(rng.integers(2, 5))
+        out.append(reservoir(nx, nx, nz, 3, model=0, sigma=sigma, wells=wells, seed=seed + i))
+    return out
+
+
 def block_laplace(nx: int, ny: int, nb: int, seed: int) -> BlockSystem:
     """The reference tests' 2-D block Laplacian (run_tests.cpp:3694-3724), rhs = 1."""
     return _wrap(_lib().rsg_block_laplace(nx, ny, nb, seed))

@@ -1,0 +1,101 @@
+"""The serving loop (rsh_server_*) through the C-ABI: requests from several resident models (each a different
+Newton iteration: values perturbed +-5 % on the device), solved alone one after another and then all submitted at
+once so that they overlap on the device; every result must be the same alone and in the mix (the kernels are
+deterministic: bit for bit), meet the 1e-8 true residual on its perturbed matrix and match the oracle's solve."""
+import numpy as np
+import pytest
+
+import ressimstudio_amd as rs
+from ressimstudio_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def models():
+    return [gen.reservoir(20, 16, 6, 3, model=0, sigma=1.0, wells=2, seed=201), gen.reservoir(30, 24, 8, 3, model=0, sigma=1.2, wells=3, seed=202),
+            gen.reservoir(12, 12, 5, 3, model=0, sigma=0.7, wells=1, seed=203), gen.reservoir(26, 18, 10, 3, model=0, sigma=1.5, wells=2, seed=204)]
+
+
+@pytest.fixture(scope="module")
+def mods():
+    return models()
+
+
+def _requests(mods):
+    # (model, perturbation seed): three Newton iterations of every model, interleaved as ensemble members would arrive
+    return [(k % len(mods), 1000 + k) for k in range(3 * len(mods))]
+
+
+def _run(srv, ids, mods, reqs, together, **kw):
+    out = {}
+    if together:
+        tickets = [(i, srv.submit(ids[m], mods[m].rhs, perturb=0.05, seed=sd, tol=TOL, max_it=2000)) for i, (m, sd) in enumerate(reqs)]
+        for i, t in tickets:
+            out[i] = srv.wait(t)
+    else:
+        for i, (m, sd) in enumerate(reqs):
+            out[i] = srv.wait(srv.submit(ids[m], mods[m].rhs, perturb=0.05, seed=sd, tol=TOL, max_it=2000))
+    return out
+
+
+@pytest.mark.parametrize("precision", [rs.FP64, rs.MIXED])
+def test_same_result_alone_and_in_the_mix(mods, orc, precision):
+    srv = rs.Server(slots=4)
+    ids = [srv.add_model(s, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, precision=precision) for s in mods]
+    reqs = _requests(mods)
+    # first Newton iteration of every model: builds its AMG hierarchy, later iterations refresh it (AMG::refresh)
+    _run(srv, ids, mods, [(m, 1) for m in range(len(mods))], together=False)
+    alone = _run(srv, ids, mods, reqs, together=False)
+    mix = _run(srv, ids, mods, reqs, together=True)
+    st = srv.stats()
+    assert st["peak_running"] >= 2, st            # requests did overlap on the device
+    for i, (m, sd) in enumerate(reqs):
+        a, b = alone[i], mix[i]
+        assert a["status"] == rs.OK and b["status"] == rs.OK
+        assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"], i
+        s = mods[m]
+        v = rs.perturbed_values(s.val, 0.05, sd)
+        sp = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=v.reshape(s.val.shape), rhs=s.rhs)
+        r = s.rhs - orc.spmv(sp, a["x"])
+        assert np.linalg.norm(r) <= 1.01 * TOL * np.linalg.norm(s.rhs), i
+        # the +-5 % per-entry perturbation makes these matrices less well conditioned than the parity systems: at a
+        # 1e-8 residual the solutions agree with the oracle's to 1e-5
+        xo, _, _, ok = orc.krylov(sp, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+        assert ok and np.linalg.norm(a["x"] - xo) <= 1e-5 * np.linalg.norm(xo), i
+    srv.close()
+
+
+def test_hbm_budget_evicts_idle_models(mods):
+    # MC-BILU0 (no AMG hierarchy whose aggregates would come from the loading request): a reloaded model solves
+    # exactly as a resident one
+    ref = rs.Server(slots=2)
+    ids = [ref.add_model(s, precond="mcbilu0") for s in mods[:3]]
+    reqs = [(k % 3, 500 + k) for k in range(6)]
+    base = _run(ref, ids, mods, reqs, together=False)
+    foot = ref.stats()["footprints"]
+    ref.close()
+    # a budget that holds the two largest models but not all three: serving them in turn must evict and reload
+    big = sorted(foot)
+    srv = rs.Server(slots=2, hbm_budget=big[2] + 1.05 * big[1])
+    ids = [srv.add_model(s, precond="mcbilu0") for s in mods[:3]]
+    got = _run(srv, ids, mods, reqs, together=True)
+    st = srv.stats()
+    assert st["evictions"] >= 1 and st["hbm_used"] <= st["hbm_budget"], st
+    for i in range(len(reqs)):
+        assert got[i]["status"] == rs.OK and np.array_equal(got[i]["x"], base[i]["x"]), i
+    srv.close()
+
+
+def test_perturbation_matches_host_formula(mods):
+    # the host restatement of the device perturbation used by the checks above
+    s = mods[2]
+    srv = rs.Server(slots=1)
+    mid = srv.add_model(s, precond="bilu0")
+    r = srv.wait(srv.submit(mid, s.rhs, perturb=0.05, seed=77, tol=TOL, max_it=2000))
+    v = rs.perturbed_values(s.val, 0.05, 77)
+    d = rs.System(rs.BSR(s.n, s.nb, s.row_ptr, s.col, v)).setup(precond="bilu0")
+    x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+    assert ok and np.array_equal(x, r["x"]) and its == r["iterations"]
+    srv.close()
