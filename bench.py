@@ -220,8 +220,8 @@ def batched_leg(rs, gen, args):
 
 def serving_leg(rs, gen, args):
     """The serving loop under Poisson arrivals: 64 resident models (5k-500k cells, log-uniform), each request one
-    Newton iteration of a random model (values perturbed +-5 % on the device from the model's base values, new
-    numeric setup with the resident AMG aggregates refreshed, CPR-BiCGSTAB to 1e-8); latency = submit -> done."""
+    Newton iteration of a random model (every block row scaled by +-5 % on the device from the model's base values,
+    new numeric setup with the resident AMG aggregates refreshed, CPR-BiCGSTAB to 1e-8); latency = submit -> done."""
     t = time.perf_counter()
     models = gen.serving_models(args.serve_models)
     cells = int(sum(m.n for m in models))
@@ -229,7 +229,7 @@ def serving_leg(rs, gen, args):
     rng = np.random.default_rng(20240606)
     lambdas = [float(x) for x in args.serve_lambdas.split(",") if x]
     out = {"workload": f"{len(models)} resident models, 5k-500k cells log-uniform ({cells} cells), nb 3, CPR(Jacobi AMG)+MC-BILU0 "
-                       f"BiCGSTAB to 1e-8, values perturbed +-5 % per request", "models": len(models), "cells": cells,
+                       f"BiCGSTAB to 1e-8, block rows scaled by +-5 % per request", "models": len(models), "cells": cells,
            "slots": args.serve_slots, "requests_per_rate": args.serve_requests, "hbm_budget_GB": 200}
     for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED)):
         srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9)

@@ -151,7 +151,8 @@ typedef struct rsh_request {
     int model;
     const double* values;          /* nnzb*nb*nb new values, or NULL for the model's base values */
     int values_on_device;
-    double perturb;                /* > 0: values *= 1 + perturb*u, u uniform in [-1,1) hashed from perturb_seed */
+    double perturb;                /* > 0: block row i *= 1 + perturb*u_i, u_i uniform in [-1,1) hashed from
+                                      (perturb_seed, i): synthetic Newton iterations of an ensemble member */
     unsigned long long perturb_seed;
     const double* rhs;             /* n*nb */
     int rhs_on_device;

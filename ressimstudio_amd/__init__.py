@@ -354,17 +354,21 @@ class Batch:
         return json.loads(buf.value.decode())
 
 
-def perturbed_values(base, amp: float, seed: int) -> np.ndarray:
-    """The values a served request with ``perturb = amp`` and ``perturb_seed = seed`` solves with:
-    base * (1 + amp * u), u uniform in [-1, 1) from a SplitMix64 hash of (seed, index) (the server's kPerturb)."""
-    b = np.ascontiguousarray(base, np.float64).reshape(-1)
+def perturbed_values(base, row_ptr, amp: float, seed: int) -> np.ndarray:
+    """The values a served request with ``perturb = amp`` and ``perturb_seed = seed`` solves with: block row i of
+    the BSR values scaled by 1 + amp * u_i, u_i uniform in [-1, 1) from a SplitMix64 hash of (seed, i) (the
+    server's kPerturbRows)."""
+    rp = np.asarray(row_ptr, np.int64)
+    n = rp.size - 1
+    b = np.ascontiguousarray(base, np.float64).reshape(int(rp[-1]), -1)
     with np.errstate(over="ignore"):
-        z = np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.arange(b.size, dtype=np.uint64) + np.uint64(1)
+        z = np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) + np.arange(n, dtype=np.uint64) + np.uint64(1)
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         z = z ^ (z >> np.uint64(31))
     u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
-    return b * (1.0 + amp * u)
+    f = 1.0 + amp * u
+    return (b * np.repeat(f, np.diff(rp))[:, None]).reshape(-1)
 
 
 class Server:

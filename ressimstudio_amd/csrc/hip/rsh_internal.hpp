@@ -201,8 +201,8 @@ void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fw
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
-// out = base * (1 + amp * u), u uniform in [-1, 1) hashed from (seed, index)
-void launchPerturb(hipStream_t s, long long n, const double* base, double* out, double amp, unsigned long long seed);
+// BSR values with block row i scaled by (1 + amp * u_i), u_i uniform in [-1, 1) hashed from (seed, i)
+void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed);
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
 // BILU0 elimination of long rows rows[r0..r1): one wavefront per row

@@ -766,18 +766,21 @@ __global__ __launch_bounds__(256) void kDenseInvert(const int* __restrict__ star
     }
 }
 
-// synthetic Newton iterations of a served model: out = base * (1 + amp * u), u uniform in [-1, 1) from a
-// SplitMix64 hash of (seed, i)
-__global__ void kPerturb(long long n, const double* __restrict__ base, double* __restrict__ out, double amp, unsigned long long seed) {
+// synthetic Newton iterations of a served model: every block row i scaled by (1 + amp * u_i), u_i uniform in
+// [-1, 1) from a SplitMix64 hash of (seed, i) (new accumulation / mobility values per cell; the pattern and the
+// character of the Jacobian stay those of the model)
+__global__ void kPerturbRows(int n, const int* __restrict__ rp, int nb2, const double* __restrict__ base, double* __restrict__ out, double amp,
+                             unsigned long long seed) {
 #pragma clang fp contract(off)
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long z = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)i + 1ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
     const double u = double(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
-    out[i] = base[i] * (1.0 + amp * u);
+    const double f = 1.0 + amp * u;
+    for (long long k = (long long)rp[i] * nb2; k < (long long)rp[i + 1] * nb2; ++k) out[k] = base[k] * f;
 }
 
 __global__ void kToFloat(long long n, const double* __restrict__ a, float* __restrict__ b) {
@@ -1701,9 +1704,9 @@ void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int
     RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, start, off, raw, inv)))
     checkLaunch();
 }
-void launchPerturb(hipStream_t s, long long n, const double* base, double* out, double amp, unsigned long long seed) {
+void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(kPerturb, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, base, out, amp, seed);
+    hipLaunchKernelGGL(kPerturbRows, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, rp, nb * nb, base, out, amp, seed);
     checkLaunch();
 }
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {

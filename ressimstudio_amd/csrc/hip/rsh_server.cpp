@@ -103,8 +103,7 @@ struct rsh_server {
         }
         return 0;
     }
-    void serve(Req& q, rsh_context* ctx) {
-        Model& m = *models[size_t(q.r.model)];
+    void serve(Req& q, Model& m, rsh_context* ctx) {
         hipStream_t st = ctx->c.stream;
         RSH_HIP(hipSetDevice(device));
         if (!m.S) {
@@ -127,7 +126,7 @@ struct rsh_server {
             src = S.val.p;
         }
         if (q.r.perturb > 0) {
-            launchPerturb(st, nv, src, S.val.p, q.r.perturb, q.r.perturb_seed);
+            launchPerturbRows(st, m.n, S.rp.p, m.nb, src, S.val.p, q.r.perturb, q.r.perturb_seed);
         } else if (src != S.val.p) {
             RSH_HIP(hipMemcpyAsync(S.val.p, src, size_t(nv) * sizeof(double), hipMemcpyDeviceToDevice, st));
         }
@@ -170,7 +169,7 @@ struct rsh_server {
             Req& q = *qp;
             Model& m = *mp;
             try {
-                serve(q, ctx);
+                serve(q, m, ctx);
             } catch (const Error& e) {
                 q.res.status = e.code;
             } catch (const std::exception&) {

@@ -1,5 +1,5 @@
 """The serving loop (rsh_server_*) through the C-ABI: requests from several resident models (each a different
-Newton iteration: values perturbed +-5 % on the device), solved alone one after another and then all submitted at
+Newton iteration: block rows scaled by +-5 % on the device), solved alone one after another and then all submitted at
 once so that they overlap on the device; every result must be the same alone and in the mix (the kernels are
 deterministic: bit for bit), meet the 1e-8 true residual on its perturbed matrix and match the oracle's solve."""
 import numpy as np
@@ -14,7 +14,7 @@ TOL = 1e-8
 
 
 def models():
-    return [gen.reservoir(20, 16, 6, 3, model=0, sigma=1.0, wells=2, seed=201), gen.reservoir(30, 24, 8, 3, model=0, sigma=1.2, wells=3, seed=202),
+    return [gen.reservoir(20, 16, 6, 3, model=0, sigma=1.0, wells=2, seed=201), gen.reservoir(30, 24, 8, 3, model=1, sigma=2.0, kvkh=0.1, wells=3, seed=202),
             gen.reservoir(12, 12, 5, 3, model=0, sigma=0.7, wells=1, seed=203), gen.reservoir(26, 18, 10, 3, model=0, sigma=1.5, wells=2, seed=204)]
 
 
@@ -56,14 +56,12 @@ def test_same_result_alone_and_in_the_mix(mods, orc, precision):
         assert a["status"] == rs.OK and b["status"] == rs.OK
         assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"], i
         s = mods[m]
-        v = rs.perturbed_values(s.val, 0.05, sd)
+        v = rs.perturbed_values(s.val, s.row_ptr, 0.05, sd)
         sp = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=v.reshape(s.val.shape), rhs=s.rhs)
         r = s.rhs - orc.spmv(sp, a["x"])
         assert np.linalg.norm(r) <= 1.01 * TOL * np.linalg.norm(s.rhs), i
-        # the +-5 % per-entry perturbation makes these matrices less well conditioned than the parity systems: at a
-        # 1e-8 residual the solutions agree with the oracle's to 1e-5
         xo, _, _, ok = orc.krylov(sp, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
-        assert ok and np.linalg.norm(a["x"] - xo) <= 1e-5 * np.linalg.norm(xo), i
+        assert ok and np.linalg.norm(a["x"] - xo) <= 1e-6 * np.linalg.norm(xo), i
     srv.close()
 
 
@@ -94,7 +92,7 @@ def test_perturbation_matches_host_formula(mods):
     srv = rs.Server(slots=1)
     mid = srv.add_model(s, precond="bilu0")
     r = srv.wait(srv.submit(mid, s.rhs, perturb=0.05, seed=77, tol=TOL, max_it=2000))
-    v = rs.perturbed_values(s.val, 0.05, 77)
+    v = rs.perturbed_values(s.val, s.row_ptr, 0.05, 77)
     d = rs.System(rs.BSR(s.n, s.nb, s.row_ptr, s.col, v)).setup(precond="bilu0")
     x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
     assert ok and np.array_equal(x, r["x"]) and its == r["iterations"]
