@@ -169,7 +169,8 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
         for (int e = 0; e < NB; ++e) acc[e] = 0;
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
-            const int c = col[slot * kWave + lane];
+            // matrix values and columns are streamed once: non-temporal loads keep the caches for the x gathers
+            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
             const TA* a = val + slot * (NB * NB * kWave) + lane;
             double xv[NB];
 #pragma unroll
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
             for (int e = 0; e < NB; ++e) {
                 double s = 0;
 #pragma unroll
-                for (int f = 0; f < NB; ++f) s += double(a[(e * NB + f) * kWave]) * xv[f];
+                for (int f = 0; f < NB; ++f) s += double(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * xv[f];
                 acc[e] += s;
             }
         }
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
         for (int e = 0; e < NB; ++e) acc[e] = TV(r[(long long)o * NB + e]);
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
-            const int c = col[slot * kWave + lane];
+            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
             const TA* a = val + slot * (NB * NB * kWave) + lane;
             TV yk[NB];
 #pragma unroll
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
             for (int e = 0; e < NB; ++e) {
                 TV s = 0;
 #pragma unroll
-                for (int f = 0; f < NB; ++f) s += TV(a[(e * NB + f) * kWave]) * yk[f];
+                for (int f = 0; f < NB; ++f) s += TV(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * yk[f];
                 acc[e] -= s;
             }
         }
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
         for (int e = 0; e < NB; ++e) acc[e] = y[(long long)wr * NB + e];
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
-            const int c = col[slot * kWave + lane];
+            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
             const TA* a = val + slot * (NB * NB * kWave) + lane;
             TV zj[NB];
 #pragma unroll
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
             for (int e = 0; e < NB; ++e) {
                 TV s = 0;
 #pragma unroll
-                for (int f = 0; f < NB; ++f) s += TV(a[(e * NB + f) * kWave]) * zj[f];
+                for (int f = 0; f < NB; ++f) s += TV(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * zj[f];
                 acc[e] -= s;
             }
         }
@@ -977,10 +978,10 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
         for (int e = 0; e < NB; ++e) acc[e] = r[(long long)rr * NB + e];
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
-            const double xk = xp[col[slot * kWave + lane]];
+            const double xk = xp[__builtin_nontemporal_load(col + slot * kWave + lane)];
             const double* a = colP + slot * (NB * kWave) + lane;
 #pragma unroll
-            for (int e = 0; e < NB; ++e) acc[e] -= a[e * kWave] * xk;
+            for (int e = 0; e < NB; ++e) acc[e] -= __builtin_nontemporal_load(a + e * kWave) * xk;
         }
 #pragma unroll
         for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = acc[e];
