@@ -74,7 +74,8 @@ typedef struct rsh_options {
                            each inner solve runs at most max_it / max_outer (>= 50) iterations */
     int reuse_amg;      /* 1: keep the AMG aggregates of the last setup while the pattern is unchanged and only
                            recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
-    int reserved[7];
+    int smooth_steps;   /* RSH_SMOOTH_JACOBI: pre- and post-smoothing steps per level (default 1, DeviceCprAmg's) */
+    int reserved[6];
 } rsh_options;
 
 /* ---- library / context */

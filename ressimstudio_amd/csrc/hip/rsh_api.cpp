@@ -457,6 +457,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC) {
             if (!S.cpr) S.cpr = std::make_unique<CprDev>();
             S.cpr->smoother = o.smoother;
+            S.cpr->steps = o.smooth_steps > 0 ? o.smooth_steps : 1;
             S.cpr->plan = &S.spmv64;
             if (S.colP.n == 0) {
                 S.colP.alloc(size_t(S.spmvHost.nslots) * S.nb * kWave);

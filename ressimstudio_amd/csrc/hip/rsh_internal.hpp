@@ -224,7 +224,8 @@ void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, con
                    const double* b, const double* xin, double* xout, bool forward);
 // meanAgg: mean aggregate size (sets the lanes per aggregate)
 void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg);
-// fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc.
+// fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc;
+// mode 2 (r in xc) out0 = x1 + w D^-1 r, out1 = b - A out0; mode 3 out0 = x1 + w D^-1 (b - A x1).
 // sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
 void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
                        const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,

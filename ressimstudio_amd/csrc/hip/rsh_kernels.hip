@@ -1132,10 +1132,15 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
 // ---------------------------------------------------------------- fused damped-Jacobi V-cycle steps (DeviceCprAmg smoothing)
 // MODE 0 (pre-smoothing from zero + residual): x1 = w D^-1 b, r = b - A x1, with x1_j formed on the fly.
 // MODE 1 (prolongation + post-smoothing): x = x1 + P xc, x2 = x + w D^-1 (b - A x), with x_j formed on the fly.
+// MODE 2 (a further pre-smoothing step + residual, r given in xc): x' = x1 + w D^-1 r, r' = b - A x'.
+// MODE 3 (a further post-smoothing step): x' = x1 + w D^-1 (b - A x1).
 template <int MODE>
 __device__ inline double jacX(int j, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
                               const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
-    return MODE == 0 ? omega * dinv[j] * b[j] : x1[j] + xc[agg[j]];
+    if (MODE == 0) return omega * dinv[j] * b[j];
+    if (MODE == 1) return x1[j] + xc[agg[j]];
+    if (MODE == 2) return x1[j] + omega * dinv[j] * xc[j];
+    return x1[j];
 }
 template <int MODE>
 __device__ inline void jacFinish(int i, double acc, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
@@ -1144,9 +1149,14 @@ __device__ inline void jacFinish(int i, double acc, double omega, const double* 
     if (MODE == 0) {
         out0[i] = omega * dinv[i] * b[i];
         out1[i] = b[i] - acc;
-    } else {
+    } else if (MODE == 1) {
         const double xi = x1[i] + xc[agg[i]];
         out0[i] = xi + omega * dinv[i] * (b[i] - acc);
+    } else if (MODE == 2) {
+        out0[i] = x1[i] + omega * dinv[i] * xc[i];
+        out1[i] = b[i] - acc;
+    } else {
+        out0[i] = x1[i] + omega * dinv[i] * (b[i] - acc);
     }
 }
 // level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
@@ -1835,23 +1845,34 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
     checkLaunch();
 }
 
+template <int MODE>
+void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const double* sval, const double* lval, const double* dinv, const double* b,
+                   const double* x1, const int* agg, const double* xc, double omega, double* out0, double* out1) {
+    const int waves = sl->nslices + sl->nlong;
+    if (!waves) return;
+    hipLaunchKernelGGL(kJacobiSliced<MODE>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
+}
 void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
                        const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
                        const double* xc, double omega, double* out0, double* out1, double meanRow) {
+    if (mode < 0 || mode > 3) throw Error(-1, "unknown Jacobi step");
     if (sl) {
-        const int waves = sl->nslices + sl->nlong;
-        if (!waves) return;
-        if (mode == 0)
-            hipLaunchKernelGGL(kJacobiSliced<0>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
-                               sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
-        else
-            hipLaunchKernelGGL(kJacobiSliced<1>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
-                               sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
+        switch (mode) {
+            case 0: jacobiSlicedT<0>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 1: jacobiSlicedT<1>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 2: jacobiSlicedT<2>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            default: jacobiSlicedT<3>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+        }
     } else {
         if (n <= 0) return;
         const int g = groupLanes(meanRow);
-        if (mode == 0) RSH_G_DISPATCH(g, (jacobiCsrT<0, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1)))
-        else RSH_G_DISPATCH(g, (jacobiCsrT<1, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1)))
+        switch (mode) {
+            case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+            case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+            case 2: RSH_G_DISPATCH(g, (jacobiCsrT<2, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+            default: RSH_G_DISPATCH(g, (jacobiCsrT<3, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+        }
     }
     checkLaunch();
 }

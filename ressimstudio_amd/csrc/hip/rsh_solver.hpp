@@ -435,6 +435,7 @@ struct AmgLevelDev {
     DBuf<int> dgsFs, dgsBs;
     std::vector<LevelRun> runsF, runsB;
     DBuf<double> x1, x2, b, r;
+    DBuf<double> x3, r2;                 // further Jacobi steps (ping-pong)
     DBuf<double> dinv;                   // 1 / a_ii (Jacobi)
     const double* result = nullptr;
 };
@@ -450,6 +451,7 @@ struct CprDev {
     DBuf<double> M0, M1;                  // [C | I] ping-pong; the inverse ends in Minv
     double* Minv = nullptr;
     int smoother = RSH_SMOOTH_GS;
+    int steps = 1;                        // Jacobi pre- and post-smoothing steps
     bool built = false;
     // level 0 values in the SpMV row-slice layout of S (same pattern) for the streamed Jacobi steps
     const SliceDev<double>* plan = nullptr;
@@ -513,6 +515,8 @@ struct CprDev {
             L->runsB = levelRuns(work(L->gsBs), 2048);
             L->x1.alloc(size_t(L->n));
             L->x2.alloc(size_t(L->n));
+            L->x3.alloc(size_t(L->n));
+            L->r2.alloc(size_t(L->n));
             L->dinv.alloc(size_t(L->n));
             L->b.alloc(size_t(L->n));
             L->r.alloc(size_t(L->n));
@@ -610,15 +614,26 @@ struct CprDev {
         }
         AmgLevelDev& C = *lv[l + 1];
         if (smoother == RSH_SMOOTH_JACOBI) {
-            // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing
+            // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing; further
+            // steps (smooth_steps > 1) ping-pong between the level's spare buffers
             const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
-            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p,
-                              double(L.nnz) / std::max(L.n, 1));
-            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
+            const double mean = double(L.nnz) / std::max(L.n, 1);
+            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p, mean);
+            double *xa = L.x1.p, *ra = L.r.p, *xb = L.x3.p, *rb = L.r2.p;
+            for (int k = 1; k < steps; ++k) {
+                launchJacobiFused(s, 2, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, xa, nullptr, ra, kOmega, xb, rb, mean);
+                std::swap(xa, xb);
+                std::swap(ra, rb);
+            }
+            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, ra, C.b.p, double(L.n) / std::max(L.nc, 1));
             cycle(l + 1, s);
-            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, L.x1.p, L.agg.p, C.result, kOmega, L.x2.p, nullptr,
-                              double(L.nnz) / std::max(L.n, 1));
-            L.result = L.x2.p;
+            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, nullptr, mean);
+            double *pa = L.x2.p, *pb = (xa == L.x1.p) ? L.x3.p : L.x1.p;
+            for (int k = 1; k < steps; ++k) {
+                launchJacobiFused(s, 3, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, pa, nullptr, nullptr, kOmega, pb, nullptr, mean);
+                std::swap(pa, pb);
+            }
+            L.result = pa;
             return;
         }
         gsSweep(s, L, nullptr, L.x1.p, true);
@@ -640,7 +655,7 @@ struct CprDev {
         for (auto& L : lv)
             b += L->rpB.bytes() + L->ciB.bytes() + L->diagB.bytes() + L->v.bytes() + L->agg.bytes() + L->rrp.bytes() + L->rci.bytes() +
                  L->apPtr.bytes() + L->apSrc.bytes() + L->cPtr.bytes() + L->cSrc.bytes() + L->ap.bytes() + L->gsF.bytes() + L->gsB.bytes() +
-                 L->x1.bytes() + L->x2.bytes() + L->b.bytes() + L->r.bytes();
+                 L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->r2.bytes() + L->b.bytes() + L->r.bytes();
         return b;
     }
 };
@@ -721,7 +736,8 @@ struct rsh_system {
         if (!useGraphs) { precond(rr, z, f32); return; }
         const std::pair<int, int> gen{ilu ? ilu->generation : 0, cpr ? cpr->generation : 0};
         if (gen != graphGen) { clearGraphs(); graphGen = gen; }
-        const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z), (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5));
+        const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z),
+                                         (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5) | (opt.smooth_steps << 7));
         auto it = graphs.find(key);
         if (it == graphs.end()) {
             hipGraph_t g = nullptr;

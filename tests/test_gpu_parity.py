@@ -184,6 +184,20 @@ def test_jacobi_smoother_converges(sysd, orc):
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("steps", [2, 3])
+def test_jacobi_smoother_steps(sysd, orc, steps):
+    # further damped-Jacobi steps per level (modes 2 and 3 of the fused kernels): same solution, fewer iterations
+    s = sysd["layered"]
+    d1 = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI)
+    _, its1, _, ok1 = d1.solve(s.rhs, TOL, 2000)
+    d = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=steps)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+    xo, _, _, _ = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 2000)
+    assert ok and ok1 and its <= its1
+    assert _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
 def test_refresh_equals_fresh_setup(sysd, orc):
     s = sysd["layered"]
     d = rs.System(s).setup(precond="cpr", reuse_amg=1)
