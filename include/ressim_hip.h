@@ -45,10 +45,12 @@ extern "C" {
 #define RSH_KRYLOV_FGMRES 1       /* flexible restarted GMRES (linsolve.cpp:218-273) */
 /* precision */
 #define RSH_FP64 0                /* every kernel in double: the drop-in for setDistributedSolver */
-#define RSH_MIXED 1               /* FP32 preconditioner (ILU factors and sweeps, AMG hierarchy and smoothers) inside
-                                     FP64 FGMRES (whose restarts recompute the FP64 true residual) or FP64 BiCGSTAB
-                                     restarted by iterative refinement on the FP64 true residual; FP64 matrix,
-                                     vectors, dots and coarsest-level solve */
+#define RSH_MIXED 1               /* FP32 preconditioner (ILU factors and sweeps, AMG level matrices and smoother
+                                     diagonals, CPR pressure column) inside FP64 FGMRES (whose restarts recompute the
+                                     FP64 true residual) or FP64 BiCGSTAB restarted by iterative refinement on the
+                                     FP64 true residual; FP64 matrix, vectors, dots and coarsest-level solve.  A
+                                     refinement step that does not lower the FP64 residual switches the rest of the
+                                     solve to the FP64 preconditioner (rsh_system_info: last_fallback_fp64) */
 #define RSH_MIXED_IR 2            /* FP64 iterative refinement (refineWithDevice, bsr.cpp:1373-1396) around inner
                                      solves on the FP32 matrix and FP32 preconditioner (the float device solve,
                                      bsr.cpp:1130-1215); FP64 true residual, dots and Krylov vectors */

@@ -187,6 +187,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-4;
     const int maxOuter = S.opt.max_outer > 0 ? S.opt.max_outer : 10;
     double rn = bn;
+    bool precF32 = true;
     for (outer = 0; outer < maxOuter; ++outer) {
         rel = rn / bn;
         if (rel <= tol) return RSH_OK;
@@ -196,8 +197,8 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         // an inner solve that stalls is restarted by the next refinement step: at most maxIt / maxOuter iterations
         const int left = std::max(1, std::min(maxIt - its, std::max(50, maxIt / maxOuter)));
         const int rc = S.opt.krylov == RSH_KRYLOV_FGMRES
-                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, true, opF32, false, inIts, inRel)
-                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, true, opF32);
+                           ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, precF32, opF32, false, inIts, inRel)
+                           : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, precF32, opF32);
         its += inIts;
         if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
         launchAxpy(st, N, 1.0, S.irD.p, x);
@@ -208,11 +209,17 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         RSH_HIP(hipStreamSynchronize(st));
         const double rnew = std::sqrt(c.hscal[0]);
         if (!(rnew < rn)) {
-            // no progress (or a non-finite correction): take the step back, keep the last residual
+            // no progress (or a non-finite correction): take the step back and restore its residual
             launchAxpy(st, N, -1.0, S.irD.p, x);
-            if (!std::isfinite(rnew)) {
-                S.spmv(x, S.irR.p, false);
-                launchSub(st, N, b, S.irR.p, S.irR.p);
+            S.spmv(x, S.irR.p, false);
+            launchSub(st, N, b, S.irR.p, S.irR.p);
+            if (precF32 || opF32) {
+                // the FP32 preconditioner (or operator) stalls on this system: continue with the FP64 ones
+                precF32 = false;
+                opF32 = false;
+                S.lastFallback = true;
+                if (its >= maxIt) break;
+                continue;
             }
             S.lastStop = "refinement stalled";
             break;
@@ -229,6 +236,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     const double t0 = nowMs();
     int its = 0, outer = 0, rc;
     double rel = 0;
+    S.lastFallback = false;
     // RSH_MIXED: FGMRES with the FP32 preconditioner (its restarts recompute the FP64 true residual), or BiCGSTAB
     // with the FP32 preconditioner restarted on the FP64 true residual (refinement with the FP64 operator), which
     // keeps it from stalling on the preconditioner's FP32 rounding; RSH_MIXED_IR: refinement around FP32-operator
@@ -463,7 +471,11 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
                 S.colP.alloc(size_t(S.spmvHost.nslots) * S.nb * kWave);
             }
             launchPackColumn0(st, S.nb, S.val.p, S.spmv64.src.p, S.spmvHost.nslots, S.colP.p);
-            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st);
+            if (f32) {
+                S.colP32.alloc(S.colP.n);
+                launchToFloat(st, (long long)S.colP.n, S.colP.p, S.colP32.p);
+            }
+            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, f32);
         }
         RSH_HIP(hipStreamSynchronize(st));
         const double t1 = nowMs();
@@ -545,7 +557,7 @@ int rsh_amg_apply(rsh_system* sys, const double* r, double* z) {
         rsh_system& S = *sys;
         CprDev& C = *S.cpr;
         RSH_HIP(hipMemcpyAsync(C.lv[0]->b.p, r, size_t(S.n) * sizeof(double), hipMemcpyHostToDevice, S.s()));
-        const double* xp = C.vcycle(S.s());
+        const double* xp = C.vcycle(S.s(), S.opt.precision != RSH_FP64);
         RSH_HIP(hipMemcpyAsync(z, xp, size_t(S.n) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
         RSH_HIP(hipStreamSynchronize(S.s()));
         return RSH_OK;
@@ -616,7 +628,7 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << (S.cpr->lv[l]->gsFs.size() - 1);
             o << "],\"coarse_dense\":" << (S.cpr->dense ? 1 : 0) << ",\"cpr_bytes\":" << S.cpr->bytes();
         }
-        o << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
+        o << ",\"last_fallback_fp64\":" << (S.lastFallback ? 1 : 0) << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
  << ",\"last_setup_ms\":" << S.lastSetupMs << ",\"last_setup_ilu_ms\":" << S.lastIluMs << ",\"last_setup_other_ms\":" << S.lastCprMs << ",\"last_solve_ms\":" << S.lastSolveMs << ",\"last_stop\":\"" << S.lastStop << "\"}";
         const std::string str = o.str();
         if (int(str.size()) + 1 > cap) return fail(RSH_ERR_ARG, "info buffer too small");
@@ -678,7 +690,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                 case 0: S.spmv(S.vb.p, S.vx.p, op32); break;
                 case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
                 case 2: S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st); break;
-                case 3: S.cpr->vcycle(st); break;
+                case 3: S.cpr->vcycle(st, f32); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;
                 default: throw Error(RSH_ERR_ARG, "unknown kernel kind");
             }

@@ -215,7 +215,8 @@ void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const doubl
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);
 void launchFixDiag(hipStream_t s, int n, const int* diag, double* v);
 void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp);
-void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp,
+// colP: the pressure column in the SpMV slice layout, FP64 or (f32) FP32 storage
+void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp,
                      double* t);
 // scalar AMG level kernels
 void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b,
@@ -227,8 +228,9 @@ void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const
 // fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc;
 // mode 2 (r in xc) out0 = x1 + w D^-1 r, out1 = b - A out0; mode 3 out0 = x1 + w D^-1 (b - A x1).
 // sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
-void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
-                       const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
+// f32: level values (sval / lval / v) and dinv stored in FP32 (vectors and sums FP64)
+void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
+                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
                        const double* xc, double omega, double* out0, double* out1, double meanRow);
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv);
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);

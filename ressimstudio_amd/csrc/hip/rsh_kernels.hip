@@ -960,9 +960,9 @@ __global__ void kCprRestrict(int n, const double* __restrict__ w, const double* 
 }
 // second-stage residual t = r - S(:, 0) x_p over the SpMV slices with the pressure column packed (nb values per
 // slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081)
-template <int NB>
+template <int NB, typename TA>
 __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
-                                                     const int* __restrict__ col, const double* __restrict__ colP, int nlong,
+                                                     const int* __restrict__ col, const TA* __restrict__ colP, int nlong,
                                                      const int* __restrict__ lrow, const int* __restrict__ lptr,
                                                      const int* __restrict__ lcol, const double* __restrict__ lvalFull,
                                                      const double* __restrict__ r, const double* __restrict__ xp, double* __restrict__ t) {
@@ -979,9 +979,9 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
             const double xk = xp[__builtin_nontemporal_load(col + slot * kWave + lane)];
-            const double* a = colP + slot * (NB * kWave) + lane;
+            const TA* a = colP + slot * (NB * kWave) + lane;
 #pragma unroll
-            for (int e = 0; e < NB; ++e) acc[e] -= __builtin_nontemporal_load(a + e * kWave) * xk;
+            for (int e = 0; e < NB; ++e) acc[e] -= double(__builtin_nontemporal_load(a + e * kWave)) * xk;
         }
 #pragma unroll
         for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = acc[e];
@@ -1134,37 +1134,37 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
 // MODE 1 (prolongation + post-smoothing): x = x1 + P xc, x2 = x + w D^-1 (b - A x), with x_j formed on the fly.
 // MODE 2 (a further pre-smoothing step + residual, r given in xc): x' = x1 + w D^-1 r, r' = b - A x'.
 // MODE 3 (a further post-smoothing step): x' = x1 + w D^-1 (b - A x1).
-template <int MODE>
-__device__ inline double jacX(int j, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
+template <int MODE, typename TD>
+__device__ inline double jacX(int j, double omega, const TD* __restrict__ dinv, const double* __restrict__ b,
                               const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
-    if (MODE == 0) return omega * dinv[j] * b[j];
+    if (MODE == 0) return omega * double(dinv[j]) * b[j];
     if (MODE == 1) return x1[j] + xc[agg[j]];
-    if (MODE == 2) return x1[j] + omega * dinv[j] * xc[j];
+    if (MODE == 2) return x1[j] + omega * double(dinv[j]) * xc[j];
     return x1[j];
 }
-template <int MODE>
-__device__ inline void jacFinish(int i, double acc, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
+template <int MODE, typename TD>
+__device__ inline void jacFinish(int i, double acc, double omega, const TD* __restrict__ dinv, const double* __restrict__ b,
                                  const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc,
                                  double* __restrict__ out0, double* __restrict__ out1) {
     if (MODE == 0) {
-        out0[i] = omega * dinv[i] * b[i];
+        out0[i] = omega * double(dinv[i]) * b[i];
         out1[i] = b[i] - acc;
     } else if (MODE == 1) {
         const double xi = x1[i] + xc[agg[i]];
-        out0[i] = xi + omega * dinv[i] * (b[i] - acc);
+        out0[i] = xi + omega * double(dinv[i]) * (b[i] - acc);
     } else if (MODE == 2) {
-        out0[i] = x1[i] + omega * dinv[i] * xc[i];
+        out0[i] = x1[i] + omega * double(dinv[i]) * xc[i];
         out1[i] = b[i] - acc;
     } else {
-        out0[i] = x1[i] + omega * dinv[i] * (b[i] - acc);
+        out0[i] = x1[i] + omega * double(dinv[i]) * (b[i] - acc);
     }
 }
 // level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
-template <int MODE>
+template <int MODE, typename TA>
 __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
-                                                        const int* __restrict__ col, const double* __restrict__ val, int nlong,
+                                                        const int* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                         const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
-                                                        const double* __restrict__ lval, const double* __restrict__ dinv,
+                                                        const TA* __restrict__ lval, const TA* __restrict__ dinv,
                                                         const double* __restrict__ b, const double* __restrict__ x1, const int* __restrict__ agg,
                                                         const double* __restrict__ xc, double omega, double* __restrict__ out0,
                                                         double* __restrict__ out1) {
@@ -1177,7 +1177,7 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
         double acc = 0;
         for (int k = 0; k < inf.x; ++k) {
             const long long at = ((long long)inf.y + k) * kWave + lane;
-            acc += val[at] * jacX<MODE>(col[at], omega, dinv, b, x1, agg, xc);
+            acc += double(val[at]) * jacX<MODE>(col[at], omega, dinv, b, x1, agg, xc);
         }
         jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
         return;
@@ -1185,22 +1185,22 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
     const int q = wid - nslices;
     if (q >= nlong) return;
     double acc = 0;
-    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += lval[p] * jacX<MODE>(lcol[p], omega, dinv, b, x1, agg, xc);
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += double(lval[p]) * jacX<MODE>(lcol[p], omega, dinv, b, x1, agg, xc);
     acc = waveSum(acc);
     if (lane == 0) jacFinish<MODE>(lrow[q], acc, omega, dinv, b, x1, agg, xc, out0, out1);
 }
 // coarse levels (CSR): G lanes per row (G a power of two near the mean row length), the row's entries strided over
 // the group so their loads are issued together, then a shuffle reduction in a fixed order
-template <int MODE, int G>
-__global__ __launch_bounds__(256) void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
-                                                  const double* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
+template <int MODE, int G, typename TA>
+__global__ __launch_bounds__(256) void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const TA* __restrict__ v,
+                                                  const TA* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
                                                   const int* __restrict__ agg, const double* __restrict__ xc, double omega,
                                                   double* __restrict__ out0, double* __restrict__ out1) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = int(t / G), sub = int(t % G);
     if (i >= n) return;                       // whole groups leave together (n * G is a multiple of G)
     double acc = 0;
-    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += v[p] * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
+    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += double(v[p]) * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
     acc = groupSum<G>(acc);
     if (sub == 0) jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
 }
@@ -1588,10 +1588,14 @@ void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double
     hipLaunchKernelGGL(kCprRestrict<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp);
 }
 template <int NB>
-void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp, double* t) {
+void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
     const int waves = sl->nslices + sl->nlong;
-    hipLaunchKernelGGL(kCprSecond<NB>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, colP,
-                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+    if (f32)
+        hipLaunchKernelGGL((kCprSecond<NB, float>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p,
+                           (const float*)colP, sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+    else
+        hipLaunchKernelGGL((kCprSecond<NB, double>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p,
+                           (const double*)colP, sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
 }
 
 
@@ -1768,10 +1772,10 @@ void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const doub
     RSH_NB_DISPATCH(nb, (cprRestrictT<NB>(s, n, w, r, rp)));
     checkLaunch();
 }
-void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const double* colP, const double* r, const double* xp, double* t) {
+void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
-    RSH_NB_DISPATCH(nb, (cprSecondT<NB>(s, sl, colP, r, xp, t)));
+    RSH_NB_DISPATCH(nb, (cprSecondT<NB>(s, sl, colP, f32, r, xp, t)));
     checkLaunch();
 }
 void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b, double* r) {
@@ -1790,11 +1794,11 @@ void restrictT(hipStream_t s, int nc, const int* rrp, const int* rci, const doub
     const long long th = (long long)nc * G;
     hipLaunchKernelGGL(kRestrict<G>, dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, nc, rrp, rci, r, bc);
 }
-template <int MODE, int G>
-void jacobiCsrT(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* dinv, const double* b, const double* x1,
+template <int MODE, int G, typename TA>
+void jacobiCsrT(hipStream_t s, int n, const int* rp, const int* ci, const TA* v, const TA* dinv, const double* b, const double* x1,
                 const int* agg, const double* xc, double omega, double* out0, double* out1) {
     const long long th = (long long)n * G;
-    hipLaunchKernelGGL((kJacobiCsr<MODE, G>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+    hipLaunchKernelGGL((kJacobiCsr<MODE, G, TA>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
 }
 // lanes per row for a mean row length
 int groupLanes(double mean) {
@@ -1845,35 +1849,46 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
     checkLaunch();
 }
 
-template <int MODE>
-void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const double* sval, const double* lval, const double* dinv, const double* b,
+template <int MODE, typename TA>
+void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, const TA* lval, const TA* dinv, const double* b,
                    const double* x1, const int* agg, const double* xc, double omega, double* out0, double* out1) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
-    hipLaunchKernelGGL(kJacobiSliced<MODE>, dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+    hipLaunchKernelGGL((kJacobiSliced<MODE, TA>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
                        sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
 }
-void launchJacobiFused(hipStream_t s, int mode, const SliceDev<double>* sl, const double* sval, const double* lval, int n, const int* rp,
-                       const int* ci, const double* v, const double* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double* out1, double meanRow) {
-    if (mode < 0 || mode > 3) throw Error(-1, "unknown Jacobi step");
+template <typename TA>
+void jacobiFusedT(hipStream_t s, int mode, const SliceDev<double>* sl, const TA* sval, const TA* lval, int n, const int* rp, const int* ci,
+                  const TA* v, const TA* dinv, const double* b, const double* x1, const int* agg, const double* xc, double omega,
+                  double* out0, double* out1, double meanRow) {
     if (sl) {
         switch (mode) {
-            case 0: jacobiSlicedT<0>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            case 1: jacobiSlicedT<1>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            case 2: jacobiSlicedT<2>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            default: jacobiSlicedT<3>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 0: jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 1: jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 2: jacobiSlicedT<2, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            default: jacobiSlicedT<3, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
         }
-    } else {
-        if (n <= 0) return;
-        const int g = groupLanes(meanRow);
-        switch (mode) {
-            case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-            case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-            case 2: RSH_G_DISPATCH(g, (jacobiCsrT<2, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-            default: RSH_G_DISPATCH(g, (jacobiCsrT<3, G>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-        }
+        return;
     }
+    if (n <= 0) return;
+    const int g = groupLanes(meanRow);
+    switch (mode) {
+        case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+        case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+        case 2: RSH_G_DISPATCH(g, (jacobiCsrT<2, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+        default: RSH_G_DISPATCH(g, (jacobiCsrT<3, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+    }
+}
+void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
+                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
+                       const double* xc, double omega, double* out0, double* out1, double meanRow) {
+    if (mode < 0 || mode > 3) throw Error(-1, "unknown Jacobi step");
+    if (f32)
+        jacobiFusedT<float>(s, mode, sl, (const float*)sval, (const float*)lval, n, rp, ci, (const float*)v, (const float*)dinv, b, x1, agg, xc,
+                            omega, out0, out1, meanRow);
+    else
+        jacobiFusedT<double>(s, mode, sl, (const double*)sval, (const double*)lval, n, rp, ci, (const double*)v, (const double*)dinv, b, x1, agg,
+                             xc, omega, out0, out1, meanRow);
     checkLaunch();
 }
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv) {

@@ -437,6 +437,7 @@ struct AmgLevelDev {
     DBuf<double> x1, x2, b, r;
     DBuf<double> x3, r2;                 // further Jacobi steps (ping-pong)
     DBuf<double> dinv;                   // 1 / a_ii (Jacobi)
+    DBuf<float> v32, dinv32;             // FP32 copies of the level's values and dinv (mixed-precision hierarchy)
     const double* result = nullptr;
 };
 
@@ -456,6 +457,8 @@ struct CprDev {
     // level 0 values in the SpMV row-slice layout of S (same pattern) for the streamed Jacobi steps
     const SliceDev<double>* plan = nullptr;
     DBuf<double> apS, apL;
+    DBuf<float> apS32, apL32;             // FP32 copies for the mixed-precision hierarchy
+    bool have32 = false;
     std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
 
     int generation = 0;
@@ -545,7 +548,8 @@ struct CprDev {
         Minv = a;
     }
     // numeric set-up on the device values of S: weights, pressure matrix, hierarchy (fresh or refreshed)
-    void setup(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* val, bool reuse, hipStream_t s) {
+    void setup(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* val, bool reuse, hipStream_t s,
+               bool f32 = false) {
         n = S.n;
         nb = S.nb;
         w.alloc(size_t(n) * nb);
@@ -580,6 +584,22 @@ struct CprDev {
             launchPackSlices(s, 1, lv[0]->v.p, plan->src.p, (long long)(apS.n / kWave), apS.p, false);
             if (plan->lsrc.n) launchPackLong(s, 1, lv[0]->v.p, plan->lsrc.p, int(plan->lsrc.n), apL.p, false);
         }
+        have32 = f32;
+        if (f32) {
+            // the mixed-precision hierarchy: FP32 level values and dinv (the coarsest inverse stays FP64)
+            if (plan) {
+                apS32.alloc(apS.n);
+                apL32.alloc(apL.n);
+                launchToFloat(s, (long long)apS.n, apS.p, apS32.p);
+                launchToFloat(s, (long long)apL.n, apL.p, apL32.p);
+            }
+            for (auto& L : lv) {
+                L->v32.alloc(L->v.n);
+                L->dinv32.alloc(L->dinv.n);
+                launchToFloat(s, (long long)L->v.n, L->v.p, L->v32.p);
+                launchToFloat(s, (long long)L->dinv.n, L->dinv.p, L->dinv32.p);
+            }
+        }
     }
     // one Gauss-Seidel sweep of level L over its level schedule (fused runs of small levels)
     static void gsSweep(hipStream_t s, AmgLevelDev& L, const double* xin, double* xout, bool fwd) {
@@ -593,7 +613,7 @@ struct CprDev {
             launchGsLevel(s, st[size_t(q) + 1] - st[size_t(q)], rows + st[size_t(q)], L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd);
         }
     }
-    void cycle(size_t l, hipStream_t s) {
+    void cycle(size_t l, hipStream_t s, bool f32 = false) {
         AmgLevelDev& L = *lv[l];
         if (l + 1 == lv.size()) {
             if (dense) {
@@ -618,19 +638,24 @@ struct CprDev {
             // steps (smooth_steps > 1) ping-pong between the level's spare buffers
             const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
             const double mean = double(L.nnz) / std::max(L.n, 1);
-            launchJacobiFused(s, 0, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p, mean);
+            const bool h = f32 && have32;
+            const void* sv = h ? static_cast<const void*>(apS32.p) : static_cast<const void*>(apS.p);
+            const void* lvv = h ? static_cast<const void*>(apL32.p) : static_cast<const void*>(apL.p);
+            const void* vv = h ? static_cast<const void*>(L.v32.p) : static_cast<const void*>(L.v.p);
+            const void* dv = h ? static_cast<const void*>(L.dinv32.p) : static_cast<const void*>(L.dinv.p);
+            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p, mean);
             double *xa = L.x1.p, *ra = L.r.p, *xb = L.x3.p, *rb = L.r2.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 2, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, xa, nullptr, ra, kOmega, xb, rb, mean);
+                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, ra, kOmega, xb, rb, mean);
                 std::swap(xa, xb);
                 std::swap(ra, rb);
             }
             launchRestrict(s, L.nc, L.rrp.p, L.rci.p, ra, C.b.p, double(L.n) / std::max(L.nc, 1));
-            cycle(l + 1, s);
-            launchJacobiFused(s, 1, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, nullptr, mean);
+            cycle(l + 1, s, f32);
+            launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, nullptr, mean);
             double *pa = L.x2.p, *pb = (xa == L.x1.p) ? L.x3.p : L.x1.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 3, sl, apS.p, apL.p, L.n, L.rp, L.ci, L.v.p, L.dinv.p, L.b.p, pa, nullptr, nullptr, kOmega, pb, nullptr, mean);
+                launchJacobiFused(s, 3, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, nullptr, nullptr, kOmega, pb, nullptr, mean);
                 std::swap(pa, pb);
             }
             L.result = pa;
@@ -639,23 +664,23 @@ struct CprDev {
         gsSweep(s, L, nullptr, L.x1.p, true);
         launchCsrSpmvResidual(s, L.n, L.rp, L.ci, L.v.p, L.x1.p, L.b.p, L.r.p);
         launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1));
-        cycle(l + 1, s);
+        cycle(l + 1, s, f32);
         launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
         gsSweep(s, L, L.x1.p, L.x2.p, false);
         L.result = L.x2.p;
     }
     void launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out);
     // one V-cycle on r_p (in lv[0].b) -> result pointer
-    const double* vcycle(hipStream_t s) {
-        cycle(0, s);
+    const double* vcycle(hipStream_t s, bool f32 = false) {
+        cycle(0, s, f32);
         return lv[0]->result;
     }
     size_t bytes() const {
-        size_t b = w.bytes() + xp.bytes() + t.bytes() + M0.bytes() + M1.bytes();
+        size_t b = w.bytes() + xp.bytes() + t.bytes() + M0.bytes() + M1.bytes() + apS.bytes() + apL.bytes() + apS32.bytes() + apL32.bytes();
         for (auto& L : lv)
             b += L->rpB.bytes() + L->ciB.bytes() + L->diagB.bytes() + L->v.bytes() + L->agg.bytes() + L->rrp.bytes() + L->rci.bytes() +
                  L->apPtr.bytes() + L->apSrc.bytes() + L->cPtr.bytes() + L->cSrc.bytes() + L->ap.bytes() + L->gsF.bytes() + L->gsB.bytes() +
-                 L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->r2.bytes() + L->b.bytes() + L->r.bytes();
+                 L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->r2.bytes() + L->b.bytes() + L->r.bytes() + L->v32.bytes() + L->dinv32.bytes();
         return b;
     }
 };
@@ -692,6 +717,7 @@ struct rsh_system {
     SliceDev<float> spmv32;
     bool have32 = false;
     DBuf<double> colP;                  // pressure column of S in the SpMV slice layout (CPR second stage)
+    DBuf<float> colP32;                 // its FP32 copy (mixed precision)
     rsh_options opt{};
     bool ready = false;
     bool valuesSet = false;
@@ -709,6 +735,7 @@ struct rsh_system {
     int lastIts = 0, lastOuter = 0;
     double lastRel = 0, lastSetupMs = 0, lastSolveMs = 0, lastIluMs = 0, lastCprMs = 0;
     std::string lastStop;
+    bool lastFallback = false;          // the last refinement fell back to the FP64 preconditioner
 
     hipStream_t s() const { return ctx->c.stream; }
 
@@ -773,8 +800,9 @@ struct rsh_system {
         // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p
         CprDev& C = *cpr;
         launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p);
-        const double* xp = C.vcycle(s());
-        launchCprSecond(s(), nb, &spmv64, colP.p, rr, xp, C.t.p);
+        const double* xp = C.vcycle(s(), f32);
+        const bool p32 = f32 && colP32.n > 0;
+        launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p);
         ilu->apply(C.t.p, z, xp, f32, s());
     }
 };

@@ -184,6 +184,24 @@ def test_jacobi_smoother_converges(sysd, orc):
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("name", ["layered", "nb4", "deepwell"])
+def test_fp32_hierarchy(sysd, orc, name):
+    # RSH_MIXED with the Jacobi V-cycle: level values, dinv and the pressure column stored in FP32 (sums in FP64);
+    # one V-cycle differs from the FP64 hierarchy's by FP32 rounding only, and the solve meets the FP64 bars
+    s = sysd[name]
+    d64 = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+    d32 = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=rs.MIXED)
+    rp = np.cos(0.11 * np.arange(s.n))
+    a64, a32 = d64.amg_apply(rp), d32.amg_apply(rp)
+    assert np.max(np.abs(a32 - a64)) <= 1e-5 * np.max(np.abs(a64))
+    if len(d64.info()["amg_levels"]) > 1:
+        assert not np.array_equal(a32, a64)  # the FP32 copies are the ones used (one level = the FP64 coarsest solve)
+    x, its, rel, ok = d32.solve(s.rhs, TOL, 3000)
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok and ok_o and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("steps", [2, 3])
 def test_jacobi_smoother_steps(sysd, orc, steps):
     # further damped-Jacobi steps per level (modes 2 and 3 of the fused kernels): same solution, fewer iterations
