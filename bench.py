@@ -287,6 +287,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="only the batched-systems leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--dump", default="", help="solve a Jacobian dumped by the engine (RESSIM_DUMP format) instead of the synthetic system")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--serving-only", action="store_true", help="only the serving-loop leg")
     ap.add_argument("--serve-models", type=int, default=64)
@@ -308,7 +309,12 @@ def main():
         print(json.dumps(serving_leg(rs, gen, args)))
         return
     t = time.perf_counter()
-    s = gen.large(scale=args.scale)
+    if args.dump:
+        from ressimstudio_amd import dump
+        s = dump.to_system(dump.read_dump(args.dump))
+        args.scale = 0.0          # not the 1.1M-cell synthetic workload: no traffic file, no batched / serving legs
+    else:
+        s = gen.large(scale=args.scale)
     log(f"workload: {s.n} cells x {s.nb}, {s.nnzb} blocks ({s.fill_blocks} well fill), generated in {time.perf_counter() - t:.1f}s")
     ctx = rs.default_context()
     sysd = rs.System(s, ctx)
@@ -418,7 +424,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator, ressimstudio_amd/csrc/gen)",
-        "config": {"workload": f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells; variant {head}: BiCGSTAB to 1e-8 "
+        "config": {"workload": (f"engine Jacobian dump {os.path.basename(args.dump)}, {s.n} cells x nb {s.nb}" if args.dump else
+                                f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells") + f"; variant {head}: BiCGSTAB to 1e-8 "
                                f"preconditioned by CPR (quasi-IMPES weights, aggregation AMG V-cycle with two damped-Jacobi steps per level) + "
                                f"multicolour BILU0; numeric setup + solve per step",
                    "cells": s.n, "block": s.nb, "blocks": s.nnzb, "tol": args.tol, "scale": args.scale},
