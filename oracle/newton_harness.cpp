@@ -6,6 +6,8 @@
 // own plug-in tests, run_tests.cpp:3755-3773, c29_tests.inc:280-296).
 //
 // usage: newton_harness [steps] [precond: cpr|cpr_mc|bilu0] [smoother: gs|jacobi]
+// With RESSIM_DEBUG set the engine prints one "it N cnv X mb Y" line per Newton iteration to stderr
+// (simulator.cpp:3207-3214); the harness marks where each run starts ("== run cpu" / "== run device").
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -55,6 +57,8 @@ int main(int argc, char** argv) {
     c.tuning.blockSolver = true;
     c.tuning.linearSolver = pc == "bilu0" ? 0 : 1;   // CPU path: BILU0 or block CPR
     c.tuning.cprWeights = 0;                           // quasi-IMPES on both paths (the hook carries no accumulation blocks)
+    std::fprintf(stderr, "== run cpu\n");
+    std::fflush(stderr);
     const Run cpu = run(c);
     dump("cpu", cpu);
 
@@ -65,6 +69,8 @@ int main(int argc, char** argv) {
     opt.reuse_amg = 0;                                 // a fresh hierarchy per solve, as the CPU path builds it
     ressim_hip::HipBlockSolver gpu(&opt, 0);
     setDistributedSolver(&gpu);
+    std::fprintf(stderr, "== run device\n");
+    std::fflush(stderr);
     const Run dev = run(c);
     setDistributedSolver(nullptr);
     dump("device", dev);

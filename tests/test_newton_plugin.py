@@ -4,28 +4,63 @@ the same engine with its CPU solver.  The harness binary (oracle/_ref/newton_har
 sources where they are mounted; the tests skip where it was never built.
 
 The reference's own plug-in tests assert FOPT within 1e-5 / 1e-6 relative and no fallbacks
-(run_tests.cpp:3771-3773, c29_tests.inc:295)."""
+(run_tests.cpp:3771-3773, c29_tests.inc:295).  Beyond them, the per-Newton-iteration CNV / MB trajectory the engine
+prints with RESSIM_DEBUG (simulator.cpp:3207-3214) is compared run against run and against the CPU run recorded in
+tests/golden/demo_newton_cpu.json (tests/golden/make_demo_newton.py)."""
 import json
+import math
 import os
+import re
 import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "newton_harness")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "demo_newton_cpu.json")
+_IT = re.compile(r"\bit (\d+) cnv ([0-9.eE+-]+) mb ([0-9.eE+-]+)")
+
+
+def harness_runs(*args, timeout=300):
+    """Runs the harness with RESSIM_DEBUG: the JSON summaries per run and, per run, the Newton solves as lists of
+    [cnv, mb] per iteration."""
+    env = dict(os.environ, RESSIM_DEBUG="1")
+    p = subprocess.run([HARNESS, *map(str, args)], capture_output=True, text=True, timeout=timeout, check=True, env=env)
+    runs = {}
+    for line in p.stdout.splitlines():
+        line = line.strip()
+        if line.startswith("{"):
+            d = json.loads(line)
+            runs[d.get("run", "meta")] = d
+    traj, cur = {}, None
+    for line in p.stderr.splitlines():
+        if line.startswith("== run "):
+            cur = traj.setdefault(line[7:].strip(), [])
+            continue
+        m = _IT.search(line)
+        if m and cur is not None:
+            it, cnv, mb = int(m.group(1)), float(m.group(2)), float(m.group(3))
+            if it == 0:
+                cur.append([])
+            cur[-1].append([cnv, mb])
+    return runs, traj
 
 
 def run_harness(*args, timeout=300):
     if not os.path.exists(HARNESS):
         pytest.skip("oracle/_ref/newton_harness not built (needs the reference sources)")
-    out = subprocess.run([HARNESS, *map(str, args)], capture_output=True, text=True, timeout=timeout, check=True).stdout
-    runs = {}
-    for line in out.splitlines():
-        line = line.strip()
-        if line.startswith("{"):
-            d = json.loads(line)
-            runs[d.get("run", "meta")] = d
-    return runs
+    return harness_runs(*args, timeout=timeout)
+
+
+def _same_trajectory(a, b):
+    # the same Newton iterations per solve; CNV and MB of every iteration within a factor 2 wherever they are above
+    # round-off (linear solves to 1e-5 relative make the converged-iteration values differ in their last digits)
+    assert [len(s) for s in a] == [len(s) for s in b]
+    for sa, sb in zip(a, b):
+        for (ca, ma), (cb, mb) in zip(sa, sb):
+            for u, v in ((ca, cb), (ma, mb)):
+                if max(u, v) > 1e-9:
+                    assert abs(math.log10(max(u, 1e-300)) - math.log10(max(v, 1e-300))) <= math.log10(2.0), (sa, sb)
 
 
 def _compare_steps(cpu, dev, tol):
@@ -41,17 +76,27 @@ def test_hook_failure_falls_back_to_the_cpu_path():
     import ressimstudio_amd as rs
     if rs.device_count() > 0:
         pytest.skip("a device is present (covered by the GPU test)")
-    r = run_harness(3, "cpr", "gs")
+    r, traj = run_harness(3, "cpr", "gs")
     cpu, dev = r["cpu"], r["device"]
     assert dev["dd_solves"] > 0 and dev["dd_fallbacks"] == dev["dd_solves"]
     assert dev["newton"] == cpu["newton"]
     _compare_steps(cpu, dev, 1e-5)
 
 
+def test_cpu_run_matches_the_recorded_trajectory():
+    # the engine's own CPU run against the recorded golden run: summary and per-iteration CNV / MB
+    r, traj = run_harness(6, "cpr", "gs")
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    assert [s["newton"] for s in r["cpu"]["steps"]] == [s["newton"] for s in g["summary"]["steps"]]
+    _compare_steps(g["summary"], r["cpu"], 1e-6)
+    _same_trajectory(g["newton_trajectory"], traj["cpu"])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("precond,smoother", [("cpr", "gs"), ("cpr_mc", "jacobi"), ("bilu0", "gs")])
 def test_demo_waterflood_with_device_solver(precond, smoother):
-    r = run_harness(6, precond, smoother)
+    r, traj = run_harness(6, precond, smoother)
     cpu, dev = r["cpu"], r["device"]
     print(json.dumps({"cpu": {k: cpu[k] for k in ("newton", "linear")}, "device": {k: dev[k] for k in ("newton", "linear", "dd_solves")},
                       "device_seconds": r["meta"]["device_seconds"]}))
@@ -59,3 +104,8 @@ def test_demo_waterflood_with_device_solver(precond, smoother):
     # identical Newton convergence: the same number of Newton iterations in every report step
     assert [s["newton"] for s in dev["steps"]] == [s["newton"] for s in cpu["steps"]]
     _compare_steps(cpu, dev, 1e-5)
+    # identical Newton / CNV convergence: iteration by iteration, against this run's CPU pass
+    _same_trajectory(traj["cpu"], traj["device"])
+    if precond != "bilu0":
+        with open(GOLDEN) as f:
+            _same_trajectory(json.load(f)["newton_trajectory"], traj["device"])
