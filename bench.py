@@ -325,6 +325,8 @@ def main():
         # head: two damped-Jacobi steps per level (one step, DeviceCprAmg's choice, is variant cpr_mc_j1)
         "cpr_mc": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=2),
         "cpr_mc_j1": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=1),
+        # DeviceCprAmg: Jacobi V-cycle + block-Jacobi second stage (no ILU)
+        "cpr_bj": rs.default_options(precond="cpr_bj", precision=FP64, krylov=BI, smoother=J, smooth_steps=1),
         "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J, smooth_steps=2),
         "ilu0": rs.default_options(precond="bilu0", precision=FP64, krylov=BI),
         "mcilu0": rs.default_options(precond="mcbilu0", precision=FP64, krylov=BI),

@@ -40,6 +40,9 @@ extern "C" {
 #define RSH_PREC_MCBILU0 2        /* multicolour block ILU(0) (MCBILU0, bsr.cpp:599-730) */
 #define RSH_PREC_CPR 3            /* BlockCPR: quasi-IMPES weights, AMG V-cycle on the pressure system, BILU0 */
 #define RSH_PREC_CPR_MC 4         /* BlockCPR with the multicolour BILU0 second stage */
+#define RSH_PREC_CPR_BJ 5         /* DeviceCprAmg (bsr.h:158-179): CPR whose second stage is block Jacobi,
+                                     z = E x_p + D_b^-1 (r - A E x_p); pair with RSH_SMOOTH_JACOBI for the reference's
+                                     device V-cycle */
 /* Krylov methods (linsolve.h:99-106) */
 #define RSH_KRYLOV_BICGSTAB 0     /* right-preconditioned BiCGSTAB from x0 = 0 (linsolve.cpp:159-216) */
 #define RSH_KRYLOV_FGMRES 1       /* flexible restarted GMRES (linsolve.cpp:218-273) */

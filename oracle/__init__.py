@@ -70,6 +70,10 @@ class Checker:
         lib.chk_cpr_aggregates.argtypes = [_vp, ctypes.c_int, _ip]
         lib.chk_cpr_amg_apply.argtypes = [_vp, _dp, _dp]
         lib.chk_cpr_free.argtypes = [_vp]
+        lib.chk_devcpr_new.restype = _vp
+        lib.chk_devcpr_new.argtypes = bsr
+        lib.chk_devcpr_apply.argtypes = [_vp, _dp, _dp]
+        lib.chk_devcpr_free.argtypes = [_vp]
         lib.chk_krylov.argtypes = bsr + [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         if hasattr(lib, "chk_bilu0_factors"):
             lib.chk_bilu0_factors.argtypes = [_vp, _dp]
@@ -110,6 +114,10 @@ class Checker:
 
     def cpr(self, s):
         return _Cpr(self, s)
+
+    def devcpr(self, s):
+        """DeviceCprAmg (bsr.cpp:1235-1371): float32 CPR with damped-Jacobi AMG and block-Jacobi second stage."""
+        return _DevCpr(self, s)
 
 
 class _Handle:
@@ -168,6 +176,23 @@ class _McBilu0(_Handle):
         perm = np.empty(self.n, np.int32)
         k = self.c.lib.chk_mcbilu0_colouring(self.h, _p(colour, ctypes.c_int), _p(perm, ctypes.c_int))
         return int(k), colour, perm
+
+
+class _DevCpr(_Handle):
+    _free = "chk_devcpr_free"
+
+    def __init__(self, c, s):
+        self.c, self.N = c, s.n * s.nb
+        n, nb, rp, ci, v = _bsr_args(s)
+        self.h = c.lib.chk_devcpr_new(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double))
+        if not self.h:
+            raise RuntimeError("DeviceCprAmg build failed")
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(self.N)
+        self.c.lib.chk_devcpr_apply(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double))
+        return z
 
 
 class _Cpr(_Handle):

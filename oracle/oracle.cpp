@@ -578,6 +578,156 @@ struct BlockCpr {
     }
 };
 
+// ---- DeviceCprAmg (bsr.h:158-179, build bsr.cpp:1235-1317, apply :1319-1371): the reference's device CPR in float32
+// - quasi-IMPES restriction W, the host AMG hierarchy with damped Jacobi (omega 0.6667f, one pre / one post step), the
+// explicit inverse of the coarsest operator (Gauss-Jordan with partial pivoting in double, m <= 1500, else 30 Jacobi
+// sweeps), E = injection into the pressure slot, block-Jacobi second stage z = z1 + D_b^-1 (r - A z1)
+struct DevCprAmg {
+    struct FCsr { int rows = 0; std::vector<int> rp, ci; std::vector<float> v;
+        void spmv(const float* x, float* y) const {                                    // float accumulation (bsr.cpp:1218-1224)
+            for (int i = 0; i < rows; ++i) { float acc = 0; for (int k = rp[size_t(i)]; k < rp[size_t(i) + 1]; ++k) acc += v[size_t(k)] * x[size_t(ci[size_t(k)])]; y[i] = acc; }
+        }
+    };
+    struct Lvl { FCsr A, P, R; std::vector<float> dinv; };
+    Bsr S;
+    std::vector<float> W, Av, bjInv;
+    std::vector<Lvl> lv;
+    FCsr coarseInv;
+    float omega = 0.6667f;
+    bool build(const Bsr& A) {
+        BlockCpr C;
+        C.S = A;
+        C.weights();
+        C.amg.setup(C.pressure());
+        S = A;
+        const int n = S.n, nb = S.nb, b2 = nb * nb;
+        W.assign(C.w.begin(), C.w.end());
+        Av.assign(S.v.begin(), S.v.end());
+        const size_t L = C.amg.lv.size();
+        lv.assign(L, Lvl{});
+        for (size_t l = 0; l < L; ++l) {
+            const Csr& H = C.amg.lv[l].A;
+            Lvl& V = lv[l];
+            V.A.rows = H.n; V.A.rp = H.rp; V.A.ci = H.ci; V.A.v.assign(H.v.begin(), H.v.end());
+            V.dinv.resize(size_t(H.n));
+            for (int i = 0; i < H.n; ++i) {
+                double d = 0;
+                for (int k = H.rp[size_t(i)]; k < H.rp[size_t(i) + 1]; ++k) if (H.ci[size_t(k)] == i) d = H.v[size_t(k)];
+                V.dinv[size_t(i)] = std::fabs(d) > 1e-300 ? float(1.0 / d) : 0.0f;
+            }
+            if (l + 1 < L) {
+                const auto& agg = C.amg.lv[l].agg;
+                const int nc = C.amg.lv[l].nc;
+                V.P.rows = H.n; V.P.rp.resize(size_t(H.n) + 1); V.P.ci.resize(size_t(H.n)); V.P.v.assign(size_t(H.n), 1.0f);
+                for (int i = 0; i <= H.n; ++i) V.P.rp[size_t(i)] = i;
+                for (int i = 0; i < H.n; ++i) V.P.ci[size_t(i)] = agg[size_t(i)];
+                V.R.rows = nc; V.R.rp.assign(size_t(nc) + 1, 0);
+                for (int i = 0; i < H.n; ++i) V.R.rp[size_t(agg[size_t(i)]) + 1]++;
+                for (int a = 0; a < nc; ++a) V.R.rp[size_t(a) + 1] += V.R.rp[size_t(a)];
+                V.R.ci.resize(size_t(H.n)); V.R.v.assign(size_t(H.n), 1.0f);
+                std::vector<int> fill(V.R.rp.begin(), V.R.rp.end() - 1);
+                for (int i = 0; i < H.n; ++i) V.R.ci[size_t(fill[size_t(agg[size_t(i)])]++)] = i;
+            }
+        }
+        const Csr& Cc = C.amg.lv.back().A;
+        const int m = Cc.n;
+        coarseInv = FCsr{};
+        if (m > 0 && m <= 1500) {
+            std::vector<double> M(size_t(m) * 2 * m, 0.0);
+            for (int i = 0; i < m; ++i) {
+                for (int k = Cc.rp[size_t(i)]; k < Cc.rp[size_t(i) + 1]; ++k) M[size_t(i) * 2 * m + size_t(Cc.ci[size_t(k)])] += Cc.v[size_t(k)];
+                M[size_t(i) * 2 * m + size_t(m + i)] = 1.0;
+            }
+            bool ok = true;
+            for (int k = 0; k < m && ok; ++k) {
+                int pr = k;
+                for (int i = k + 1; i < m; ++i) if (std::fabs(M[size_t(i) * 2 * m + k]) > std::fabs(M[size_t(pr) * 2 * m + k])) pr = i;
+                if (std::fabs(M[size_t(pr) * 2 * m + k]) < 1e-300) { ok = false; break; }
+                if (pr != k) for (int j = 0; j < 2 * m; ++j) std::swap(M[size_t(k) * 2 * m + j], M[size_t(pr) * 2 * m + j]);
+                const double piv = 1.0 / M[size_t(k) * 2 * m + k];
+                for (int j = 0; j < 2 * m; ++j) M[size_t(k) * 2 * m + j] *= piv;
+                for (int i = 0; i < m; ++i) {
+                    if (i == k) continue;
+                    const double f = M[size_t(i) * 2 * m + k];
+                    if (f == 0) continue;
+                    for (int j = 0; j < 2 * m; ++j) M[size_t(i) * 2 * m + j] -= f * M[size_t(k) * 2 * m + j];
+                }
+            }
+            if (ok) {
+                coarseInv.rows = m; coarseInv.rp.resize(size_t(m) + 1); coarseInv.ci.resize(size_t(m) * m); coarseInv.v.resize(size_t(m) * m);
+                for (int i = 0; i < m; ++i) {
+                    coarseInv.rp[size_t(i)] = i * m;
+                    for (int j = 0; j < m; ++j) { coarseInv.ci[size_t(i) * m + j] = j; coarseInv.v[size_t(i) * m + j] = float(M[size_t(i) * 2 * m + size_t(m + j)]); }
+                }
+                coarseInv.rp[size_t(m)] = m * m;
+            }
+        }
+        bjInv.resize(size_t(n) * b2);
+        for (int i = 0; i < n; ++i) {
+            double blk[kMaxB * kMaxB];
+            const double* D = S.blk(S.diag[size_t(i)]);
+            std::memcpy(blk, D, sizeof(double) * size_t(b2));
+            if (!invertBlock(blk, nb)) { for (int e = 0; e < b2; ++e) blk[e] = 0; for (int e = 0; e < nb; ++e) blk[e * nb + e] = 1.0 / std::max(1e-30, std::fabs(D[e * nb + e])); }
+            for (int e = 0; e < b2; ++e) bjInv[size_t(i) * b2 + e] = float(blk[e]);
+        }
+        return true;
+    }
+    void apply(const float* r, float* z) const {
+        const int n = S.n, nb = S.nb, b2 = nb * nb;
+        const size_t L = lv.size();
+        std::vector<std::vector<float>> x(L), b(L), t(L);
+        for (size_t l = 0; l < L; ++l) { x[l].assign(size_t(lv[l].A.rows), 0.f); b[l] = x[l]; t[l] = x[l]; }
+        for (int i = 0; i < n; ++i) { float acc = 0; for (int e = 0; e < nb; ++e) acc += W[size_t(i) * nb + e] * r[size_t(i) * nb + e]; b[0][size_t(i)] = acc; }
+        for (size_t l = 0; l + 1 < L; ++l) {
+            const Lvl& V = lv[l];
+            const size_t nl = x[l].size();
+            for (size_t i = 0; i < nl; ++i) x[l][i] = omega * V.dinv[i] * b[l][i];
+            V.A.spmv(x[l].data(), t[l].data());
+            for (size_t i = 0; i < nl; ++i) t[l][i] = b[l][i] - t[l][i];
+            V.R.spmv(t[l].data(), b[l + 1].data());
+        }
+        {
+            const Lvl& V = lv[L - 1];
+            const size_t nl = x[L - 1].size();
+            if (!coarseInv.rp.empty()) coarseInv.spmv(b[L - 1].data(), x[L - 1].data());
+            else {
+                for (size_t i = 0; i < nl; ++i) x[L - 1][i] = omega * V.dinv[i] * b[L - 1][i];
+                for (int s2 = 0; s2 < 30; ++s2) {
+                    V.A.spmv(x[L - 1].data(), t[L - 1].data());
+                    for (size_t i = 0; i < nl; ++i) x[L - 1][i] += omega * V.dinv[i] * (b[L - 1][i] - t[L - 1][i]);
+                }
+            }
+        }
+        for (size_t l = L - 1; l-- > 0;) {
+            const Lvl& V = lv[l];
+            const size_t nl = x[l].size();
+            V.P.spmv(x[l + 1].data(), t[l].data());
+            for (size_t i = 0; i < nl; ++i) x[l][i] += t[l][i];
+            V.A.spmv(x[l].data(), t[l].data());
+            for (size_t i = 0; i < nl; ++i) x[l][i] += omega * V.dinv[i] * (b[l][i] - t[l][i]);
+        }
+        std::vector<float> z1(size_t(n) * nb, 0.f), tt(size_t(n) * nb);
+        for (int i = 0; i < n; ++i) z1[size_t(i) * nb] = 1.0f * x[0][size_t(i)];
+        for (int i = 0; i < n; ++i) {
+            float acc[kMaxB] = {};
+            for (int k = S.rp[size_t(i)]; k < S.rp[size_t(i) + 1]; ++k) {
+                const float* B = Av.data() + size_t(k) * b2;
+                const float* xk = z1.data() + size_t(S.ci[size_t(k)]) * nb;
+                for (int e = 0; e < nb; ++e) for (int f = 0; f < nb; ++f) acc[e] += B[e * nb + f] * xk[f];
+            }
+            for (int e = 0; e < nb; ++e) tt[size_t(i) * nb + e] = r[size_t(i) * nb + e] - acc[e];
+        }
+        for (int i = 0; i < n; ++i) {
+            const float* D = bjInv.data() + size_t(i) * b2;
+            for (int e = 0; e < nb; ++e) {
+                float s2 = 0;
+                for (int f = 0; f < nb; ++f) s2 += D[e * nb + f] * tt[size_t(i) * nb + f];
+                z[size_t(i) * nb + e] = z1[size_t(i) * nb + e] + s2;
+            }
+        }
+    }
+};
+
 // ---- Krylov (linsolve.cpp:147-273)
 struct Prec { virtual ~Prec() = default; virtual void apply(const double* r, double* z) const = 0; };
 template <typename P> struct PrecOf : Prec { const P& p; explicit PrecOf(const P& x) : p(x) {} void apply(const double* r, double* z) const override { p.apply(r, z); } };
@@ -773,5 +923,21 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
     std::copy(xx.begin(), xx.end(), x);
     return conv ? 1 : 0;
 }
+
+
+void* chk_devcpr_new(int n, int nb, const int* rp, const int* ci, const double* v) {
+    auto* d = new DevCprAmg;
+    if (!d->build(makeBsr(n, nb, rp, ci, v))) { delete d; return nullptr; }
+    return d;
+}
+int chk_devcpr_apply(void* h, const double* r, double* z) {
+    auto* d = static_cast<DevCprAmg*>(h);
+    const size_t N = size_t(d->S.n) * d->S.nb;
+    std::vector<float> rf(r, r + N), zf(N);
+    d->apply(rf.data(), zf.data());
+    for (size_t i = 0; i < N; ++i) z[i] = zf[i];
+    return 0;
+}
+void chk_devcpr_free(void* h) { delete static_cast<DevCprAmg*>(h); }
 
 }  // extern "C"

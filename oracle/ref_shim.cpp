@@ -112,6 +112,24 @@ int chk_cpr_aggregates(void* h, int l, int* agg) {
 int chk_cpr_amg_apply(void* h, const double* r, double* z) { static_cast<CprH*>(h)->P.amg().apply(r, z); return 0; }
 void chk_cpr_free(void* h) { delete static_cast<CprH*>(h); }
 
+struct DevH { BSR S; DeviceCprAmg D; std::vector<float> Av; };
+void* chk_devcpr_new(int n, int nb, const int* rp, const int* ci, const double* v) {
+    auto* d = new DevH;
+    d->S = makeBsr(n, nb, rp, ci, v);
+    if (!d->D.build(d->S)) { delete d; return nullptr; }
+    d->Av.assign(d->S.val.begin(), d->S.val.end());
+    return d;
+}
+int chk_devcpr_apply(void* h, const double* r, double* z) {
+    auto* d = static_cast<DevH*>(h);
+    const size_t N = size_t(d->S.n) * d->S.nb;
+    std::vector<float> rf(r, r + N), zf(N);
+    d->D.apply(d->S, d->Av, rf.data(), zf.data());
+    for (size_t i = 0; i < N; ++i) z[i] = zf[i];
+    return 0;
+}
+void chk_devcpr_free(void* h) { delete static_cast<DevH*>(h); }
+
 int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond, int method,
                int restart, double tol, int maxit, int* its, double* relres) {
     BSR A = makeBsr(n, nb, rp, ci, v);

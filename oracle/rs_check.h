@@ -44,6 +44,10 @@ int chk_cpr_level_matrix(void* h, int l, int* rp, int* ci, double* v);
 int chk_cpr_aggregates(void* h, int l, int* agg);
 int chk_cpr_amg_apply(void* h, const double* r, double* z);
 void chk_cpr_free(void* h);
+/* DeviceCprAmg (bsr.cpp:1235-1371): the reference's float32 device CPR with the block-Jacobi second stage */
+void* chk_devcpr_new(int n, int nb, const int* rp, const int* ci, const double* v);
+int chk_devcpr_apply(void* h, const double* r, double* z);
+void chk_devcpr_free(void* h);
 
 int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond,
                int method, int restart, double tol, int maxit, int* its, double* relres);

@@ -441,7 +441,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         rsh_options o;
         rsh_default_options(&o);
         if (opt) o = *opt;
-        if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_MC) return fail(RSH_ERR_ARG, "unknown preconditioner");
+        if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_BJ) return fail(RSH_ERR_ARG, "unknown preconditioner");
         if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR) return fail(RSH_ERR_ARG, "unknown precision");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
         rsh_system& S = *sys;
@@ -454,7 +454,8 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             S.have32 = true;
         }
         const bool wantMc = o.precond == RSH_PREC_MCBILU0 || o.precond == RSH_PREC_CPR_MC;
-        if (!S.ilu || S.ilu->mc != wantMc) {
+        const bool wantIlu = o.precond != RSH_PREC_CPR_BJ;
+        if (wantIlu && (!S.ilu || S.ilu->mc != wantMc)) {
             S.ilu = std::make_unique<IluDev>();
             S.ilu->keepEntries = S.keepIluEntries;
             S.ilu->fuseMax = S.iluFuseMax;
@@ -462,7 +463,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             S.ilu->gate = S.gate;
             S.ilu->build(S.pat, S.rp.p, S.ci.p, S.diag.p, wantMc, st);
         }
-        if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC) {
+        if (o.precond == RSH_PREC_CPR || o.precond == RSH_PREC_CPR_MC || o.precond == RSH_PREC_CPR_BJ) {
             if (!S.cpr) S.cpr = std::make_unique<CprDev>();
             S.cpr->smoother = o.smoother;
             S.cpr->steps = o.smooth_steps > 0 ? o.smooth_steps : 1;
@@ -477,9 +478,18 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             }
             S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, f32);
         }
+        if (!wantIlu) {
+            // block-Jacobi second stage: D_b^-1 of the natural blocks (FP32 copy for the mixed precision)
+            S.bjInv.alloc(size_t(S.n) * S.nb * S.nb);
+            launchBlockDiagInv(st, S.nb, S.n, S.diag.p, S.val.p, S.bjInv.p);
+            if (f32) {
+                S.bjInv32.alloc(S.bjInv.n);
+                launchToFloat(st, (long long)S.bjInv.n, S.bjInv.p, S.bjInv32.p);
+            }
+        }
         RSH_HIP(hipStreamSynchronize(st));
         const double t1 = nowMs();
-        S.ilu->factor(S.val.p, f32, st);
+        if (wantIlu) S.ilu->factor(S.val.p, f32, st);
         RSH_HIP(hipStreamSynchronize(st));
         S.lastIluMs = nowMs() - t1;
         S.lastCprMs = t1 - t0;
@@ -689,7 +699,10 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
             switch (kind) {
                 case 0: S.spmv(S.vb.p, S.vx.p, op32); break;
                 case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
-                case 2: S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st); break;
+                case 2:
+                    if (!S.ilu || S.opt.precond == RSH_PREC_CPR_BJ) throw Error(RSH_ERR_ARG, "no ILU preconditioner set up");
+                    S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st);
+                    break;
                 case 3: S.cpr->vcycle(st, f32); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;
                 default: throw Error(RSH_ERR_ARG, "unknown kernel kind");

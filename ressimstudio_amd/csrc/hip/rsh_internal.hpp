@@ -201,6 +201,9 @@ void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fw
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
+// block-Jacobi second stage: D_b^-1 per block row (diag positions of a natural BSR), and z = D_b^-1 t + E xp
+void launchBlockDiagInv(hipStream_t s, int nb, int n, const int* diag, const double* val, double* out);
+void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32, const double* t, const double* xp, double* z);
 // BSR values with block row i scaled by (1 + amp * u_i), u_i uniform in [-1, 1) hashed from (seed, i)
 void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed);
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,

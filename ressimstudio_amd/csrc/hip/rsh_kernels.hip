@@ -784,6 +784,45 @@ __global__ void kPerturbRows(int n, const int* __restrict__ rp, int nb2, const d
     for (long long k = (long long)rp[i] * nb2; k < (long long)rp[i + 1] * nb2; ++k) out[k] = base[k] * f;
 }
 
+// ---------------------------------------------------------------- block-Jacobi second stage (DeviceCprAmg, bsr.cpp:1304-1311, 1359-1367)
+// D_b^-1 per block row; a singular block falls back to the inverted absolute diagonal, as the reference does
+template <int NB>
+__global__ void kBlockDiagInv(int n, const int* __restrict__ diag, const double* __restrict__ val, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* D = val + (long long)diag[i] * NB * NB;
+    double a[NB * NB];
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) a[e] = D[e];
+    if (!invertBlockDev<NB>(a)) {
+#pragma unroll
+        for (int e = 0; e < NB * NB; ++e) a[e] = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) a[e * NB + e] = 1.0 / fmax(1e-30, fabs(D[e * NB + e]));
+    }
+#pragma unroll
+    for (int e = 0; e < NB * NB; ++e) out[(long long)i * NB * NB + e] = a[e];
+}
+// z_i = D_i^-1 t_i, then z_i0 += xp_i (z1 = E xp); TD: storage of D^-1
+template <int NB, typename TD>
+__global__ void kBlockJacobi(int n, const TD* __restrict__ dinv, const double* __restrict__ t, const double* __restrict__ xp, double* __restrict__ z) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TD* D = dinv + (long long)i * NB * NB;
+    double ti[NB];
+#pragma unroll
+    for (int f = 0; f < NB; ++f) ti[f] = t[(long long)i * NB + f];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        double s = 0;
+#pragma unroll
+        for (int f = 0; f < NB; ++f) s += double(D[e * NB + f]) * ti[f];
+        z[(long long)i * NB + e] = (e == 0 ? xp[i] : 0.0) + s;
+    }
+}
+
 __global__ void kToFloat(long long n, const double* __restrict__ a, float* __restrict__ b) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = float(a[i]);
@@ -1722,6 +1761,25 @@ void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int
 void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed) {
     if (n <= 0) return;
     hipLaunchKernelGGL(kPerturbRows, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, rp, nb * nb, base, out, amp, seed);
+    checkLaunch();
+}
+template <int NB>
+void blockDiagInvT(hipStream_t s, int n, const int* diag, const double* val, double* out) {
+    hipLaunchKernelGGL(kBlockDiagInv<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, val, out);
+}
+void launchBlockDiagInv(hipStream_t s, int nb, int n, const int* diag, const double* val, double* out) {
+    if (n <= 0) return;
+    RSH_NB_DISPATCH(nb, (blockDiagInvT<NB>(s, n, diag, val, out)))
+    checkLaunch();
+}
+template <int NB>
+void blockJacobiT(hipStream_t s, int n, const void* dinv, bool f32, const double* t, const double* xp, double* z) {
+    if (f32) hipLaunchKernelGGL((kBlockJacobi<NB, float>), dim3((n + 255) / 256), dim3(256), 0, s, n, (const float*)dinv, t, xp, z);
+    else hipLaunchKernelGGL((kBlockJacobi<NB, double>), dim3((n + 255) / 256), dim3(256), 0, s, n, (const double*)dinv, t, xp, z);
+}
+void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32, const double* t, const double* xp, double* z) {
+    if (n <= 0) return;
+    RSH_NB_DISPATCH(nb, (blockJacobiT<NB>(s, n, dinv, f32, t, xp, z)))
     checkLaunch();
 }
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {

@@ -718,6 +718,8 @@ struct rsh_system {
     bool have32 = false;
     DBuf<double> colP;                  // pressure column of S in the SpMV slice layout (CPR second stage)
     DBuf<float> colP32;                 // its FP32 copy (mixed precision)
+    DBuf<double> bjInv;                 // block-Jacobi second stage: D_b^-1 per block row
+    DBuf<float> bjInv32;
     rsh_options opt{};
     bool ready = false;
     bool valuesSet = false;
@@ -797,12 +799,18 @@ struct rsh_system {
             ilu->apply(rr, z, nullptr, f32, s());
             return;
         }
-        // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p
+        // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p; DeviceCprAmg
+        // (bsr.cpp:1319-1371) ends with the block-Jacobi second stage z = E x_p + D_b^-1 t instead of the BILU0
         CprDev& C = *cpr;
         launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p);
         const double* xp = C.vcycle(s(), f32);
         const bool p32 = f32 && colP32.n > 0;
         launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p);
+        if (pc == RSH_PREC_CPR_BJ) {
+            const bool b32 = f32 && bjInv32.n > 0;
+            launchBlockJacobi(s(), nb, n, b32 ? static_cast<const void*>(bjInv32.p) : static_cast<const void*>(bjInv.p), b32, C.t.p, xp, z);
+            return;
+        }
         ilu->apply(C.t.p, z, xp, f32, s());
     }
 };

@@ -184,6 +184,22 @@ def test_jacobi_smoother_converges(sysd, orc):
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("name", ["small", "layered", "nb2", "nb4", "laplace"])
+def test_device_cpr_block_jacobi(sysd, orc, name):
+    # RSH_PREC_CPR_BJ with the Jacobi V-cycle is DeviceCprAmg (bsr.cpp:1235-1371), which the reference runs in float32:
+    # the FP64 apply agrees with the float32 oracle (itself bit-identical to the compiled reference) to FP32 rounding
+    s = sysd[name]
+    d = rs.System(s).setup(precond="cpr_bj", smoother=rs.SMOOTH_JACOBI)
+    o = orc.devcpr(s)
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    z, zo = d.precond(r), o.apply(r)
+    assert np.max(np.abs(z - zo)) <= 2e-5 * np.max(np.abs(zo))
+    x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok and ok_o and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("name", ["layered", "nb4", "deepwell"])
 def test_fp32_hierarchy(sysd, orc, name):
     # RSH_MIXED with the Jacobi V-cycle: level values, dinv and the pressure column stored in FP32 (sums in FP64);

@@ -297,3 +297,24 @@ def test_planted_mistake_is_caught(orc):
     bad.val[5, 1, 1] *= -1
     zb = orc.bilu0(bad, 1).apply(r)
     assert np.max(np.abs(z - zb)) > 1e-6 * np.max(np.abs(z))
+
+
+@pytest.mark.parametrize("name", ["small", "layered"])
+def test_devcpr_restatement_equals_reference(orc, ref, name):
+    # DeviceCprAmg (bsr.cpp:1235-1371) in float32: the restatement reproduces the compiled reference bit for bit
+    s = {"small": gen.small(), "layered": gen.reservoir(12, 30, 14, 3, model=1, sigma=2.5, kvkh=0.1, wells=5, seed=7)}[name]
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    assert np.array_equal(orc.devcpr(s).apply(r), ref.devcpr(s).apply(r))
+
+
+def test_devcpr_block_jacobi_second_stage(orc):
+    # what the mathematics fixes: with the AMG correction removed (r in the null space of the restriction), the
+    # second stage is z = D_b^-1 r exactly (in float32); a planted change of the block inverse breaks it
+    s = gen.small()
+    d = orc.devcpr(s)
+    z = d.apply(np.zeros(s.n * s.nb))
+    assert np.all(z == 0)
+    # linearity of the whole preconditioner (float32 rounding only)
+    r1, r2 = np.sin(np.arange(s.n * s.nb)), np.cos(0.3 * np.arange(s.n * s.nb))
+    z12 = d.apply(r1 + r2)
+    assert np.max(np.abs(z12 - d.apply(r1) - d.apply(r2))) <= 1e-5 * np.max(np.abs(z12))
