@@ -67,6 +67,7 @@ typedef struct rsh_context rsh_context;
 typedef struct rsh_system rsh_system;
 typedef struct rsh_batch rsh_batch;
 typedef struct rsh_server rsh_server;
+typedef struct rsh_wells rsh_wells;
 
 typedef struct rsh_options {
     int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
@@ -125,6 +126,21 @@ int rsh_memcpy_d2h(rsh_context* ctx, void* dst, const void* src, size_t bytes);
  * stream, bracketed by HIP events.  kind: 0 BSR SpMV (precision of the system setup), 1 preconditioner apply,
  * 2 ILU forward+backward sweeps, 3 AMG V-cycle, 4 copy kernel of `bytes` (bandwidth calibration). */
 int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double* bytes_per_launch);
+
+/* ---- well Schur reduction on the device (replaces the host-side WellSchur::build / reduceRhs / recover,
+ * bsr.cpp:733-990, that the engine runs before calling the hook, simulator.cpp:3320-3330): the engine's scalar
+ * Jacobian (cells in blocks of nv unknowns, then one BHP row per well; row_ptr / col of n = na*nv + nw rows) is
+ * handed over as it is assembled.  The reduced block pattern and the entry maps are built once per pattern; each
+ * Newton iteration uploads only J's values (rsh_wells_set_values) and solves with the full right-hand side
+ * (rsh_wells_solve), getting back cells and BHPs.  rsh_wells_system gives the reduced resident system for
+ * rsh_system_setup (and any other rsh_system call). */
+int rsh_wells_create(rsh_context* ctx, int n, const int* row_ptr, const int* col, int na, int nv, int nw, rsh_wells** out);
+int rsh_wells_destroy(rsh_wells* w);
+int rsh_wells_system(rsh_wells* w, rsh_system** sys);
+int rsh_wells_set_values(rsh_wells* w, const double* jval, int on_device);          /* RSH_ERR_UNSUPPORTED: bad well row */
+int rsh_wells_solve(rsh_wells* w, const double* b_full, double* x_full, int on_device, double tol, int max_it, int* iterations,
+                    double* rel_res);
+int rsh_wells_reduced_values(rsh_wells* w, double* out);                         /* S's values to the host (checks) */
 
 /* ---- batched solves of many independent systems (no counterpart in the reference, whose engine solves one
  * system per call; this serves its ensemble callers, runBatch study.cpp:326-364, in one launch stream).

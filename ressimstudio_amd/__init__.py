@@ -41,7 +41,8 @@ EXPORTED = [
     "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
     "rsh_batch_info", "rsh_server_create", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
-    "rsh_server_stats",
+    "rsh_server_stats", "rsh_wells_create", "rsh_wells_destroy", "rsh_wells_system", "rsh_wells_set_values", "rsh_wells_solve",
+    "rsh_wells_reduced_values",
 ]
 
 
@@ -123,6 +124,12 @@ def lib():
         L.rsh_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
         L.rsh_server_wait.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(Result)]
         L.rsh_server_stats.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+        L.rsh_wells_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.rsh_wells_destroy.argtypes = [_vp]
+        L.rsh_wells_system.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.rsh_wells_set_values.argtypes = [_vp, _vp, ctypes.c_int]
+        L.rsh_wells_solve.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.rsh_wells_reduced_values.argtypes = [_vp, _dp]
         _LIB = L
     return _LIB
 
@@ -422,6 +429,55 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+class Wells:
+    """Well Schur reduction on the device (rsh_wells_*): the engine's scalar Jacobian (cells in blocks of nv, then
+    one BHP row per well) in, cells and BHPs out; ``setup`` / ``info`` act on the reduced resident system."""
+
+    def __init__(self, row_ptr, col, na: int, nv: int, nw: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        self.col = np.ascontiguousarray(col, np.int32)
+        self.n = len(self.row_ptr) - 1
+        h = _vp()
+        _check(lib().rsh_wells_create(self.ctx.h, self.n, _p(self.row_ptr, ctypes.c_int), _p(self.col, ctypes.c_int), na, nv, nw, ctypes.byref(h)),
+               "rsh_wells_create")
+        self.h = h
+        sh = _vp()
+        _check(lib().rsh_wells_system(self.h, ctypes.byref(sh)), "rsh_wells_system")
+        self.sys_h = sh
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rsh_wells_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_values(self, jval):
+        v = np.ascontiguousarray(jval, np.float64)
+        _check(lib().rsh_wells_set_values(self.h, v.ctypes.data, 0), "rsh_wells_set_values")
+
+    def setup(self, options: Options | None = None, **kw):
+        o = options or default_options(**kw)
+        _check(lib().rsh_system_setup(self.sys_h, ctypes.byref(o)), "rsh_system_setup")
+        return self
+
+    def solve(self, b_full, tol=1e-8, max_it=1000):
+        b = np.ascontiguousarray(b_full, np.float64)
+        x = np.zeros(self.n)
+        its = ctypes.c_int(0)
+        rel = ctypes.c_double(0)
+        rc = lib().rsh_wells_solve(self.h, b.ctypes.data, x.ctypes.data, 0, tol, max_it, ctypes.byref(its), ctypes.byref(rel))
+        _check(rc, "rsh_wells_solve")
+        return x, int(its.value), float(rel.value)
+
+    def reduced_values(self, nnzb: int, nb: int):
+        out = np.empty(nnzb * nb * nb)
+        _check(lib().rsh_wells_reduced_values(self.h, _p(out, ctypes.c_double)), "rsh_wells_reduced_values")
+        return out
 
 
 class HipBlockSolver:

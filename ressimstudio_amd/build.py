@@ -10,7 +10,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-HIP_SOURCES = ["csrc/hip/rsh_kernels.hip", "csrc/hip/rsh_setup.cpp", "csrc/hip/rsh_api.cpp", "csrc/hip/rsh_batch.cpp", "csrc/hip/rsh_server.cpp"]
+HIP_SOURCES = ["csrc/hip/rsh_kernels.hip", "csrc/hip/rsh_setup.cpp", "csrc/hip/rsh_api.cpp", "csrc/hip/rsh_batch.cpp", "csrc/hip/rsh_server.cpp", "csrc/hip/rsh_wells.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              # packed FP32 instructions stay disabled (hipcc's host pass warns about the feature: harmless)
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -201,6 +201,14 @@ void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fw
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
+// well Schur reduction of a scalar Jacobian on the device (see rsh_wells.cpp for the maps)
+void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const double* jval, double* dinv, int* bad);
+void launchSchurValues(hipStream_t s, long long nvals, const int* direct, const int* termPtr, const int4* terms, const double* jval,
+                       const double* dinv, double* sval);
+void launchSchurRhs(hipStream_t s, int nc, const int* cwPtr, const int* cwWell, const int* cwSrc, const double* jval, const double* dinv,
+                    const double* b, double* bc);
+void launchSchurRecover(hipStream_t s, int nc, int nw, const int* wcPtr, const int* wcCol, const int* wcSrc, const double* jval, const double* dinv,
+                        const double* b, const double* xc, double* x);
 // block-Jacobi second stage: D_b^-1 per block row (diag positions of a natural BSR), and z = D_b^-1 t + E xp
 void launchBlockDiagInv(hipStream_t s, int nb, int n, const int* diag, const double* val, double* out);
 void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32, const double* t, const double* xp, double* z);

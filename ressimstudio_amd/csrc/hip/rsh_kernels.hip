@@ -7,6 +7,8 @@
 // Kernels that restate a reference loop keep its order of floating-point operations and disable contraction
 // (#pragma clang fp contract(off)), so that on rows handled by one lane the results equal the reference's
 // (bsr.cpp:14-56 block products, :67-87 inversion, :183-215 row elimination, :437-596 sweeps; amg.cpp:98-131).
+#include <algorithm>
+
 #include "rsh_internal.hpp"
 
 namespace rsh {
@@ -821,6 +823,59 @@ __global__ void kBlockJacobi(int n, const TD* __restrict__ dinv, const double* _
         for (int f = 0; f < NB; ++f) s += double(D[e * NB + f]) * ti[f];
         z[(long long)i * NB + e] = (e == 0 ? xp[i] : 0.0) + s;
     }
+}
+
+// ---------------------------------------------------------------- well Schur reduction on the device (WellSchur, bsr.cpp:733-990)
+// The scalar Jacobian J: cells in blocks of nv unknowns (rows / columns < nc = na nv), then one BHP row per well.
+// dinv[w] = 1 / J_ww (refused: |J_ww| <= 1e-12 max |J_w,cells| or non-finite -> bad = 1)
+__global__ void kWellDinv(int nw, const int* __restrict__ wdiag, const int* __restrict__ wcPtr, const int* __restrict__ wcSrc,
+                          const double* __restrict__ jval, double* __restrict__ dinv, int* __restrict__ bad) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const double d = wdiag[w] >= 0 ? jval[wdiag[w]] : 0.0;
+    double rowMax = 0;
+    for (int q = wcPtr[w]; q < wcPtr[w + 1]; ++q) rowMax = fmax(rowMax, fabs(jval[wcSrc[q]]));
+    if (!(fabs(d) > 1e-12 * fmax(rowMax, 1e-300)) || !isfinite(d)) { bad[0] = 1; dinv[w] = 0.0; return; }
+    dinv[w] = 1.0 / d;
+}
+// S value t = (J cell entry mapped to t, if any) - sum over its fill terms of (J_cw dinv_w) J_wc, in the reference's
+// order (bsr.cpp:889-907: the cell entries first, then the wells coupled to the row in row order)
+__global__ void kSchurValues(long long nvals, const int* __restrict__ direct, const int* __restrict__ termPtr, const int4* __restrict__ terms,
+                             const double* __restrict__ jval, const double* __restrict__ dinv, double* __restrict__ sval) {
+#pragma clang fp contract(off)
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvals) return;
+    double acc = 0.0;
+    const int d = direct[t];
+    if (d >= 0) acc += jval[d];
+    for (int k = termPtr[t]; k < termPtr[t + 1]; ++k) {
+        const int4 q = terms[k];                 // x: J index of the cell -> well entry, y: well, z: J index of well -> cell
+        const double f = jval[q.x] * dinv[q.y];
+        acc -= f * jval[q.z];
+    }
+    sval[t] = acc;
+}
+// bc_r = b_r - sum_w (J_rw dinv_w) b_w (reduceRhs, bsr.cpp:973-980)
+__global__ void kSchurRhs(int nc, const int* __restrict__ cwPtr, const int* __restrict__ cwWell, const int* __restrict__ cwSrc,
+                          const double* __restrict__ jval, const double* __restrict__ dinv, const double* __restrict__ b, double* __restrict__ bc) {
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nc) return;
+    double s = b[r];
+    for (int p = cwPtr[r]; p < cwPtr[r + 1]; ++p) s -= jval[cwSrc[p]] * dinv[cwWell[p]] * b[nc + cwWell[p]];
+    bc[r] = s;
+}
+// x = (x_c, x_w), x_w = (b_w - sum J_wc x_c) dinv_w (recover, bsr.cpp:982-990)
+__global__ void kSchurRecover(int nc, int nw, const int* __restrict__ wcPtr, const int* __restrict__ wcCol, const int* __restrict__ wcSrc,
+                              const double* __restrict__ jval, const double* __restrict__ dinv, const double* __restrict__ b,
+                              const double* __restrict__ xc, double* __restrict__ x) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) x[i] = xc[i];
+    if (i >= nw) return;
+    double s = b[nc + i];
+    for (int p = wcPtr[i]; p < wcPtr[i + 1]; ++p) s -= jval[wcSrc[p]] * xc[wcCol[p]];
+    x[nc + i] = s * dinv[i];
 }
 
 __global__ void kToFloat(long long n, const double* __restrict__ a, float* __restrict__ b) {
@@ -1780,6 +1835,30 @@ void blockJacobiT(hipStream_t s, int n, const void* dinv, bool f32, const double
 void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32, const double* t, const double* xp, double* z) {
     if (n <= 0) return;
     RSH_NB_DISPATCH(nb, (blockJacobiT<NB>(s, n, dinv, f32, t, xp, z)))
+    checkLaunch();
+}
+void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const double* jval, double* dinv, int* bad) {
+    if (nw <= 0) return;
+    hipLaunchKernelGGL(kWellDinv, dim3((nw + 63) / 64), dim3(64), 0, s, nw, wdiag, wcPtr, wcSrc, jval, dinv, bad);
+    checkLaunch();
+}
+void launchSchurValues(hipStream_t s, long long nvals, const int* direct, const int* termPtr, const int4* terms, const double* jval,
+                       const double* dinv, double* sval) {
+    if (nvals <= 0) return;
+    hipLaunchKernelGGL(kSchurValues, dim3(unsigned((nvals + 255) / 256)), dim3(256), 0, s, nvals, direct, termPtr, terms, jval, dinv, sval);
+    checkLaunch();
+}
+void launchSchurRhs(hipStream_t s, int nc, const int* cwPtr, const int* cwWell, const int* cwSrc, const double* jval, const double* dinv,
+                    const double* b, double* bc) {
+    if (nc <= 0) return;
+    hipLaunchKernelGGL(kSchurRhs, dim3((nc + 255) / 256), dim3(256), 0, s, nc, cwPtr, cwWell, cwSrc, jval, dinv, b, bc);
+    checkLaunch();
+}
+void launchSchurRecover(hipStream_t s, int nc, int nw, const int* wcPtr, const int* wcCol, const int* wcSrc, const double* jval, const double* dinv,
+                        const double* b, const double* xc, double* x) {
+    const int m = std::max(nc, nw);
+    if (m <= 0) return;
+    hipLaunchKernelGGL(kSchurRecover, dim3((m + 255) / 256), dim3(256), 0, s, nc, nw, wcPtr, wcCol, wcSrc, jval, dinv, b, xc, x);
     checkLaunch();
 }
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {

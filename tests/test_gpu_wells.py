@@ -1,0 +1,72 @@
+"""The well Schur reduction on the device (rsh_wells_*): the engine's scalar Jacobian in, cells and BHPs out.  The
+reduced values are the host reduction's (WellSchur::build order of operations, bsr.cpp:733-840) bit for bit, on the
+reference engine's own demo Jacobian (RESSIM_DUMP) and on seeded systems; the solve through it satisfies the full
+scalar system."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import ressimstudio_amd as rs
+from ressimstudio_amd import dump, gen
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-8
+
+
+def _demo():
+    z = np.load(os.path.join(HERE, "golden", "demo_jacobian.npz"))
+    d = {k: (int(z[k]) if z[k].ndim == 0 else z[k]) for k in z.files}
+    d["acc"] = None
+    return d
+
+
+def _generated(seed):
+    s = gen.reservoir(14, 12, 9, 3, model=1, sigma=1.5, kvkh=0.1, wells=3, seed=seed)
+    f = s.full
+    return {"n": f["n"], "na": f["na"], "nv": f["nv"], "nw": f["nw"], "row_ptr": f["row_ptr"], "col": f["col"], "val": f["val"], "rhs": f["rhs"], "acc": None}
+
+
+@pytest.mark.parametrize("which", ["demo", "generated"])
+def test_device_reduction_equals_host(which):
+    d = _demo() if which == "demo" else _generated(31)
+    host = dump.to_system(d)
+    W = rs.Wells(d["row_ptr"], d["col"], d["na"], d["nv"], d["nw"])
+    W.set_values(d["val"])
+    sv = W.reduced_values(host.nnzb, host.nb)
+    assert np.array_equal(sv, np.asarray(host.val).reshape(-1))
+    # solve through the device reduction: the full solution (BHPs included) satisfies J x = b
+    W.setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+    x, its, rel = W.solve(d["rhs"], TOL, 2000)
+    J = sp.csr_matrix((d["val"], d["col"], d["row_ptr"]), shape=(d["n"], d["n"]))
+    assert np.linalg.norm(J @ x - d["rhs"]) <= 1e-6 * np.linalg.norm(d["rhs"])
+    # the host path (host reduction + rsh_solve + host recovery) gives the same solution
+    xh, _, _, ok = rs.System(host).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2).solve(host.rhs, TOL, 2000)
+    assert ok
+    xf = host.recover(d["rhs"], xh)
+    assert np.linalg.norm(x - xf) <= 1e-10 * np.linalg.norm(xf)
+
+
+def test_new_values_same_pattern():
+    d = _generated(32)
+    W = rs.Wells(d["row_ptr"], d["col"], d["na"], d["nv"], d["nw"])
+    v2 = d["val"] * (1.0 + 0.03 * np.sin(np.arange(d["val"].size)))
+    W.set_values(v2)
+    host = gen.from_csr(d["row_ptr"], d["col"], v2, d["rhs"], d["na"], d["nv"], d["nw"])
+    assert np.array_equal(W.reduced_values(host.nnzb, host.nb), np.asarray(host.val).reshape(-1))
+
+
+def test_rejects_a_well_row_without_diagonal():
+    d = _generated(33)
+    v = d["val"].copy()
+    n, nc = d["n"], d["na"] * d["nv"]
+    for r in range(nc, n):           # zero the first well's diagonal
+        for p in range(d["row_ptr"][r], d["row_ptr"][r + 1]):
+            if d["col"][p] == r:
+                v[p] = 0.0
+        break
+    W = rs.Wells(d["row_ptr"], d["col"], d["na"], d["nv"], d["nw"])
+    with pytest.raises(rs.RshError):
+        W.set_values(v)
