@@ -442,12 +442,14 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         rsh_default_options(&o);
         if (opt) o = *opt;
         if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_BJ) return fail(RSH_ERR_ARG, "unknown preconditioner");
-        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR) return fail(RSH_ERR_ARG, "unknown precision");
+        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR && o.precision != RSH_MIXED_AMG)
+            return fail(RSH_ERR_ARG, "unknown precision");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
         rsh_system& S = *sys;
         hipStream_t st = S.s();
         const double t0 = nowMs();
-        const bool f32 = o.precision != RSH_FP64;             // FP32 preconditioner
+        const bool f32 = o.precision == RSH_MIXED || o.precision == RSH_MIXED_IR;   // FP32 ILU factors / block-Jacobi
+        const bool a32 = o.precision != RSH_FP64;                                    // FP32 CPR pressure stage
         if (o.precision == RSH_MIXED_IR && !S.have32) {       // FP32 operator of the inner solves
             uploadSlices(S.spmvHost, S.spmv32, S.nb, st);
             packSlices(st, S.nb, S.val.p, S.spmv32, S.spmvHost.nslots, true);
@@ -472,11 +474,11 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
                 S.colP.alloc(size_t(S.spmvHost.nslots) * S.nb * kWave);
             }
             launchPackColumn0(st, S.nb, S.val.p, S.spmv64.src.p, S.spmvHost.nslots, S.colP.p);
-            if (f32) {
+            if (a32) {
                 S.colP32.alloc(S.colP.n);
                 launchToFloat(st, (long long)S.colP.n, S.colP.p, S.colP32.p);
             }
-            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, f32);
+            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, a32);
         }
         if (!wantIlu) {
             // block-Jacobi second stage: D_b^-1 of the natural blocks (FP32 copy for the mixed precision)
@@ -552,7 +554,7 @@ int rsh_precond_apply(rsh_system* sys, const double* r, double* z) {
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
         RSH_HIP(hipMemcpyAsync(S.vb.p, r, size_t(S.N) * sizeof(double), hipMemcpyHostToDevice, S.s()));
-        S.precondG(S.vb.p, S.vx.p, S.opt.precision != RSH_FP64);
+        S.precondG(S.vb.p, S.vx.p, S.precF32());
         RSH_HIP(hipMemcpyAsync(z, S.vx.p, size_t(S.N) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
         RSH_HIP(hipStreamSynchronize(S.s()));
         return RSH_OK;
@@ -567,7 +569,7 @@ int rsh_amg_apply(rsh_system* sys, const double* r, double* z) {
         rsh_system& S = *sys;
         CprDev& C = *S.cpr;
         RSH_HIP(hipMemcpyAsync(C.lv[0]->b.p, r, size_t(S.n) * sizeof(double), hipMemcpyHostToDevice, S.s()));
-        const double* xp = C.vcycle(S.s(), S.opt.precision != RSH_FP64);
+        const double* xp = C.vcycle(S.s(), S.amgF32(S.precF32()));
         RSH_HIP(hipMemcpyAsync(z, xp, size_t(S.n) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
         RSH_HIP(hipStreamSynchronize(S.s()));
         return RSH_OK;
@@ -636,6 +638,8 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << S.cpr->lv[l]->nnz;
             o << "],\"gs_levels\":[";
             for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << (S.cpr->lv[l]->gsFs.size() - 1);
+            o << "],\"amg_sliced\":[";
+            for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << (S.cpr->lv[l]->plan ? 1 : 0);
             o << "],\"coarse_dense\":" << (S.cpr->dense ? 1 : 0) << ",\"cpr_bytes\":" << S.cpr->bytes();
         }
         o << ",\"last_fallback_fp64\":" << (S.lastFallback ? 1 : 0) << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
@@ -690,7 +694,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
         launchFill(st, S.vb.p, 1.0, S.N);
-        const bool f32 = S.ready && S.opt.precision != RSH_FP64;          // preconditioner precision
+        const bool f32 = S.ready && S.precF32();                          // preconditioner precision
         const bool op32 = S.ready && S.opt.precision == RSH_MIXED_IR;     // operator precision
         const double nb2 = double(S.nb) * S.nb;
         const double pa = f32 ? 4.0 : 8.0;
@@ -703,7 +707,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                     if (!S.ilu || S.opt.precond == RSH_PREC_CPR_BJ) throw Error(RSH_ERR_ARG, "no ILU preconditioner set up");
                     S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st);
                     break;
-                case 3: S.cpr->vcycle(st, f32); break;
+                case 3: S.cpr->vcycle(st, S.amgF32(f32)); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;
                 default: throw Error(RSH_ERR_ARG, "unknown kernel kind");
             }

@@ -181,7 +181,7 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
 void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
                       const KScalars* gate = nullptr);
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
-                       const double* xp, const KScalars* gate = nullptr);
+                       const double* xp, const KScalars* gate = nullptr, bool writeY = true);
 // a run of small levels [l0, l1) swept by one workgroup (level starts on the device); its rows are the working rows
 // [wbase, wbase + wcount), wcount <= kRunRowsMax, perm maps working rows to original rows
 constexpr int kRunRowsMax = 2048;
@@ -225,7 +225,9 @@ void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, co
 void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w);
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);
 void launchFixDiag(hipStream_t s, int n, const int* diag, double* v);
-void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp);
+// r_p = w^T r; with x1 given also level 0's first damped-Jacobi iterate x1 = omega D^-1 r_p (dinv FP32 when f32)
+void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp, const void* dinv = nullptr,
+                       bool f32 = false, double omega = 0, double* x1 = nullptr);
 // colP: the pressure column in the SpMV slice layout, FP64 or (f32) FP32 storage
 void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp,
                      double* t);
@@ -234,15 +236,19 @@ void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, c
                            double* r);
 void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
-// meanAgg: mean aggregate size (sets the lanes per aggregate)
-void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg);
-// fused damped-Jacobi steps: mode 0 x1 = w D^-1 b, r = b - A x1; mode 1 x2 = x + w D^-1 (b - A x), x = x1 + P xc;
-// mode 2 (r in xc) out0 = x1 + w D^-1 r, out1 = b - A out0; mode 3 out0 = x1 + w D^-1 (b - A x1).
+// meanAgg: mean aggregate size (sets the lanes per aggregate); with x1 given also the coarse level's first
+// damped-Jacobi iterate x1 = omega D^-1 bc
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg,
+                    const void* dinv = nullptr, bool f32 = false, double omega = 0, double* x1 = nullptr);
+// x1 = omega D^-1 b
+void launchJacobiInit(hipStream_t s, int n, bool f32, const void* dinv, const double* b, double omega, double* x1);
+// damped-Jacobi steps from the first iterate x1: mode 0 out0 = b - A x1 (residual); mode 1 x = x1 + P xc,
+// out0 = x + w D^-1 (b - A x); mode 2 out0 = x1 + w D^-1 (b - A x1).
 // sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
 // f32: level values (sval / lval / v) and dinv stored in FP32 (vectors and sums FP64)
 void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
                        const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double* out1, double meanRow);
+                       const double* xc, double omega, double* out0, double meanRow);
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv);
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);
 void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y);

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
                                                        const int* __restrict__ lcol, const TA* __restrict__ lval,
                                                        const TA* __restrict__ ldval, TV* __restrict__ y, double* __restrict__ z,
                                                        const double* __restrict__ xp, const int* __restrict__ owner,
-                                                       const int* __restrict__ lowner, const KScalars* __restrict__ gate) {
+                                                       const int* __restrict__ lowner, const KScalars* __restrict__ gate, int writeY) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -370,8 +370,10 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
         for (int f = 0; f < NB; ++f) s += TV(d[(e * NB + f) * dstride]) * acc[f];
         t[e] = s;
     }
+    if (writeY) {   // the last launch of the sweep has no later reader of y
 #pragma unroll
-    for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = t[e];
+        for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = t[e];
+    }
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
         double v = double(t[e]);
@@ -1042,8 +1044,10 @@ __global__ void kFixDiag(int n, const int* __restrict__ diag, double* __restrict
     double& d = v[diag[i]];
     if (fabs(d) < 1e-300) d = 1.0;
 }
-template <int NB>
-__global__ void kCprRestrict(int n, const double* __restrict__ w, const double* __restrict__ r, double* __restrict__ rp) {
+// r_p = w^T r per block row; with x1 given also the first damped-Jacobi iterate of level 0, x1 = omega D^-1 r_p
+template <int NB, typename TD>
+__global__ void kCprRestrict(int n, const double* __restrict__ w, const double* __restrict__ r, double* __restrict__ rp,
+                             const TD* __restrict__ dinv, double omega, double* __restrict__ x1) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1051,6 +1055,7 @@ __global__ void kCprRestrict(int n, const double* __restrict__ w, const double* 
 #pragma unroll
     for (int e = 0; e < NB; ++e) s += w[(long long)i * NB + e] * r[(long long)i * NB + e];
     rp[i] = s;
+    if (x1) x1[i] = omega * double(dinv[i]) * s;
 }
 // second-stage residual t = r - S(:, 0) x_p over the SpMV slices with the pressure column packed (nb values per
 // slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081)
@@ -1131,17 +1136,28 @@ __global__ void kGsLevel(int nrows, const int* __restrict__ rows, const int* __r
     }
     xout[i] = s / v[diag[i]];
 }
-// restriction: the sum of the fine residuals of each aggregate, G lanes per aggregate
-template <int G>
+// restriction: the sum of the fine residuals of each aggregate, G lanes per aggregate; with x1 given also the
+// coarse level's first damped-Jacobi iterate x1 = omega D^-1 b_c
+template <int G, typename TD>
 __global__ __launch_bounds__(256) void kRestrict(int nc, const int* __restrict__ rrp, const int* __restrict__ rci, const double* __restrict__ r,
-                                                 double* __restrict__ bc) {
+                                                 double* __restrict__ bc, const TD* __restrict__ dinv, double omega, double* __restrict__ x1) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int a = int(t / G), sub = int(t % G);
     if (a >= nc) return;
     double s = 0;
     for (int q = rrp[a] + sub; q < rrp[a + 1]; q += G) s += r[rci[q]];
     s = groupSum<G>(s);
-    if (sub == 0) bc[a] = s;
+    if (sub == 0) {
+        bc[a] = s;
+        if (x1) x1[a] = omega * double(dinv[a]) * s;
+    }
+}
+// x1 = omega D^-1 b (the first damped-Jacobi iterate from zero)
+template <typename TD>
+__global__ void kJacobiInit(int n, const TD* __restrict__ dinv, const double* __restrict__ b, double omega, double* __restrict__ x1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x1[i] = omega * double(dinv[i]) * b[i];
 }
 __global__ void kProlongAdd(int n, const int* __restrict__ agg, const double* __restrict__ xc, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1224,33 +1240,38 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
 
 
 // ---------------------------------------------------------------- fused damped-Jacobi V-cycle steps (DeviceCprAmg smoothing)
-// MODE 0 (pre-smoothing from zero + residual): x1 = w D^-1 b, r = b - A x1, with x1_j formed on the fly.
-// MODE 1 (prolongation + post-smoothing): x = x1 + P xc, x2 = x + w D^-1 (b - A x), with x_j formed on the fly.
-// MODE 2 (a further pre-smoothing step + residual, r given in xc): x' = x1 + w D^-1 r, r' = b - A x'.
-// MODE 3 (a further post-smoothing step): x' = x1 + w D^-1 (b - A x1).
+// The first iterate from zero, x1 = w D^-1 b, is written by the restriction that produces b, so every step gathers
+// one vector per entry:
+// MODE 0 (residual): out0 = b - A x1.
+// MODE 1 (prolongation + smoothing step): x = x1 + P xc, out0 = x + w D^-1 (b - A x), with x_j formed on the fly.
+// MODE 2 (smoothing step): out0 = x1 + w D^-1 (b - A x1).
 template <int MODE, typename TD>
-__device__ inline double jacX(int j, double omega, const TD* __restrict__ dinv, const double* __restrict__ b,
-                              const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
-    if (MODE == 0) return omega * double(dinv[j]) * b[j];
+__device__ inline double jacX(int j, const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
     if (MODE == 1) return x1[j] + xc[agg[j]];
-    if (MODE == 2) return x1[j] + omega * double(dinv[j]) * xc[j];
     return x1[j];
 }
+// the row's own operands (dinv_i, b_i, x1_i and the coarse term) do not depend on the row's entries: their loads
+// are issued before the entry loop so that they overlap the gathers
+struct JacSelf { double d, b, x, c; };
 template <int MODE, typename TD>
-__device__ inline void jacFinish(int i, double acc, double omega, const TD* __restrict__ dinv, const double* __restrict__ b,
-                                 const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc,
-                                 double* __restrict__ out0, double* __restrict__ out1) {
+__device__ inline JacSelf jacSelf(int i, const TD* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
+                                  const int* __restrict__ agg, const double* __restrict__ xc) {
+    JacSelf o;
+    o.d = MODE == 0 ? 0.0 : double(dinv[i]);
+    o.b = b[i];
+    o.x = MODE == 0 ? 0.0 : x1[i];
+    o.c = MODE == 1 ? xc[agg[i]] : 0.0;
+    return o;
+}
+template <int MODE>
+__device__ inline void jacFinish(int i, double acc, double omega, const JacSelf& o, double* __restrict__ out0) {
     if (MODE == 0) {
-        out0[i] = omega * double(dinv[i]) * b[i];
-        out1[i] = b[i] - acc;
+        out0[i] = o.b - acc;
     } else if (MODE == 1) {
-        const double xi = x1[i] + xc[agg[i]];
-        out0[i] = xi + omega * double(dinv[i]) * (b[i] - acc);
-    } else if (MODE == 2) {
-        out0[i] = x1[i] + omega * double(dinv[i]) * xc[i];
-        out1[i] = b[i] - acc;
+        const double xi = o.x + o.c;
+        out0[i] = xi + omega * o.d * (o.b - acc);
     } else {
-        out0[i] = x1[i] + omega * double(dinv[i]) * (b[i] - acc);
+        out0[i] = o.x + omega * o.d * (o.b - acc);
     }
 }
 // level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
@@ -1260,28 +1281,31 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
                                                         const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
                                                         const TA* __restrict__ lval, const TA* __restrict__ dinv,
                                                         const double* __restrict__ b, const double* __restrict__ x1, const int* __restrict__ agg,
-                                                        const double* __restrict__ xc, double omega, double* __restrict__ out0,
-                                                        double* __restrict__ out1) {
+                                                        const double* __restrict__ xc, double omega, double* __restrict__ out0) {
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wid < nslices) {
         const int i = row[(long long)wid * kWave + lane];
         if (i < 0) return;
+        const JacSelf self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
         const int2 inf = info[wid];
         double acc = 0;
         for (int k = 0; k < inf.x; ++k) {
             const long long at = ((long long)inf.y + k) * kWave + lane;
-            acc += double(val[at]) * jacX<MODE>(col[at], omega, dinv, b, x1, agg, xc);
+            acc += double(val[at]) * jacX<MODE, TA>(col[at], x1, agg, xc);
         }
-        jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
+        jacFinish<MODE>(i, acc, omega, self, out0);
         return;
     }
     const int q = wid - nslices;
     if (q >= nlong) return;
+    const int i = lrow[q];
+    JacSelf self{};
+    if (lane == 0) self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
     double acc = 0;
-    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += double(lval[p]) * jacX<MODE>(lcol[p], omega, dinv, b, x1, agg, xc);
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += double(lval[p]) * jacX<MODE, TA>(lcol[p], x1, agg, xc);
     acc = waveSum(acc);
-    if (lane == 0) jacFinish<MODE>(lrow[q], acc, omega, dinv, b, x1, agg, xc, out0, out1);
+    if (lane == 0) jacFinish<MODE>(i, acc, omega, self, out0);
 }
 // coarse levels (CSR): G lanes per row (G a power of two near the mean row length), the row's entries strided over
 // the group so their loads are issued together, then a shuffle reduction in a fixed order
@@ -1289,14 +1313,16 @@ template <int MODE, int G, typename TA>
 __global__ __launch_bounds__(256) void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const TA* __restrict__ v,
                                                   const TA* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
                                                   const int* __restrict__ agg, const double* __restrict__ xc, double omega,
-                                                  double* __restrict__ out0, double* __restrict__ out1) {
+                                                  double* __restrict__ out0) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = int(t / G), sub = int(t % G);
     if (i >= n) return;                       // whole groups leave together (n * G is a multiple of G)
+    JacSelf self{};
+    if (sub == 0) self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
     double acc = 0;
-    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += double(v[p]) * jacX<MODE>(ci[p], omega, dinv, b, x1, agg, xc);
+    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += double(v[p]) * jacX<MODE, TA>(ci[p], x1, agg, xc);
     acc = groupSum<G>(acc);
-    if (sub == 0) jacFinish<MODE>(i, acc, omega, dinv, b, x1, agg, xc, out0, out1);
+    if (sub == 0) jacFinish<MODE>(i, acc, omega, self, out0);
 }
 __global__ void kDinv(int n, const int* __restrict__ diag, const double* __restrict__ v, double* __restrict__ dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1656,12 +1682,13 @@ void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q
                        d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y, d.owner.p, d.lowner.p, gate);
 }
 template <int NB, typename TA, typename TV>
-void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp, const KScalars* gate) {
+void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp, const KScalars* gate,
+             bool writeY) {
     const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
     if (!waves) return;
     hipLaunchKernelGGL((kIluBackward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p,
                        d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, y, z, xp, d.owner.p,
-                       d.lowner.p, gate);
+                       d.lowner.p, gate, writeY ? 1 : 0);
 }
 
 template <int NB>
@@ -1678,8 +1705,9 @@ void cprPressureT(hipStream_t s, int n, const int* rp, const double* val, const 
     hipLaunchKernelGGL(kCprPressure<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, rp, val, w, ap);
 }
 template <int NB>
-void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double* rp) {
-    hipLaunchKernelGGL(kCprRestrict<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp);
+void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double* rp, const void* dinv, bool f32, double omega, double* x1) {
+    if (f32) hipLaunchKernelGGL((kCprRestrict<NB, float>), dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp, (const float*)dinv, omega, x1);
+    else hipLaunchKernelGGL((kCprRestrict<NB, double>), dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp, (const double*)dinv, omega, x1);
 }
 template <int NB>
 void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
@@ -1761,9 +1789,9 @@ void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s
     checkLaunch();
 }
 void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp,
-                       const KScalars* gate) {
-    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp, gate)))
-    else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp, gate)))
+                       const KScalars* gate, bool writeY) {
+    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp, gate, writeY)))
+    else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp, gate, writeY)))
     checkLaunch();
 }
 
@@ -1905,8 +1933,10 @@ void launchFixDiag(hipStream_t s, int n, const int* diag, double* v) {
     hipLaunchKernelGGL(kFixDiag, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, v);
     checkLaunch();
 }
-void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp) {
-    RSH_NB_DISPATCH(nb, (cprRestrictT<NB>(s, n, w, r, rp)));
+void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp, const void* dinv, bool f32, double omega,
+                       double* x1) {
+    if (!x1) dinv = nullptr;
+    RSH_NB_DISPATCH(nb, (cprRestrictT<NB>(s, n, w, r, rp, dinv, f32, omega, x1)));
     checkLaunch();
 }
 void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
@@ -1927,15 +1957,18 @@ void launchGsLevel(hipStream_t s, int nrows, const int* rows, const int* rp, con
     checkLaunch();
 }
 template <int G>
-void restrictT(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc) {
+void restrictT(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, const void* dinv, bool f32, double omega,
+               double* x1) {
     const long long th = (long long)nc * G;
-    hipLaunchKernelGGL(kRestrict<G>, dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, nc, rrp, rci, r, bc);
+    const dim3 grid(unsigned((th + 255) / 256));
+    if (f32) hipLaunchKernelGGL((kRestrict<G, float>), grid, dim3(256), 0, s, nc, rrp, rci, r, bc, (const float*)dinv, omega, x1);
+    else hipLaunchKernelGGL((kRestrict<G, double>), grid, dim3(256), 0, s, nc, rrp, rci, r, bc, (const double*)dinv, omega, x1);
 }
 template <int MODE, int G, typename TA>
 void jacobiCsrT(hipStream_t s, int n, const int* rp, const int* ci, const TA* v, const TA* dinv, const double* b, const double* x1,
-                const int* agg, const double* xc, double omega, double* out0, double* out1) {
+                const int* agg, const double* xc, double omega, double* out0) {
     const long long th = (long long)n * G;
-    hipLaunchKernelGGL((kJacobiCsr<MODE, G, TA>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1);
+    hipLaunchKernelGGL((kJacobiCsr<MODE, G, TA>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0);
 }
 // lanes per row for a mean row length
 int groupLanes(double mean) {
@@ -1951,9 +1984,17 @@ int groupLanes(double mean) {
         case 32: { constexpr int G = 32; CALL; } break;          \
         default: { constexpr int G = 64; CALL; } break;          \
     }
-void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg) {
+void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const double* r, double* bc, double meanAgg, const void* dinv,
+                    bool f32, double omega, double* x1) {
     if (nc <= 0) return;
-    RSH_G_DISPATCH(groupLanes(meanAgg), (restrictT<G>(s, nc, rrp, rci, r, bc)))
+    if (!x1) dinv = nullptr;
+    RSH_G_DISPATCH(groupLanes(meanAgg), (restrictT<G>(s, nc, rrp, rci, r, bc, dinv, f32, omega, x1)))
+    checkLaunch();
+}
+void launchJacobiInit(hipStream_t s, int n, bool f32, const void* dinv, const double* b, double omega, double* x1) {
+    if (n <= 0) return;
+    if (f32) hipLaunchKernelGGL(kJacobiInit<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, (const float*)dinv, b, omega, x1);
+    else hipLaunchKernelGGL(kJacobiInit<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, (const double*)dinv, b, omega, x1);
     checkLaunch();
 }
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x) {
@@ -1988,44 +2029,42 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
 
 template <int MODE, typename TA>
 void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, const TA* lval, const TA* dinv, const double* b,
-                   const double* x1, const int* agg, const double* xc, double omega, double* out0, double* out1) {
+                   const double* x1, const int* agg, const double* xc, double omega, double* out0) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
     hipLaunchKernelGGL((kJacobiSliced<MODE, TA>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
-                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0, out1);
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0);
 }
 template <typename TA>
 void jacobiFusedT(hipStream_t s, int mode, const SliceDev<double>* sl, const TA* sval, const TA* lval, int n, const int* rp, const int* ci,
                   const TA* v, const TA* dinv, const double* b, const double* x1, const int* agg, const double* xc, double omega,
-                  double* out0, double* out1, double meanRow) {
+                  double* out0, double meanRow) {
     if (sl) {
         switch (mode) {
-            case 0: jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            case 1: jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            case 2: jacobiSlicedT<2, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
-            default: jacobiSlicedT<3, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0, out1); break;
+            case 0: jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
+            case 1: jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
+            default: jacobiSlicedT<2, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
         }
         return;
     }
     if (n <= 0) return;
     const int g = groupLanes(meanRow);
     switch (mode) {
-        case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-        case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-        case 2: RSH_G_DISPATCH(g, (jacobiCsrT<2, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
-        default: RSH_G_DISPATCH(g, (jacobiCsrT<3, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0, out1))) break;
+        case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
+        case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
+        default: RSH_G_DISPATCH(g, (jacobiCsrT<2, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
     }
 }
 void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
                        const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double* out1, double meanRow) {
-    if (mode < 0 || mode > 3) throw Error(-1, "unknown Jacobi step");
+                       const double* xc, double omega, double* out0, double meanRow) {
+    if (mode < 0 || mode > 2) throw Error(-1, "unknown Jacobi step");
     if (f32)
         jacobiFusedT<float>(s, mode, sl, (const float*)sval, (const float*)lval, n, rp, ci, (const float*)v, (const float*)dinv, b, x1, agg, xc,
-                            omega, out0, out1, meanRow);
+                            omega, out0, meanRow);
     else
         jacobiFusedT<double>(s, mode, sl, (const double*)sval, (const double*)lval, n, rp, ci, (const double*)v, (const double*)dinv, b, x1, agg,
-                             xc, omega, out0, out1, meanRow);
+                             xc, omega, out0, meanRow);
     checkLaunch();
 }
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv) {

@@ -413,7 +413,7 @@ struct IluDev {
             if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
             launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
-                              xp, gate);
+                              xp, gate, k != 0);
         }
     }
     size_t bytes() const {
@@ -435,10 +435,30 @@ struct AmgLevelDev {
     DBuf<int> dgsFs, dgsBs;
     std::vector<LevelRun> runsF, runsB;
     DBuf<double> x1, x2, b, r;
-    DBuf<double> x3, r2;                 // further Jacobi steps (ping-pong)
+    DBuf<double> x3;                     // further Jacobi steps (ping-pong)
     DBuf<double> dinv;                   // 1 / a_ii (Jacobi)
     DBuf<float> v32, dinv32;             // FP32 copies of the level's values and dinv (mixed-precision hierarchy)
     const double* result = nullptr;
+    // the level's values in a row-slice layout for the streamed Jacobi steps: level 0 uses the SpMV layout of S (same
+    // pattern); coarse levels (nullptr) stay CSR with G lanes per row, which keeps more loads in flight on a few
+    // hundred thousand rows than one row per lane does
+    const SliceDev<double>* plan = nullptr;
+    DBuf<double> pS, pL;
+    DBuf<float> pS32, pL32;
+    // repack the slice values from the CSR values v (after every numeric set-up)
+    void packPlan(hipStream_t s, bool f32) {
+        if (!plan) return;
+        pS.alloc(plan->src.n);
+        pL.alloc(plan->lsrc.n);
+        launchPackSlices(s, 1, v.p, plan->src.p, (long long)(pS.n / kWave), pS.p, false);
+        if (plan->lsrc.n) launchPackLong(s, 1, v.p, plan->lsrc.p, int(plan->lsrc.n), pL.p, false);
+        if (f32) {
+            pS32.alloc(pS.n);
+            pL32.alloc(pL.n);
+            launchToFloat(s, (long long)pS.n, pS.p, pS32.p);
+            if (pL.n) launchToFloat(s, (long long)pL.n, pL.p, pL32.p);
+        }
+    }
 };
 
 struct CprDev {
@@ -454,10 +474,8 @@ struct CprDev {
     int smoother = RSH_SMOOTH_GS;
     int steps = 1;                        // Jacobi pre- and post-smoothing steps
     bool built = false;
-    // level 0 values in the SpMV row-slice layout of S (same pattern) for the streamed Jacobi steps
+    // the SpMV row-slice layout of S (same pattern as level 0), for level 0's streamed Jacobi steps
     const SliceDev<double>* plan = nullptr;
-    DBuf<double> apS, apL;
-    DBuf<float> apS32, apL32;             // FP32 copies for the mixed-precision hierarchy
     bool have32 = false;
     std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
 
@@ -519,7 +537,6 @@ struct CprDev {
             L->x1.alloc(size_t(L->n));
             L->x2.alloc(size_t(L->n));
             L->x3.alloc(size_t(L->n));
-            L->r2.alloc(size_t(L->n));
             L->dinv.alloc(size_t(L->n));
             L->b.alloc(size_t(L->n));
             L->r.alloc(size_t(L->n));
@@ -578,21 +595,11 @@ struct CprDev {
         }
         factorCoarsest(s);
         for (auto& L : lv) launchDinv(s, L->n, L->diag, L->v.p, L->dinv.p);
-        if (plan) {
-            apS.alloc(size_t(plan->val.n / (size_t(nb) * nb)));
-            apL.alloc(plan->lsrc.n);
-            launchPackSlices(s, 1, lv[0]->v.p, plan->src.p, (long long)(apS.n / kWave), apS.p, false);
-            if (plan->lsrc.n) launchPackLong(s, 1, lv[0]->v.p, plan->lsrc.p, int(plan->lsrc.n), apL.p, false);
-        }
+        lv[0]->plan = plan;
+        for (auto& L : lv) L->packPlan(s, f32);
         have32 = f32;
         if (f32) {
             // the mixed-precision hierarchy: FP32 level values and dinv (the coarsest inverse stays FP64)
-            if (plan) {
-                apS32.alloc(apS.n);
-                apL32.alloc(apL.n);
-                launchToFloat(s, (long long)apS.n, apS.p, apS32.p);
-                launchToFloat(s, (long long)apL.n, apL.p, apL32.p);
-            }
             for (auto& L : lv) {
                 L->v32.alloc(L->v.n);
                 L->dinv32.alloc(L->dinv.n);
@@ -634,28 +641,30 @@ struct CprDev {
         }
         AmgLevelDev& C = *lv[l + 1];
         if (smoother == RSH_SMOOTH_JACOBI) {
-            // pre-smoothing from zero fused with the residual, prolongation fused with the post-smoothing; further
-            // steps (smooth_steps > 1) ping-pong between the level's spare buffers
-            const SliceDev<double>* sl = (l == 0 && plan) ? plan : nullptr;
+            // the first iterate x1 = w D^-1 b is in L.x1 (written with b); the pre-smoothing steps end with the
+            // residual, the prolongation is fused with the first post-smoothing step; further steps (smooth_steps
+            // > 1) ping-pong between the level's spare buffers
+            const SliceDev<double>* sl = L.plan;
             const double mean = double(L.nnz) / std::max(L.n, 1);
-            const bool h = f32 && have32;
-            const void* sv = h ? static_cast<const void*>(apS32.p) : static_cast<const void*>(apS.p);
-            const void* lvv = h ? static_cast<const void*>(apL32.p) : static_cast<const void*>(apL.p);
+            const bool h = use32(f32);
+            const void* sv = h ? static_cast<const void*>(L.pS32.p) : static_cast<const void*>(L.pS.p);
+            const void* lvv = h ? static_cast<const void*>(L.pL32.p) : static_cast<const void*>(L.pL.p);
             const void* vv = h ? static_cast<const void*>(L.v32.p) : static_cast<const void*>(L.v.p);
-            const void* dv = h ? static_cast<const void*>(L.dinv32.p) : static_cast<const void*>(L.dinv.p);
-            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, nullptr, nullptr, nullptr, kOmega, L.x1.p, L.r.p, mean);
-            double *xa = L.x1.p, *ra = L.r.p, *xb = L.x3.p, *rb = L.r2.p;
+            const void* dv = dinvOf(l, f32);
+            double *xa = L.x1.p, *xb = L.x3.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, ra, kOmega, xb, rb, mean);
+                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, nullptr, kOmega, xb, mean);
                 std::swap(xa, xb);
-                std::swap(ra, rb);
             }
-            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, ra, C.b.p, double(L.n) / std::max(L.nc, 1));
+            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, nullptr, kOmega, L.r.p, mean);
+            const bool smoothedC = l + 2 < lv.size();   // the coarse level is smoothed too: its x1 comes with its b
+            launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1), smoothedC ? dinvOf(l + 1, f32) : nullptr,
+                           h, kOmega, smoothedC ? C.x1.p : nullptr);
             cycle(l + 1, s, f32);
-            launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, nullptr, mean);
+            launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, mean);
             double *pa = L.x2.p, *pb = (xa == L.x1.p) ? L.x3.p : L.x1.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 3, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, nullptr, nullptr, kOmega, pb, nullptr, mean);
+                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, nullptr, nullptr, kOmega, pb, mean);
                 std::swap(pa, pb);
             }
             L.result = pa;
@@ -670,17 +679,25 @@ struct CprDev {
         L.result = L.x2.p;
     }
     void launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out);
-    // one V-cycle on r_p (in lv[0].b) -> result pointer
-    const double* vcycle(hipStream_t s, bool f32 = false) {
+    bool use32(bool f32) const { return f32 && have32; }
+    const void* dinvOf(size_t l, bool f32) const {
+        return use32(f32) ? static_cast<const void*>(lv[l]->dinv32.p) : static_cast<const void*>(lv[l]->dinv.p);
+    }
+    // level 0 is smoothed by damped Jacobi: its first iterate is needed with r_p
+    bool jacobi0() const { return smoother == RSH_SMOOTH_JACOBI && lv.size() > 1; }
+    // one V-cycle on r_p (in lv[0].b) -> result pointer; x1Ready: level 0's first Jacobi iterate was written with r_p
+    const double* vcycle(hipStream_t s, bool f32 = false, bool x1Ready = false) {
+        if (jacobi0() && !x1Ready) launchJacobiInit(s, lv[0]->n, use32(f32), dinvOf(0, f32), lv[0]->b.p, kOmega, lv[0]->x1.p);
         cycle(0, s, f32);
         return lv[0]->result;
     }
     size_t bytes() const {
-        size_t b = w.bytes() + xp.bytes() + t.bytes() + M0.bytes() + M1.bytes() + apS.bytes() + apL.bytes() + apS32.bytes() + apL32.bytes();
+        size_t b = w.bytes() + xp.bytes() + t.bytes() + M0.bytes() + M1.bytes();
         for (auto& L : lv)
             b += L->rpB.bytes() + L->ciB.bytes() + L->diagB.bytes() + L->v.bytes() + L->agg.bytes() + L->rrp.bytes() + L->rci.bytes() +
                  L->apPtr.bytes() + L->apSrc.bytes() + L->cPtr.bytes() + L->cSrc.bytes() + L->ap.bytes() + L->gsF.bytes() + L->gsB.bytes() +
-                 L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->r2.bytes() + L->b.bytes() + L->r.bytes() + L->v32.bytes() + L->dinv32.bytes();
+                 L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->b.bytes() + L->r.bytes() + L->v32.bytes() + L->dinv32.bytes() +
+                 L->pS.bytes() + L->pL.bytes() + L->pS32.bytes() + L->pL32.bytes();
         return b;
     }
 };
@@ -740,6 +757,10 @@ struct rsh_system {
     bool lastFallback = false;          // the last refinement fell back to the FP64 preconditioner
 
     hipStream_t s() const { return ctx->c.stream; }
+    // FP32 ILU factors / block-Jacobi inverses (the preconditioner is FP32 when the solve asks for it)
+    bool precF32() const { return opt.precision == RSH_MIXED || opt.precision == RSH_MIXED_IR; }
+    // FP32-stored CPR pressure stage for a preconditioner apply in the given precision
+    bool amgF32(bool f32) const { return f32 || opt.precision == RSH_MIXED_AMG; }
 
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
 
@@ -766,7 +787,8 @@ struct rsh_system {
         const std::pair<int, int> gen{ilu ? ilu->generation : 0, cpr ? cpr->generation : 0};
         if (gen != graphGen) { clearGraphs(); graphGen = gen; }
         const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z),
-                                         (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5) | (opt.smooth_steps << 7));
+                                         (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5) | (amgF32(f32) ? 1 << 6 : 0) |
+                                             (opt.smooth_steps << 7));
         auto it = graphs.find(key);
         if (it == graphs.end()) {
             hipGraph_t g = nullptr;
@@ -802,9 +824,12 @@ struct rsh_system {
         // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p; DeviceCprAmg
         // (bsr.cpp:1319-1371) ends with the block-Jacobi second stage z = E x_p + D_b^-1 t instead of the BILU0
         CprDev& C = *cpr;
-        launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p);
-        const double* xp = C.vcycle(s(), f32);
-        const bool p32 = f32 && colP32.n > 0;
+        const bool a32 = amgF32(f32);
+        const bool j0 = C.jacobi0();
+        launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p, j0 ? C.dinvOf(0, a32) : nullptr, C.use32(a32), CprDev::kOmega,
+                          j0 ? C.lv[0]->x1.p : nullptr);
+        const double* xp = C.vcycle(s(), a32, j0);
+        const bool p32 = a32 && colP32.n > 0;
         launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p);
         if (pc == RSH_PREC_CPR_BJ) {
             const bool b32 = f32 && bjInv32.n > 0;

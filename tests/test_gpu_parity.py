@@ -27,6 +27,8 @@ def systems():
         "deepwell": gen.reservoir(24, 24, 60, 3, model=1, sigma=2.0, kvkh=0.1, wells=3, seed=9),
         "deepwell_nb2": gen.reservoir(12, 12, 40, 2, model=0, sigma=1.0, wells=2, seed=10),
         "deepwell_nb4": gen.reservoir(10, 10, 48, 4, model=0, sigma=1.0, wells=2, seed=11),
+        # 60,000 cells: the first coarse AMG level is large enough for its own row-slice layout
+        "medium": gen.reservoir(60, 50, 20, 3, model=1, sigma=2.5, kvkh=0.1, wells=4, seed=12),
     }
 
 
@@ -142,7 +144,7 @@ def test_fgmres_matches_oracle_solution(sysd, orc, name, prec):
 
 
 MIXED_MODES = {"fp32prec_bicgstab": (rs.MIXED, rs.KRYLOV_BICGSTAB), "fp32prec_fgmres": (rs.MIXED, rs.KRYLOV_FGMRES),
-               "refine_bicgstab": (rs.MIXED_IR, rs.KRYLOV_BICGSTAB)}
+               "refine_bicgstab": (rs.MIXED_IR, rs.KRYLOV_BICGSTAB), "fp32amg_bicgstab": (rs.MIXED_AMG, rs.KRYLOV_BICGSTAB)}
 
 
 @pytest.mark.parametrize("mode", list(MIXED_MODES))
@@ -264,3 +266,24 @@ def test_hook_adapter(sysd, orc):
     ok2, x2, _ = hook.solve(s, s.rhs, TOL, 500)     # cached system: values re-uploaded, hierarchy refreshed
     assert ok2 and np.array_equal(x, x2)
     assert hook.device_solves == 2 and hook.failures == 0
+
+
+def test_medium_system_jacobi_cpr(sysd, orc):
+    # a 60,000-cell system (five AMG levels): DeviceCprAmg's apply agrees with the float32 oracle to FP32 rounding,
+    # and the Jacobi-smoothed CPR solves with the FP64 and the FP32-stored pressure stage agree with the FP64 oracle
+    # solution
+    s = sysd["medium"]
+    d = rs.System(s).setup(precond="cpr_bj", smoother=rs.SMOOTH_JACOBI)
+    info = d.info()
+    assert info["amg_sliced"][0] == 1 and len(info["amg_levels"]) >= 3
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    z, zo = d.precond(r), orc.devcpr(s).apply(r)
+    assert np.max(np.abs(z - zo)) <= 2e-5 * np.max(np.abs(zo))
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok_o
+    for precision in (rs.FP64, rs.MIXED_AMG):
+        d = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=precision)
+        x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+        assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+        print(f"medium cpr_mc precision {precision}: {its} its")
