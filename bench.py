@@ -244,7 +244,7 @@ def serving_leg(rs, gen, args):
         for lam in lambdas:
             gaps = rng.exponential(1.0 / lam, args.serve_requests)
             picks = rng.integers(0, len(models), args.serve_requests)
-            tickets = []
+            tickets, sent = [], []
             t0 = time.perf_counter()
             due = t0
             for g, k in zip(gaps, picks):
@@ -252,8 +252,9 @@ def serving_leg(rs, gen, args):
                 d = due - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                tickets.append(srv.submit(ids[k], models[k].rhs, perturb=0.05, seed=1000 * int(k) + newton[k], tol=args.tol,
-                                          max_it=args.max_it, want_x=False))
+                seed = 1000 * int(k) + newton[k]
+                tickets.append(srv.submit(ids[k], models[k].rhs, perturb=0.05, seed=seed, tol=args.tol, max_it=args.max_it, want_x=False))
+                sent.append((int(k), seed))
                 newton[k] = 2 + (newton[k] - 1) % 11      # 4-12 Newton iterations per step: cycle through the seeds
             res = [srv.wait(tk) for tk in tickets]
             t1 = time.perf_counter()
@@ -262,6 +263,9 @@ def serving_leg(rs, gen, args):
             row = {"lambda": lam, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                    "mean_service_ms": float(np.mean([r["service_ms"] for r in res])), "throughput_req_s": len(res) / (t1 - t0),
                    "converged": ok, "requests": len(res), "iterations_mean": float(np.mean([r["iterations"] for r in res]))}
+            if ok < len(res):
+                row["not_converged"] = [{"model": k, "seed": sd, "status": r["status"], "rel_res": r["rel_res"]}
+                                        for (k, sd), r in zip(sent, res) if r["status"] != 0]
             rows.append(row)
             log(f"serving {name} lambda {lam:g}/s: p50 {row['p50_ms']:.1f} ms, p99 {row['p99_ms']:.1f} ms, throughput "
                 f"{row['throughput_req_s']:.1f} req/s, {ok}/{len(res)} converged")

@@ -174,6 +174,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     const long long N = S.N;
     S.ensureVec(S.irR);  // residual
     S.ensureVec(S.irD);  // correction
+    S.ensureVec(S.irX);  // the iterate before the correction (a non-finite correction cannot be subtracted again)
     launchFill(st, x, 0.0, N);
     launchCopy(st, S.irR.p, b, N);
     launchDots(st, c, N, 1, b, b, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
@@ -201,6 +202,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
                            : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, precF32, opF32);
         its += inIts;
         if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
+        launchCopy(st, S.irX.p, x, N);
         launchAxpy(st, N, 1.0, S.irD.p, x);
         S.spmv(x, S.irR.p, false);
         launchSub(st, N, b, S.irR.p, S.irR.p);
@@ -209,8 +211,8 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         RSH_HIP(hipStreamSynchronize(st));
         const double rnew = std::sqrt(c.hscal[0]);
         if (!(rnew < rn)) {
-            // no progress (or a non-finite correction): take the step back and restore its residual
-            launchAxpy(st, N, -1.0, S.irD.p, x);
+            // no progress (or a non-finite correction): restore the iterate and its residual
+            launchCopy(st, x, S.irX.p, N);
             S.spmv(x, S.irR.p, false);
             launchSub(st, N, b, S.irR.p, S.irR.p);
             if (precF32 || opF32) {

@@ -35,6 +35,7 @@ struct rsh_batch {
     std::vector<Group> groups;
     long long N = 0, totalVals = 0;
     DBuf<double> x, b, r, rh, p, v, sv, t, ph, sh, tolD, xo, rt, ax, normD;
+    DBuf<double> xPrev, rPrev;    // refinement: iterate and residual before the last correction
     DBuf<int> keepD;
     double* hnorm = nullptr;
     DBuf<int4> task;
@@ -53,6 +54,8 @@ struct rsh_batch {
     std::vector<double> lastRel, lastDoneMs;
     double lastSolveMs = 0, lastSetupMs = 0;
     int lastSweeps = 0, lastOuter = 0;
+    bool prec32 = false;          // the preconditioner precision of the running solve
+    bool lastFallback = false;    // the last refinement fell back to the FP64 preconditioner
 
     hipStream_t s() const { return ctx->c.stream; }
     ~rsh_batch() {
@@ -83,7 +86,7 @@ struct rsh_batch {
     }
     bool f32() const { return opt.precision == RSH_MIXED; }
     void precondAll(const double* in, double* out) {
-        for (auto& g : groups) g.S->precond(in + g.off, out + g.off, f32());
+        for (auto& g : groups) g.S->precond(in + g.off, out + g.off, prec32);
     }
     void spmvAll(const double* in, double* out, bool gated = true) {
         for (auto& g : groups) g.S->spmv(in + g.off, out + g.off, false, gated);
@@ -111,7 +114,7 @@ struct rsh_batch {
         RSH_HIP(hipMemcpyAsync(hks, ks.p, sizeof(KScalars) * size_t(ns), hipMemcpyDeviceToHost, st));
     }
     std::vector<int> currentKey() const {
-        std::vector<int> k{chunk, opt.precision, opt.precond};
+        std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0};
         for (auto& g : groups) k.push_back(g.S->ilu ? g.S->ilu->generation : -1);
         return k;
     }
@@ -195,6 +198,8 @@ struct rsh_batch {
         lastRel.assign(size_t(ns), 0.0);
         lastStatus.assign(size_t(ns), RSH_OK);
         lastOuter = 0;
+        lastFallback = false;
+        prec32 = f32();
         if (!f32()) {
             lastSweeps = bicgstabCore(b.p, tolV, maxIt, t0, lastDoneMs);
             const double now = nowMs() - t0;
@@ -218,14 +223,14 @@ struct rsh_batch {
         std::vector<double> bn, rn;
         norms(b.p, bn);
         rn = bn;
-        std::vector<int> keep(size_t(ns), 0);
+        std::vector<int> keep(size_t(ns), 0), stuck(size_t(ns), 0);
         std::vector<double> itol(size_t(ns), 1.0), innerDone(size_t(ns), 0.0);
         int used = 0;
         for (int outer = 0;; ++outer) {
             const double now = nowMs() - t0;
             int active = 0;
             for (int i = 0; i < ns; ++i) {
-                const bool done = bn[size_t(i)] == 0.0 || rn[size_t(i)] / bn[size_t(i)] <= tolV[size_t(i)];
+                const bool done = bn[size_t(i)] == 0.0 || rn[size_t(i)] / bn[size_t(i)] <= tolV[size_t(i)] || stuck[size_t(i)];
                 keep[size_t(i)] = done ? 0 : 1;
                 if (done && lastDoneMs[size_t(i)] == 0.0) lastDoneMs[size_t(i)] = now;
                 if (!done) {
@@ -246,13 +251,30 @@ struct rsh_batch {
                 lastIts[size_t(i)] += hks[i].it;
                 if (hks[i].state == kBreak) lastStatus[size_t(i)] = RSH_ERR_BREAKDOWN;
             }
+            launchCopy(st, xPrev.p, xo.p, N);
+            launchCopy(st, rPrev.p, rt.p, N);
             launchBAdd(st, T, task.p, x.p, xo.p);
             spmvAll(xo.p, ax.p, false);    // true residual of every system (no gating by the inner solve's states)
             launchSub(st, N, b.p, ax.p, rt.p);
             std::vector<double> rnew;
             norms(rt.p, rnew);
-            for (int i = 0; i < ns; ++i)
-                if (keep[size_t(i)]) rn[size_t(i)] = rnew[size_t(i)];
+            bool stalled = false;
+            for (int i = 0; i < ns; ++i) {
+                if (!keep[size_t(i)]) continue;
+                if (rnew[size_t(i)] < rn[size_t(i)]) { rn[size_t(i)] = rnew[size_t(i)]; continue; }
+                // no progress (or a non-finite correction): this system keeps its previous iterate and residual
+                const Sys& q = sys[size_t(i)];
+                const size_t bytes = sizeof(double) * size_t(q.n) * q.nb;
+                RSH_HIP(hipMemcpyAsync(xo.p + q.off, xPrev.p + q.off, bytes, hipMemcpyDeviceToDevice, st));
+                RSH_HIP(hipMemcpyAsync(rt.p + q.off, rPrev.p + q.off, bytes, hipMemcpyDeviceToDevice, st));
+                if (!prec32) stuck[size_t(i)] = 1;   // stalled with the FP64 preconditioner as well: give up on it
+                stalled = true;
+            }
+            if (stalled && prec32) {
+                // the FP32 preconditioner stalls on some system: the remaining steps use the FP64 one
+                prec32 = false;
+                lastFallback = true;
+            }
         }
         const double now = nowMs() - t0;
         for (int i = 0; i < ns; ++i) {
@@ -519,7 +541,7 @@ int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt) {
         }
         batch->opt = *opt;
         if (opt->precision == RSH_MIXED)
-            for (auto* bf : {&batch->xo, &batch->rt, &batch->ax}) bf->alloc(size_t(batch->N));
+            for (auto* bf : {&batch->xo, &batch->rt, &batch->ax, &batch->xPrev, &batch->rPrev}) bf->alloc(size_t(batch->N));
         if (!batch->tuned) batch->tune();
         RSH_HIP(hipStreamSynchronize(batch->s()));
         batch->ready = true;
@@ -608,7 +630,7 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
             }
             o << "}";
         }
-        o << "],\"last_sweeps\":" << batch->lastSweeps << ",\"last_outer\":" << batch->lastOuter << ",\"last_solve_ms\":" << batch->lastSolveMs << ",\"last_setup_ms\":" << batch->lastSetupMs
+        o << "],\"last_sweeps\":" << batch->lastSweeps << ",\"last_outer\":" << batch->lastOuter << ",\"last_fallback_fp64\":" << (batch->lastFallback ? 1 : 0) << ",\"last_solve_ms\":" << batch->lastSolveMs << ",\"last_setup_ms\":" << batch->lastSetupMs
           << ",\"tuned\":" << (batch->tuned ? 1 : 0) << "}";
         const std::string str = o.str();
         if (int(str.size()) + 1 > cap) return fail(RSH_ERR_ARG, "buffer too small");

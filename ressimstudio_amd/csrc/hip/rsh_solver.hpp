@@ -747,7 +747,7 @@ struct rsh_system {
     std::vector<int> rowSys;            // member of a batch: block row -> system of the batch
     const KScalars* gate = nullptr;     // member of a batch: the systems' states (stopped systems are skipped)
     // Krylov workspace
-    DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w, irR, irD;
+    DBuf<double> vb, vx, r, rh, p, v, sv, t, ph, sh, w, irR, irD, irX;
     DBuf<double> V, Z, hcol, yv;
     int restartAlloc = 0;
     // last solve

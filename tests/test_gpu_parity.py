@@ -287,3 +287,18 @@ def test_medium_system_jacobi_cpr(sysd, orc):
         assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
         print(f"medium cpr_mc precision {precision}: {its} its")
+
+
+def test_mixed_refinement_recovers_from_a_diverged_inner_solve(orc):
+    # a served request that made the FP32-preconditioned inner BiCGSTAB return a non-finite correction: the
+    # refinement keeps the previous iterate, falls back to the FP64 preconditioner and converges
+    import dataclasses
+    m = gen.serving_models(10)[9]
+    pv = rs.perturbed_values(m.val, m.row_ptr, 0.05, 9002)
+    s = dataclasses.replace(m, val=pv.reshape(m.val.shape), full=None, handle=None)
+    d = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=rs.MIXED)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+    info = d.info()
+    print(f"{its} its, {info['last_outer']} refinements, FP64 fallback {info['last_fallback_fp64']}")
+    assert ok and np.all(np.isfinite(x))
+    assert np.linalg.norm(s.rhs - orc.spmv(s, x)) <= 1.01 * TOL * np.linalg.norm(s.rhs)
