@@ -57,10 +57,10 @@ extern "C" {
 #define RSH_MIXED_IR 2            /* FP64 iterative refinement (refineWithDevice, bsr.cpp:1373-1396) around inner
                                      solves on the FP32 matrix and FP32 preconditioner (the float device solve,
                                      bsr.cpp:1130-1215); FP64 true residual, dots and Krylov vectors */
-#define RSH_MIXED_AMG 3           /* FP32-stored CPR pressure stage only (AMG level matrices and smoother diagonals,
-                                     CPR pressure column; FP64 accumulation and vectors) with FP64 BILU0 factors and
-                                     an FP64 Krylov method: the preconditioner stays one fixed linear operator, so no
-                                     refinement is needed */
+#define RSH_MIXED_STORE 3         /* FP32-stored preconditioner (BILU0 factors and dense well-group inverses, AMG level
+                                     matrices and smoother diagonals, CPR pressure column, block-Jacobi inverses) with
+                                     FP64 vectors and FP64 arithmetic inside an FP64 Krylov method: the preconditioner
+                                     stays one fixed linear operator, so no refinement is needed */
 /* AMG smoothers */
 #define RSH_SMOOTH_GS 0           /* level-scheduled Gauss-Seidel: forward pre-, backward post-sweep, the exact
                                      arithmetic of AMG::cycle (amg.cpp:303-329) */
@@ -76,7 +76,7 @@ typedef struct rsh_wells rsh_wells;
 typedef struct rsh_options {
     int precond;        /* RSH_PREC_* (default RSH_PREC_CPR) */
     int krylov;         /* RSH_KRYLOV_* (default BiCGSTAB; with RSH_MIXED_IR the method of the inner solves) */
-    int precision;      /* RSH_FP64, RSH_MIXED, RSH_MIXED_IR or RSH_MIXED_AMG */
+    int precision;      /* RSH_FP64, RSH_MIXED, RSH_MIXED_IR or RSH_MIXED_STORE */
     int smoother;       /* RSH_SMOOTH_* */
     int restart;        /* FGMRES restart length (default 30) */
     int max_outer;      /* refinement steps of RSH_MIXED (BiCGSTAB) and RSH_MIXED_IR (default 10) */

@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libressim_hip.so")
 
 PREC_BILU0, PREC_MCBILU0, PREC_CPR, PREC_CPR_MC, PREC_CPR_BJ = 1, 2, 3, 4, 5
 KRYLOV_BICGSTAB, KRYLOV_FGMRES = 0, 1
-FP64, MIXED, MIXED_IR, MIXED_AMG = 0, 1, 2, 3
+FP64, MIXED, MIXED_IR, MIXED_STORE = 0, 1, 2, 3
 SMOOTH_GS, SMOOTH_JACOBI = 0, 1
 PRECONDITIONERS = {"bilu0": PREC_BILU0, "mcbilu0": PREC_MCBILU0, "cpr": PREC_CPR, "cpr_mc": PREC_CPR_MC, "cpr_bj": PREC_CPR_BJ}
 

@@ -444,14 +444,14 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         rsh_default_options(&o);
         if (opt) o = *opt;
         if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_BJ) return fail(RSH_ERR_ARG, "unknown preconditioner");
-        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR && o.precision != RSH_MIXED_AMG)
+        if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR && o.precision != RSH_MIXED_STORE)
             return fail(RSH_ERR_ARG, "unknown precision");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
         rsh_system& S = *sys;
         hipStream_t st = S.s();
         const double t0 = nowMs();
-        const bool f32 = o.precision == RSH_MIXED || o.precision == RSH_MIXED_IR;   // FP32 ILU factors / block-Jacobi
-        const bool a32 = o.precision != RSH_FP64;                                    // FP32 CPR pressure stage
+        const bool f32 = o.precision != RSH_FP64;             // FP32 copies of the preconditioner's factors and inverses
+        const bool a32 = f32;                                 // ... and of the CPR pressure stage
         if (o.precision == RSH_MIXED_IR && !S.have32) {       // FP32 operator of the inner solves
             uploadSlices(S.spmvHost, S.spmv32, S.nb, st);
             packSlices(st, S.nb, S.val.p, S.spmv32, S.spmvHost.nslots, true);
@@ -699,7 +699,9 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         const bool f32 = S.ready && S.precF32();                          // preconditioner precision
         const bool op32 = S.ready && S.opt.precision == RSH_MIXED_IR;     // operator precision
         const double nb2 = double(S.nb) * S.nb;
-        const double pa = f32 ? 4.0 : 8.0;
+        const int ifp = S.iluFp(f32);
+        const double pa = ifp != kIluFp64 ? 4.0 : 8.0;       // factor bytes per value
+        const double pv = ifp == kIluFp32 ? 4.0 : 8.0;       // working-vector bytes per entry
         double bytes = 0;
         auto once = [&]() {
             switch (kind) {
@@ -707,7 +709,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                 case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
                 case 2:
                     if (!S.ilu || S.opt.precond == RSH_PREC_CPR_BJ) throw Error(RSH_ERR_ARG, "no ILU preconditioner set up");
-                    S.ilu->apply(S.vb.p, S.vx.p, nullptr, f32, st);
+                    S.ilu->apply(S.vb.p, S.vx.p, nullptr, ifp, st);
                     break;
                 case 3: S.cpr->vcycle(st, S.amgF32(f32)); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;
@@ -716,7 +718,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         };
         // algorithmic bytes: values + indices + vectors
         if (kind == 0) bytes = S.nnzb * (nb2 * (op32 ? 4.0 : 8.0) + 4.0) + (S.n + 1) * 4.0 + 2.0 * S.N * 8.0;
-        else if (kind == 2) bytes = S.nnzb * (nb2 * pa + 4.0) + 2.0 * (S.n + 1) * 4.0 + 4.0 * S.N * (f32 ? 4.0 : 8.0);
+        else if (kind == 2) bytes = S.nnzb * (nb2 * pa + 4.0) + 2.0 * (S.n + 1) * 4.0 + 4.0 * S.N * pv;
         else if (kind == 4) bytes = 2.0 * S.N * 8.0;
         once();   // warm-up
         hipEvent_t e0, e1;

@@ -530,8 +530,8 @@ int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt) {
         if (!batch || !opt) return fail(RSH_ERR_ARG, "null argument");
         if (opt->precond != RSH_PREC_BILU0 && opt->precond != RSH_PREC_MCBILU0)
             return fail(RSH_ERR_UNSUPPORTED, "batched solves support the BILU0 and MC-BILU0 preconditioners");
-        if (opt->precision != RSH_FP64 && opt->precision != RSH_MIXED)
-            return fail(RSH_ERR_UNSUPPORTED, "batched solves run FP64 BiCGSTAB with an FP64 or FP32 preconditioner");
+        if (opt->precision != RSH_FP64 && opt->precision != RSH_MIXED && opt->precision != RSH_MIXED_STORE)
+            return fail(RSH_ERR_UNSUPPORTED, "batched solves run FP64 BiCGSTAB with an FP64, FP32 or FP32-stored preconditioner");
         if (opt->krylov != RSH_KRYLOV_BICGSTAB) return fail(RSH_ERR_UNSUPPORTED, "batched solves use BiCGSTAB");
         RSH_HIP(hipSetDevice(batch->ctx->c.device));
         const double t0 = nowMs();

@@ -177,15 +177,18 @@ void launchGatherBlocks(hipStream_t s, int nb, const double* src, const int* map
 // y = A x over a slice layout (all levels at once); TA value type, TV vector type
 void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x,
                       double* y, const KScalars* gate = nullptr);
-// ILU sweeps of one level
-void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
+// precision of an ILU apply: FP64 factors and vectors; FP32 factors and FP32 working vector (FP32 arithmetic);
+// FP32-stored factors with the FP64 working vector and FP64 arithmetic (an exactly linear FP64 operator)
+enum IluFp { kIluFp64 = 0, kIluFp32 = 1, kIluStore32 = 2 };
+// ILU sweeps of one level (fp: IluFp)
+void launchIluForward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
                       const KScalars* gate = nullptr);
-void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
+void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
                        const double* xp, const KScalars* gate = nullptr, bool writeY = true);
 // a run of small levels [l0, l1) swept by one workgroup (level starts on the device); its rows are the working rows
 // [wbase, wbase + wcount), wcount <= kRunRowsMax, perm maps working rows to original rows
 constexpr int kRunRowsMax = 2048;
-void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
+void launchIluFused(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
                     int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp);
 // dense groups of a run: largest group (block rows); the dense product runs one workgroup of kDenseThreads per
 // tile of kWave scalar rows of a group
@@ -193,7 +196,7 @@ constexpr int kDenseMaxRows = 256;
 constexpr int kDenseThreads = 1024;
 // a run swept through its dense groups: outside contributions of the tasks [t0, t0+ns) / long rows [u0, u0+nq) into
 // rt, then per task {group, first scalar row} the dense product into y (and z after the backward sweep)
-void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
+void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
                     const KScalars* gate = nullptr);

@@ -1782,15 +1782,17 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
     RSH_NB_DISPATCH(nb, (spmvT<NB>(s, f32, d64, d32, x, y, gate)));
     checkLaunch();
 }
-void launchIluForward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
+void launchIluForward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
                       const KScalars* gate) {
-    if (f32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y, gate)))
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y, gate)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (double*)y, gate)))
     else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y, gate)))
     checkLaunch();
 }
-void launchIluBackward(hipStream_t s, int nb, bool f32, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp,
+void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp,
                        const KScalars* gate, bool writeY) {
-    if (f32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp, gate, writeY)))
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (float*)y, z, xp, gate, writeY)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluBwdT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, (double*)y, z, xp, gate, writeY)))
     else RSH_NB_DISPATCH(nb, (iluBwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, (double*)y, z, xp, gate, writeY)))
     checkLaunch();
 }
@@ -1812,15 +1814,17 @@ void runDenseT(hipStream_t s, bool fwd, int ntasks, const int2* tasks, const int
     else hipLaunchKernelGGL((kRunDense<NB, TM, TV, false>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp,
                             gowner, gate);
 }
-void launchRunDense(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
+void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner, const KScalars* gate) {
     if (ns + nq > 0) {
-        if (f32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt, gate)))
+        if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt, gate)))
+        else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
         else RSH_NB_DISPATCH(nb, (runOutsideT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
         checkLaunch();
     }
-    if (f32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate)))
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, double>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate)))
     else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate)))
     checkLaunch();
 }
@@ -1895,11 +1899,12 @@ void launchToFloat(hipStream_t s, long long n, const double* a, float* b) {
     checkLaunch();
 }
 
-void launchIluFused(hipStream_t s, int nb, bool f32, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
+void launchIluFused(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
                     int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp) {
     if (l1 <= l0) return;
     if (wcount > kRunRowsMax) throw Error(-1, "fused level run longer than the LDS buffer");
-    if (f32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (float*)y, z, xp)))
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (float*)y, z, xp)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (double*)y, z, xp)))
     else RSH_NB_DISPATCH(nb, (iluFusedT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (double*)y, z, xp)))
     checkLaunch();
 }

@@ -381,11 +381,14 @@ struct IluDev {
         denseU.factor(nb, true, lu.p, f32, s);
         RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
-    void apply(const double* r, double* z, const double* xp, bool f32, hipStream_t s) const {
+    // fp: IluFp (kIluFp32 and kIluStore32 need the FP32 factors of a factor(..., true))
+    void apply(const double* r, double* z, const double* xp, int fp, hipStream_t s) const {
+        const bool f32 = fp != kIluFp64;          // FP32-stored factors
+        const bool v32 = fp == kIluFp32;          // FP32 working vectors
         const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
         const void* Up = f32 ? static_cast<const void*>(&U32) : static_cast<const void*>(&U64);
-        void* y = f32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
-        void* rt = f32 ? static_cast<void*>(rt32.p) : static_cast<void*>(rt64.p);
+        void* y = v32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
+        void* rt = v32 ? static_cast<void*>(rt32.p) : static_cast<void*>(rt64.p);
         auto sweepRun = [&](const LevelRun& run, size_t ri, bool fwd) {
             const DenseSide& D = fwd ? denseL : denseU;
             const LevelSched& sc = fwd ? sL : sU;
@@ -394,25 +397,25 @@ struct IluDev {
             if (D.runTask0[ri] >= 0) {
                 const int t0 = sc.sStart[size_t(run.l0)], u0 = sc.qStart[size_t(run.l0)];
                 const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
-                launchRunDense(s, nb, f32, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
+                launchRunDense(s, nb, fp, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
                                D.runTask1[ri] - D.runTask0[ri], D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r,
                                rt, y, z, xp, D.gowner.p, gate);
                 return;
             }
-            launchIluFused(s, nb, f32, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
+            launchIluFused(s, nb, fp, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
         };
         for (size_t ri = 0; ri < runsL.size(); ++ri) {
             const LevelRun& run = runsL[ri];
             if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
-            launchIluForward(s, nb, f32, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
+            launchIluForward(s, nb, fp, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
                              gate);
         }
         for (size_t k = runsU.size(); k-- > 0;) {
             const LevelRun& run = runsU[k];
             if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
-            launchIluBackward(s, nb, f32, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
+            launchIluBackward(s, nb, fp, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
                               xp, gate, k != 0);
         }
     }
@@ -757,10 +760,12 @@ struct rsh_system {
     bool lastFallback = false;          // the last refinement fell back to the FP64 preconditioner
 
     hipStream_t s() const { return ctx->c.stream; }
-    // FP32 ILU factors / block-Jacobi inverses (the preconditioner is FP32 when the solve asks for it)
+    // the FP32 preconditioner of the mixed modes (asked for by the solve; FP32 ILU arithmetic)
     bool precF32() const { return opt.precision == RSH_MIXED || opt.precision == RSH_MIXED_IR; }
-    // FP32-stored CPR pressure stage for a preconditioner apply in the given precision
-    bool amgF32(bool f32) const { return f32 || opt.precision == RSH_MIXED_AMG; }
+    // FP32-stored CPR pressure stage / block-Jacobi inverses for an apply in the given precision
+    bool amgF32(bool f32) const { return f32 || opt.precision == RSH_MIXED_STORE; }
+    // the ILU apply's IluFp for an apply in the given precision
+    int iluFp(bool f32) const { return f32 ? kIluFp32 : opt.precision == RSH_MIXED_STORE ? kIluStore32 : kIluFp64; }
 
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
 
@@ -818,7 +823,7 @@ struct rsh_system {
     void precond(const double* rr, double* z, bool f32) {
         const int pc = opt.precond;
         if (pc == RSH_PREC_BILU0 || pc == RSH_PREC_MCBILU0) {
-            ilu->apply(rr, z, nullptr, f32, s());
+            ilu->apply(rr, z, nullptr, iluFp(f32), s());
             return;
         }
         // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p; DeviceCprAmg
@@ -832,11 +837,11 @@ struct rsh_system {
         const bool p32 = a32 && colP32.n > 0;
         launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p);
         if (pc == RSH_PREC_CPR_BJ) {
-            const bool b32 = f32 && bjInv32.n > 0;
+            const bool b32 = a32 && bjInv32.n > 0;
             launchBlockJacobi(s(), nb, n, b32 ? static_cast<const void*>(bjInv32.p) : static_cast<const void*>(bjInv.p), b32, C.t.p, xp, z);
             return;
         }
-        ilu->apply(C.t.p, z, xp, f32, s());
+        ilu->apply(C.t.p, z, xp, iluFp(f32), s());
     }
 };
 

@@ -31,7 +31,7 @@ def _true_rel(orc, s, x):
 
 
 @pytest.mark.parametrize("precond", ["mcbilu0", "bilu0"])
-@pytest.mark.parametrize("precision", [rs.FP64, rs.MIXED])
+@pytest.mark.parametrize("precision", [rs.FP64, rs.MIXED, rs.MIXED_STORE])
 def test_batch_matches_oracle_per_system(systems, orc, precond, precision):
     B = rs.Batch(systems).setup(precond=precond, precision=precision)
     xs, its, rel, st = B.solve([s.rhs for s in systems], TOL, 3000)

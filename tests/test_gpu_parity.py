@@ -144,7 +144,7 @@ def test_fgmres_matches_oracle_solution(sysd, orc, name, prec):
 
 
 MIXED_MODES = {"fp32prec_bicgstab": (rs.MIXED, rs.KRYLOV_BICGSTAB), "fp32prec_fgmres": (rs.MIXED, rs.KRYLOV_FGMRES),
-               "refine_bicgstab": (rs.MIXED_IR, rs.KRYLOV_BICGSTAB), "fp32amg_bicgstab": (rs.MIXED_AMG, rs.KRYLOV_BICGSTAB)}
+               "refine_bicgstab": (rs.MIXED_IR, rs.KRYLOV_BICGSTAB), "fp32store_bicgstab": (rs.MIXED_STORE, rs.KRYLOV_BICGSTAB)}
 
 
 @pytest.mark.parametrize("mode", list(MIXED_MODES))
@@ -270,7 +270,7 @@ def test_hook_adapter(sysd, orc):
 
 def test_medium_system_jacobi_cpr(sysd, orc):
     # a 60,000-cell system (five AMG levels): DeviceCprAmg's apply agrees with the float32 oracle to FP32 rounding,
-    # and the Jacobi-smoothed CPR solves with the FP64 and the FP32-stored pressure stage agree with the FP64 oracle
+    # and the Jacobi-smoothed CPR solves with the FP64 and the FP32-stored preconditioner agree with the FP64 oracle
     # solution
     s = sysd["medium"]
     d = rs.System(s).setup(precond="cpr_bj", smoother=rs.SMOOTH_JACOBI)
@@ -281,7 +281,7 @@ def test_medium_system_jacobi_cpr(sysd, orc):
     assert np.max(np.abs(z - zo)) <= 2e-5 * np.max(np.abs(zo))
     xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
     assert ok_o
-    for precision in (rs.FP64, rs.MIXED_AMG):
+    for precision in (rs.FP64, rs.MIXED_STORE):
         d = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=precision)
         x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
         assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
@@ -302,3 +302,22 @@ def test_mixed_refinement_recovers_from_a_diverged_inner_solve(orc):
     print(f"{its} its, {info['last_outer']} refinements, FP64 fallback {info['last_fallback_fp64']}")
     assert ok and np.all(np.isfinite(x))
     assert np.linalg.norm(s.rhs - orc.spmv(s, x)) <= 1.01 * TOL * np.linalg.norm(s.rhs)
+
+
+@pytest.mark.parametrize("prec", ["bilu0", "mcbilu0", "cpr_mc", "cpr_bj"])
+@pytest.mark.parametrize("name", ["layered", "deepwell", "nb4"])
+def test_fp32_storage_preconditioner_is_linear(sysd, name, prec):
+    # RSH_MIXED_STORE keeps every preconditioner array in FP32 but computes in FP64 with FP64 vectors: the apply is an
+    # exactly linear operator (to FP64 rounding), unlike FP32 arithmetic, and stays close to the FP64 apply
+    s = sysd[name]
+    kw = dict(precond=prec, smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+    d = rs.System(s).setup(precision=rs.MIXED_STORE, **kw)
+    d64 = rs.System(s).setup(precision=rs.FP64, **kw)
+    rng = np.random.default_rng(3)
+    r1, r2 = rng.standard_normal(s.n * s.nb), rng.standard_normal(s.n * s.nb)
+    z1, z2, z12 = d.precond(r1), d.precond(r2), d.precond(2.0 * r1 - 3.0 * r2)
+    lin = np.linalg.norm(z12 - (2.0 * z1 - 3.0 * z2)) / np.linalg.norm(z12)
+    near = np.linalg.norm(z1 - d64.precond(r1)) / np.linalg.norm(z1)
+    print(f"{name}/{prec}: linearity {lin:.1e}, FP32 storage vs FP64 {near:.1e}")
+    assert lin <= 1e-10
+    assert near <= 1e-3
