@@ -232,7 +232,7 @@ def serving_leg(rs, gen, args):
     out = {"workload": f"{len(models)} resident models, 5k-500k cells log-uniform ({cells} cells), nb 3, CPR(Jacobi AMG)+MC-BILU0 "
                        f"BiCGSTAB to 1e-8, block rows scaled by +-5 % per request", "models": len(models), "cells": cells,
            "slots": args.serve_slots, "requests_per_rate": args.serve_requests, "hbm_budget_GB": 200}
-    for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED)):
+    for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED), ("store32", rs.MIXED_STORE)):
         srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9)
         ids = [srv.add_model(m, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=prec) for m in models]
         t = time.perf_counter()

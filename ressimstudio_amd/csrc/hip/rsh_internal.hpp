@@ -245,17 +245,19 @@ void launchRestrict(hipStream_t s, int nc, const int* rrp, const int* rci, const
                     const void* dinv = nullptr, bool f32 = false, double omega = 0, double* x1 = nullptr);
 // x1 = omega D^-1 b
 void launchJacobiInit(hipStream_t s, int n, bool f32, const void* dinv, const double* b, double omega, double* x1);
-// damped-Jacobi steps from the first iterate x1: mode 0 out0 = b - A x1 (residual); mode 1 x = x1 + P xc,
-// out0 = x + w D^-1 (b - A x); mode 2 out0 = x1 + w D^-1 (b - A x1).
+// damped-Jacobi steps from the first iterate x1: mode 0 out0 = b - A x1 (residual); mode 1 out0 = x1 + w D^-1 (b - A x1).
 // sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
-// f32: level values (sval / lval / v) and dinv stored in FP32 (vectors and sums FP64)
+// f32: level values (sval / lval / v) and dinv stored in FP32 (vectors and sums FP64).  pfine (CSR levels): also
+// pfine[j] += out0[a] for the members j of aggregate a (prp / pci), the prolongation into the finer level's iterate
 void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
-                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double meanRow);
+                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, double omega, double* out0,
+                       double meanRow, const int* prp = nullptr, const int* pci = nullptr, double* pfine = nullptr);
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv);
 void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, double* x);
 void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y);
-void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y);
+// with pfine given, also pfine[j] += y[a] for the members j of aggregate a (prp / pci)
+void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y, const int* prp = nullptr,
+                      const int* pci = nullptr, double* pfine = nullptr);
 void launchGalerkin(hipStream_t s, int nap, const int* apPtr, const int* apSrc, const double* fine, double* ap, int nnzc,
                     const int* cPtr, const int* cSrc, double* coarse);
 void launchDenseFromCsr(hipStream_t s, int m, const int* rp, const int* ci, const double* v, double* M);

@@ -1164,16 +1164,23 @@ __global__ void kProlongAdd(int n, const int* __restrict__ agg, const double* __
     if (i >= n) return;
     x[i] = x[i] + xc[agg[i]];
 }
-// y = M x for a dense m x ld row-major matrix (the explicit coarsest inverse): one wavefront per row
+// y = M x for a dense m x ld row-major matrix (the explicit coarsest inverse): one wavefront per row; with pfine
+// given, also the prolongation into the finer level's iterate (pfine at the members prp / pci of aggregate row)
 __global__ __launch_bounds__(kBlock) void kDenseGemv(int m, int ld, const double* __restrict__ M, const double* __restrict__ x,
-                                                     double* __restrict__ y) {
+                                                     double* __restrict__ y, const int* __restrict__ prp, const int* __restrict__ pci,
+                                                     double* __restrict__ pfine) {
     const int rowi = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (rowi >= m) return;
     double s = 0;
     for (int j = lane; j < m; j += kWave) s += M[(long long)rowi * ld + j] * x[j];
-    s = waveSum(s);
+    s = waveSum(s);                            // every lane holds the sum
     if (lane == 0) y[rowi] = s;
+    if (pfine)
+        for (int q = prp[rowi] + lane; q < prp[rowi + 1]; q += kWave) {
+            const int j = pci[q];
+            pfine[j] = pfine[j] + s;
+        }
 }
 __global__ void kGalerkinSum(int count, const int* __restrict__ ptr, const int* __restrict__ srcIdx, const double* __restrict__ src,
                              double* __restrict__ dst) {
@@ -1240,39 +1247,25 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
 
 
 // ---------------------------------------------------------------- fused damped-Jacobi V-cycle steps (DeviceCprAmg smoothing)
-// The first iterate from zero, x1 = w D^-1 b, is written by the restriction that produces b, so every step gathers
-// one vector per entry:
+// The first iterate from zero, x1 = w D^-1 b, is written by the restriction that produces b, and the prolongation
+// x += P x_c by the coarse level's last kernel, so every step gathers one vector per entry:
 // MODE 0 (residual): out0 = b - A x1.
-// MODE 1 (prolongation + smoothing step): x = x1 + P xc, out0 = x + w D^-1 (b - A x), with x_j formed on the fly.
-// MODE 2 (smoothing step): out0 = x1 + w D^-1 (b - A x1).
+// MODE 1 (smoothing step): out0 = x1 + w D^-1 (b - A x1).
+// the row's own operands do not depend on the row's entries: their loads are issued before the entry loop so that
+// they overlap the gathers
+struct JacSelf { double d, b, x; };
 template <int MODE, typename TD>
-__device__ inline double jacX(int j, const double* __restrict__ x1, const int* __restrict__ agg, const double* __restrict__ xc) {
-    if (MODE == 1) return x1[j] + xc[agg[j]];
-    return x1[j];
-}
-// the row's own operands (dinv_i, b_i, x1_i and the coarse term) do not depend on the row's entries: their loads
-// are issued before the entry loop so that they overlap the gathers
-struct JacSelf { double d, b, x, c; };
-template <int MODE, typename TD>
-__device__ inline JacSelf jacSelf(int i, const TD* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
-                                  const int* __restrict__ agg, const double* __restrict__ xc) {
+__device__ inline JacSelf jacSelf(int i, const TD* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1) {
     JacSelf o;
     o.d = MODE == 0 ? 0.0 : double(dinv[i]);
     o.b = b[i];
     o.x = MODE == 0 ? 0.0 : x1[i];
-    o.c = MODE == 1 ? xc[agg[i]] : 0.0;
     return o;
 }
 template <int MODE>
-__device__ inline void jacFinish(int i, double acc, double omega, const JacSelf& o, double* __restrict__ out0) {
-    if (MODE == 0) {
-        out0[i] = o.b - acc;
-    } else if (MODE == 1) {
-        const double xi = o.x + o.c;
-        out0[i] = xi + omega * o.d * (o.b - acc);
-    } else {
-        out0[i] = o.x + omega * o.d * (o.b - acc);
-    }
+__device__ inline double jacValue(double acc, double omega, const JacSelf& o) {
+    if (MODE == 0) return o.b - acc;
+    return o.x + omega * o.d * (o.b - acc);
 }
 // level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
 template <int MODE, typename TA>
@@ -1280,49 +1273,56 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
                                                         const int* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                         const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
                                                         const TA* __restrict__ lval, const TA* __restrict__ dinv,
-                                                        const double* __restrict__ b, const double* __restrict__ x1, const int* __restrict__ agg,
-                                                        const double* __restrict__ xc, double omega, double* __restrict__ out0) {
+                                                        const double* __restrict__ b, const double* __restrict__ x1, double omega,
+                                                        double* __restrict__ out0) {
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wid < nslices) {
         const int i = row[(long long)wid * kWave + lane];
         if (i < 0) return;
-        const JacSelf self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
+        const JacSelf self = jacSelf<MODE>(i, dinv, b, x1);
         const int2 inf = info[wid];
         double acc = 0;
         for (int k = 0; k < inf.x; ++k) {
             const long long at = ((long long)inf.y + k) * kWave + lane;
-            acc += double(val[at]) * jacX<MODE, TA>(col[at], x1, agg, xc);
+            acc += double(val[at]) * x1[col[at]];
         }
-        jacFinish<MODE>(i, acc, omega, self, out0);
+        out0[i] = jacValue<MODE>(acc, omega, self);
         return;
     }
     const int q = wid - nslices;
     if (q >= nlong) return;
     const int i = lrow[q];
     JacSelf self{};
-    if (lane == 0) self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
+    if (lane == 0) self = jacSelf<MODE>(i, dinv, b, x1);
     double acc = 0;
-    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += double(lval[p]) * jacX<MODE, TA>(lcol[p], x1, agg, xc);
+    for (int p = lptr[q] + lane; p < lptr[q + 1]; p += kWave) acc += double(lval[p]) * x1[lcol[p]];
     acc = waveSum(acc);
-    if (lane == 0) jacFinish<MODE>(i, acc, omega, self, out0);
+    if (lane == 0) out0[i] = jacValue<MODE>(acc, omega, self);
 }
 // coarse levels (CSR): G lanes per row (G a power of two near the mean row length), the row's entries strided over
-// the group so their loads are issued together, then a shuffle reduction in a fixed order
+// the group so their loads are issued together, then a shuffle reduction in a fixed order.  With pfine given (the
+// last post-smoothing step of a coarse level) the group also adds the new value to the finer level's iterate at
+// the members of its aggregate (prp / pci): the prolongation x_f += P x_c, every fine row written by one group.
 template <int MODE, int G, typename TA>
 __global__ __launch_bounds__(256) void kJacobiCsr(int n, const int* __restrict__ rp, const int* __restrict__ ci, const TA* __restrict__ v,
                                                   const TA* __restrict__ dinv, const double* __restrict__ b, const double* __restrict__ x1,
-                                                  const int* __restrict__ agg, const double* __restrict__ xc, double omega,
-                                                  double* __restrict__ out0) {
+                                                  double omega, double* __restrict__ out0, const int* __restrict__ prp,
+                                                  const int* __restrict__ pci, double* __restrict__ pfine) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = int(t / G), sub = int(t % G);
     if (i >= n) return;                       // whole groups leave together (n * G is a multiple of G)
-    JacSelf self{};
-    if (sub == 0) self = jacSelf<MODE>(i, dinv, b, x1, agg, xc);
+    const JacSelf self = jacSelf<MODE>(i, dinv, b, x1);
     double acc = 0;
-    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += double(v[p]) * jacX<MODE, TA>(ci[p], x1, agg, xc);
-    acc = groupSum<G>(acc);
-    if (sub == 0) jacFinish<MODE>(i, acc, omega, self, out0);
+    for (int p = rp[i] + sub; p < rp[i + 1]; p += G) acc += double(v[p]) * x1[ci[p]];
+    acc = groupSum<G>(acc);                   // every lane of the group holds the sum
+    const double val = jacValue<MODE>(acc, omega, self);
+    if (sub == 0) out0[i] = val;
+    if (pfine)
+        for (int q = prp[i] + sub; q < prp[i + 1]; q += G) {
+            const int j = pci[q];
+            pfine[j] = pfine[j] + val;
+        }
 }
 __global__ void kDinv(int n, const int* __restrict__ diag, const double* __restrict__ v, double* __restrict__ dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1971,9 +1971,10 @@ void restrictT(hipStream_t s, int nc, const int* rrp, const int* rci, const doub
 }
 template <int MODE, int G, typename TA>
 void jacobiCsrT(hipStream_t s, int n, const int* rp, const int* ci, const TA* v, const TA* dinv, const double* b, const double* x1,
-                const int* agg, const double* xc, double omega, double* out0) {
+                double omega, double* out0, const int* prp, const int* pci, double* pfine) {
     const long long th = (long long)n * G;
-    hipLaunchKernelGGL((kJacobiCsr<MODE, G, TA>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0);
+    hipLaunchKernelGGL((kJacobiCsr<MODE, G, TA>), dim3(unsigned((th + 255) / 256)), dim3(256), 0, s, n, rp, ci, v, dinv, b, x1, omega, out0, prp,
+                       pci, pfine);
 }
 // lanes per row for a mean row length
 int groupLanes(double mean) {
@@ -2008,9 +2009,9 @@ void launchProlongAdd(hipStream_t s, int n, const int* agg, const double* xc, do
     checkLaunch();
 }
 void launchDenseGemv(hipStream_t s, int m, const double* M, const double* x, double* y) { launchDenseGemv2(s, m, m, M, x, y); }
-void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y) {
+void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const double* x, double* y, const int* prp, const int* pci, double* pfine) {
     if (m <= 0) return;
-    hipLaunchKernelGGL(kDenseGemv, dim3((m + 3) / 4), dim3(kBlock), 0, s, m, ld, M, x, y);
+    hipLaunchKernelGGL(kDenseGemv, dim3((m + 3) / 4), dim3(kBlock), 0, s, m, ld, M, x, y, prp, pci, pfine);
     checkLaunch();
 }
 void launchGalerkin(hipStream_t s, int nap, const int* apPtr, const int* apSrc, const double* fine, double* ap, int nnzc, const int* cPtr,
@@ -2034,42 +2035,37 @@ void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, doubl
 
 template <int MODE, typename TA>
 void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, const TA* lval, const TA* dinv, const double* b,
-                   const double* x1, const int* agg, const double* xc, double omega, double* out0) {
+                   const double* x1, double omega, double* out0) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
     hipLaunchKernelGGL((kJacobiSliced<MODE, TA>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
-                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, agg, xc, omega, out0);
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
 }
 template <typename TA>
 void jacobiFusedT(hipStream_t s, int mode, const SliceDev<double>* sl, const TA* sval, const TA* lval, int n, const int* rp, const int* ci,
-                  const TA* v, const TA* dinv, const double* b, const double* x1, const int* agg, const double* xc, double omega,
-                  double* out0, double meanRow) {
+                  const TA* v, const TA* dinv, const double* b, const double* x1, double omega, double* out0, double meanRow,
+                  const int* prp, const int* pci, double* pfine) {
     if (sl) {
-        switch (mode) {
-            case 0: jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
-            case 1: jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
-            default: jacobiSlicedT<2, TA>(s, sl, sval, lval, dinv, b, x1, agg, xc, omega, out0); break;
-        }
+        if (pfine) throw Error(-1, "the sliced level-0 Jacobi step has no coarser level to prolong from");
+        if (mode == 0) jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+        else jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
         return;
     }
     if (n <= 0) return;
     const int g = groupLanes(meanRow);
-    switch (mode) {
-        case 0: RSH_G_DISPATCH(g, (jacobiCsrT<0, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
-        case 1: RSH_G_DISPATCH(g, (jacobiCsrT<1, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
-        default: RSH_G_DISPATCH(g, (jacobiCsrT<2, G, TA>(s, n, rp, ci, v, dinv, b, x1, agg, xc, omega, out0))) break;
-    }
+    if (mode == 0) RSH_G_DISPATCH(g, (jacobiCsrT<0, G, TA>(s, n, rp, ci, v, dinv, b, x1, omega, out0, prp, pci, pfine)))
+    else RSH_G_DISPATCH(g, (jacobiCsrT<1, G, TA>(s, n, rp, ci, v, dinv, b, x1, omega, out0, prp, pci, pfine)))
 }
 void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
-                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, const int* agg,
-                       const double* xc, double omega, double* out0, double meanRow) {
-    if (mode < 0 || mode > 2) throw Error(-1, "unknown Jacobi step");
+                       const int* ci, const void* v, const void* dinv, const double* b, const double* x1, double omega, double* out0,
+                       double meanRow, const int* prp, const int* pci, double* pfine) {
+    if (mode < 0 || mode > 1) throw Error(-1, "unknown Jacobi step");
     if (f32)
-        jacobiFusedT<float>(s, mode, sl, (const float*)sval, (const float*)lval, n, rp, ci, (const float*)v, (const float*)dinv, b, x1, agg, xc,
-                            omega, out0, meanRow);
+        jacobiFusedT<float>(s, mode, sl, (const float*)sval, (const float*)lval, n, rp, ci, (const float*)v, (const float*)dinv, b, x1, omega, out0,
+                            meanRow, prp, pci, pfine);
     else
-        jacobiFusedT<double>(s, mode, sl, (const double*)sval, (const double*)lval, n, rp, ci, (const double*)v, (const double*)dinv, b, x1, agg,
-                             xc, omega, out0, meanRow);
+        jacobiFusedT<double>(s, mode, sl, (const double*)sval, (const double*)lval, n, rp, ci, (const double*)v, (const double*)dinv, b, x1, omega,
+                             out0, meanRow, prp, pci, pfine);
     checkLaunch();
 }
 void launchDinv(hipStream_t s, int n, const int* diag, const double* v, double* dinv) {

@@ -623,12 +623,17 @@ struct CprDev {
             launchGsLevel(s, st[size_t(q) + 1] - st[size_t(q)], rows + st[size_t(q)], L.rp, L.ci, L.diag, L.v.p, L.b.p, xin, xout, fwd);
         }
     }
-    void cycle(size_t l, hipStream_t s, bool f32 = false) {
+    // V-cycle from level l.  pfine (Jacobi smoothing): the finer level's iterate, to which this level's last kernel
+    // adds P x_l (the prolongation); returns whether it did (else the caller prolongs)
+    bool cycle(size_t l, hipStream_t s, bool f32 = false, double* pfine = nullptr) {
         AmgLevelDev& L = *lv[l];
+        const int* prp = pfine ? lv[l - 1]->rrp.p : nullptr;   // aggregate members in the finer level
+        const int* pci = pfine ? lv[l - 1]->rci.p : nullptr;
         if (l + 1 == lv.size()) {
             if (dense) {
-                launchDenseGemvLd(s, L, L.b.p, L.x1.p);
+                launchDenseGemv2(s, m, 2 * m, Minv + m, L.b.p, L.x1.p, prp, pci, pfine);
                 L.result = L.x1.p;
+                return pfine != nullptr;
             } else {
                 // 20 symmetric Gauss-Seidel sweeps from zero (amg.cpp:315-316)
                 launchFill(s, L.x1.p, 0.0, L.n);
@@ -640,13 +645,13 @@ struct CprDev {
                 }
                 L.result = a;
             }
-            return;
+            return false;
         }
         AmgLevelDev& C = *lv[l + 1];
         if (smoother == RSH_SMOOTH_JACOBI) {
             // the first iterate x1 = w D^-1 b is in L.x1 (written with b); the pre-smoothing steps end with the
-            // residual, the prolongation is fused with the first post-smoothing step; further steps (smooth_steps
-            // > 1) ping-pong between the level's spare buffers
+            // residual; the coarse level's last kernel adds P x_c to the pre-smoothed iterate; the post-smoothing
+            // steps ping-pong between it and L.x2, the last one adding P x_l to the finer level's iterate
             const SliceDev<double>* sl = L.plan;
             const double mean = double(L.nnz) / std::max(L.n, 1);
             const bool h = use32(f32);
@@ -656,22 +661,23 @@ struct CprDev {
             const void* dv = dinvOf(l, f32);
             double *xa = L.x1.p, *xb = L.x3.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, nullptr, kOmega, xb, mean);
+                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, kOmega, xb, mean);
                 std::swap(xa, xb);
             }
-            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, nullptr, nullptr, kOmega, L.r.p, mean);
+            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, kOmega, L.r.p, mean);
             const bool smoothedC = l + 2 < lv.size();   // the coarse level is smoothed too: its x1 comes with its b
             launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1), smoothedC ? dinvOf(l + 1, f32) : nullptr,
                            h, kOmega, smoothedC ? C.x1.p : nullptr);
-            cycle(l + 1, s, f32);
-            launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, L.agg.p, C.result, kOmega, L.x2.p, mean);
-            double *pa = L.x2.p, *pb = (xa == L.x1.p) ? L.x3.p : L.x1.p;
-            for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 2, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, nullptr, nullptr, kOmega, pb, mean);
+            if (!cycle(l + 1, s, f32, xa)) launchProlongAdd(s, L.n, L.agg.p, C.result, xa);
+            double *pa = xa, *pb = L.x2.p;
+            for (int k = 0; k < steps; ++k) {
+                const bool last = k + 1 == steps;
+                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, kOmega, pb, mean, last ? prp : nullptr,
+                                  last ? pci : nullptr, last ? pfine : nullptr);
                 std::swap(pa, pb);
             }
             L.result = pa;
-            return;
+            return pfine != nullptr && !sl;
         }
         gsSweep(s, L, nullptr, L.x1.p, true);
         launchCsrSpmvResidual(s, L.n, L.rp, L.ci, L.v.p, L.x1.p, L.b.p, L.r.p);
@@ -680,8 +686,8 @@ struct CprDev {
         launchProlongAdd(s, L.n, L.agg.p, C.result, L.x1.p);
         gsSweep(s, L, L.x1.p, L.x2.p, false);
         L.result = L.x2.p;
+        return false;
     }
-    void launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out);
     bool use32(bool f32) const { return f32 && have32; }
     const void* dinvOf(size_t l, bool f32) const {
         return use32(f32) ? static_cast<const void*>(lv[l]->dinv32.p) : static_cast<const void*>(lv[l]->dinv.p);
@@ -704,12 +710,6 @@ struct CprDev {
         return b;
     }
 };
-
-// dense y = Minv b with the inverse in the right half of [I | C^-1] (leading dimension 2m)
-inline void CprDev::launchDenseGemvLd(hipStream_t s, AmgLevelDev& L, const double* bv, double* out) {
-    launchDenseGemv2(s, m, 2 * m, Minv + m, bv, out);
-    (void)L;
-}
 
 inline double nowMs() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
