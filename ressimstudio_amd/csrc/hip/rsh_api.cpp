@@ -631,7 +631,8 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             for (size_t i = 0; i < S.ilu->runsL.size(); ++i) { fusedL += S.ilu->runsL[i].fused; denseRunsL += S.ilu->denseL.runTask0[i] >= 0; }
             for (size_t i = 0; i < S.ilu->runsU.size(); ++i) { fusedU += S.ilu->runsU[i].fused; denseRunsU += S.ilu->denseU.runTask0[i] >= 0; }
             o << ",\"ilu_fused_runs\":[" << fusedL << "," << fusedU << "],\"ilu_dense_runs\":[" << denseRunsL << "," << denseRunsU
-              << "],\"ilu_dense_groups\":[" << S.ilu->denseL.ngroups << "," << S.ilu->denseU.ngroups << "]";
+              << "],\"ilu_dense_groups\":[" << S.ilu->denseL.ngroups << "," << S.ilu->denseU.ngroups << "],\"ilu_factor_groups\":"
+              << (S.ilu->fgStart.n ? S.ilu->fgStart.n - 1 : 0);
         }
         if (S.cpr) {
             o << ",\"amg_levels\":[";

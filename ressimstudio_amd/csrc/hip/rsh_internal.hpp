@@ -221,9 +221,14 @@ void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* 
                    const double* b, const double* xin, double* xout, bool forward);
 // BILU0 elimination of long rows rows[r0..r1): one wavefront per row
 void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg);
-// BILU0 numeric factorisation of the rows of one level (in place, natural layout)
+// BILU0 numeric factorisation of the rows of one level (in place, natural layout); stop >= 0: only the lower entries
+// in columns below stop, and the pivot is left for kIluGroupFactor
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag,
-                      double* val, int* nreg);
+                      double* val, int* nreg, int stop = -1);
+// right-looking elimination of the groups [g0, g1) (members grows[gstart[g] .. gstart[g+1]) in ascending row order,
+// member-pair block table gidx at gidxOff[g]), one workgroup per group
+void launchIluGroupFactor(hipStream_t s, int nb, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
+                          const int* diag, double* val, int* nreg);
 // CPR kernels
 void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w);
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);

@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
 // earlier levels.
 template <int NB>
 __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
-                            const int* __restrict__ diag, double* __restrict__ val, int* __restrict__ nreg) {
+                            const int* __restrict__ diag, double* __restrict__ val, int* __restrict__ nreg, int stop) {
 #pragma clang fp contract(off)
     const int idx = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= r1) return;
@@ -398,6 +398,7 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
     constexpr int B2 = NB * NB;
     for (int p = rp[i]; p < pd; ++p) {
         const int k = ci[p];
+        if (stop >= 0 && k >= stop) break;   // the rest of the row is eliminated by kIluGroupFactor
         double* ap = val + (long long)p * B2;
         const double* dk = val + (long long)diag[k] * B2;
         double L[B2];
@@ -442,6 +443,7 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
             }
         }
     }
+    if (stop >= 0) return;
     double d[B2];
     double* dp = val + (long long)pd * B2;
 #pragma unroll
@@ -450,6 +452,87 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
 #pragma unroll
     for (int e = 0; e < B2; ++e) dp[e] = d[e];
     if (reg && nreg) atomicAdd(nreg, 1);
+}
+
+// The rows of a group coupled inside a run of small levels (a well's perforated cells), once the contributions
+// of the rows before the run are in (kBilu0Level with stop), are eliminated right-looking: for every member k
+// in ascending order, invert its pivot, form L_ik = A_ik U_kk^-1 for the later members i, then A_ij -= L_ik U_kj
+// for the later members i, j.  Every entry receives the same updates in the same order and with the same block
+// arithmetic as the row-by-row elimination of bilu0Row (bsr.cpp:183-215), so the factors are bit-identical; the
+// dependent steps are one per member instead of one merge walk per entry pair.  One workgroup per group;
+// idx[a * K + b] is the factor block of members (a, b) or -1.
+template <int NB>
+__global__ __launch_bounds__(256) void kIluGroupFactor(const int* __restrict__ gstart, const int* __restrict__ grows,
+                                                       const long long* __restrict__ gidxOff, const int* __restrict__ gidx,
+                                                       const int* __restrict__ diag, double* val, int* __restrict__ nreg) {
+#pragma clang fp contract(off)
+    constexpr int B2 = NB * NB;
+    const int g = blockIdx.x;
+    const int m0 = gstart[g], K = gstart[g + 1] - m0;
+    const int* idx = gidx + gidxOff[g];
+    for (int t = 0; t < K; ++t) {
+        const int k = grows[m0 + t];
+        double* dk = val + (long long)diag[k] * B2;
+        if (threadIdx.x == 0) {
+            double d[B2];
+#pragma unroll
+            for (int e = 0; e < B2; ++e) d[e] = dk[e];
+            const int reg = invertPivotDev<NB>(d);
+#pragma unroll
+            for (int e = 0; e < B2; ++e) dk[e] = d[e];
+            if (reg && nreg) atomicAdd(nreg, 1);
+        }
+        __syncthreads();
+        // L_ak = A_ak U_kk^-1 for the later members a
+        for (int a = t + 1 + int(threadIdx.x); a < K; a += int(blockDim.x)) {
+            const int p = idx[(long long)a * K + t];
+            if (p < 0) continue;
+            double* ap = val + (long long)p * B2;
+            double L[B2];
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) rr[j] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = ap[ii * NB + kk];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) rr[j] += f * dk[kk * NB + j];
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) L[ii * NB + j] = rr[j];
+            }
+#pragma unroll
+            for (int e = 0; e < B2; ++e) ap[e] = L[e];
+        }
+        __syncthreads();
+        // A_ab -= L_ak U_kb for the later members a, b (entries of the pattern only)
+        const int r = K - t - 1;
+        for (int pr = int(threadIdx.x); pr < r * r; pr += int(blockDim.x)) {
+            const int a = t + 1 + pr / r, b = t + 1 + pr % r;
+            const int pl = idx[(long long)a * K + t], pu = idx[(long long)t * K + b], pab = idx[(long long)a * K + b];
+            if (pl < 0 || pu < 0 || pab < 0) continue;
+            const double* L = val + (long long)pl * B2;
+            const double* u = val + (long long)pu * B2;
+            double* as = val + (long long)pab * B2;
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) rr[jj] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = L[ii * NB + kk];
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) rr[jj] += f * u[kk * NB + jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) as[ii * NB + jj] -= rr[jj];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 
@@ -1692,9 +1775,14 @@ void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q
 }
 
 template <int NB>
-void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg, int stop) {
     const int cnt = r1 - r0;
-    hipLaunchKernelGGL(kBilu0Level<NB>, dim3((cnt + 127) / 128), dim3(128), 0, s, rows, r0, r1, rp, ci, diag, val, nreg);
+    hipLaunchKernelGGL(kBilu0Level<NB>, dim3((cnt + 127) / 128), dim3(128), 0, s, rows, r0, r1, rp, ci, diag, val, nreg, stop);
+}
+template <int NB>
+void iluGroupFactorT(hipStream_t s, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
+                     const int* diag, double* val, int* nreg) {
+    hipLaunchKernelGGL(kIluGroupFactor<NB>, dim3(g1 - g0), dim3(256), 0, s, gstart + g0, grows, gidxOff + g0, gidx, diag, val, nreg);
 }
 template <int NB>
 void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w) {
@@ -1919,10 +2007,17 @@ void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1,
     RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg)));
     checkLaunch();
 }
-void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
+                      int stop) {
     const int cnt = r1 - r0;
     if (cnt <= 0) return;
-    RSH_NB_DISPATCH(nb, (bilu0LevelT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg)));
+    RSH_NB_DISPATCH(nb, (bilu0LevelT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, stop)));
+    checkLaunch();
+}
+void launchIluGroupFactor(hipStream_t s, int nb, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
+                          const int* diag, double* val, int* nreg) {
+    if (g1 <= g0) return;
+    RSH_NB_DISPATCH(nb, (iluGroupFactorT<NB>(s, g0, g1, gstart, grows, gidxOff, gidx, diag, val, nreg)));
     checkLaunch();
 }
 void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w) {

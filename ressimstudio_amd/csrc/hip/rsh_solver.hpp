@@ -96,6 +96,8 @@ void packSlices(hipStream_t s, int nb, const double* src, SliceDev<TA>& d, long 
 // rows, with the dense inverse of each group (recomputed after every factorisation).
 struct DenseSide {
     std::vector<int> runTask0, runTask1;   // per run: its range in tasks, or -1 when swept level by level
+    std::vector<int> runGroup0, runGroup1; // per run: its range of groups, or -1
+    std::vector<int> hStart, hRows;        // host copies of start / rows (working rows, ascending per group)
     int ngroups = 0, nmap = 0;
     long long elems = 0;
     DBuf<int> start, rows;
@@ -118,6 +120,8 @@ struct DenseSide {
         elems = 0;
         runTask0.assign(runs.size(), -1);
         runTask1.assign(runs.size(), -1);
+        runGroup0.assign(runs.size(), -1);
+        runGroup1.assign(runs.size(), -1);
         constexpr long long kBudget = 1ll << 26;   // dense elements per direction (512 MiB in FP64)
         for (size_t ri = 0; ri < runs.size(); ++ri) {
             const LevelRun& run = runs[ri];
@@ -155,6 +159,7 @@ struct DenseSide {
             const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.005 * double(maxRows) * nb;
             if (maxRows > size_t(kDenseMaxRows) || elems + add > kBudget || denseCost >= levelCost) continue;
             runTask0[ri] = int(htasks.size());
+            runGroup0[ri] = ngroups;
             for (const auto& g : groups) {
                 const int gi = ngroups++;
                 const int K = int(g.size()) * nb;
@@ -175,8 +180,11 @@ struct DenseSide {
                 for (int a0 = 0; a0 < K; a0 += kWave) htasks.push_back(make_int2(gi, a0));
             }
             runTask1[ri] = int(htasks.size());
+            runGroup1[ri] = ngroups;
         }
         nmap = int(hmap.size());
+        hStart = hstart;
+        hRows = hrows;
         start.upload(hstart, s);
         rows.upload(hrows, s);
         off.upload(hoff, s);
@@ -223,6 +231,11 @@ struct IluDev {
     // factorisation schedule: per level the rows eliminated one per thread, then the long rows one per wavefront
     DBuf<int> fShort, fLong;
     std::vector<int> fShortStart, fLongStart;
+    // multicolour order: the forward runs swept through dense groups are factorised group by group (the rows before
+    // the run first, then kIluGroupFactor); per run its range of factor groups, or -1
+    std::vector<int> fgRun0, fgRun1;
+    DBuf<int> fgStart, fgRows, fgIdx;
+    DBuf<long long> fgIdxOff;
     // sweep schedule: runs of levels (fused small levels or single launches) and device copies of level starts
     std::vector<LevelRun> runsL, runsU;
     DBuf<int> dLs, dLq, dUs, dUq;
@@ -303,6 +316,7 @@ struct IluDev {
             runsU = levelRuns(wU, fuseMax, &rowsL, kRunRowsMax);
             denseL.build(runsL, EL, lstart, nb, false, s, rowSys);
             denseU.build(runsU, EU, lstart, nb, true, s, rowSys);
+            buildFactorGroups(s);
             if (keepEntries) {
                 keptL = EL;
                 keptU = EU;
@@ -348,6 +362,46 @@ struct IluDev {
     }
     // new level chunking on the built schedule (needs keepEntries at build); the dense groups are refactored from
     // the current factors
+    // the member-pair block tables of the dense groups of the forward runs (multicolour order, where working rows
+    // are the factorisation rows and every row before a run precedes all of its rows)
+    void buildFactorGroups(hipStream_t s) {
+        fgRun0.assign(runsL.size(), -1);
+        fgRun1.assign(runsL.size(), -1);
+        if (!mc) return;
+        std::vector<int> st(1, 0), rw, idx;
+        std::vector<long long> off;
+        std::vector<int> loc(static_cast<size_t>(n), -1);
+        int ng = 0;
+        for (size_t ri = 0; ri < runsL.size(); ++ri) {
+            if (denseL.runGroup0[ri] < 0) continue;
+            fgRun0[ri] = ng;
+            for (int g = denseL.runGroup0[ri]; g < denseL.runGroup1[ri]; ++g) {
+                const int m0 = denseL.hStart[size_t(g)], K = denseL.hStart[size_t(g) + 1] - m0;
+                for (int a = 0; a < K; ++a) loc[size_t(denseL.hRows[size_t(m0 + a)])] = a;
+                const size_t base = idx.size();
+                off.push_back((long long)base);
+                idx.resize(base + size_t(K) * K, -1);
+                for (int a = 0; a < K; ++a) {
+                    const int i = denseL.hRows[size_t(m0 + a)];
+                    for (int p = fpat.rp[size_t(i)]; p < fpat.rp[size_t(i) + 1]; ++p) {
+                        const int b = loc[size_t(fpat.ci[size_t(p)])];
+                        if (b >= 0) idx[base + size_t(a) * K + size_t(b)] = p;
+                    }
+                }
+                for (int a = 0; a < K; ++a) {
+                    rw.push_back(denseL.hRows[size_t(m0 + a)]);
+                    loc[size_t(denseL.hRows[size_t(m0 + a)])] = -1;
+                }
+                st.push_back(int(rw.size()));
+                ++ng;
+            }
+            fgRun1[ri] = ng;
+        }
+        fgStart.upload(st, s);
+        fgRows.upload(rw, s);
+        fgIdx.upload(idx, s);
+        fgIdxOff.upload(off, s);
+    }
     void retune(int newFuseMax, hipStream_t s) {
         if (!keepEntries) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a schedule built with keepEntries");
         fuseMax = newFuseMax;
@@ -355,6 +409,7 @@ struct IluDev {
         runsU = levelRuns(workU, fuseMax, &rowsPerLevel, kRunRowsMax);
         denseL.build(runsL, keptL, lstart, nb, false, s, rowSys);
         denseU.build(runsU, keptU, lstart, nb, true, s, rowSys);
+        buildFactorGroups(s);
         denseL.factor(nb, false, lu.p, have32, s);
         denseU.factor(nb, true, lu.p, have32, s);
         ++generation;
@@ -365,10 +420,19 @@ struct IluDev {
         if (mc) launchGatherBlocks(s, nb, val, fmap.p, int(nvals / (size_t(nb) * nb)), lu.p);
         else RSH_HIP(hipMemcpyAsync(lu.p, val, nvals * sizeof(double), hipMemcpyDeviceToDevice, s));
         nreg.zero(s);
-        const int nl = int(lstart.size()) - 1;
-        for (int l = 0; l < nl; ++l) {
-            launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
-            launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+        for (size_t ri = 0; ri < runsL.size(); ++ri) {
+            const LevelRun& run = runsL[ri];
+            if (ri < fgRun0.size() && fgRun0[ri] >= 0) {
+                // the run's rows: the contributions of the rows before the run, then the groups right-looking
+                const int w0 = lstart[size_t(run.l0)], w1 = lstart[size_t(run.l1)];
+                launchBilu0Level(s, nb, nullptr, w0, w1, rp, ci, diag, lu.p, nreg.p, w0);
+                launchIluGroupFactor(s, nb, fgRun0[ri], fgRun1[ri], fgStart.p, fgRows.p, fgIdxOff.p, fgIdx.p, diag, lu.p, nreg.p);
+                continue;
+            }
+            for (int l = run.l0; l < run.l1; ++l) {
+                launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+            }
         }
         packSlices(s, nb, lu.p, L64, hL.nslots, false);
         packSlices(s, nb, lu.p, U64, hU.nslots, false);

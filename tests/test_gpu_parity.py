@@ -321,3 +321,28 @@ def test_fp32_storage_preconditioner_is_linear(sysd, name, prec):
     print(f"{name}/{prec}: linearity {lin:.1e}, FP32 storage vs FP64 {near:.1e}")
     assert lin <= 1e-10
     assert near <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["layered", "deepwell", "deepwell_nb2", "deepwell_nb4"])
+def test_mcbilu0_factors_bit_identical(sysd, orc, name):
+    # the multicolour factors, with the well-clique runs eliminated group by group (right-looking), equal the
+    # natural-order BILU0 of P A P^T bit for bit (MC-BILU0 = BILU0(PAP^T), c18_tests.inc:205-215)
+    s = sysd[name]
+    d = rs.System(s).setup(precond="mcbilu0")
+    if name.startswith("deepwell"):
+        assert d.info()["ilu_factor_groups"] > 0
+    k, colour, perm = orc.mcbilu0(s).colouring()
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(s.n)
+    rp, cols, vals = [0], [], []
+    for inew in range(s.n):
+        i = perm[inew]
+        p0, p1 = int(s.row_ptr[i]), int(s.row_ptr[i + 1])
+        c = iperm[s.col[p0:p1]]
+        o = np.argsort(c, kind="stable")
+        cols.append(c[o])
+        vals.append(s.val[p0:p1][o])
+        rp.append(rp[-1] + p1 - p0)
+    ps = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=np.array(rp, np.int32), col=np.concatenate(cols).astype(np.int32),
+                         val=np.concatenate(vals), rhs=np.zeros(s.n * s.nb))
+    assert np.array_equal(d.bilu0_factors(), orc.bilu0(ps, 1).factors())
