@@ -149,8 +149,9 @@ int rsh_wells_reduced_values(rsh_wells* w, double* out);                        
 /* ---- batched solves of many independent systems (no counterpart in the reference, whose engine solves one
  * system per call; this serves its ensemble callers, runBatch study.cpp:326-364, in one launch stream).
  * Systems of one block size are concatenated into one block-diagonal resident system; each system keeps its own
- * BiCGSTAB scalars and convergence, and its preconditioner is the one it gets alone (BILU0 or MC-BILU0, FP64 or
- * FP32 factors with RSH_MIXED).  One BiCGSTAB iteration over all systems is one captured HIP graph.  The level
+ * BiCGSTAB scalars and convergence, and its preconditioner is the one it gets alone (BILU0 or MC-BILU0; FP64,
+ * FP32 with refinement (RSH_MIXED) or FP32-stored (RSH_MIXED_STORE)).  One BiCGSTAB iteration over all systems is
+ * one captured HIP graph.  The level
  * chunking of the sweeps and the entries per workgroup of the vector kernels are tuned at the first setup. */
 int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, const int* const* row_ptr, const int* const* col,
                      rsh_batch** out);
