@@ -10,6 +10,9 @@ using namespace rsh;
 
 namespace {
 
+// iterations between host reads of the Krylov scalars while the residual is far from the tolerance
+constexpr int kCheckEvery = 4;
+
 void syncScalars(rsh_system& S) {
     Ctx& c = S.ctx->c;
     RSH_HIP(hipMemcpyAsync(c.hks, c.ks.p, sizeof(KScalars), hipMemcpyDeviceToHost, c.stream));
@@ -40,6 +43,10 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     its = 0;
     rel = 0;
     if (c.hks->bn == 0) { S.lastStop = "zero rhs"; return RSH_OK; }
+    // The stopping tests run on the device (the epilogues freeze the state), so the host reads the scalars only
+    // every few iterations while the residual is far from the tolerance; iterations launched after a stop change
+    // nothing.
+    int nextCheck = 1;
     for (int it = 1; it <= maxIt; ++it) {
         launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
         launchBicgP(st, N, c.ks.p, S.r.p, S.v.p, S.p.p);
@@ -48,18 +55,19 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
         launchDots(st, c, N, 1, S.rh.p, S.v.p, nullptr, nullptr, c.scal.p, kEpAlpha, c.ks.p);
         launchBicgS(st, c, N, c.ks.p, S.r.p, S.v.p, S.sv.p);
         launchBicgXs(st, N, c.ks.p, S.ph.p, x);
-        syncScalars(S);
-        if (c.hks->state != kRun) break;
         S.precondG(S.sv.p, S.sh.p, precF32);
         S.spmv(S.sh.p, S.t.p, opF32);
         launchDots(st, c, N, 2, S.t.p, S.t.p, S.t.p, S.sv.p, c.scal.p, kEpOmega, c.ks.p);
         launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, x, S.r.p);
+        if (it < nextCheck && it < maxIt) continue;
         syncScalars(S);
         if (c.hks->state != kRun) break;
+        const double relNow = c.hks->rn / c.hks->bn;
+        nextCheck = it + (relNow > 30.0 * tol ? kCheckEvery : 1);
     }
     const KScalars& k = *c.hks;
     its = k.it;
-    if (k.state == kConvS) { rel = k.sn / k.bn; S.lastStop = "converged (s)"; return RSH_OK; }
+    if (k.state == kConvS || k.state == kDoneS) { rel = k.sn / k.bn; S.lastStop = "converged (s)"; return RSH_OK; }
     rel = k.bn > 0 ? k.rn / k.bn : 0;
     if (k.state == kConv) { S.lastStop = "converged"; return RSH_OK; }
     if (k.state == kBreak) { S.lastStop = "breakdown"; return RSH_ERR_BREAKDOWN; }

@@ -1425,8 +1425,16 @@ __global__ void kCopyVec(const double2* __restrict__ s, double2* __restrict__ d,
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
+// Every state but kRun freezes the scalars, so iterations launched after the solve stopped change nothing: the
+// early exit on ||s|| (kConvS) lets kBicgXs add alpha ph once, and the next kEpOmega of that iteration freezes it
+// (kDoneS) before kBicgXs could run again.
 __device__ void runEpilogue(int ep, const double* d, KScalars* ks) {
-    if (!ks || ks->state != kRun) return;
+    if (!ks) return;
+    if (ks->state == kConvS) {
+        if (ep == kEpOmega) ks->state = kDoneS;
+        return;
+    }
+    if (ks->state != kRun) return;
     switch (ep) {
         case kEpRho:
             ks->rhoN = d[0];
@@ -1567,11 +1575,7 @@ __global__ __launch_bounds__(kBlock) void kBicgXR(long long n, const KScalars* k
 // kernels, so each workgroup reads its system's scalars once.  Reductions: one partial per task (fixed tree), then
 // one thread per system adds its tasks' partials in task order and runs the system's epilogue.
 __device__ inline void batchEpilogue(int ep, const double* d, KScalars* ks) {
-    if (ks->state == kConvS) {            // the x += alpha ph of the early exit has run: freeze the system
-        if (ep == kEpOmega) ks->state = kDoneS;
-        return;
-    }
-    if (ep == kEpBn) {
+    if (ep == kEpBn && ks->state == kRun) {
         ks->bn = sqrt(d[0]);
         if (ks->bn == 0) ks->state = kConv;   // zero right-hand side: x = 0
         return;
