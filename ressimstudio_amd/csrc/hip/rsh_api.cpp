@@ -46,19 +46,19 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     // The stopping tests run on the device (the epilogues freeze the state), so the host reads the scalars only
     // every few iterations while the residual is far from the tolerance; iterations launched after a stop change
     // nothing.
+    // rho = (rh, r) of the first iteration; the later ones come with the closing reduction of the previous iteration
+    launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
     int nextCheck = 1;
     for (int it = 1; it <= maxIt; ++it) {
-        launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
         launchBicgP(st, N, c.ks.p, S.r.p, S.v.p, S.p.p);
         S.precondG(S.p.p, S.ph.p, precF32);
         S.spmv(S.ph.p, S.v.p, opF32);
         launchDots(st, c, N, 1, S.rh.p, S.v.p, nullptr, nullptr, c.scal.p, kEpAlpha, c.ks.p);
         launchBicgS(st, c, N, c.ks.p, S.r.p, S.v.p, S.sv.p);
-        launchBicgXs(st, N, c.ks.p, S.ph.p, x);
         S.precondG(S.sv.p, S.sh.p, precF32);
         S.spmv(S.sh.p, S.t.p, opF32);
         launchDots(st, c, N, 2, S.t.p, S.t.p, S.t.p, S.sv.p, c.scal.p, kEpOmega, c.ks.p);
-        launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, x, S.r.p);
+        launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, S.rh.p, x, S.r.p);
         if (it < nextCheck && it < maxIt) continue;
         syncScalars(S);
         if (c.hks->state != kRun) break;

@@ -65,7 +65,7 @@ struct KScalars {
 };
 // kDoneS: converged on ||s|| with the final update applied (batched solves, where other systems keep iterating)
 enum KState : int { kRun = 0, kConvS = 1, kConv = 2, kBreak = 3, kOmegaZero = 4, kDoneS = 5 };
-enum Epilogue : int { kEpNone = 0, kEpRho, kEpAlpha, kEpSn, kEpOmega, kEpRn, kEpBn, kEpNorm };
+enum Epilogue : int { kEpNone = 0, kEpRho, kEpAlpha, kEpSn, kEpOmega, kEpRn, kEpBn, kEpNorm, kEpRnRho };
 
 struct Ctx {
     int device = 0;
@@ -274,9 +274,10 @@ void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, con
                 double* out, int epilogue, KScalars* ks);
 void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p);
 void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv);
-void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* ph, double* x);
+// x += alpha ph + omega sh, r = s - omega t, then ||r|| and the next iteration's rho = (rh, r) (epilogue kEpRnRho);
+// after the early exit on ||s|| only x += alpha ph
 void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv,
-                  const double* t, double* x, double* r);
+                  const double* t, const double* rh, double* x, double* r);
 void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y);                      // y += a x
 void launchAxpyDev(hipStream_t s, long long n, const double* a, double sign, const double* x, double* y); // y += sign*a[0] x
 void launchScaleTo(hipStream_t s, long long n, const double* num, const double* x, double* y, bool invert);  // y = x / num[0]

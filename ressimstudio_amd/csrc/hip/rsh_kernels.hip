@@ -1426,12 +1426,12 @@ __global__ void kCopyVec(const double2* __restrict__ s, double2* __restrict__ d,
 }
 
 // Every state but kRun freezes the scalars, so iterations launched after the solve stopped change nothing: the
-// early exit on ||s|| (kConvS) lets kBicgXs add alpha ph once, and the next kEpOmega of that iteration freezes it
-// (kDoneS) before kBicgXs could run again.
+// early exit on ||s|| (kConvS) lets the x update add alpha ph once, and the closing reduction of that iteration
+// (kEpRn / kEpRnRho) freezes it (kDoneS) before the next iteration's update could run again.
 __device__ void runEpilogue(int ep, const double* d, KScalars* ks) {
     if (!ks) return;
     if (ks->state == kConvS) {
-        if (ep == kEpOmega) ks->state = kDoneS;
+        if (ep == kEpRn || ep == kEpRnRho) ks->state = kDoneS;
         return;
     }
     if (ks->state != kRun) return;
@@ -1461,6 +1461,16 @@ __device__ void runEpilogue(int ep, const double* d, KScalars* ks) {
             ks->it += 1;
             if (ks->rn / ks->bn < ks->tol) ks->state = kConv;
             else if (ks->omega == 0) ks->state = kOmegaZero;
+            break;
+        case kEpRnRho:   // kEpRn, then the next iteration's kEpRho with d[1] = (rh, r)
+            ks->rn = sqrt(d[0]);
+            ks->it += 1;
+            if (ks->rn / ks->bn < ks->tol) { ks->state = kConv; break; }
+            if (ks->omega == 0) { ks->state = kOmegaZero; break; }
+            ks->rhoN = d[1];
+            if (fabs(d[1]) < 1e-300) { ks->state = kBreak; break; }
+            ks->beta = (d[1] / ks->rho) * (ks->alpha / ks->omega);
+            ks->rho = d[1];
             break;
         case kEpBn: ks->bn = sqrt(d[0]); break;
         default: break;
@@ -1545,30 +1555,29 @@ __global__ __launch_bounds__(kBlock) void kBicgS(long long n, const KScalars* ks
     }
     reducePartial<1>(acc, partial);
 }
-// early exit on ||s||: x += alpha ph
-__global__ void kBicgXs(long long n, const KScalars* __restrict__ ks, const double* __restrict__ ph, double* __restrict__ x) {
-#pragma clang fp contract(off)
-    if (ks->state != kConvS) return;
-    const double alpha = ks->alpha;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        x[i] += alpha * ph[i];
-}
-// x += alpha ph + omega sh; r = s - omega t; with (r, r)
+// x += alpha ph + omega sh; r = s - omega t; with (r, r) and the next iteration's (rh, r) (accumulated with a fused
+// multiply-add, as kDots accumulates it, so the value is the one kDots gives); after the early exit on ||s||
+// (kConvS) only x += alpha ph
 __global__ __launch_bounds__(kBlock) void kBicgXR(long long n, const KScalars* ks, const double* __restrict__ ph, const double* __restrict__ sh,
-                                                  const double* __restrict__ sv, const double* __restrict__ t, double* __restrict__ x,
-                                                  double* __restrict__ r, double* partial) {
+                                                  const double* __restrict__ sv, const double* __restrict__ t, const double* __restrict__ rh,
+                                                  double* __restrict__ x, double* __restrict__ r, double* partial) {
 #pragma clang fp contract(off)
-    double acc[1] = {0};
-    if (ks->state == kRun) {
+    double acc[2] = {0, 0};
+    const int state = ks->state;
+    if (state == kRun) {
         const double alpha = ks->alpha, omega = ks->omega;
         for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
             x[i] += alpha * ph[i] + omega * sh[i];
             const double ri = sv[i] - omega * t[i];
             r[i] = ri;
             acc[0] += ri * ri;
+            acc[1] = fma(rh[i], ri, acc[1]);
         }
+    } else if (state == kConvS) {
+        const double alpha = ks->alpha;
+        for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) x[i] += alpha * ph[i];
     }
-    reducePartial<1>(acc, partial);
+    reducePartial<2>(acc, partial);
 }
 // ---------------------------------------------------------------- batched BiCGSTAB (independent systems, per-system scalars)
 // The flat vectors hold every system; task t = {system, first entry, end entry} is one workgroup of the vector
@@ -2206,15 +2215,11 @@ void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double*
     hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, c.red.p, nullptr, int(kEpSn), ks);
     checkLaunch();
 }
-void launchBicgXs(hipStream_t s, long long n, const KScalars* ks, const double* ph, double* x) {
-    hipLaunchKernelGGL(kBicgXs, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, ks, ph, x);
-    checkLaunch();
-}
 void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv, const double* t,
-                  double* x, double* r) {
+                  const double* rh, double* x, double* r) {
     const int g = redGrid(n);
-    hipLaunchKernelGGL(kBicgXR, dim3(g), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, x, r, c.red.p);
-    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, c.red.p, nullptr, int(kEpRn), ks);
+    hipLaunchKernelGGL(kBicgXR, dim3(g), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, rh, x, r, c.red.p);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 2, g, c.red.p, nullptr, int(kEpRnRho), ks);
     checkLaunch();
 }
 void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y) {
