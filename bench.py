@@ -234,7 +234,8 @@ def serving_leg(rs, gen, args):
            "slots": args.serve_slots, "requests_per_rate": args.serve_requests, "hbm_budget_GB": 200}
     for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED), ("store32", rs.MIXED_STORE)):
         srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9)
-        ids = [srv.add_model(m, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, precision=prec) for m in models]
+        ids = [srv.add_model(m, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0, precision=prec)
+               for m in models]
         t = time.perf_counter()
         warm = [srv.submit(i, m.rhs, perturb=0.05, seed=1, tol=args.tol, max_it=args.max_it, want_x=False) for i, m in zip(ids, models)]
         for tk in warm:
@@ -327,21 +328,25 @@ def main():
 
     J, GS, FP64, MIXED, BI, FG = rs.SMOOTH_JACOBI, rs.SMOOTH_GS, rs.FP64, rs.MIXED, rs.KRYLOV_BICGSTAB, rs.KRYLOV_FGMRES
     all_cfgs = {
-        # head: two damped-Jacobi steps per level (one step, DeviceCprAmg's choice, is variant cpr_mc_j1)
-        "cpr_mc": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=2),
+        # head: two Jacobi steps per level with weight 1 (DeviceCprAmg's one step with weight 0.6667 is cpr_mc_j1; the
+        # weight 1 takes 7 % fewer iterations on average over four seeds of this workload)
+        "cpr_mc": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
+        "cpr_mc_w067": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=2),
         "cpr_mc_j1": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=1),
-        # DeviceCprAmg: Jacobi V-cycle + block-Jacobi second stage (no ILU)
+        # DeviceCprAmg: Jacobi V-cycle (one step, weight 0.6667) + block-Jacobi second stage (no ILU)
         "cpr_bj": rs.default_options(precond="cpr_bj", precision=FP64, krylov=BI, smoother=J, smooth_steps=1),
-        "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J, smooth_steps=2),
+        "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
         "ilu0": rs.default_options(precond="bilu0", precision=FP64, krylov=BI),
         "mcilu0": rs.default_options(precond="mcbilu0", precision=FP64, krylov=BI),
-        # FP32 preconditioner inside FP64 BiCGSTAB / FGMRES(30); FP64 refinement around FP32-operator inner solves
-        "cpr_mc_mixed": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=BI, smoother=J, smooth_steps=2),
-        "cpr_mc_mixed_fgmres": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=FG, smoother=J, restart=30, smooth_steps=2),
-        "cpr_mc_mixed_ir": rs.default_options(precond="cpr_mc", precision=rs.MIXED_IR, krylov=BI, smoother=J, smooth_steps=2),
         # FP32-stored preconditioner (factors, hierarchy, pressure column) with FP64 vectors and arithmetic, FP64 BiCGSTAB
-        "cpr_mc_store32": rs.default_options(precond="cpr_mc", precision=rs.MIXED_STORE, krylov=BI, smoother=J, smooth_steps=2),
-        "cpr_mixed": rs.default_options(precond="cpr", precision=MIXED, krylov=BI, smoother=J, smooth_steps=2),
+        "cpr_mc_store32": rs.default_options(precond="cpr_mc", precision=rs.MIXED_STORE, krylov=BI, smoother=J, smooth_steps=2,
+                                             jacobi_omega=1.0),
+        # FP32 preconditioner inside FP64 BiCGSTAB / FGMRES(30); FP64 refinement around FP32-operator inner solves
+        "cpr_mc_mixed": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
+        "cpr_mc_mixed_fgmres": rs.default_options(precond="cpr_mc", precision=MIXED, krylov=FG, smoother=J, restart=30, smooth_steps=2,
+                                                  jacobi_omega=1.0),
+        "cpr_mc_mixed_ir": rs.default_options(precond="cpr_mc", precision=rs.MIXED_IR, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
+        "cpr_mixed": rs.default_options(precond="cpr", precision=MIXED, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
         "cpr_exact_gs": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=GS),
     }
     names = [c for c in args.configs.split(",") if c] or [n for n in all_cfgs if n != "cpr_exact_gs" or args.extra]
@@ -446,7 +451,7 @@ def main():
         "data": "synthetic (seeded generator, ressimstudio_amd/csrc/gen)",
         "config": {"workload": (f"engine Jacobian dump {os.path.basename(args.dump)}, {s.n} cells x nb {s.nb}" if args.dump else
                                 f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells") + f"; variant {head}: BiCGSTAB to 1e-8 "
-                               f"preconditioned by CPR (quasi-IMPES weights, aggregation AMG V-cycle with two damped-Jacobi steps per level) + "
+                               f"preconditioned by CPR (quasi-IMPES weights, aggregation AMG V-cycle with two Jacobi steps (weight 1) per level) + "
                                f"multicolour BILU0; numeric setup + solve per step",
                    "cells": s.n, "block": s.nb, "blocks": s.nnzb, "tol": args.tol, "scale": args.scale},
         "iterations": main_r["iterations"],

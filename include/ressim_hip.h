@@ -85,7 +85,8 @@ typedef struct rsh_options {
     int reuse_amg;      /* 1: keep the AMG aggregates of the last setup while the pattern is unchanged and only
                            recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
     int smooth_steps;   /* RSH_SMOOTH_JACOBI: pre- and post-smoothing steps per level (default 1, DeviceCprAmg's) */
-    int reserved[6];
+    double jacobi_omega;/* RSH_SMOOTH_JACOBI: the damping weight (<= 0: 0.6667, DeviceCprAmg's, bsr.h:173) */
+    int reserved[4];
 } rsh_options;
 
 /* ---- library / context */

@@ -479,6 +479,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             if (!S.cpr) S.cpr = std::make_unique<CprDev>();
             S.cpr->smoother = o.smoother;
             S.cpr->steps = o.smooth_steps > 0 ? o.smooth_steps : 1;
+            S.cpr->omega = o.jacobi_omega > 0 ? o.jacobi_omega : CprDev::kOmegaDefault;
             S.cpr->plan = &S.spmv64;
             if (S.colP.n == 0) {
                 S.colP.alloc(size_t(S.spmvHost.nslots) * S.nb * kWave);
@@ -505,6 +506,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         RSH_HIP(hipStreamSynchronize(st));
         S.lastIluMs = nowMs() - t1;
         S.lastCprMs = t1 - t0;
+        if (std::memcmp(&S.opt, &o, sizeof o) != 0) S.clearGraphs();   // captured applies bake the options in
         S.opt = o;
         S.ready = true;
         S.lastSetupMs = nowMs() - t0;

@@ -529,7 +529,8 @@ struct AmgLevelDev {
 };
 
 struct CprDev {
-    static constexpr double kOmega = 0.6667;   // damped-Jacobi weight of the reference's device V-cycle (bsr.h:173)
+    static constexpr double kOmegaDefault = 0.6667;   // damped-Jacobi weight of the reference's device V-cycle (bsr.h:173)
+    double omega = kOmegaDefault;
     int n = 0, nb = 1;
     DBuf<double> w, xp, t;
     std::vector<std::unique_ptr<AmgLevelDev>> lv;
@@ -725,18 +726,18 @@ struct CprDev {
             const void* dv = dinvOf(l, f32);
             double *xa = L.x1.p, *xb = L.x3.p;
             for (int k = 1; k < steps; ++k) {
-                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, kOmega, xb, mean);
+                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, omega, xb, mean);
                 std::swap(xa, xb);
             }
-            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, kOmega, L.r.p, mean);
+            launchJacobiFused(s, 0, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, omega, L.r.p, mean);
             const bool smoothedC = l + 2 < lv.size();   // the coarse level is smoothed too: its x1 comes with its b
             launchRestrict(s, L.nc, L.rrp.p, L.rci.p, L.r.p, C.b.p, double(L.n) / std::max(L.nc, 1), smoothedC ? dinvOf(l + 1, f32) : nullptr,
-                           h, kOmega, smoothedC ? C.x1.p : nullptr);
+                           h, omega, smoothedC ? C.x1.p : nullptr);
             if (!cycle(l + 1, s, f32, xa)) launchProlongAdd(s, L.n, L.agg.p, C.result, xa);
             double *pa = xa, *pb = L.x2.p;
             for (int k = 0; k < steps; ++k) {
                 const bool last = k + 1 == steps;
-                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, kOmega, pb, mean, last ? prp : nullptr,
+                launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, omega, pb, mean, last ? prp : nullptr,
                                   last ? pci : nullptr, last ? pfine : nullptr);
                 std::swap(pa, pb);
             }
@@ -760,7 +761,7 @@ struct CprDev {
     bool jacobi0() const { return smoother == RSH_SMOOTH_JACOBI && lv.size() > 1; }
     // one V-cycle on r_p (in lv[0].b) -> result pointer; x1Ready: level 0's first Jacobi iterate was written with r_p
     const double* vcycle(hipStream_t s, bool f32 = false, bool x1Ready = false) {
-        if (jacobi0() && !x1Ready) launchJacobiInit(s, lv[0]->n, use32(f32), dinvOf(0, f32), lv[0]->b.p, kOmega, lv[0]->x1.p);
+        if (jacobi0() && !x1Ready) launchJacobiInit(s, lv[0]->n, use32(f32), dinvOf(0, f32), lv[0]->b.p, omega, lv[0]->x1.p);
         cycle(0, s, f32);
         return lv[0]->result;
     }
@@ -895,7 +896,7 @@ struct rsh_system {
         CprDev& C = *cpr;
         const bool a32 = amgF32(f32);
         const bool j0 = C.jacobi0();
-        launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p, j0 ? C.dinvOf(0, a32) : nullptr, C.use32(a32), CprDev::kOmega,
+        launchCprRestrict(s(), nb, n, C.w.p, rr, C.lv[0]->b.p, j0 ? C.dinvOf(0, a32) : nullptr, C.use32(a32), C.omega,
                           j0 ? C.lv[0]->x1.p : nullptr);
         const double* xp = C.vcycle(s(), a32, j0);
         const bool p32 = a32 && colP32.n > 0;
