@@ -404,6 +404,23 @@ int rsh_system_create(rsh_context* ctx, int n, int nb, const int* row_ptr, const
                         S->spmvHost.col[at] = rr >= 0 ? rr : 0;
                     }
                 }
+        {
+            // the columns relative to the lane's row fit 16 bits when no slice row reaches further than 32767 block
+            // rows (the 7-point stencil of grids up to nx * ny = 32767 cells per layer; far couplings are long rows)
+            SliceHost& h = S->spmvHost;
+            std::vector<short> d(h.col.size());
+            bool fits = true;
+            for (int sl = 0; sl < h.nslices && fits; ++sl)
+                for (int k = 0; k < h.width[size_t(sl)] && fits; ++k)
+                    for (int lane = 0; lane < kSliceRows; ++lane) {
+                        const size_t at = (size_t(h.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
+                        const int rr = h.row[size_t(sl) * kSliceRows + size_t(lane)];
+                        const long long delta = (long long)h.col[at] - (rr >= 0 ? rr : -1);
+                        if (delta < -32768 || delta > 32767) { fits = false; break; }
+                        d[at] = short(delta);
+                    }
+            if (fits) h.dcol = std::move(d);
+        }
         uploadSlices(S->spmvHost, S->spmv64, nb, st);
         RSH_HIP(hipStreamSynchronize(st));
         *out = S.release();
@@ -633,7 +650,8 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
         rsh_system& S = *sys;
         std::ostringstream o;
         o << "{\"n\":" << S.n << ",\"nb\":" << S.nb << ",\"nnzb\":" << S.nnzb << ",\"spmv_slices\":" << S.spmvHost.nslices
-          << ",\"spmv_slots\":" << S.spmvHost.nslots << ",\"spmv_long_rows\":" << S.spmvHost.lrow.size();
+          << ",\"spmv_slots\":" << S.spmvHost.nslots << ",\"spmv_long_rows\":" << S.spmvHost.lrow.size()
+          << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0);
         if (S.ilu) {
             o << ",\"ilu_levels\":" << S.ilu->sL.levels() << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
               << ",\"ilu_long_rows_L\":" << S.ilu->hL.lrow.size() << ",\"ilu_long_rows_U\":" << S.ilu->hU.lrow.size() << ",\"ilu_bytes\":" << S.ilu->bytes();

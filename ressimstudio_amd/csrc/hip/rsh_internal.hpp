@@ -91,6 +91,7 @@ struct SliceHost {
     std::vector<int> width, off;   // per slice
     std::vector<int> row, orig;    // per slice*64+lane: working row, original row (-1 = idle lane)
     std::vector<int> col, src;     // per slot*64+lane: working column, source block index (-1 = padding)
+    std::vector<short> dcol;       // optional: col - row as 16 bits (idle lanes: 1, i.e. column 0), when all fit
     // long rows
     std::vector<int> lrow, lorig, lptr, lcol, lsrc;
     std::vector<int> dsrc;         // optional per slice*64+lane: diagonal block (ILU backward sweep)
@@ -104,12 +105,13 @@ struct SliceDev {
     int nslices = 0, nlong = 0, nb = 1;
     DBuf<int2> info;               // (width, slot offset)
     DBuf<int> row, orig, col, src, dsrc;
+    DBuf<short> dcol;              // the columns relative to the lane's row (16-bit layout), or empty
     DBuf<TA> val, dval;
     DBuf<int> lrow, lorig, lptr, lcol, lsrc, ldsrc;
     DBuf<TA> lval, ldval;
     DBuf<int> owner, lowner;
     size_t bytes() const {
-        return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
+        return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + dcol.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
                lrow.bytes() + lorig.bytes() + lptr.bytes() + lcol.bytes() + lsrc.bytes() + ldsrc.bytes() + lval.bytes() + ldval.bytes();
     }
 };

@@ -149,12 +149,21 @@ inline int gridFor(long long work, int per = kBlock) {
     return int(g);
 }
 
+// a slot's column: absolute (CT = int) or relative to the lane's row r (CT = short, the 16-bit layout: half the index
+// bytes); NT: non-temporal load (streamed once)
+template <typename CT, bool NT>
+__device__ inline int slotCol(const CT* __restrict__ col, long long at, int r) {
+    const CT c = NT ? __builtin_nontemporal_load(col + at) : col[at];
+    if constexpr (sizeof(CT) == sizeof(int)) return int(c);
+    else return r + int(c);
+}
+
 // ---------------------------------------------------------------- BSR SpMV over slices (+ long rows)
 // y_i = sum_p A_p x_col(p) (bsr.cpp:109-126); per lane the blocks are visited in column order and each block
 // product is formed as s_e = sum_f a_ef x_f before y_e += s_e, as gemvAdd does.
-template <int NB, typename TA>
+template <int NB, typename TA, typename CT>
 __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
-                                                      const int* __restrict__ col, const TA* __restrict__ val, int nlong,
+                                                      const CT* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                       const int* __restrict__ lrow, const int* __restrict__ lptr,
                                                       const int* __restrict__ lcol, const TA* __restrict__ lval,
                                                       const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ owner,
@@ -172,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
             // matrix values and columns are streamed once: non-temporal loads keep the caches for the x gathers
-            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
+            const int c = slotCol<CT, true>(col, slot * kWave + lane, r);
             const TA* a = val + slot * (NB * NB * kWave) + lane;
             double xv[NB];
 #pragma unroll
@@ -1142,9 +1151,9 @@ __global__ void kCprRestrict(int n, const double* __restrict__ w, const double* 
 }
 // second-stage residual t = r - S(:, 0) x_p over the SpMV slices with the pressure column packed (nb values per
 // slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081)
-template <int NB, typename TA>
+template <int NB, typename TA, typename CT>
 __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
-                                                     const int* __restrict__ col, const TA* __restrict__ colP, int nlong,
+                                                     const CT* __restrict__ col, const TA* __restrict__ colP, int nlong,
                                                      const int* __restrict__ lrow, const int* __restrict__ lptr,
                                                      const int* __restrict__ lcol, const double* __restrict__ lvalFull,
                                                      const double* __restrict__ r, const double* __restrict__ xp, double* __restrict__ t) {
@@ -1160,7 +1169,7 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
         for (int e = 0; e < NB; ++e) acc[e] = r[(long long)rr * NB + e];
         for (int k = 0; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
-            const double xk = xp[__builtin_nontemporal_load(col + slot * kWave + lane)];
+            const double xk = xp[slotCol<CT, true>(col, slot * kWave + lane, rr)];
             const TA* a = colP + slot * (NB * kWave) + lane;
 #pragma unroll
             for (int e = 0; e < NB; ++e) acc[e] -= double(__builtin_nontemporal_load(a + e * kWave)) * xk;
@@ -1351,9 +1360,9 @@ __device__ inline double jacValue(double acc, double omega, const JacSelf& o) {
     return o.x + omega * o.d * (o.b - acc);
 }
 // level 0 over the SpMV row slices (the pressure matrix has the pattern of S), long rows one wavefront each
-template <int MODE, typename TA>
+template <int MODE, typename TA, typename CT>
 __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
-                                                        const int* __restrict__ col, const TA* __restrict__ val, int nlong,
+                                                        const CT* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                         const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
                                                         const TA* __restrict__ lval, const TA* __restrict__ dinv,
                                                         const double* __restrict__ b, const double* __restrict__ x1, double omega,
@@ -1368,7 +1377,7 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
         double acc = 0;
         for (int k = 0; k < inf.x; ++k) {
             const long long at = ((long long)inf.y + k) * kWave + lane;
-            acc += double(val[at]) * x1[col[at]];
+            acc += double(val[at]) * x1[slotCol<CT, false>(col, at, i)];
         }
         out0[i] = jacValue<MODE>(acc, omega, self);
         return;
@@ -1753,21 +1762,22 @@ __global__ void kMultiAxpy(long long n, int k, const double* __restrict__ V, lon
     }
 }
 
+template <int NB, typename TA>
+void spmvLayoutT(hipStream_t s, const SliceDev<TA>& d, const double* x, double* y, const KScalars* gate) {
+    const int waves = d.nslices + d.nlong;
+    if (!waves) return;
+    const dim3 grid((waves + 3) / 4);
+    if (d.dcol.n)
+        hipLaunchKernelGGL((kSpmvSliced<NB, TA, short>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.dcol.p, d.val.p, d.nlong,
+                           d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
+    else
+        hipLaunchKernelGGL((kSpmvSliced<NB, TA, int>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p, d.val.p, d.nlong,
+                           d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
+}
 template <int NB>
 void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y, const KScalars* gate) {
-    if (f32) {
-        const SliceDev<float>& d = *d32;
-        const int waves = d.nslices + d.nlong;
-        if (!waves) return;
-        hipLaunchKernelGGL((kSpmvSliced<NB, float>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
-                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
-    } else {
-        const SliceDev<double>& d = *d64;
-        const int waves = d.nslices + d.nlong;
-        if (!waves) return;
-        hipLaunchKernelGGL((kSpmvSliced<NB, double>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p,
-                           d.val.p, d.nlong, d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
-    }
+    if (f32) spmvLayoutT<NB, float>(s, *d32, x, y, gate);
+    else spmvLayoutT<NB, double>(s, *d64, x, y, gate);
 }
 
 template <int NB, typename TA, typename TV>
@@ -1810,15 +1820,21 @@ void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double
     if (f32) hipLaunchKernelGGL((kCprRestrict<NB, float>), dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp, (const float*)dinv, omega, x1);
     else hipLaunchKernelGGL((kCprRestrict<NB, double>), dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp, (const double*)dinv, omega, x1);
 }
+template <int NB, typename TA, typename CT>
+void cprSecondCT(hipStream_t s, const SliceDev<double>* sl, const CT* col, const TA* colP, const double* r, const double* xp, double* t) {
+    const int waves = sl->nslices + sl->nlong;
+    hipLaunchKernelGGL((kCprSecond<NB, TA, CT>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, col, colP,
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+}
 template <int NB>
 void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
-    const int waves = sl->nslices + sl->nlong;
-    if (f32)
-        hipLaunchKernelGGL((kCprSecond<NB, float>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p,
-                           (const float*)colP, sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
-    else
-        hipLaunchKernelGGL((kCprSecond<NB, double>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p,
-                           (const double*)colP, sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+    if (sl->dcol.n) {
+        if (f32) cprSecondCT<NB, float, short>(s, sl, sl->dcol.p, (const float*)colP, r, xp, t);
+        else cprSecondCT<NB, double, short>(s, sl, sl->dcol.p, (const double*)colP, r, xp, t);
+    } else {
+        if (f32) cprSecondCT<NB, float, int>(s, sl, sl->col.p, (const float*)colP, r, xp, t);
+        else cprSecondCT<NB, double, int>(s, sl, sl->col.p, (const double*)colP, r, xp, t);
+    }
 }
 
 
@@ -2146,8 +2162,13 @@ void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, co
                    const double* x1, double omega, double* out0) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
-    hipLaunchKernelGGL((kJacobiSliced<MODE, TA>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
-                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
+    const dim3 grid((waves + 3) / 4);
+    if (sl->dcol.n)
+        hipLaunchKernelGGL((kJacobiSliced<MODE, TA, short>), grid, dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->dcol.p, sval,
+                           sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
+    else
+        hipLaunchKernelGGL((kJacobiSliced<MODE, TA, int>), grid, dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+                           sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
 }
 template <typename TA>
 void jacobiFusedT(hipStream_t s, int mode, const SliceDev<double>* sl, const TA* sval, const TA* lval, int n, const int* rp, const int* ci,

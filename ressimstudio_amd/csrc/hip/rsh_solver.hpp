@@ -66,6 +66,7 @@ void uploadSlices(const SliceHost& h, SliceDev<TA>& d, int nb, hipStream_t s) {
     d.row.upload(h.row, s);
     d.orig.upload(h.orig, s);
     d.col.upload(h.col, s);
+    d.dcol.upload(h.dcol, s);
     d.src.upload(h.src, s);
     d.dsrc.upload(h.dsrc, s);
     d.lrow.upload(h.lrow, s);

@@ -55,6 +55,7 @@ def test_spmv_matches_oracle(sysd, orc, name):
     y = d.spmv(x)
     yo = orc.spmv(s, x)
     info = d.info()
+    assert info["spmv_col16"] == 1     # these grids fit the 16-bit relative columns
     if info["spmv_long_rows"] == 0:
         assert np.array_equal(y, yo), "rows handled by one lane keep the reference's order: bit-identical"
     assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
