@@ -238,9 +238,10 @@ void launchFixDiag(hipStream_t s, int n, const int* diag, double* v);
 // r_p = w^T r; with x1 given also level 0's first damped-Jacobi iterate x1 = omega D^-1 r_p (dinv FP32 when f32)
 void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const double* r, double* rp, const void* dinv = nullptr,
                        bool f32 = false, double omega = 0, double* x1 = nullptr);
-// colP: the pressure column in the SpMV slice layout, FP64 or (f32) FP32 storage
+// colP: the pressure column in the SpMV slice layout, FP64 or (f32) FP32 storage; y0 (optional): also t of the rows
+// with iperm[row] < w1 into y0[iperm[row]] (the ILU's first level when it has no lower entries)
 void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp,
-                     double* t);
+                     double* t, const int* iperm = nullptr, int w1 = 0, double* y0 = nullptr);
 // scalar AMG level kernels
 void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b,
                            double* r);

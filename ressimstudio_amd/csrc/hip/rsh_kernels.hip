@@ -1150,13 +1150,16 @@ __global__ void kCprRestrict(int n, const double* __restrict__ w, const double* 
     if (x1) x1[i] = omega * double(dinv[i]) * s;
 }
 // second-stage residual t = r - S(:, 0) x_p over the SpMV slices with the pressure column packed (nb values per
-// slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081)
+// slot): acc_e -= S_p[e][0] x_p,col in block order (bsr.cpp:1071-1081).  With y0 given, t of the rows whose ILU
+// working index w = iperm[row] is below w1 also goes to y0[w]: the forward sweep of a first level without lower
+// entries only copies t there.
 template <int NB, typename TA, typename CT>
 __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
                                                      const CT* __restrict__ col, const TA* __restrict__ colP, int nlong,
                                                      const int* __restrict__ lrow, const int* __restrict__ lptr,
                                                      const int* __restrict__ lcol, const double* __restrict__ lvalFull,
-                                                     const double* __restrict__ r, const double* __restrict__ xp, double* __restrict__ t) {
+                                                     const double* __restrict__ r, const double* __restrict__ xp, double* __restrict__ t,
+                                                     const int* __restrict__ iperm, int w1, double* __restrict__ y0) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -1176,6 +1179,13 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
         }
 #pragma unroll
         for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = acc[e];
+        if (y0) {
+            const int w = iperm[rr];
+            if (w < w1) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) y0[(long long)w * NB + e] = acc[e];
+            }
+        }
         return;
     }
     const int q = wid - nslices;
@@ -1193,8 +1203,13 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
     for (int e = 0; e < NB; ++e) part[e] = waveSum(part[e]);
     if (lane == 0) {
         const int rr = lrow[q];
+        const int w = y0 ? iperm[rr] : w1;
 #pragma unroll
-        for (int e = 0; e < NB; ++e) t[(long long)rr * NB + e] = r[(long long)rr * NB + e] - part[e];
+        for (int e = 0; e < NB; ++e) {
+            const double v = r[(long long)rr * NB + e] - part[e];
+            t[(long long)rr * NB + e] = v;
+            if (w < w1) y0[(long long)w * NB + e] = v;
+        }
     }
 }
 
@@ -1821,19 +1836,21 @@ void cprRestrictT(hipStream_t s, int n, const double* w, const double* r, double
     else hipLaunchKernelGGL((kCprRestrict<NB, double>), dim3((n + 255) / 256), dim3(256), 0, s, n, w, r, rp, (const double*)dinv, omega, x1);
 }
 template <int NB, typename TA, typename CT>
-void cprSecondCT(hipStream_t s, const SliceDev<double>* sl, const CT* col, const TA* colP, const double* r, const double* xp, double* t) {
+void cprSecondCT(hipStream_t s, const SliceDev<double>* sl, const CT* col, const TA* colP, const double* r, const double* xp, double* t,
+                 const int* iperm, int w1, double* y0) {
     const int waves = sl->nslices + sl->nlong;
     hipLaunchKernelGGL((kCprSecond<NB, TA, CT>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, col, colP,
-                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t);
+                       sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, sl->lval.p, r, xp, t, iperm, w1, y0);
 }
 template <int NB>
-void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
+void cprSecondT(hipStream_t s, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t,
+                const int* iperm, int w1, double* y0) {
     if (sl->dcol.n) {
-        if (f32) cprSecondCT<NB, float, short>(s, sl, sl->dcol.p, (const float*)colP, r, xp, t);
-        else cprSecondCT<NB, double, short>(s, sl, sl->dcol.p, (const double*)colP, r, xp, t);
+        if (f32) cprSecondCT<NB, float, short>(s, sl, sl->dcol.p, (const float*)colP, r, xp, t, iperm, w1, y0);
+        else cprSecondCT<NB, double, short>(s, sl, sl->dcol.p, (const double*)colP, r, xp, t, iperm, w1, y0);
     } else {
-        if (f32) cprSecondCT<NB, float, int>(s, sl, sl->col.p, (const float*)colP, r, xp, t);
-        else cprSecondCT<NB, double, int>(s, sl, sl->col.p, (const double*)colP, r, xp, t);
+        if (f32) cprSecondCT<NB, float, int>(s, sl, sl->col.p, (const float*)colP, r, xp, t, iperm, w1, y0);
+        else cprSecondCT<NB, double, int>(s, sl, sl->col.p, (const double*)colP, r, xp, t, iperm, w1, y0);
     }
 }
 
@@ -2068,10 +2085,12 @@ void launchCprRestrict(hipStream_t s, int nb, int n, const double* w, const doub
     RSH_NB_DISPATCH(nb, (cprRestrictT<NB>(s, n, w, r, rp, dinv, f32, omega, x1)));
     checkLaunch();
 }
-void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t) {
+void launchCprSecond(hipStream_t s, int nb, const SliceDev<double>* sl, const void* colP, bool f32, const double* r, const double* xp, double* t,
+                     const int* iperm, int w1, double* y0) {
     const int waves = sl->nslices + sl->nlong;
     if (!waves) return;
-    RSH_NB_DISPATCH(nb, (cprSecondT<NB>(s, sl, colP, f32, r, xp, t)));
+    if (!y0) iperm = nullptr;
+    RSH_NB_DISPATCH(nb, (cprSecondT<NB>(s, sl, colP, f32, r, xp, t, iperm, w1, y0)));
     checkLaunch();
 }
 void launchCsrSpmvResidual(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, const double* b, double* r) {

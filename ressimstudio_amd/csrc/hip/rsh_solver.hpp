@@ -213,6 +213,8 @@ struct IluDev {
     int n = 0, nb = 1, colours = 0;
     std::vector<int> lstart;            // level (or colour) starts in working numbering
     DBuf<int> perm;                     // working row -> original row
+    DBuf<int> iperm;                    // original row -> working row
+    bool firstCopy = false;             // the first level has no lower entries: its forward sweep copies r
     // factorisation matrix: natural mode = the system's pattern (rows visited in level order),
     // multicolour mode = the colour-permuted pattern P A P^T
     HostBsr fpat;
@@ -294,6 +296,11 @@ struct IluDev {
         EL.orig = ord;
         EU.orig = ord;
         perm.upload(ord, s);
+        {
+            std::vector<int> ip(static_cast<size_t>(n));
+            for (int w = 0; w < n; ++w) ip[size_t(ord[size_t(w)])] = w;
+            iperm.upload(ip, s);
+        }
         std::vector<int> lenL(static_cast<size_t>(n)), lenU(static_cast<size_t>(n));
         for (int w = 0; w < n; ++w) { lenL[size_t(w)] = EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)]; lenU[size_t(w)] = EU.ptr[size_t(w) + 1] - EU.ptr[size_t(w)]; }
         buildSlices(EL, lstart, rowCap(lenL), hL, sL);
@@ -329,6 +336,7 @@ struct IluDev {
             dLq.upload(sL.qStart, s);
             dUs.upload(sU.sStart, s);
             dUq.upload(sU.qStart, s);
+            updateFirstCopy();
             // factor rows (F numbering) per level, long rows apart
             const HostBsr& F = mc ? fpat : S;
             std::vector<int> sh, lo;
@@ -403,6 +411,11 @@ struct IluDev {
         fgIdx.upload(idx, s);
         fgIdxOff.upload(off, s);
     }
+    // the first level's forward sweep is a copy when it runs alone and none of its rows has lower entries
+    void updateFirstCopy() {
+        firstCopy = lstart.size() > 1 && !runsL.empty() && !runsL[0].fused && runsL[0].l1 == 1 && sL.qStart[1] == sL.qStart[0];
+        for (int sl = sL.sStart[0]; firstCopy && sl < sL.sStart[1]; ++sl) firstCopy = hL.width[size_t(sl)] == 0;
+    }
     void retune(int newFuseMax, hipStream_t s) {
         if (!keepEntries) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a schedule built with keepEntries");
         fuseMax = newFuseMax;
@@ -411,6 +424,7 @@ struct IluDev {
         denseL.build(runsL, keptL, lstart, nb, false, s, rowSys);
         denseU.build(runsU, keptU, lstart, nb, true, s, rowSys);
         buildFactorGroups(s);
+        updateFirstCopy();
         denseL.factor(nb, false, lu.p, have32, s);
         denseU.factor(nb, true, lu.p, have32, s);
         ++generation;
@@ -447,7 +461,8 @@ struct IluDev {
         RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     // fp: IluFp (kIluFp32 and kIluStore32 need the FP32 factors of a factor(..., true))
-    void apply(const double* r, double* z, const double* xp, int fp, hipStream_t s) const {
+    // firstDone: y of the first level (firstCopy) is already r in the working numbering
+    void apply(const double* r, double* z, const double* xp, int fp, hipStream_t s, bool firstDone = false) const {
         const bool f32 = fp != kIluFp64;          // FP32-stored factors
         const bool v32 = fp == kIluFp32;          // FP32 working vectors
         const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
@@ -473,6 +488,7 @@ struct IluDev {
             const LevelRun& run = runsL[ri];
             if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
+            if (l == 0 && firstDone) continue;   // the caller wrote y of the first level
             launchIluForward(s, nb, fp, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
                              gate);
         }
@@ -901,13 +917,16 @@ struct rsh_system {
                           j0 ? C.lv[0]->x1.p : nullptr);
         const double* xp = C.vcycle(s(), a32, j0);
         const bool p32 = a32 && colP32.n > 0;
-        launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p);
+        // the ILU's first level without lower entries: its forward copy of t is done here (FP64 working vector)
+        const bool pre = pc != RSH_PREC_CPR_BJ && ilu && ilu->firstCopy && iluFp(f32) != kIluFp32;
+        launchCprSecond(s(), nb, &spmv64, p32 ? static_cast<const void*>(colP32.p) : static_cast<const void*>(colP.p), p32, rr, xp, C.t.p,
+                        pre ? ilu->iperm.p : nullptr, pre ? ilu->lstart[1] : 0, pre ? ilu->y64.p : nullptr);
         if (pc == RSH_PREC_CPR_BJ) {
             const bool b32 = a32 && bjInv32.n > 0;
             launchBlockJacobi(s(), nb, n, b32 ? static_cast<const void*>(bjInv32.p) : static_cast<const void*>(bjInv.p), b32, C.t.p, xp, z);
             return;
         }
-        ilu->apply(C.t.p, z, xp, iluFp(f32), s());
+        ilu->apply(C.t.p, z, xp, iluFp(f32), s(), pre);
     }
 };
 

@@ -347,3 +347,23 @@ def test_mcbilu0_factors_bit_identical(sysd, orc, name):
     ps = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=np.array(rp, np.int32), col=np.concatenate(cols).astype(np.int32),
                          val=np.concatenate(vals), rhs=np.zeros(s.n * s.nb))
     assert np.array_equal(d.bilu0_factors(), orc.bilu0(ps, 1).factors())
+
+
+@pytest.mark.parametrize("name", ["layered", "deepwell", "medium"])
+def test_jacobi_weight_option(sysd, orc, name):
+    # rsh_options.jacobi_omega: the V-cycle's damping weight (default DeviceCprAmg's 0.6667); weight 1 converges to the
+    # FP64 oracle solution like the default, and a changed weight changes the apply (the captured apply is rebuilt)
+    s = sysd[name]
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok_o
+    d = rs.System(s)
+    r = np.cos(0.3 * np.arange(s.n * s.nb))
+    zs = []
+    for w in (0.0, 1.0):
+        d.setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=w)
+        zs.append(d.precond(r))
+        x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+        assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+        print(f"{name} weight {w or 0.6667}: {its} its")
+    assert not np.array_equal(zs[0], zs[1])
