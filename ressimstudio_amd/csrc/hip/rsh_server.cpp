@@ -62,10 +62,11 @@ struct rsh_server {
     }
     size_t measure(const Model& m) const {
         const rsh_system& S = *m.S;
-        size_t b = S.val.bytes() + S.spmv64.bytes() + S.spmv32.bytes() + S.colP.bytes() + S.rp.bytes() + S.ci.bytes() + S.diag.bytes();
+        size_t b = S.val.bytes() + S.spmv64.bytes() + S.spmv32.bytes() + S.colP.bytes() + S.colP32.bytes() + S.bjInv.bytes() + S.bjInv32.bytes() +
+                   S.rp.bytes() + S.ci.bytes() + S.diag.bytes();
         if (S.ilu) b += S.ilu->bytes();
         if (S.cpr) b += S.cpr->bytes();
-        for (const DBuf<double>* v : {&S.vb, &S.vx, &S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh, &S.w, &S.irR, &S.irD, &S.V, &S.Z})
+        for (const DBuf<double>* v : {&S.vb, &S.vx, &S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh, &S.w, &S.irR, &S.irD, &S.irX, &S.V, &S.Z})
             b += v->bytes();
         return b + m.base.bytes() + m.x.bytes() + m.b.bytes();
     }
