@@ -155,8 +155,10 @@ def batched_leg(rs, gen, args):
         rs._check(rc, "batch solve")
 
     xs = {}
-    for name, prec in (("batched_mixed", rs.MIXED), ("batched_store32", rs.MIXED_STORE), ("batched_fp64", rs.FP64)):
-        o = rs.default_options(precond="mcbilu0", precision=prec)
+    legs = (("batched_mixed", "mcbilu0", rs.MIXED), ("batched_store32", "mcbilu0", rs.MIXED_STORE), ("batched_fp64", "mcbilu0", rs.FP64),
+            ("batched_cpr_fp64", "cpr_mc", rs.FP64), ("batched_cpr_store32", "cpr_mc", rs.MIXED_STORE))
+    for name, pc, prec in legs:
+        o = rs.default_options(precond=pc, precision=prec, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
         t = time.perf_counter()
         bstep(o)                      # symbolic set-up and tuning (once per pattern) outside the timed region
         log(f"{name}: first step (symbolic + tuning) {time.perf_counter() - t:.1f}s")
@@ -175,45 +177,53 @@ def batched_leg(rs, gen, args):
                      "latency_p50_ms": float(np.percentile(lat, 50)), "latency_p99_ms": float(np.percentile(lat, 99)),
                      "iterations_mean": float(its.mean()), "iterations_max": int(its.max()), "sweeps": info["last_sweeps"],
                      "outer": info["last_outer"], "max_rel_res": float(rel.max()), "all_converged": bool((st == 0).all()),
-                     "chunk": info["chunk"], "fuse_max": [g.get("fuse_max") for g in info["groups"]]}
+                     "chunk": info["chunk"], "fuse_max": [g.get("fuse_max") for g in info["groups"]],
+                     "amg": [[g.get("amg_levels"), g.get("amg_coarsest")] for g in info["groups"]] if pc.startswith("cpr") else None}
         log(f"{name}: {m / dt:.1f} solves/s ({dt * 1e3:.1f} ms per batch: setup {info['last_setup_ms']:.1f} + solve {info['last_solve_ms']:.1f}), "
             f"latency p50 {out[name]['latency_p50_ms']:.1f} ms p99 {out[name]['latency_p99_ms']:.1f} ms, its mean {its.mean():.0f} max {its.max()}")
     B.close()
-    # the single-system path looped over the same systems (resident inputs, same preconditioner, FP64)
-    o = rs.default_options(precond="mcbilu0", precision=rs.FP64)
-    sysd, devs = [], []
-    t = time.perf_counter()
-    for s in systems:
-        d = rs.System(s, ctx)
-        dv = DeviceInputs(rs, ctx, s)
-        step(rs, d, dv, o, args.tol, args.max_it)
-        sysd.append(d)
-        devs.append(dv)
-    log(f"loop: symbolic set-up of {m} systems {time.perf_counter() - t:.1f}s")
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    lits = []
-    for d, dv in zip(sysd, devs):
-        k, _ = step(rs, d, dv, o, args.tol, args.max_it)
-        lits.append(k)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    err = 0.0
-    off = 0
-    for s, dv in zip(systems, devs):
-        xl = dv.x_host()
-        n = s.n * s.nb
-        err = max(err, float(np.linalg.norm(xs["batched_fp64"][off:off + n] - xl) / max(np.linalg.norm(xl), 1e-300)))
-        off += n
-    out["loop_fp64"] = {"solves_per_s": m / dt, "ms_total": dt * 1e3, "iterations_mean": float(np.mean(lits)),
-                        "max_rel_diff_batched_fp64_vs_loop": err}
+
+    def loop_leg(name, o, ref):
+        """the single-system path looped over the same systems (resident inputs, same options)"""
+        sysd, devs = [], []
+        t = time.perf_counter()
+        for s in systems:
+            d = rs.System(s, ctx)
+            dv = DeviceInputs(rs, ctx, s)
+            step(rs, d, dv, o, args.tol, args.max_it)
+            sysd.append(d)
+            devs.append(dv)
+        log(f"{name}: symbolic set-up of {m} systems {time.perf_counter() - t:.1f}s")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        lits = []
+        for d, dv in zip(sysd, devs):
+            k, _ = step(rs, d, dv, o, args.tol, args.max_it)
+            lits.append(k)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        err = 0.0
+        off = 0
+        for s, dv in zip(systems, devs):
+            xl = dv.x_host()
+            n = s.n * s.nb
+            err = max(err, float(np.linalg.norm(xs[ref][off:off + n] - xl) / max(np.linalg.norm(xl), 1e-300)))
+            off += n
+        out[name] = {"solves_per_s": m / dt, "ms_total": dt * 1e3, "iterations_mean": float(np.mean(lits)),
+                     f"max_rel_diff_{ref}_vs_loop": err}
+        log(f"{name}: {m / dt:.1f} solves/s ({dt * 1e3:.0f} ms for {m} systems), {ref} vs loop max rel diff {err:.2e}")
+        for d, dv in zip(sysd, devs):
+            dv.free()
+            d.close()
+
+    loop_leg("loop_fp64", rs.default_options(precond="mcbilu0", precision=rs.FP64), "batched_fp64")
+    loop_leg("loop_cpr_fp64", rs.default_options(precond="cpr_mc", precision=rs.FP64, smoother=rs.SMOOTH_JACOBI, smooth_steps=2,
+                                                 jacobi_omega=1.0), "batched_cpr_fp64")
     out["speedup_mixed_vs_loop"] = out["batched_mixed"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
     out["speedup_store32_vs_loop"] = out["batched_store32"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
     out["speedup_fp64_vs_loop"] = out["batched_fp64"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
-    log(f"loop_fp64: {m / dt:.1f} solves/s ({dt * 1e3:.0f} ms for {m} systems), batched vs loop max rel diff {err:.2e}")
-    for d, dv in zip(sysd, devs):
-        dv.free()
-        d.close()
+    out["speedup_cpr_fp64_vs_loop_cpr"] = out["batched_cpr_fp64"]["solves_per_s"] / out["loop_cpr_fp64"]["solves_per_s"]
+    out["speedup_cpr_store32_vs_loop_cpr"] = out["batched_cpr_store32"]["solves_per_s"] / out["loop_cpr_fp64"]["solves_per_s"]
     for p in ptrs:
         L.rsh_device_free(ctx.h, p)
     return out

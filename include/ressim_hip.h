@@ -150,10 +150,14 @@ int rsh_wells_reduced_values(rsh_wells* w, double* out);                        
 /* ---- batched solves of many independent systems (no counterpart in the reference, whose engine solves one
  * system per call; this serves its ensemble callers, runBatch study.cpp:326-364, in one launch stream).
  * Systems of one block size are concatenated into one block-diagonal resident system; each system keeps its own
- * BiCGSTAB scalars and convergence, and its preconditioner is the one it gets alone (BILU0 or MC-BILU0; FP64,
- * FP32 with refinement (RSH_MIXED) or FP32-stored (RSH_MIXED_STORE)).  One BiCGSTAB iteration over all systems is
- * one captured HIP graph.  The level
- * chunking of the sweeps and the entries per workgroup of the vector kernels are tuned at the first setup. */
+ * BiCGSTAB scalars and convergence.  Every preconditioner of the single-system path is accepted (BILU0, MC-BILU0
+ * and the three CPR forms; FP64, FP32 with refinement (RSH_MIXED) or FP32-stored (RSH_MIXED_STORE)): the ILU forms
+ * are the ones each system gets alone; CPR builds one pressure hierarchy per block-size group whose aggregates
+ * never span two systems, so the preconditioner stays block-diagonal (a system's result does not depend on the
+ * others) while the group's coarsest level holds every system's coarsest aggregates.  BiCGSTAB only (FGMRES and
+ * RSH_MIXED_IR are rejected).  One BiCGSTAB iteration over all systems is one captured HIP graph.  The level
+ * chunking of the sweeps is tuned per group (cached), the entries per workgroup of the vector kernels at the
+ * first setup. */
 int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, const int* const* row_ptr, const int* const* col,
                      rsh_batch** out);
 int rsh_batch_destroy(rsh_batch* batch);

@@ -115,7 +115,10 @@ struct rsh_batch {
     }
     std::vector<int> currentKey() const {
         std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0};
-        for (auto& g : groups) k.push_back(g.S->ilu ? g.S->ilu->generation : -1);
+        for (auto& g : groups) {
+            k.push_back(g.S->ilu ? g.S->ilu->generation : -1);
+            k.push_back(g.S->cpr ? g.S->cpr->generation : -1);
+        }
         return k;
     }
     void iterationGraphed() {
@@ -333,17 +336,25 @@ int log2Bucket(long long v) {
 // Tunes (1) the level chunking of each group's ILU sweeps: the largest level (in tasks) swept inside a run of small
 // levels, i.e. how many levels one workgroup or one dense-group pass covers; (2) the entries per workgroup of the
 // batched vector kernels and reductions (rows per wavefront = entries / 256 threads).  Candidates are timed on the
-// resident data; the fastest wins.
+// resident data; the fastest wins.  Runs at every set-up: the choices are cached per (block size, ordering, size
+// bucket), so a set-up that builds new factors (another preconditioner) only looks its choice up; groups whose
+// preconditioner has no ILU (block-Jacobi second stage) are skipped.
 void rsh_batch::tune() {
     hipStream_t st = s();
     std::lock_guard<std::mutex> lk(g_tuneMu);
-    // every system running while the candidates are timed
-    std::vector<double> tol(sys.size(), 1e-8);
-    RSH_HIP(hipMemcpyAsync(tolD.p, tol.data(), sizeof(double) * tol.size(), hipMemcpyHostToDevice, st));
-    launchBInit(st, int(sys.size()), tolD.p, ks.p);
-    fuseChosen.assign(groups.size(), 16);
+    bool armed = false;
+    auto arm = [&] {
+        // every system running while the candidates are timed
+        if (armed) return;
+        std::vector<double> tol(sys.size(), 1e-8);
+        RSH_HIP(hipMemcpyAsync(tolD.p, tol.data(), sizeof(double) * tol.size(), hipMemcpyHostToDevice, st));
+        launchBInit(st, int(sys.size()), tolD.p, ks.p);
+        armed = true;
+    };
+    fuseChosen.assign(groups.size(), 0);
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         Group& g = groups[gi];
+        if (!g.S->ilu) continue;
         IluDev& I = *g.S->ilu;
         const auto key = std::make_pair(g.nb * 1000 + (I.mc ? 1 : 0), log2Bucket(g.S->n));
         auto it = g_fuseCache.find(key);
@@ -351,6 +362,7 @@ void rsh_batch::tune() {
         if (it != g_fuseCache.end()) {
             best = it->second;
         } else {
+            arm();
             double bestMs = 1e300;
             launchFill(st, p.p + g.off, 1.0, g.N);
             for (int cand : {4, 16, 64}) {
@@ -363,27 +375,30 @@ void rsh_batch::tune() {
         if (I.fuseMax != best) I.retune(best, st);
         fuseChosen[gi] = best;
     }
-    const int ck = log2Bucket(N);
-    auto it = g_chunkCache.find(ck);
-    if (it != g_chunkCache.end()) {
-        if (chunk != it->second) buildTasks(it->second);
-    } else {
-        double bestMs = 1e300;
-        int best = chunk;
-        launchFill(st, r.p, 1.0, N);
-        launchFill(st, v.p, 1.0, N);
-        for (int cand : {2048, 8192, 32768}) {
-            buildTasks(cand);
-            launchBInit(st, int(sys.size()), tolD.p, ks.p);
-            const double ms = timeMs([&] {
-                launchBDots(st, T, task.p, 2, r.p, v.p, r.p, r.p, partial.p);
-                launchBBicgP(st, T, task.p, ks.p, r.p, v.p, p.p);
-                launchBBicgS(st, T, task.p, ks.p, r.p, v.p, sv.p, partial.p);
-            }, 5);
-            if (ms < bestMs) { bestMs = ms; best = cand; }
+    if (!tuned) {
+        const int ck = log2Bucket(N);
+        auto it = g_chunkCache.find(ck);
+        if (it != g_chunkCache.end()) {
+            if (chunk != it->second) buildTasks(it->second);
+        } else {
+            arm();
+            double bestMs = 1e300;
+            int best = chunk;
+            launchFill(st, r.p, 1.0, N);
+            launchFill(st, v.p, 1.0, N);
+            for (int cand : {2048, 8192, 32768}) {
+                buildTasks(cand);
+                launchBInit(st, int(sys.size()), tolD.p, ks.p);
+                const double ms = timeMs([&] {
+                    launchBDots(st, T, task.p, 2, r.p, v.p, r.p, r.p, partial.p);
+                    launchBBicgP(st, T, task.p, ks.p, r.p, v.p, p.p);
+                    launchBBicgS(st, T, task.p, ks.p, r.p, v.p, sv.p, partial.p);
+                }, 5);
+                if (ms < bestMs) { bestMs = ms; best = cand; }
+            }
+            g_chunkCache[ck] = best;
+            if (chunk != best) buildTasks(best);
         }
-        g_chunkCache[ck] = best;
-        if (chunk != best) buildTasks(best);
     }
     RSH_HIP(hipStreamSynchronize(st));
     tuned = true;
@@ -528,8 +543,6 @@ int rsh_batch_set_values_device(rsh_batch* batch, const double* val_dev) {
 int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt) {
     return guarded([&] {
         if (!batch || !opt) return fail(RSH_ERR_ARG, "null argument");
-        if (opt->precond != RSH_PREC_BILU0 && opt->precond != RSH_PREC_MCBILU0)
-            return fail(RSH_ERR_UNSUPPORTED, "batched solves support the BILU0 and MC-BILU0 preconditioners");
         if (opt->precision != RSH_FP64 && opt->precision != RSH_MIXED && opt->precision != RSH_MIXED_STORE)
             return fail(RSH_ERR_UNSUPPORTED, "batched solves run FP64 BiCGSTAB with an FP64, FP32 or FP32-stored preconditioner");
         if (opt->krylov != RSH_KRYLOV_BICGSTAB) return fail(RSH_ERR_UNSUPPORTED, "batched solves use BiCGSTAB");
@@ -542,7 +555,7 @@ int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt) {
         batch->opt = *opt;
         if (opt->precision == RSH_MIXED)
             for (auto* bf : {&batch->xo, &batch->rt, &batch->ax, &batch->xPrev, &batch->rPrev}) bf->alloc(size_t(batch->N));
-        if (!batch->tuned) batch->tune();
+        batch->tune();
         RSH_HIP(hipStreamSynchronize(batch->s()));
         batch->ready = true;
         batch->lastSetupMs = nowMs() - t0;
@@ -628,6 +641,9 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
                 o << ",\"ilu_levels\":" << g.S->ilu->sL.levels() << ",\"fuse_max\":" << g.S->ilu->fuseMax << ",\"dense_groups\":["
                   << g.S->ilu->denseL.ngroups << "," << g.S->ilu->denseU.ngroups << "]";
             }
+            if (g.S->cpr && g.S->cpr->built)
+                o << ",\"amg_levels\":" << g.S->cpr->lv.size() << ",\"amg_coarsest\":" << g.S->cpr->m << ",\"amg_dense_coarsest\":"
+                  << (g.S->cpr->dense ? 1 : 0);
             o << "}";
         }
         o << "],\"last_sweeps\":" << batch->lastSweeps << ",\"last_outer\":" << batch->lastOuter << ",\"last_fallback_fp64\":" << (batch->lastFallback ? 1 : 0) << ",\"last_solve_ms\":" << batch->lastSolveMs << ",\"last_setup_ms\":" << batch->lastSetupMs

@@ -49,6 +49,44 @@ def test_batch_matches_oracle_per_system(systems, orc, precond, precision):
     assert np.all(lat > 0) and np.all(lat <= info["last_solve_ms"] * (1 + 1e-5) + 1e-3)
 
 
+@pytest.mark.parametrize("precond,smoother", [("cpr_mc", rs.SMOOTH_JACOBI), ("cpr", rs.SMOOTH_GS), ("cpr_bj", rs.SMOOTH_JACOBI)])
+@pytest.mark.parametrize("precision", [rs.FP64, rs.MIXED, rs.MIXED_STORE])
+def test_batch_cpr_matches_oracle_per_system(systems, orc, precond, smoother, precision):
+    """CPR inside the batch: one pressure hierarchy per block-size group, built on the group's block-diagonal
+    matrix (aggregates never span two systems); every system converges and matches the oracle's FP64 solve."""
+    B = rs.Batch(systems).setup(precond=precond, precision=precision, smoother=smoother, smooth_steps=2, jacobi_omega=1.0)
+    xs, its, rel, st = B.solve([s.rhs for s in systems], TOL, 3000)
+    info = B.info()
+    for g in info["groups"]:
+        assert g["amg_levels"] >= 1 and g["amg_coarsest"] >= 1
+    for k, s in enumerate(systems):
+        xo, _, _, ok_o = orc.krylov(s, s.rhs, "mcbilu0", "bicgstab", TOL, 3000)
+        assert st[k] == rs.OK and ok_o
+        assert _true_rel(orc, s, xs[k]) <= 1.01 * TOL, k
+        assert np.linalg.norm(xs[k] - xo) <= 1e-6 * np.linalg.norm(xo), k
+    # the pressure correction pays: fewer iterations than the ILU-only batch on the same systems
+    B2 = rs.Batch(systems).setup(precond="mcbilu0", precision=precision)
+    _, its2, _, _ = B2.solve([s.rhs for s in systems], TOL, 3000)
+    assert its.sum() < its2.sum()
+
+
+@pytest.mark.parametrize("precond", ["mcbilu0", "cpr_mc"])
+def test_batch_systems_do_not_interact(systems, precond):
+    """A system's result does not depend on the other systems of its batch: a new right-hand side for system 0
+    leaves every other system's solution, iteration count and residual bit-identical (block-diagonal
+    preconditioner, per-system Krylov scalars, converged systems frozen)."""
+    B = rs.Batch(systems).setup(precond=precond, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
+    bs = [s.rhs.copy() for s in systems]
+    x1, i1, r1, s1 = B.solve(bs, TOL, 3000)
+    bs[0] = 3.0 * bs[0] + np.roll(bs[0], 7)
+    x2, i2, r2, s2 = B.solve(bs, TOL, 3000)
+    assert s2[0] == rs.OK
+    for k in range(1, len(systems)):
+        assert s1[k] == rs.OK and s2[k] == rs.OK
+        assert i1[k] == i2[k] and r1[k] == r2[k], k
+        assert np.array_equal(x1[k], x2[k]), k
+
+
 def test_batch_equals_single_system_path(systems):
     B = rs.Batch(systems).setup(precond="mcbilu0")
     xs, its, _, _ = B.solve([s.rhs for s in systems], TOL, 3000)
@@ -93,4 +131,6 @@ def test_batch_new_values_same_pattern(systems, orc):
 def test_batch_rejects_unsupported_options(systems):
     B = rs.Batch(systems[:2])
     with pytest.raises(rs.RshError):
-        B.setup(precond="cpr")
+        B.setup(precond="cpr", krylov=rs.KRYLOV_FGMRES)
+    with pytest.raises(rs.RshError):
+        B.setup(precond="cpr", precision=rs.MIXED_IR)
