@@ -201,7 +201,7 @@ constexpr int kDenseThreads = 1024;
 void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
-                    const KScalars* gate = nullptr);
+                    const KScalars* gate = nullptr, void* ycopy = nullptr);
 // numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);

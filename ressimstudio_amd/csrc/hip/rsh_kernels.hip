@@ -722,14 +722,16 @@ __global__ __launch_bounds__(256) void kRunOutside(int t0, int ns, int u0, int n
 
 // group g: rows rows[start[g] .. start[g+1]) (working numbering, ascending), dense K x K at off[g], K = k NB, column-major
 // with exact zeros outside the block triangle.  One workgroup per 64-row tile of a group: its 16 wavefronts split the
-// tile's non-zero columns, each lane one row; the partial sums are added in a fixed order.
+// tile's non-zero columns, each lane one row; the partial sums are added in a fixed order.  With ycopy given (forward
+// only) the result is also written in the run's local numbering, where the backward product of the same run reads
+// its input when the run's rows have no upper couplings outside the run (the backward outside pass would only copy y).
 template <int NB, typename TM, typename TV, bool FWD>
 __global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restrict__ tasks, const int* __restrict__ start,
                                                           const int* __restrict__ rows, const long long* __restrict__ off,
                                                           const TM* __restrict__ inv, int wbase, const TV* __restrict__ rt, TV* y,
                                                           double* __restrict__ z, const int* __restrict__ perm,
                                                           const double* __restrict__ xp, const int* __restrict__ gowner,
-                                                          const KScalars* __restrict__ gate) {
+                                                          const KScalars* __restrict__ gate, TV* __restrict__ ycopy) {
     constexpr int W = kDenseThreads / kWave;
     __shared__ TV v[kDenseMaxRows * NB];
     __shared__ TV part[W][kWave];
@@ -762,6 +764,7 @@ __global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restric
     for (int w = 0; w < W; ++w) out += part[w][lane];
     const int blk = a / NB, w = rows[r0 + blk], e = a - blk * NB;
     y[(long long)w * NB + e] = out;
+    if (FWD && ycopy) ycopy[(w - wbase) * NB + e] = out;
     if (!FWD) {
         const int o = perm[w];
         double val = double(out);
@@ -1942,24 +1945,26 @@ void runOutsideT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int t0, int ns,
 }
 template <int NB, typename TM, typename TV>
 void runDenseT(hipStream_t s, bool fwd, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const TM* inv,
-               int wbase, const TV* rt, TV* y, double* z, const int* perm, const double* xp, const int* gowner, const KScalars* gate) {
+               int wbase, const TV* rt, TV* y, double* z, const int* perm, const double* xp, const int* gowner, const KScalars* gate,
+               TV* ycopy) {
     if (fwd) hipLaunchKernelGGL((kRunDense<NB, TM, TV, true>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp,
-                                gowner, gate);
+                                gowner, gate, ycopy);
     else hipLaunchKernelGGL((kRunDense<NB, TM, TV, false>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, off, inv, wbase, rt, y, z, perm, xp,
-                            gowner, gate);
+                            gowner, gate, static_cast<TV*>(nullptr));
 }
 void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
-                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner, const KScalars* gate) {
+                    const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner, const KScalars* gate,
+                    void* ycopy) {
     if (ns + nq > 0) {
         if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt, gate)))
         else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
         else RSH_NB_DISPATCH(nb, (runOutsideT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
         checkLaunch();
     }
-    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate)))
-    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, double>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate)))
-    else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate)))
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate, (float*)ycopy)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, double>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
+    else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
     checkLaunch();
 }
 template <int NB>

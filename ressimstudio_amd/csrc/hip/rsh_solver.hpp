@@ -98,6 +98,7 @@ void packSlices(hipStream_t s, int nb, const double* src, SliceDev<TA>& d, long 
 struct DenseSide {
     std::vector<int> runTask0, runTask1;   // per run: its range in tasks, or -1 when swept level by level
     std::vector<int> runGroup0, runGroup1; // per run: its range of groups, or -1
+    std::vector<char> noOutside;           // per run: no entry of its rows couples to a row outside the run
     std::vector<int> hStart, hRows;        // host copies of start / rows (working rows, ascending per group)
     int ngroups = 0, nmap = 0;
     long long elems = 0;
@@ -123,6 +124,7 @@ struct DenseSide {
         runTask1.assign(runs.size(), -1);
         runGroup0.assign(runs.size(), -1);
         runGroup1.assign(runs.size(), -1);
+        noOutside.assign(runs.size(), 0);
         constexpr long long kBudget = 1ll << 26;   // dense elements per direction (512 MiB in FP64)
         for (size_t ri = 0; ri < runs.size(); ++ri) {
             const LevelRun& run = runs[ri];
@@ -159,6 +161,11 @@ struct DenseSide {
             // levels deep (a level step costs about 2.5 us, the dense product about 4 us + 10 ns per scalar row)
             const double levelCost = 2.5 * (run.l1 - run.l0), denseCost = 4.0 + 0.005 * double(maxRows) * nb;
             if (maxRows > size_t(kDenseMaxRows) || elems + add > kBudget || denseCost >= levelCost) continue;
+            bool inside = true;
+            for (int w = wb; inside && w < we; ++w)
+                for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p)
+                    if (E.col[size_t(p)] < wb || E.col[size_t(p)] >= we) { inside = false; break; }
+            noOutside[ri] = inside ? 1 : 0;
             runTask0[ri] = int(htasks.size());
             runGroup0[ri] = ngroups;
             for (const auto& g : groups) {
@@ -245,6 +252,13 @@ struct IluDev {
     DenseSide denseL, denseU;
     DBuf<double> rt64;                  // outside contributions of the run being swept
     DBuf<float> rt32;
+    // the last forward and the first backward step are the same run of dense groups whose rows have no upper
+    // couplings outside it (the well-clique colours at the end of the multicolour order): the forward product
+    // writes its result in the run's numbering (tail64 / tail32) and the backward product starts from it, without
+    // the backward outside pass (which would only copy y)
+    bool tailCopy = false;
+    DBuf<double> tail64;
+    DBuf<float> tail32;
     // level chunking: levels with at most fuseMax tasks are swept in runs (one workgroup or dense groups)
     int fuseMax = 16;
     // batched systems: original row -> system, and the systems' BiCGSTAB states (slices of stopped systems skipped)
@@ -358,6 +372,7 @@ struct IluDev {
         y64.alloc(size_t(n + 1) * nb);
         y64.zero(s);
         rt64.alloc(size_t(kRunRowsMax) * nb);
+        tail64.alloc(size_t(kRunRowsMax) * nb);
         nreg.alloc(1);
     }
     void ensure32(hipStream_t s) {
@@ -367,6 +382,7 @@ struct IluDev {
         y32.alloc(size_t(n + 1) * nb);
         y32.zero(s);
         rt32.alloc(size_t(kRunRowsMax) * nb);
+        tail32.alloc(size_t(kRunRowsMax) * nb);
         have32 = true;
     }
     // new level chunking on the built schedule (needs keepEntries at build); the dense groups are refactored from
@@ -415,6 +431,12 @@ struct IluDev {
     void updateFirstCopy() {
         firstCopy = lstart.size() > 1 && !runsL.empty() && !runsL[0].fused && runsL[0].l1 == 1 && sL.qStart[1] == sL.qStart[0];
         for (int sl = sL.sStart[0]; firstCopy && sl < sL.sStart[1]; ++sl) firstCopy = hL.width[size_t(sl)] == 0;
+        tailCopy = false;
+        if (!runsL.empty() && !runsU.empty()) {
+            const size_t a = runsL.size() - 1, b = runsU.size() - 1;
+            tailCopy = runsL[a].fused && runsU[b].fused && runsL[a].l0 == runsU[b].l0 && runsL[a].l1 == runsU[b].l1 && denseL.runTask0[a] >= 0 &&
+                       denseU.runTask0[b] >= 0 && denseU.noOutside[b];
+        }
     }
     void retune(int newFuseMax, hipStream_t s) {
         if (!keepEntries) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a schedule built with keepEntries");
@@ -469,6 +491,7 @@ struct IluDev {
         const void* Up = f32 ? static_cast<const void*>(&U32) : static_cast<const void*>(&U64);
         void* y = v32 ? static_cast<void*>(y32.p) : static_cast<void*>(y64.p);
         void* rt = v32 ? static_cast<void*>(rt32.p) : static_cast<void*>(rt64.p);
+        void* tail = v32 ? static_cast<void*>(tail32.p) : static_cast<void*>(tail64.p);
         auto sweepRun = [&](const LevelRun& run, size_t ri, bool fwd) {
             const DenseSide& D = fwd ? denseL : denseU;
             const LevelSched& sc = fwd ? sL : sU;
@@ -477,9 +500,12 @@ struct IluDev {
             if (D.runTask0[ri] >= 0) {
                 const int t0 = sc.sStart[size_t(run.l0)], u0 = sc.qStart[size_t(run.l0)];
                 const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
-                launchRunDense(s, nb, fp, sl, fwd, t0, sc.sStart[size_t(run.l1)] - t0, u0, sc.qStart[size_t(run.l1)] - u0, wb, we,
-                               D.runTask1[ri] - D.runTask0[ri], D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r,
-                               rt, y, z, xp, D.gowner.p, gate);
+                const bool tailStep = tailCopy && ri + 1 == (fwd ? runsL.size() : runsU.size());
+                const bool skipOutside = tailStep && !fwd;
+                launchRunDense(s, nb, fp, sl, fwd, t0, skipOutside ? 0 : sc.sStart[size_t(run.l1)] - t0, u0,
+                               skipOutside ? 0 : sc.qStart[size_t(run.l1)] - u0, wb, we, D.runTask1[ri] - D.runTask0[ri],
+                               D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r, skipOutside ? tail : rt, y, z, xp,
+                               D.gowner.p, gate, tailStep && fwd ? tail : nullptr);
                 return;
             }
             launchIluFused(s, nb, fp, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
