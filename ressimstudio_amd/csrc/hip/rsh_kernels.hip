@@ -178,9 +178,37 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
         double acc[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) acc[e] = 0;
-        for (int k = 0; k < inf.x; ++k) {
+        // two slots per round: both slots' columns and blocks, then both x gathers, are in flight before the first
+        // product; the products are formed and added slot by slot in column order as before
+        int k = 0;
+        for (; k + 2 <= inf.x; k += 2) {
             const long long slot = (long long)inf.y + k;
-            // matrix values and columns are streamed once: non-temporal loads keep the caches for the x gathers
+            double av[2][NB * NB], xv[2][NB];
+            int c[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                // matrix values and columns are streamed once: non-temporal loads keep the caches for the x gathers
+                c[u] = slotCol<CT, true>(col, (slot + u) * kWave + lane, r);
+                const TA* a = val + (slot + u) * (NB * NB * kWave) + lane;
+#pragma unroll
+                for (int ef = 0; ef < NB * NB; ++ef) av[u][ef] = double(__builtin_nontemporal_load(a + ef * kWave));
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int f = 0; f < NB; ++f) xv[u][f] = x[(long long)c[u] * NB + f];
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int e = 0; e < NB; ++e) {
+                    double s = 0;
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) s += av[u][e * NB + f] * xv[u][f];
+                    acc[e] += s;
+                }
+        }
+        for (; k < inf.x; ++k) {
+            const long long slot = (long long)inf.y + k;
             const int c = slotCol<CT, true>(col, slot * kWave + lane, r);
             const TA* a = val + slot * (NB * NB * kWave) + lane;
             double xv[NB];
@@ -226,6 +254,55 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
 }
 
 // ---------------------------------------------------------------- BILU0 level sweeps
+// One lane's slots of a sweep: acc_e -= sum_f a_ef y_col,f per block in slot (column) order, s_e formed first as
+// in the block products of bsr.cpp:14-56.  Two slots per round: both slots' columns and blocks, then both
+// gathers of y, are in flight before the first product; the arithmetic is the same slot by slot.
+template <int NB, typename TA, typename TV>
+__device__ inline void sliceRowSweep(const int* __restrict__ col, const TA* __restrict__ val, int2 inf, int lane, const TV* y, TV (&acc)[NB]) {
+#pragma clang fp contract(off)
+    int k = 0;
+    for (; k + 2 <= inf.x; k += 2) {
+        const long long slot = (long long)inf.y + k;
+        TV av[2][NB * NB], yk[2][NB];
+        int c[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            c[u] = __builtin_nontemporal_load(col + (slot + u) * kWave + lane);
+            const TA* a = val + (slot + u) * (NB * NB * kWave) + lane;
+#pragma unroll
+            for (int ef = 0; ef < NB * NB; ++ef) av[u][ef] = TV(__builtin_nontemporal_load(a + ef * kWave));
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int f = 0; f < NB; ++f) yk[u][f] = y[(long long)c[u] * NB + f];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV s = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) s += av[u][e * NB + f] * yk[u][f];
+                acc[e] -= s;
+            }
+    }
+    for (; k < inf.x; ++k) {
+        const long long slot = (long long)inf.y + k;
+        const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
+        const TA* a = val + slot * (NB * NB * kWave) + lane;
+        TV yk[NB];
+#pragma unroll
+        for (int f = 0; f < NB; ++f) yk[f] = y[(long long)c * NB + f];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            TV s = 0;
+#pragma unroll
+            for (int f = 0; f < NB; ++f) s += TV(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * yk[f];
+            acc[e] -= s;
+        }
+    }
+}
+
 // Forward (bsr.cpp:575-583): y_i = r_i - sum_{k<i} L_ik y_k, y in the working (level-ordered) numbering, r read
 // through the original row index.  TA: factor storage, TV: working-vector type (accumulation type).
 template <int NB, typename TA, typename TV>
@@ -250,21 +327,7 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
         TV acc[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) acc[e] = TV(r[(long long)o * NB + e]);
-        for (int k = 0; k < inf.x; ++k) {
-            const long long slot = (long long)inf.y + k;
-            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
-            const TA* a = val + slot * (NB * NB * kWave) + lane;
-            TV yk[NB];
-#pragma unroll
-            for (int f = 0; f < NB; ++f) yk[f] = y[(long long)c * NB + f];
-#pragma unroll
-            for (int e = 0; e < NB; ++e) {
-                TV s = 0;
-#pragma unroll
-                for (int f = 0; f < NB; ++f) s += TV(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * yk[f];
-                acc[e] -= s;
-            }
-        }
+        sliceRowSweep<NB, TA, TV>(col, val, inf, lane, y, acc);
 #pragma unroll
         for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
         return;
@@ -325,21 +388,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
         const int2 inf = info[sl];
 #pragma unroll
         for (int e = 0; e < NB; ++e) acc[e] = y[(long long)wr * NB + e];
-        for (int k = 0; k < inf.x; ++k) {
-            const long long slot = (long long)inf.y + k;
-            const int c = __builtin_nontemporal_load(col + slot * kWave + lane);
-            const TA* a = val + slot * (NB * NB * kWave) + lane;
-            TV zj[NB];
-#pragma unroll
-            for (int f = 0; f < NB; ++f) zj[f] = y[(long long)c * NB + f];
-#pragma unroll
-            for (int e = 0; e < NB; ++e) {
-                TV s = 0;
-#pragma unroll
-                for (int f = 0; f < NB; ++f) s += TV(__builtin_nontemporal_load(a + (e * NB + f) * kWave)) * zj[f];
-                acc[e] -= s;
-            }
-        }
+        sliceRowSweep<NB, TA, TV>(col, val, inf, lane, y, acc);
         d = dval + (long long)sl * (NB * NB * kWave) + lane;
         dstride = kWave;
     } else {
@@ -1173,7 +1222,28 @@ __global__ __launch_bounds__(kBlock) void kCprSecond(int nslices, const int2* __
         double acc[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) acc[e] = r[(long long)rr * NB + e];
-        for (int k = 0; k < inf.x; ++k) {
+        // four slots per round: columns and pressure-column values, then the x_p gathers, all in flight before
+        // the first product; the updates are applied slot by slot as before
+        int k = 0;
+        for (; k + 4 <= inf.x; k += 4) {
+            const long long slot = (long long)inf.y + k;
+            double av[4][NB], xk[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c[u] = slotCol<CT, true>(col, (slot + u) * kWave + lane, rr);
+                const TA* a = colP + (slot + u) * (NB * kWave) + lane;
+#pragma unroll
+                for (int e = 0; e < NB; ++e) av[u][e] = double(__builtin_nontemporal_load(a + e * kWave));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xk[u] = xp[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] -= av[u][e] * xk[u];
+        }
+        for (; k < inf.x; ++k) {
             const long long slot = (long long)inf.y + k;
             const double xk = xp[slotCol<CT, true>(col, slot * kWave + lane, rr)];
             const TA* a = colP + slot * (NB * kWave) + lane;
@@ -1393,7 +1463,24 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
         const JacSelf self = jacSelf<MODE>(i, dinv, b, x1);
         const int2 inf = info[wid];
         double acc = 0;
-        for (int k = 0; k < inf.x; ++k) {
+        // four slots per round: their column and value loads, then their x gathers, are all in flight before the
+        // first product (the products are still added one by one in slot order)
+        int k = 0;
+        for (; k + 4 <= inf.x; k += 4) {
+            const long long at = ((long long)inf.y + k) * kWave + lane;
+            double v[4], xv[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v[u] = double(val[at + u * kWave]);
+                c[u] = slotCol<CT, false>(col, at + u * kWave, i);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xv[u] = x1[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u] * xv[u];
+        }
+        for (; k < inf.x; ++k) {
             const long long at = ((long long)inf.y + k) * kWave + lane;
             acc += double(val[at]) * x1[slotCol<CT, false>(col, at, i)];
         }
