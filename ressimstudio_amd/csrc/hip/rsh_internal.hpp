@@ -203,7 +203,7 @@ void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
                     const KScalars* gate = nullptr, void* ycopy = nullptr);
 // numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
-void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
+void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRows, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
 // well Schur reduction of a scalar Jacobian on the device (see rsh_wells.cpp for the maps)

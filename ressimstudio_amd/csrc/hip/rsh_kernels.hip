@@ -834,83 +834,112 @@ __global__ void kDenseGather(int count, int nb, const int4* __restrict__ map, co
     raw[off[m.x] + (m.z * nb + f) * K + m.y * nb + e] = lu[(long long)m.w * nb2 + ef];
 }
 
-// one workgroup per group: X = L^-1 (lower) or X = (I + D^-1 U)^-1 D^-1 (upper; D^-1 on raw's diagonal blocks)
-template <int NB, bool UPPER>
-__global__ __launch_bounds__(256) void kDenseInvert(const int* __restrict__ start, const long long* __restrict__ off, double* raw,
-                                                    double* inv) {
-    const int g = blockIdx.x;
+// dense NB x NB blocks (i, j) of a K x K column-major matrix
+template <int NB>
+__device__ inline void denseBlk(const double* B, long long K, int i, int j, double (&o)[NB][NB]) {
+#pragma unroll
+    for (int e = 0; e < NB; ++e)
+#pragma unroll
+        for (int f = 0; f < NB; ++f) o[e][f] = B[(j * NB + f) * K + i * NB + e];
+}
+template <int NB>
+__device__ inline void densePut(double* B, long long K, int i, int j, const double (&o)[NB][NB]) {
+#pragma unroll
+    for (int e = 0; e < NB; ++e)
+#pragma unroll
+        for (int f = 0; f < NB; ++f) B[(j * NB + f) * K + i * NB + e] = o[e][f];
+}
+template <int NB>
+__device__ inline void denseMul(const double (&a)[NB][NB], const double (&b)[NB][NB], double (&o)[NB][NB]) {
+#pragma unroll
+    for (int e = 0; e < NB; ++e)
+#pragma unroll
+        for (int f = 0; f < NB; ++f) {
+            double s = 0;
+#pragma unroll
+            for (int h = 0; h < NB; ++h) s += a[e][h] * b[h][f];
+            o[e][f] = s;
+        }
+}
+// upper sweep, before its inversion: W_im = D^-1_i U_im (i < m) in place; grid (blocks over k^2, groups)
+template <int NB>
+__global__ __launch_bounds__(256) void kDenseScaleU(const int* __restrict__ start, const long long* __restrict__ off, double* raw) {
+    const int g = blockIdx.y;
     const int k = start[g + 1] - start[g];
     const long long K = (long long)k * NB;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * k) return;
+    const int i = t % k, m = t / k;
+    if (i >= m) return;
     double* M = raw + off[g];
+    double d[NB][NB], u[NB][NB], w[NB][NB];
+    denseBlk<NB>(M, K, i, i, d);
+    denseBlk<NB>(M, K, i, m, u);
+    denseMul<NB>(d, u, w);
+    densePut<NB>(M, K, i, m, w);
+}
+// X = L^-1 (lower) or X = (I + W)^-1 D^-1 (upper, W from kDenseScaleU; D^-1 on raw's diagonal blocks).  Grid
+// (P, groups): workgroup p of a group owns the block columns j = p, p + P, ...; the elimination never couples two
+// columns of X (X_ij -= M_im X_mj), so each block gets the same updates in the same order as in one workgroup.
+template <int NB, bool UPPER>
+__global__ __launch_bounds__(256) void kDenseInvert(const int* __restrict__ start, const long long* __restrict__ off, const double* raw,
+                                                    double* inv) {
+    const int g = blockIdx.y, p = blockIdx.x, P = gridDim.x;
+    const int k = start[g + 1] - start[g];
+    if (p >= k) return;                                  // the whole workgroup
+    const int nc = (k - p + P - 1) / P;                  // block columns owned
+    const long long K = (long long)k * NB;
+    const double* M = raw + off[g];
     double* X = inv + off[g];
-    auto blk = [&](double* B, int i, int j, double (&o)[NB][NB]) {
-#pragma unroll
-        for (int e = 0; e < NB; ++e)
-#pragma unroll
-            for (int f = 0; f < NB; ++f) o[e][f] = B[(j * NB + f) * K + i * NB + e];
-    };
-    auto put = [&](double* B, int i, int j, const double (&o)[NB][NB]) {
-#pragma unroll
-        for (int e = 0; e < NB; ++e)
-#pragma unroll
-            for (int f = 0; f < NB; ++f) B[(j * NB + f) * K + i * NB + e] = o[e][f];
-    };
-    auto mul = [](const double (&a)[NB][NB], const double (&b)[NB][NB], double (&o)[NB][NB]) {
-#pragma unroll
-        for (int e = 0; e < NB; ++e)
-#pragma unroll
-            for (int f = 0; f < NB; ++f) {
-                double s = 0;
-#pragma unroll
-                for (int h = 0; h < NB; ++h) s += a[e][h] * b[h][f];
-                o[e][f] = s;
-            }
-    };
-    for (long long t = threadIdx.x; t < K * K; t += blockDim.x) X[t] = (t / K == t % K) ? 1.0 : 0.0;
-    if (UPPER) {
-        // W_im = D^-1_i U_im (i < m), in place
-        for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
-            const int i = t % k, m = t / k;
-            if (i >= m) continue;
-            double d[NB][NB], u[NB][NB], w[NB][NB];
-            blk(M, i, i, d);
-            blk(M, i, m, u);
-            mul(d, u, w);
-            put(M, i, m, w);
-        }
+    for (long long t = threadIdx.x; t < (long long)nc * NB * K; t += blockDim.x) {
+        const long long c = t / (NB * K), rem = t % (NB * K);
+        const long long col = (p + c * P) * NB + rem / K, row = rem % K;
+        X[col * K + row] = row == col ? 1.0 : 0.0;
     }
     __syncthreads();
     // right-looking elimination: step m finalises block row m of X, then removes its coupling from the rows after
     // it (lower: rows i > m, columns j <= m; upper: rows i < m, columns j >= m)
     for (int s = 0; s < k; ++s) {
         const int m = UPPER ? k - 1 - s : s;
-        const int ni = UPPER ? m : k - 1 - m, nj = UPPER ? k - m : m + 1;
-        for (int t = threadIdx.x; t < ni * nj; t += blockDim.x) {
-            const int i = UPPER ? t / nj : m + 1 + t / nj;
-            const int j = UPPER ? m + t % nj : t % nj;
-            double a[NB][NB], x[NB][NB], p[NB][NB], c[NB][NB];
-            blk(M, i, m, a);
-            blk(X, m, j, x);
-            mul(a, x, p);
-            blk(X, i, j, c);
+        int c0, na, ni, i0;
+        if (UPPER) {
+            c0 = m <= p ? 0 : (m - p + P - 1) / P;
+            na = nc - c0;
+            ni = m;
+            i0 = 0;
+        } else {
+            c0 = 0;
+            na = m >= p ? (m - p) / P + 1 : 0;
+            ni = k - 1 - m;
+            i0 = m + 1;
+        }
+        if (na <= 0 || ni <= 0) continue;                // uniform over the workgroup
+        for (int t = threadIdx.x; t < ni * na; t += blockDim.x) {
+            const int i = i0 + t / na;
+            const int j = p + (c0 + t % na) * P;
+            double a[NB][NB], x[NB][NB], q[NB][NB], c[NB][NB];
+            denseBlk<NB>(M, K, i, m, a);
+            denseBlk<NB>(X, K, m, j, x);
+            denseMul<NB>(a, x, q);
+            denseBlk<NB>(X, K, i, j, c);
 #pragma unroll
             for (int e = 0; e < NB; ++e)
 #pragma unroll
-                for (int f = 0; f < NB; ++f) c[e][f] -= p[e][f];
-            put(X, i, j, c);
+                for (int f = 0; f < NB; ++f) c[e][f] -= q[e][f];
+            densePut<NB>(X, K, i, j, c);
         }
         __syncthreads();
     }
     if (UPPER) {
         // X_ij <- X_ij D^-1_j (i <= j)
-        for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
-            const int i = t % k, j = t / k;
+        for (int t = threadIdx.x; t < k * nc; t += blockDim.x) {
+            const int i = t % k, j = p + (t / k) * P;
             if (i > j) continue;
             double x[NB][NB], d[NB][NB], o[NB][NB];
-            blk(X, i, j, x);
-            blk(M, j, j, d);
-            mul(x, d, o);
-            put(X, i, j, o);
+            denseBlk<NB>(X, K, i, j, x);
+            denseBlk<NB>(M, K, j, j, d);
+            denseMul<NB>(x, d, o);
+            densePut<NB>(X, K, i, j, o);
         }
     }
 }
@@ -2054,21 +2083,29 @@ void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd,
     else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
     checkLaunch();
 }
+// block columns per workgroup: enough workgroups per group that a step's updates are a few blocks per thread
 template <int NB>
-void denseInvertT(hipStream_t s, bool upper, int ngroups, const int* start, const long long* off, double* raw, double* inv) {
-    if (upper) hipLaunchKernelGGL((kDenseInvert<NB, true>), dim3(ngroups), dim3(256), 0, s, start, off, raw, inv);
-    else hipLaunchKernelGGL((kDenseInvert<NB, false>), dim3(ngroups), dim3(256), 0, s, start, off, raw, inv);
+void denseInvertT(hipStream_t s, bool upper, int ngroups, int maxRows, const int* start, const long long* off, double* raw, double* inv) {
+    const int P = std::max(1, std::min(maxRows, 32));
+    if (upper) {
+        const unsigned bx = unsigned(((long long)maxRows * maxRows + 255) / 256);
+        hipLaunchKernelGGL((kDenseScaleU<NB>), dim3(std::max(1u, bx), unsigned(ngroups)), dim3(256), 0, s, start, off, raw);
+        hipLaunchKernelGGL((kDenseInvert<NB, true>), dim3(unsigned(P), unsigned(ngroups)), dim3(256), 0, s, start, off, raw, inv);
+    } else {
+        hipLaunchKernelGGL((kDenseInvert<NB, false>), dim3(unsigned(P), unsigned(ngroups)), dim3(256), 0, s, start, off, raw, inv);
+    }
 }
-void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, const int* start, const long long* off, int nmap, const int4* map,
-                       const double* lu, double* raw, long long rawElems, double* inv) {
+void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRows, const int* start, const long long* off, int nmap,
+                       const int4* map, const double* lu, double* raw, long long rawElems, double* inv) {
     if (ngroups == 0) return;
+    if (ngroups > 65535) throw Error(-1, "too many dense groups for one launch");
     launchFill(s, raw, 0.0, rawElems);
     if (nmap > 0) {
         const long long t = (long long)nmap * nb * nb;
         hipLaunchKernelGGL(kDenseGather, dim3(unsigned((t + 255) / 256)), dim3(256), 0, s, nmap, nb, map, start, off, lu, raw);
         checkLaunch();
     }
-    RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, start, off, raw, inv)))
+    RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, maxRows, start, off, raw, inv)))
     checkLaunch();
 }
 void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed) {

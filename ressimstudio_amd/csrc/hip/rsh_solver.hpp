@@ -100,7 +100,7 @@ struct DenseSide {
     std::vector<int> runGroup0, runGroup1; // per run: its range of groups, or -1
     std::vector<char> noOutside;           // per run: no entry of its rows couples to a row outside the run
     std::vector<int> hStart, hRows;        // host copies of start / rows (working rows, ascending per group)
-    int ngroups = 0, nmap = 0;
+    int ngroups = 0, nmap = 0, maxRows = 0;   // maxRows: rows of the largest group
     long long elems = 0;
     DBuf<int> start, rows;
     DBuf<long long> off;
@@ -191,6 +191,8 @@ struct DenseSide {
             runGroup1[ri] = ngroups;
         }
         nmap = int(hmap.size());
+        maxRows = 0;
+        for (size_t g = 0; g + 1 < hstart.size(); ++g) maxRows = std::max(maxRows, hstart[g + 1] - hstart[g]);
         hStart = hstart;
         hRows = hrows;
         start.upload(hstart, s);
@@ -205,7 +207,7 @@ struct DenseSide {
     }
     void factor(int nb, bool upper, const double* lu, bool f32, hipStream_t s) {
         if (!ngroups) return;
-        launchDenseGroups(s, nb, upper, ngroups, start.p, off.p, nmap, map.p, lu, raw.p, elems, inv.p);
+        launchDenseGroups(s, nb, upper, ngroups, maxRows, start.p, off.p, nmap, map.p, lu, raw.p, elems, inv.p);
         if (f32) {
             inv32.alloc(size_t(elems));
             launchToFloat(s, elems, inv.p, inv32.p);
