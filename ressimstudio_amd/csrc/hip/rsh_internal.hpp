@@ -269,7 +269,8 @@ void launchDenseGemv2(hipStream_t s, int m, int ld, const double* M, const doubl
 void launchGalerkin(hipStream_t s, int nap, const int* apPtr, const int* apSrc, const double* fine, double* ap, int nnzc,
                     const int* cPtr, const int* cSrc, double* coarse);
 void launchDenseFromCsr(hipStream_t s, int m, const int* rp, const int* ci, const double* v, double* M);
-void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, double* Mout);
+// steps 1: column k; steps 2: columns k and k + 1 in one launch (same values as two one-step launches)
+void launchGaussJordanStep(hipStream_t s, int m, int k, int steps, const double* Min, double* Mout);
 // vectors and reductions
 void launchFill(hipStream_t s, double* x, double v, long long n);
 void launchCopy(hipStream_t s, double* dst, const double* src, long long n);

@@ -517,10 +517,12 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
 // in ascending order, invert its pivot, form L_ik = A_ik U_kk^-1 for the later members i, then A_ij -= L_ik U_kj
 // for the later members i, j.  Every entry receives the same updates in the same order and with the same block
 // arithmetic as the row-by-row elimination of bilu0Row (bsr.cpp:183-215), so the factors are bit-identical; the
-// dependent steps are one per member instead of one merge walk per entry pair.  One workgroup per group;
-// idx[a * K + b] is the factor block of members (a, b) or -1.
+// dependent steps are one per member instead of one merge walk per entry pair.  One workgroup of kGroupThreads
+// per group (a step's trailing update is (K - t - 1)^2 independent block pairs); idx[a * K + b] is the factor
+// block of members (a, b) or -1.
+constexpr int kGroupThreads = 1024;
 template <int NB>
-__global__ __launch_bounds__(256) void kIluGroupFactor(const int* __restrict__ gstart, const int* __restrict__ grows,
+__global__ __launch_bounds__(kGroupThreads) void kIluGroupFactor(const int* __restrict__ gstart, const int* __restrict__ grows,
                                                        const long long* __restrict__ gidxOff, const int* __restrict__ gidx,
                                                        const int* __restrict__ diag, double* val, int* __restrict__ nreg) {
 #pragma clang fp contract(off)
@@ -1454,6 +1456,72 @@ __global__ __launch_bounds__(kBlock) void kGaussJordanStep(int m, int k, const d
     }
 }
 
+// argmax |v_i| over the rows a thread was handed (first index on ties), then over the workgroup (LDS tree, first
+// index on ties): the result does not depend on how the rows were split
+__device__ inline int gjArgmax(double best, int bi, double* sv, int* si) {
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double a = sv[threadIdx.x + o];
+            const int ia = si[threadIdx.x + o];
+            if (a > sv[threadIdx.x] || (a == sv[threadIdx.x] && ia < si[threadIdx.x])) { sv[threadIdx.x] = a; si[threadIdx.x] = ia; }
+        }
+        __syncthreads();
+    }
+    const int r = si[0];
+    __syncthreads();
+    return r;
+}
+// Two Gauss-Jordan steps (k, k + 1) in one launch, bit-identical to two kGaussJordanStep launches: every value of
+// the intermediate matrix B (after step k) that step k + 1 needs is recomputed from Min with the same expressions
+// (B_rc = Min[pr][c] / piv for r = k, Min[src(r)][c] - Min[src(r)][k] * (Min[pr][c] / piv) otherwise), the pivot
+// of step k + 1 is searched on B's column k + 1 with the same first-index rule.
+__global__ __launch_bounds__(kBlock) void kGaussJordanStep2(int m, int k, const double* __restrict__ Min, double* __restrict__ Mout) {
+    __shared__ double sv[kBlock];
+    __shared__ int si[kBlock];
+    const int ld = 2 * m;
+    double best = -1;
+    int bi = k;
+    for (int i = k + threadIdx.x; i < m; i += kBlock) {
+        const double a = fabs(Min[(long long)i * ld + k]);
+        if (a > best) { best = a; bi = i; }
+    }
+    const int pr1 = gjArgmax(best, bi, sv, si);
+    double piv1 = Min[(long long)pr1 * ld + k];
+    if (fabs(piv1) < 1e-300) piv1 = piv1 < 0 ? -1e-300 : 1e-300;
+    auto b1 = [&](int r, int c) -> double {                   // the matrix after step k
+        const double pk = Min[(long long)pr1 * ld + c] / piv1;
+        if (r == k) return pk;
+        const int src = r == pr1 ? k : r;
+        const double f = Min[(long long)src * ld + k];
+        return Min[(long long)src * ld + c] - f * pk;
+    };
+    const int k2 = k + 1;
+    best = -1;
+    bi = k2;
+    for (int i = k2 + threadIdx.x; i < m; i += kBlock) {
+        const double a = fabs(b1(i, k2));
+        if (a > best) { best = a; bi = i; }
+    }
+    const int pr2 = gjArgmax(best, bi, sv, si);
+    double piv2 = b1(pr2, k2);
+    if (fabs(piv2) < 1e-300) piv2 = piv2 < 0 ? -1e-300 : 1e-300;
+    const long long total = (long long)m * ld;
+    for (long long idx = blockIdx.x * (long long)kBlock + threadIdx.x; idx < total; idx += (long long)gridDim.x * kBlock) {
+        const int i = int(idx / ld), j = int(idx % ld);
+        const double pk = b1(pr2, j) / piv2;
+        double out;
+        if (i == k2) out = pk;
+        else {
+            const int src = i == pr2 ? k2 : i;
+            const double f = b1(src, k2);
+            out = b1(src, j) - f * pk;
+        }
+        Mout[idx] = out;
+    }
+}
 
 // ---------------------------------------------------------------- fused damped-Jacobi V-cycle steps (DeviceCprAmg smoothing)
 // The first iterate from zero, x1 = w D^-1 b, is written by the restriction that produces b, and the prolongation
@@ -1939,7 +2007,7 @@ void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, 
 template <int NB>
 void iluGroupFactorT(hipStream_t s, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
                      const int* diag, double* val, int* nreg) {
-    hipLaunchKernelGGL(kIluGroupFactor<NB>, dim3(g1 - g0), dim3(256), 0, s, gstart + g0, grows, gidxOff + g0, gidx, diag, val, nreg);
+    hipLaunchKernelGGL(kIluGroupFactor<NB>, dim3(g1 - g0), dim3(kGroupThreads), 0, s, gstart + g0, grows, gidxOff + g0, gidx, diag, val, nreg);
 }
 template <int NB>
 void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w) {
@@ -2298,10 +2366,15 @@ void launchDenseFromCsr(hipStream_t s, int m, const int* rp, const int* ci, cons
     hipLaunchKernelGGL(kDenseFromCsr, dim3(gridFor(total)), dim3(kBlock), 0, s, m, rp, ci, v, M);
     checkLaunch();
 }
-void launchGaussJordanStep(hipStream_t s, int m, int k, const double* Min, double* Mout) {
+void launchGaussJordanStep(hipStream_t s, int m, int k, int steps, const double* Min, double* Mout) {
     const long long total = (long long)m * 2 * m;
     int g = int(std::min<long long>((total + kBlock * 4 - 1) / (kBlock * 4), 2048));
-    hipLaunchKernelGGL(kGaussJordanStep, dim3(g), dim3(kBlock), 0, s, m, k, Min, Mout);
+    if (steps == 2) {
+        if (k + 1 >= m) throw Error(-1, "two Gauss-Jordan steps past the last column");
+        hipLaunchKernelGGL(kGaussJordanStep2, dim3(g), dim3(kBlock), 0, s, m, k, Min, Mout);
+    } else {
+        hipLaunchKernelGGL(kGaussJordanStep, dim3(g), dim3(kBlock), 0, s, m, k, Min, Mout);
+    }
     checkLaunch();
 }
 

@@ -671,9 +671,11 @@ struct CprDev {
         launchDenseFromCsr(s, m, C.rp, C.ci, C.v.p, M0.p);
         double* a = M0.p;
         double* b = M1.p;
-        for (int k = 0; k < m; ++k) {
-            launchGaussJordanStep(s, m, k, a, b);
+        for (int k = 0; k < m;) {
+            const int steps = k + 1 < m ? 2 : 1;
+            launchGaussJordanStep(s, m, k, steps, a, b);
             std::swap(a, b);
+            k += steps;
         }
         Minv = a;
     }
