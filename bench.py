@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="only the serving-loop leg")
     ap.add_argument("--serve-models", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-lambdas", default="10,25,50,200,2000")
     args = ap.parse_args()
     if args.gpus != 1:
