@@ -86,7 +86,12 @@ typedef struct rsh_options {
                            recompute the Galerkin operators (AMG::refresh, amg.cpp:257-271); 0: fresh setup */
     int smooth_steps;   /* RSH_SMOOTH_JACOBI: pre- and post-smoothing steps per level (default 1, DeviceCprAmg's) */
     double jacobi_omega;/* RSH_SMOOTH_JACOBI: the damping weight (<= 0: 0.6667, DeviceCprAmg's, bsr.h:173) */
-    int reserved[4];
+    int ilu_lean;       /* 1: the BILU0 / MC-BILU0 factors (and CPR's second stage) factorised lean, as the reference's
+                           C45 low-memory mode (BILU0::setLean / BlockCPR::setLean, bsr.cpp:216-263, 358-389; the
+                           engine's default from 500 000 cells, simulator.cpp:3318): the matrix rounded to float32,
+                           every row eliminated in double against the already-rounded earlier rows and stored
+                           rounded; the sweeps read float32 factors with FP64 vectors and arithmetic.  Default 0 */
+    int reserved[3];
 } rsh_options;
 
 /* ---- library / context */

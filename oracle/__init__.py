@@ -18,7 +18,7 @@ ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libressim_ref.so")
 REF_FAST_LIB = os.path.join(HERE, "_ref", "libressim_ref_fast.so")
 
-PREC = {"bilu0": 1, "bilu0_f32factors": 2, "bilu0f": 3, "mcbilu0": 4, "cpr": 5}
+PREC = {"bilu0": 1, "bilu0_f32factors": 2, "bilu0f": 3, "mcbilu0": 4, "cpr": 5, "bilu0_lean": 6, "cpr_lean": 7}
 METHOD = {"bicgstab": 0, "gmres": 1}
 
 _ip = ctypes.POINTER(ctypes.c_int)

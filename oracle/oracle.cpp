@@ -133,13 +133,19 @@ static void spmv(const Bsr& A, const double* x, double* y) {
 }
 
 // ---- BILU0 (bsr.cpp:183-215 row elimination, 437-596 sweeps)
+// mode 1: FP64; 2: FP64 factorisation, float32 factors in the sweeps (C38, bsr.cpp:413-419); 3: BILU0f, all float
+// (bsr.cpp:1087-1121); 4: lean (C45, bilu0RowF bsr.cpp:216-263 with setLean, :358-389): the matrix is rounded to
+// float32, each row is eliminated in double against the already-rounded factors of the earlier rows, its pivot
+// block inverted in double, and the finished row stored rounded; swept like mode 2 (float factors, double vectors)
 struct Bilu0 {
     Bsr LU;                          // L strictly lower (unit diag implied), U_ii^-1 on the diagonal, U strictly upper
-    std::vector<float> F;            // float32 copy of the factors (modes 2 and 3)
+    std::vector<float> F;            // float32 copy of the factors (modes 2, 3 and 4)
     int mode = 1, nReg = 0;
+    static double toF(double x) { return double(float(x)); }
     void factor(const Bsr& A) {
         LU = A;
         const int n = LU.n, b = LU.nb, b2 = b * b;
+        if (mode == 4) for (double& x : LU.v) x = toF(x);
         std::vector<int> pos(size_t(n), -1);
         double L[kMaxB * kMaxB];
         nReg = 0;
@@ -159,8 +165,10 @@ struct Bilu0 {
             }
             for (int p = r0; p < r1; ++p) pos[size_t(LU.ci[size_t(p)])] = -1;
             nReg += invertPivot(LU.blk(LU.diag[size_t(i)]), b);
+            if (mode == 4)
+                for (size_t e = size_t(r0) * b2; e < size_t(r1) * b2; ++e) LU.v[e] = toF(LU.v[e]);
         }
-        if (mode == 2 || mode == 3) { F.resize(LU.v.size()); for (size_t e = 0; e < F.size(); ++e) F[e] = float(LU.v[e]); }
+        if (mode >= 2) { F.resize(LU.v.size()); for (size_t e = 0; e < F.size(); ++e) F[e] = float(LU.v[e]); }
     }
     void apply(const double* r, double* z) const {
         const int n = LU.n, b = LU.nb;
@@ -195,7 +203,7 @@ struct Bilu0 {
             for (int e = 0; e < b; ++e) zi[e] = r[size_t(i) * b + e];
             for (int p = LU.rp[size_t(i)]; p < LU.diag[size_t(i)]; ++p) {
                 const double* zk = z + size_t(LU.ci[size_t(p)]) * b;
-                if (mode == 2) blockMulAdd(F.data() + size_t(p) * b2, zk, zi, b, -1.0);
+                if (mode == 2 || mode == 4) blockMulAdd(F.data() + size_t(p) * b2, zk, zi, b, -1.0);
                 else blockMulSubY(LU.blk(p), zk, zi, b);
             }
         }
@@ -205,11 +213,11 @@ struct Bilu0 {
             double* zi = z + size_t(i) * b;
             for (int p = LU.diag[size_t(i)] + 1; p < LU.rp[size_t(i) + 1]; ++p) {
                 const double* zj = z + size_t(LU.ci[size_t(p)]) * b;
-                if (mode == 2) blockMulAdd(F.data() + size_t(p) * b2, zj, zi, b, -1.0);
+                if (mode == 2 || mode == 4) blockMulAdd(F.data() + size_t(p) * b2, zj, zi, b, -1.0);
                 else blockMulSubY(LU.blk(p), zj, zi, b);
             }
             for (int e = 0; e < b; ++e) t[e] = 0;
-            if (mode == 2) blockMulAdd(F.data() + size_t(LU.diag[size_t(i)]) * b2, zi, t, b, 1.0);
+            if (mode == 2 || mode == 4) blockMulAdd(F.data() + size_t(LU.diag[size_t(i)]) * b2, zi, t, b, 1.0);
             else blockMulAdd(LU.blk(LU.diag[size_t(i)]), zi, t, b, 1.0);
             for (int e = 0; e < b; ++e) zi[e] = t[e];
         }
@@ -551,12 +559,13 @@ struct BlockCpr {
         findDiag(Ap);
         return Ap;
     }
+    bool lean = false;               // C45: the BILU0 stage factorised lean (setLean, bsr.cpp:1055)
     void setup(const Bsr& A, bool reuse) {
         S = A;
         weights();
         Csr Ap = pressure();
         if (!(reuse && amg.refresh(Ap))) amg.setup(std::move(Ap));
-        ilu.mode = 1;
+        ilu.mode = lean ? 4 : 1;
         ilu.factor(S);
     }
     void apply(const double* r, double* z) const {
@@ -914,6 +923,8 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
         case 1: case 2: case 3: ilu.mode = precond; ilu.factor(A); M.reset(new PrecOf<Bilu0>(ilu)); break;
         case 4: mc.factor(A); M.reset(new PrecOf<McBilu0>(mc)); break;
         case 5: cpr.setup(A, false); M.reset(new PrecOf<BlockCpr>(cpr)); break;
+        case 6: ilu.mode = 4; ilu.factor(A); M.reset(new PrecOf<Bilu0>(ilu)); break;
+        case 7: cpr.lean = true; cpr.setup(A, false); M.reset(new PrecOf<BlockCpr>(cpr)); break;
         default: return -1;
     }
     std::vector<double> bb(b, b + size_t(n) * nb), xx;

@@ -35,6 +35,7 @@ void* chk_bilu0_new(int n, int nb, const int* rp, const int* ci, const double* v
     BSR A = makeBsr(n, nb, rp, ci, v);
     auto* h = new Bilu0H; h->mode = mode;
     if (mode == 3) { h->f.factor(A); return h; }
+    if (mode == 4) { h->d.setLean(true); h->d.factor(A); return h; }
     if (mode == 2) {
         // float factors exist only in the contiguous level-ordered layout (bsr.cpp:413-419): force it on
         setenv("RESSIM_LEVEL_ILU", "2", 1);
@@ -145,6 +146,8 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
         case 3: { auto* p = new BILU0f; p->factor(A); M.reset(p); break; }
         case 4: { auto* p = new MCBILU0; p->factor(A); M.reset(p); break; }
         case 5: { auto* p = new BlockCPR; p->setup(A); M.reset(p); break; }
+        case 6: { auto* p = new BILU0; p->setLean(true); p->factor(A); M.reset(p); break; }
+        case 7: { auto* p = new BlockCPR; p->setLean(true); p->setup(A); M.reset(p); break; }
         default: return -1;
     }
     std::vector<double> bb(b, b + size_t(n) * nb), xx;

@@ -55,7 +55,8 @@ class RshError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int), ("krylov", ctypes.c_int), ("precision", ctypes.c_int), ("smoother", ctypes.c_int),
                 ("restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("inner_tol", ctypes.c_double), ("reuse_amg", ctypes.c_int),
-                ("smooth_steps", ctypes.c_int), ("jacobi_omega", ctypes.c_double), ("reserved", ctypes.c_int * 4)]
+                ("smooth_steps", ctypes.c_int), ("jacobi_omega", ctypes.c_double), ("ilu_lean", ctypes.c_int),
+                ("reserved", ctypes.c_int * 3)]
 
 
 class Request(ctypes.Structure):

@@ -519,7 +519,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         }
         RSH_HIP(hipStreamSynchronize(st));
         const double t1 = nowMs();
-        if (wantIlu) S.ilu->factor(S.val.p, f32, st);
+        if (wantIlu) S.ilu->factor(S.val.p, f32, st, o.ilu_lean != 0);
         RSH_HIP(hipStreamSynchronize(st));
         S.lastIluMs = nowMs() - t1;
         S.lastCprMs = t1 - t0;
@@ -738,7 +738,7 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                 case 1: S.precondG(S.vb.p, S.vx.p, f32); break;
                 case 2:
                     if (!S.ilu || S.opt.precond == RSH_PREC_CPR_BJ) throw Error(RSH_ERR_ARG, "no ILU preconditioner set up");
-                    S.ilu->apply(S.vb.p, S.vx.p, nullptr, ifp, st);
+                    S.ilu->apply(S.vb.p, S.vx.p, nullptr, ifp, st, false, S.iluInv64(f32));
                     break;
                 case 3: S.cpr->vcycle(st, S.amgF32(f32)); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;

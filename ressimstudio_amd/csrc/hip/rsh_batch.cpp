@@ -114,7 +114,12 @@ struct rsh_batch {
         RSH_HIP(hipMemcpyAsync(hks, ks.p, sizeof(KScalars) * size_t(ns), hipMemcpyDeviceToHost, st));
     }
     std::vector<int> currentKey() const {
-        std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0};
+        // every option a captured kernel bakes in (the Jacobi weight by its bit pattern)
+        std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0, opt.ilu_lean, opt.smoother, opt.smooth_steps};
+        int w[2];
+        std::memcpy(w, &opt.jacobi_omega, sizeof w);
+        k.push_back(w[0]);
+        k.push_back(w[1]);
         for (auto& g : groups) {
             k.push_back(g.S->ilu ? g.S->ilu->generation : -1);
             k.push_back(g.S->cpr ? g.S->cpr->generation : -1);

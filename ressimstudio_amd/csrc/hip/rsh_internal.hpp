@@ -201,7 +201,7 @@ constexpr int kDenseThreads = 1024;
 void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
-                    const KScalars* gate = nullptr, void* ycopy = nullptr);
+                    const KScalars* gate = nullptr, void* ycopy = nullptr, bool inv64 = false);
 // numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRows, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);
@@ -222,15 +222,19 @@ void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
 // BILU0 elimination of long rows rows[r0..r1): one wavefront per row
-void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg);
+void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
+                         bool lean = false);
 // BILU0 numeric factorisation of the rows of one level (in place, natural layout); stop >= 0: only the lower entries
 // in columns below stop, and the pivot is left for kIluGroupFactor
+// (lean: every finished row is stored rounded to float32, the C45 lean factorisation)
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag,
-                      double* val, int* nreg, int stop = -1);
+                      double* val, int* nreg, int stop = -1, bool lean = false);
 // right-looking elimination of the groups [g0, g1) (members grows[gstart[g] .. gstart[g+1]) in ascending row order,
 // member-pair block table gidx at gidxOff[g]), one workgroup per group
 void launchIluGroupFactor(hipStream_t s, int nb, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
-                          const int* diag, double* val, int* nreg);
+                          const int* diag, double* val, int* nreg, const int* rp, bool lean);
+// x = double(float(x)) in place (the lean factorisation's rounded input)
+void launchRoundToFloat(hipStream_t s, long long n, double* x);
 // CPR kernels
 void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w);
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);

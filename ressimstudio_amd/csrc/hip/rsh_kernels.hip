@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
 // earlier levels.
 template <int NB>
 __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const int* __restrict__ rp, const int* __restrict__ ci,
-                            const int* __restrict__ diag, double* __restrict__ val, int* __restrict__ nreg, int stop) {
+                            const int* __restrict__ diag, double* __restrict__ val, int* __restrict__ nreg, int stop, int lean) {
 #pragma clang fp contract(off)
     const int idx = r0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= r1) return;
@@ -510,6 +510,14 @@ __global__ void kBilu0Level(const int* __restrict__ rows, int r0, int r1, const 
 #pragma unroll
     for (int e = 0; e < B2; ++e) dp[e] = d[e];
     if (reg && nreg) atomicAdd(nreg, 1);
+    // lean (C45, bsr.cpp:216-263): the finished row is stored rounded to float32; later rows read it rounded
+    if (lean)
+        for (long long e = (long long)rp[i] * B2; e < (long long)pe * B2; ++e) val[e] = double(float(val[e]));
+}
+// lean factorisation: the matrix rounded to float32 before the elimination
+__global__ void kRoundToFloat(long long n, double* __restrict__ x) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = double(float(x[i]));
 }
 
 // The rows of a group coupled inside a run of small levels (a well's perforated cells), once the contributions
@@ -524,7 +532,8 @@ constexpr int kGroupThreads = 1024;
 template <int NB>
 __global__ __launch_bounds__(kGroupThreads) void kIluGroupFactor(const int* __restrict__ gstart, const int* __restrict__ grows,
                                                        const long long* __restrict__ gidxOff, const int* __restrict__ gidx,
-                                                       const int* __restrict__ diag, double* val, int* __restrict__ nreg) {
+                                                       const int* __restrict__ diag, double* val, int* __restrict__ nreg,
+                                                       const int* __restrict__ rp, int lean) {
 #pragma clang fp contract(off)
     constexpr int B2 = NB * NB;
     const int g = blockIdx.x;
@@ -543,6 +552,12 @@ __global__ __launch_bounds__(kGroupThreads) void kIluGroupFactor(const int* __re
             if (reg && nreg) atomicAdd(nreg, 1);
         }
         __syncthreads();
+        if (lean) {
+            // member k's row is finished (its earlier members' updates and its pivot): stored rounded to float32
+            // before any later member reads it (C45, bsr.cpp:216-263)
+            for (long long e = (long long)rp[k] * B2 + threadIdx.x; e < (long long)rp[k + 1] * B2; e += blockDim.x) val[e] = double(float(val[e]));
+            __syncthreads();
+        }
         // L_ak = A_ak U_kk^-1 for the later members a
         for (int a = t + 1 + int(threadIdx.x); a < K; a += int(blockDim.x)) {
             const int p = idx[(long long)a * K + t];
@@ -1089,7 +1104,7 @@ __global__ __launch_bounds__(kFuseThreads) void kGsFused(int l0, int l1, const i
 template <int NB>
 __global__ __launch_bounds__(kWave) void kBilu0LongRows(const int* __restrict__ rows, int r0, const int* __restrict__ rp,
                                                         const int* __restrict__ ci, const int* __restrict__ diag, double* val,
-                                                        int* __restrict__ nreg) {
+                                                        int* __restrict__ nreg, int lean) {
 #pragma clang fp contract(off)
     constexpr int B2 = NB * NB;
     const int i = rows[r0 + blockIdx.x];
@@ -1160,6 +1175,11 @@ __global__ __launch_bounds__(kWave) void kBilu0LongRows(const int* __restrict__ 
 #pragma unroll
         for (int e = 0; e < B2; ++e) dp[e] = d[e];
         if (reg && nreg) atomicAdd(nreg, 1);
+    }
+    if (lean) {
+        // the finished row stored rounded to float32 (lean, C45)
+        __syncthreads();
+        for (long long e = (long long)rp[i] * B2 + lane; e < (long long)pe * B2; e += kWave) val[e] = double(float(val[e]));
     }
 }
 
@@ -2000,14 +2020,16 @@ void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q
 }
 
 template <int NB>
-void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg, int stop) {
+void bilu0LevelT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg, int stop,
+                 int lean) {
     const int cnt = r1 - r0;
-    hipLaunchKernelGGL(kBilu0Level<NB>, dim3((cnt + 127) / 128), dim3(128), 0, s, rows, r0, r1, rp, ci, diag, val, nreg, stop);
+    hipLaunchKernelGGL(kBilu0Level<NB>, dim3((cnt + 127) / 128), dim3(128), 0, s, rows, r0, r1, rp, ci, diag, val, nreg, stop, lean);
 }
 template <int NB>
 void iluGroupFactorT(hipStream_t s, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
-                     const int* diag, double* val, int* nreg) {
-    hipLaunchKernelGGL(kIluGroupFactor<NB>, dim3(g1 - g0), dim3(kGroupThreads), 0, s, gstart + g0, grows, gidxOff + g0, gidx, diag, val, nreg);
+                     const int* diag, double* val, int* nreg, const int* rp, int lean) {
+    hipLaunchKernelGGL(kIluGroupFactor<NB>, dim3(g1 - g0), dim3(kGroupThreads), 0, s, gstart + g0, grows, gidxOff + g0, gidx, diag, val, nreg,
+                       rp, lean);
 }
 template <int NB>
 void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w) {
@@ -2051,8 +2073,9 @@ void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, c
     else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
 }
 template <int NB>
-void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
-    hipLaunchKernelGGL(kBilu0LongRows<NB>, dim3(r1 - r0), dim3(kWave), 0, s, rows, r0, rp, ci, diag, val, nreg);
+void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
+                int lean) {
+    hipLaunchKernelGGL(kBilu0LongRows<NB>, dim3(r1 - r0), dim3(kWave), 0, s, rows, r0, rp, ci, diag, val, nreg, lean);
 }
 
 inline void checkLaunch() {
@@ -2139,7 +2162,7 @@ void runDenseT(hipStream_t s, bool fwd, int ntasks, const int2* tasks, const int
 void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner, const KScalars* gate,
-                    void* ycopy) {
+                    void* ycopy, bool inv64) {
     if (ns + nq > 0) {
         if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const float*)y, (float*)rt, gate)))
         else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runOutsideT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, t0, ns, u0, nq, wbase, wend, r, (const double*)y, (double*)rt, gate)))
@@ -2147,7 +2170,7 @@ void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd,
         checkLaunch();
     }
     if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, float>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const float*)rt, (float*)y, z, perm, xp, gowner, gate, (float*)ycopy)))
-    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, double>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
+    else if (fp == kIluStore32 && !inv64) RSH_NB_DISPATCH(nb, (runDenseT<NB, float, double>(s, fwd, ntasks, tasks, start, rows, off, (const float*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
     else RSH_NB_DISPATCH(nb, (runDenseT<NB, double, double>(s, fwd, ntasks, tasks, start, rows, off, (const double*)inv, wbase, (const double*)rt, (double*)y, z, perm, xp, gowner, gate, (double*)ycopy)))
     checkLaunch();
 }
@@ -2245,22 +2268,28 @@ void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* 
     hipLaunchKernelGGL(kGsFused, dim3(1), dim3(kFuseThreads), 0, s, l0, l1, lstart, rows, rp, ci, diag, v, b, xin, xout, forward ? 1 : 0);
     checkLaunch();
 }
-void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg) {
+void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
+                         bool lean) {
     if (r1 <= r0) return;
-    RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg)));
+    RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, lean ? 1 : 0)));
     checkLaunch();
 }
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
-                      int stop) {
+                      int stop, bool lean) {
     const int cnt = r1 - r0;
     if (cnt <= 0) return;
-    RSH_NB_DISPATCH(nb, (bilu0LevelT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, stop)));
+    RSH_NB_DISPATCH(nb, (bilu0LevelT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, stop, lean ? 1 : 0)));
     checkLaunch();
 }
 void launchIluGroupFactor(hipStream_t s, int nb, int g0, int g1, const int* gstart, const int* grows, const long long* gidxOff, const int* gidx,
-                          const int* diag, double* val, int* nreg) {
+                          const int* diag, double* val, int* nreg, const int* rp, bool lean) {
     if (g1 <= g0) return;
-    RSH_NB_DISPATCH(nb, (iluGroupFactorT<NB>(s, g0, g1, gstart, grows, gidxOff, gidx, diag, val, nreg)));
+    RSH_NB_DISPATCH(nb, (iluGroupFactorT<NB>(s, g0, g1, gstart, grows, gidxOff, gidx, diag, val, nreg, rp, lean ? 1 : 0)));
+    checkLaunch();
+}
+void launchRoundToFloat(hipStream_t s, long long n, double* x) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(kRoundToFloat, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, x);
     checkLaunch();
 }
 void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w) {

@@ -454,23 +454,29 @@ struct IluDev {
         ++generation;
     }
     // numeric factorisation from the system's values (natural layout), then the sweep layouts
-    void factor(const double* val, bool f32, hipStream_t s) {
+    // lean: the C45 lean factorisation (input and every finished row rounded to float32; the factors are then
+    // float32 values, applied from the FP32 slices)
+    bool lean = false;
+    void factor(const double* val, bool f32, hipStream_t s, bool leanFactors = false) {
         const size_t nvals = lu.n;
+        lean = leanFactors;
+        if (lean) f32 = true;
         if (mc) launchGatherBlocks(s, nb, val, fmap.p, int(nvals / (size_t(nb) * nb)), lu.p);
         else RSH_HIP(hipMemcpyAsync(lu.p, val, nvals * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (lean) launchRoundToFloat(s, (long long)nvals, lu.p);
         nreg.zero(s);
         for (size_t ri = 0; ri < runsL.size(); ++ri) {
             const LevelRun& run = runsL[ri];
             if (ri < fgRun0.size() && fgRun0[ri] >= 0) {
                 // the run's rows: the contributions of the rows before the run, then the groups right-looking
                 const int w0 = lstart[size_t(run.l0)], w1 = lstart[size_t(run.l1)];
-                launchBilu0Level(s, nb, nullptr, w0, w1, rp, ci, diag, lu.p, nreg.p, w0);
-                launchIluGroupFactor(s, nb, fgRun0[ri], fgRun1[ri], fgStart.p, fgRows.p, fgIdxOff.p, fgIdx.p, diag, lu.p, nreg.p);
+                launchBilu0Level(s, nb, nullptr, w0, w1, rp, ci, diag, lu.p, nreg.p, w0, lean);
+                launchIluGroupFactor(s, nb, fgRun0[ri], fgRun1[ri], fgStart.p, fgRows.p, fgIdxOff.p, fgIdx.p, diag, lu.p, nreg.p, rp, lean);
                 continue;
             }
             for (int l = run.l0; l < run.l1; ++l) {
-                launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
-                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p);
+                launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, -1, lean);
+                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, lean);
             }
         }
         packSlices(s, nb, lu.p, L64, hL.nslots, false);
@@ -486,7 +492,8 @@ struct IluDev {
     }
     // fp: IluFp (kIluFp32 and kIluStore32 need the FP32 factors of a factor(..., true))
     // firstDone: y of the first level (firstCopy) is already r in the working numbering
-    void apply(const double* r, double* z, const double* xp, int fp, hipStream_t s, bool firstDone = false) const {
+    // inv64: with FP32-stored factors (kIluStore32), the dense group inverses in FP64 (lean factors in FP64 precision)
+    void apply(const double* r, double* z, const double* xp, int fp, hipStream_t s, bool firstDone = false, bool inv64 = false) const {
         const bool f32 = fp != kIluFp64;          // FP32-stored factors
         const bool v32 = fp == kIluFp32;          // FP32 working vectors
         const void* Lp = f32 ? static_cast<const void*>(&L32) : static_cast<const void*>(&L64);
@@ -501,13 +508,14 @@ struct IluDev {
             const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)];
             if (D.runTask0[ri] >= 0) {
                 const int t0 = sc.sStart[size_t(run.l0)], u0 = sc.qStart[size_t(run.l0)];
-                const void* inv = f32 ? static_cast<const void*>(D.inv32.p) : static_cast<const void*>(D.inv.p);
+                const bool i64 = !f32 || (inv64 && fp == kIluStore32);
+                const void* inv = i64 ? static_cast<const void*>(D.inv.p) : static_cast<const void*>(D.inv32.p);
                 const bool tailStep = tailCopy && ri + 1 == (fwd ? runsL.size() : runsU.size());
                 const bool skipOutside = tailStep && !fwd;
                 launchRunDense(s, nb, fp, sl, fwd, t0, skipOutside ? 0 : sc.sStart[size_t(run.l1)] - t0, u0,
                                skipOutside ? 0 : sc.qStart[size_t(run.l1)] - u0, wb, we, D.runTask1[ri] - D.runTask0[ri],
                                D.tasks.p + D.runTask0[ri], D.start.p, D.rows.p, D.off.p, inv, perm.p, r, skipOutside ? tail : rt, y, z, xp,
-                               D.gowner.p, gate, tailStep && fwd ? tail : nullptr);
+                               D.gowner.p, gate, tailStep && fwd ? tail : nullptr, i64);
                 return;
             }
             launchIluFused(s, nb, fp, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
@@ -877,7 +885,13 @@ struct rsh_system {
     // FP32-stored CPR pressure stage / block-Jacobi inverses for an apply in the given precision
     bool amgF32(bool f32) const { return f32 || opt.precision == RSH_MIXED_STORE; }
     // the ILU apply's IluFp for an apply in the given precision
-    int iluFp(bool f32) const { return f32 ? kIluFp32 : opt.precision == RSH_MIXED_STORE ? kIluStore32 : kIluFp64; }
+    // (lean factors are float32 values: the FP32 slices give the same apply as the FP64 ones at half the bytes)
+    int iluFp(bool f32) const {
+        return f32 ? kIluFp32 : (opt.precision == RSH_MIXED_STORE || opt.ilu_lean) ? kIluStore32 : kIluFp64;
+    }
+    // lean factors in FP64 precision: FP32 factor slices, but the dense group inverses (formed in FP64 from the
+    // rounded factors) stay FP64, so the apply is the reference's float-factor sweep up to FP64 rounding
+    bool iluInv64(bool f32) const { return !f32 && opt.ilu_lean && opt.precision == RSH_FP64; }
 
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
 
@@ -935,7 +949,7 @@ struct rsh_system {
     void precond(const double* rr, double* z, bool f32) {
         const int pc = opt.precond;
         if (pc == RSH_PREC_BILU0 || pc == RSH_PREC_MCBILU0) {
-            ilu->apply(rr, z, nullptr, iluFp(f32), s());
+            ilu->apply(rr, z, nullptr, iluFp(f32), s(), false, iluInv64(f32));
             return;
         }
         // CPR (bsr.cpp:1061-1084): r_p = w^T r -> V-cycle -> t = r - S(:,0) x_p -> BILU0 -> z_0 += x_p; DeviceCprAmg
@@ -956,7 +970,7 @@ struct rsh_system {
             launchBlockJacobi(s(), nb, n, b32 ? static_cast<const void*>(bjInv32.p) : static_cast<const void*>(bjInv.p), b32, C.t.p, xp, z);
             return;
         }
-        ilu->apply(C.t.p, z, xp, iluFp(f32), s(), pre);
+        ilu->apply(C.t.p, z, xp, iluFp(f32), s(), pre, iluInv64(f32));
     }
 };
 

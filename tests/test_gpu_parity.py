@@ -349,6 +349,81 @@ def test_mcbilu0_factors_bit_identical(sysd, orc, name):
     assert np.array_equal(d.bilu0_factors(), orc.bilu0(ps, 1).factors())
 
 
+def _colour_permuted(orc, s):
+    """P A P^T in the oracle's multicolour order, and the order itself (new row -> old row)"""
+    _, _, perm = orc.mcbilu0(s).colouring()
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(s.n)
+    rp, cols, vals = [0], [], []
+    for inew in range(s.n):
+        i = perm[inew]
+        p0, p1 = int(s.row_ptr[i]), int(s.row_ptr[i + 1])
+        c = iperm[s.col[p0:p1]]
+        o = np.argsort(c, kind="stable")
+        cols.append(c[o])
+        vals.append(s.val[p0:p1][o])
+        rp.append(rp[-1] + p1 - p0)
+    ps = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=np.array(rp, np.int32), col=np.concatenate(cols).astype(np.int32),
+                         val=np.concatenate(vals), rhs=np.zeros(s.n * s.nb))
+    return ps, perm
+
+
+@pytest.mark.parametrize("name", ["small", "layered", "nb2", "nb4", "deepwell"])
+def test_lean_bilu0_factors_and_apply(sysd, orc, name):
+    # rsh_options.ilu_lean, the reference's C45 lean factorisation (BILU0::setLean, bilu0RowF bsr.cpp:216-263): the
+    # device factors equal the oracle's lean factors bit for bit (every finished row rounded to float32, later rows
+    # eliminated against the rounded values; the oracle's lean equals the compiled reference's), and the apply
+    # (float32 factors, FP64 vectors and arithmetic) equals the oracle's lean apply to FP64 rounding
+    s = sysd[name]
+    d = rs.System(s).setup(precond="bilu0", ilu_lean=1)
+    o = orc.bilu0(s, 4)
+    lu = d.bilu0_factors()
+    assert np.array_equal(lu, o.factors())
+    assert np.array_equal(lu, lu.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(lu, orc.bilu0(s, 1).factors())
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    z, zo = d.precond(r), o.apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("name", ["layered", "deepwell", "deepwell_nb4"])
+def test_lean_mcbilu0_factors_and_apply(sysd, orc, name):
+    # lean multicolour factors (the well-clique groups eliminated right-looking, each member's row rounded when it is
+    # finished) equal the lean BILU0 of P A P^T bit for bit; the apply, with the dense group inverses formed in FP64
+    # from the rounded factors, equals the lean sweeps of P A P^T to FP64 rounding
+    s = sysd[name]
+    d = rs.System(s).setup(precond="mcbilu0", ilu_lean=1)
+    ps, perm = _colour_permuted(orc, s)
+    o = orc.bilu0(ps, 4)
+    assert np.array_equal(d.bilu0_factors(), o.factors())
+    r = np.sin(0.37 * np.arange(s.n * s.nb)) + 0.5
+    nb = s.nb
+    rp_ = r.reshape(s.n, nb)[perm].reshape(-1)
+    zp = o.apply(rp_).reshape(s.n, nb)
+    zo = np.empty_like(zp)
+    zo[perm] = zp
+    zo = zo.reshape(-1)
+    z = d.precond(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("name", ["layered", "deepwell", "medium"])
+@pytest.mark.parametrize("precond,precision", [("bilu0", rs.FP64), ("cpr_mc", rs.FP64), ("cpr_mc", rs.MIXED_STORE)])
+def test_lean_solves(sysd, orc, name, precond, precision):
+    # lean factors inside the FP64 BiCGSTAB (alone and as CPR's second stage, also with the FP32-stored CPR):
+    # true residual <= 1e-8 and the solution within 1e-6 of the FP64 oracle's
+    s = sysd[name]
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok_o
+    d = rs.System(s).setup(precond=precond, precision=precision, ilu_lean=1, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
+    x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+    assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    if precond == "bilu0":
+        _, its_o, _, _ = orc.krylov(s, s.rhs, "bilu0_lean", "bicgstab", TOL, 3000)
+        assert abs(its - its_o) <= max(3, its_o // 5), (its, its_o)
+
+
 @pytest.mark.parametrize("name", ["layered", "deepwell", "medium"])
 def test_jacobi_weight_option(sysd, orc, name):
     # rsh_options.jacobi_omega: the V-cycle's damping weight (default DeviceCprAmg's 0.6667); weight 1 converges to the

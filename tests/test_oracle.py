@@ -96,11 +96,11 @@ def test_oracle_equals_compiled_reference(orc, ref, seed):
               gen.reservoir(11, 7, 5, 1, model=0, sigma=1.0, wells=1, seed=seed)]:
         x = np.random.default_rng(seed).standard_normal(s.n * s.nb)
         assert np.array_equal(orc.spmv(s, x), ref.spmv(s, x))
-        for mode in (1, 3):
+        for mode in (1, 3, 4):
             assert np.array_equal(orc.bilu0(s, mode).apply(x), ref.bilu0(s, mode).apply(x))
         assert np.array_equal(orc.mcbilu0(s).apply(x), ref.mcbilu0(s).apply(x))
         assert np.array_equal(orc.cpr(s).apply(x), ref.cpr(s).apply(x))
-        for prec in ["bilu0", "mcbilu0", "cpr", "bilu0f"]:
+        for prec in ["bilu0", "mcbilu0", "cpr", "bilu0f", "bilu0_lean", "cpr_lean"]:
             a = orc.krylov(s, s.rhs, prec, "bicgstab", 1e-9, 500)
             b = ref.krylov(s, s.rhs, prec, "bicgstab", 1e-9, 500)
             assert a[1:] == b[1:] and np.array_equal(a[0], b[0])
@@ -113,6 +113,22 @@ def test_float_factor_mode_equals_reference(orc, ref):
     r = np.sin(0.3 * np.arange(s.n * s.nb))
     a, b = orc.bilu0(s, 2).apply(r), ref.bilu0(s, 2).apply(r)
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+def test_lean_factorisation(orc, ref):
+    # C45 lean BILU0 (bilu0RowF, bsr.cpp:216-263): the factors are float32 values, and they differ from rounding
+    # the FP64 factors (C38) because every row is eliminated against the rounded earlier rows; on a 24,000-cell grid
+    # the reference factorises in its level-ordered layout (bitwise the natural order) and sweeps with float factors
+    s = gen.reservoir(40, 30, 20, 3, model=0, sigma=1.5, wells=2, seed=12)
+    lean, c38 = orc.bilu0(s, 4), orc.bilu0(s, 2)
+    f = lean.factors()
+    assert np.array_equal(f, f.astype(np.float32).astype(np.float64))
+    assert not np.array_equal(f, c38.factors().astype(np.float32).astype(np.float64))
+    r = np.sin(0.3 * np.arange(s.n * s.nb))
+    a, b = lean.apply(r), ref.bilu0(s, 4).apply(r)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    fp64 = orc.bilu0(s, 1).apply(r)
+    assert 1e-9 < np.max(np.abs(a - fp64)) / np.max(np.abs(fp64)) < 1e-3
 
 
 def test_cpr_refresh_equals_reference(orc, ref):
