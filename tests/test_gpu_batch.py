@@ -71,6 +71,23 @@ def test_batch_cpr_matches_oracle_per_system(systems, orc, precond, smoother, pr
 
 
 @pytest.mark.parametrize("precond", ["mcbilu0", "cpr_mc"])
+def test_batch_lean_factors(systems, orc, precond):
+    """The lean factorisation (rsh_options.ilu_lean) inside the batch: every system converges to the oracle's FP64
+    solution, and switching lean off again rebuilds the captured iteration (the FP64 factors give other iterates)."""
+    B = rs.Batch(systems).setup(precond=precond, ilu_lean=1, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
+    xs, its, rel, st = B.solve([s.rhs for s in systems], TOL, 3000)
+    for k, s in enumerate(systems):
+        xo, _, _, ok_o = orc.krylov(s, s.rhs, "mcbilu0", "bicgstab", TOL, 3000)
+        assert st[k] == rs.OK and ok_o
+        assert _true_rel(orc, s, xs[k]) <= 1.01 * TOL, k
+        assert np.linalg.norm(xs[k] - xo) <= 1e-6 * np.linalg.norm(xo), k
+    B.setup(precond=precond, ilu_lean=0, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
+    xs2, _, _, st2 = B.solve([s.rhs for s in systems], TOL, 3000)
+    assert all(int(v) == rs.OK for v in st2)
+    assert any(not np.array_equal(a, b) for a, b in zip(xs, xs2))
+
+
+@pytest.mark.parametrize("precond", ["mcbilu0", "cpr_mc"])
 def test_batch_systems_do_not_interact(systems, precond):
     """A system's result does not depend on the other systems of its batch: a new right-hand side for system 0
     leaves every other system's solution, iteration count and residual bit-identical (block-diagonal
