@@ -109,6 +109,14 @@ int rsh_system_destroy(rsh_system* sys);
 /* new values on the same pattern (host or device pointer to nnzb*nb*nb doubles) */
 int rsh_system_set_values(rsh_system* sys, const double* val);
 int rsh_system_set_values_device(rsh_system* sys, const double* val_dev);
+/* true-IMPES CPR weights (BlockCPR::setup's accBlocks, bsr.cpp:993-1030; the engine's cprWeights = 1,
+ * simulator.cpp:3299-3316): n accumulation blocks, nb x nb row-major like the Jacobian's blocks (host or device
+ * pointer), used by every later CPR set-up until replaced; null clears them (quasi-IMPES, the default).  A cell whose
+ * block is ill-conditioned (|B|_1 |B^-1|_1 > 1e10) keeps its quasi-IMPES weight (counted in rsh_system_info's
+ * cpr_impes_fallbacks).  The GpuBlockSolver hook carries no accumulation blocks; this serves engine dumps and
+ * callers that have them. */
+int rsh_system_set_accumulation(rsh_system* sys, const double* acc);
+int rsh_system_set_accumulation_device(rsh_system* sys, const double* acc_dev);
 /* build (first call) or refresh the preconditioner for the current values */
 int rsh_system_setup(rsh_system* sys, const rsh_options* opt);
 /* solve A x = b to relative residual tol (the recursive residual of BiCGSTAB as linsolve.cpp:210-212, the FGMRES

@@ -61,6 +61,9 @@ class Checker:
         lib.chk_mcbilu0_free.argtypes = [_vp]
         lib.chk_cpr_new.restype = _vp
         lib.chk_cpr_new.argtypes = bsr
+        lib.chk_cpr_new_acc.restype = _vp
+        lib.chk_cpr_new_acc.argtypes = bsr + [_dp]
+        lib.chk_cpr_fallbacks.argtypes = [_vp]
         lib.chk_cpr_refresh.argtypes = [_vp, _dp]
         lib.chk_cpr_apply.argtypes = [_vp, _dp, _dp]
         lib.chk_cpr_weights.argtypes = [_vp, _dp]
@@ -112,8 +115,9 @@ class Checker:
     def mcbilu0(self, s):
         return _McBilu0(self, s)
 
-    def cpr(self, s):
-        return _Cpr(self, s)
+    def cpr(self, s, acc=None):
+        """BlockCPR (bsr.cpp:993-1084); acc: n accumulation blocks for true-IMPES weights"""
+        return _Cpr(self, s, acc)
 
     def devcpr(self, s):
         """DeviceCprAmg (bsr.cpp:1235-1371): float32 CPR with damped-Jacobi AMG and block-Jacobi second stage."""
@@ -198,10 +202,18 @@ class _DevCpr(_Handle):
 class _Cpr(_Handle):
     _free = "chk_cpr_free"
 
-    def __init__(self, c, s):
+    def __init__(self, c, s, acc=None):
         self.c, self.N, self.n, self.nb = c, s.n * s.nb, s.n, s.nb
         n, nb, rp, ci, v = _bsr_args(s)
-        self.h = c.lib.chk_cpr_new(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double))
+        if acc is None:
+            self.h = c.lib.chk_cpr_new(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double))
+        else:
+            self._acc = np.ascontiguousarray(acc, np.float64).reshape(-1)
+            self.h = c.lib.chk_cpr_new_acc(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double),
+                                           _p(self._acc, ctypes.c_double))
+
+    def fallbacks(self):
+        return int(self.c.lib.chk_cpr_fallbacks(self.h))
 
     def refresh(self, val):
         v = np.ascontiguousarray(val, np.float64).reshape(-1)

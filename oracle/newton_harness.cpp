@@ -52,15 +52,19 @@ int main(int argc, char** argv) {
     const int steps = argc > 1 ? std::atoi(argv[1]) : 6;
     const std::string pc = argc > 2 ? argv[2] : "cpr";
     const std::string sm = argc > 3 ? argv[3] : "gs";
+    // "cpu-true-impes": the CPU run only, with true-IMPES weights (the engine then also writes the accumulation
+    // blocks next to a RESSIM_DUMP file, simulator.cpp:3313-3315)
+    const bool cpuTrueImpes = argc > 4 && std::string(argv[4]) == "cpu-true-impes";
     Case c = makeDemoCase();
     if (steps > 0 && int(c.schedule.size()) > steps) c.schedule.resize(size_t(steps));
     c.tuning.blockSolver = true;
     c.tuning.linearSolver = pc == "bilu0" ? 0 : 1;   // CPU path: BILU0 or block CPR
-    c.tuning.cprWeights = 0;                           // quasi-IMPES on both paths (the hook carries no accumulation blocks)
+    c.tuning.cprWeights = cpuTrueImpes ? 1 : 0;       // else quasi-IMPES on both paths (the hook carries no accumulation blocks)
     std::fprintf(stderr, "== run cpu\n");
     std::fflush(stderr);
     const Run cpu = run(c);
     dump("cpu", cpu);
+    if (cpuTrueImpes) return 0;
 
     rsh_options opt;
     rsh_default_options(&opt);

@@ -532,14 +532,37 @@ struct BlockCpr {
     std::vector<double> w;
     Amg amg;
     Bilu0 ilu;
+    // acc (optional, n nb x nb blocks): true-IMPES (bsr.cpp:1005-1021): the accumulation block B is inverted and,
+    // if its 1-norm condition estimate |B|_1 |B^-1|_1 is at most 1e10 (or nb = 1), its inverse replaces D^-1; a
+    // singular scalar block is used as it is; otherwise (a fallback) the quasi-IMPES diagonal block
+    std::vector<double> acc;
+    int fallbacks = 0;
     void weights() {
         const int n = S.n, b = S.nb, b2 = b * b;
         w.assign(size_t(n) * b, 1.0);
+        fallbacks = 0;
         for (int i = 0; i < n; ++i) {
             double D[kMaxB * kMaxB];
             std::memcpy(D, S.blk(S.diag[size_t(i)]), sizeof(double) * size_t(b2));
             double* wi = w.data() + size_t(i) * b;
-            if (invertBlock(D, b)) { for (int e = 0; e < b; ++e) wi[e] = D[e]; }          // first row of D^-1
+            bool inverted = false;
+            if (!acc.empty()) {
+                const double* B = acc.data() + size_t(i) * b2;
+                double inv[kMaxB * kMaxB];
+                std::memcpy(inv, B, sizeof(double) * size_t(b2));
+                if (invertBlock(inv, b)) {
+                    double n1 = 0, ni = 0;
+                    for (int j = 0; j < b; ++j) {
+                        double c = 0, d = 0;
+                        for (int r = 0; r < b; ++r) { c += std::fabs(B[r * b + j]); d += std::fabs(inv[r * b + j]); }
+                        n1 = std::max(n1, c); ni = std::max(ni, d);
+                    }
+                    if (b == 1 || n1 * ni <= 1e10) { std::memcpy(D, inv, sizeof(double) * size_t(b2)); inverted = true; }
+                    else ++fallbacks;
+                } else if (b == 1) std::memcpy(D, B, sizeof(double) * size_t(b2));
+                else ++fallbacks;
+            }
+            if (inverted || invertBlock(D, b)) { for (int e = 0; e < b; ++e) wi[e] = D[e]; }          // first row of D^-1
             else {
                 double s = 0; for (int e = 0; e < b; ++e) s += S.blk(S.diag[size_t(i)])[e * b];
                 for (int e = 0; e < b; ++e) wi[e] = std::fabs(s) > 1e-300 ? 1.0 / s : 1.0;
@@ -885,6 +908,13 @@ void* chk_cpr_new(int n, int nb, const int* rp, const int* ci, const double* v) 
     h->setup(makeBsr(n, nb, rp, ci, v), false);
     return h;
 }
+void* chk_cpr_new_acc(int n, int nb, const int* rp, const int* ci, const double* v, const double* acc) {
+    auto* h = new BlockCpr;
+    h->acc.assign(acc, acc + size_t(n) * nb * nb);
+    h->setup(makeBsr(n, nb, rp, ci, v), false);
+    return h;
+}
+int chk_cpr_fallbacks(void* h) { return static_cast<BlockCpr*>(h)->fallbacks; }
 int chk_cpr_refresh(void* h, const double* v) {
     auto* c = static_cast<BlockCpr*>(h);
     Bsr A = c->S;

@@ -76,6 +76,13 @@ void* chk_cpr_new(int n, int nb, const int* rp, const int* ci, const double* v) 
     c->P.setup(c->S);
     return c;
 }
+void* chk_cpr_new_acc(int n, int nb, const int* rp, const int* ci, const double* v, const double* acc) {
+    auto* c = new CprH;
+    c->S = makeBsr(n, nb, rp, ci, v);
+    c->P.setup(c->S, acc);
+    return c;
+}
+int chk_cpr_fallbacks(void* h) { return static_cast<CprH*>(h)->P.trueImpesFallbacks(); }
 int chk_cpr_refresh(void* h, const double* v) {
     auto* c = static_cast<CprH*>(h);
     std::copy(v, v + c->S.val.size(), c->S.val.begin());

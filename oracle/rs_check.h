@@ -35,6 +35,9 @@ int chk_mcbilu0_colouring(void* h, int* colour, int* perm);
 void chk_mcbilu0_free(void* h);
 
 void* chk_cpr_new(int n, int nb, const int* rp, const int* ci, const double* v);
+// true-IMPES weights from n accumulation blocks (BlockCPR::setup accBlocks, bsr.cpp:993-1030)
+void* chk_cpr_new_acc(int n, int nb, const int* rp, const int* ci, const double* v, const double* acc);
+int chk_cpr_fallbacks(void* h);
 int chk_cpr_refresh(void* h, const double* v);
 int chk_cpr_apply(void* h, const double* r, double* z);
 int chk_cpr_weights(void* h, double* w);

@@ -36,7 +36,7 @@ OK, ERR_ARG, ERR_DEVICE, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_NONFINITE, ERR_UN
 EXPORTED = [
     "rsh_version", "rsh_last_error", "rsh_device_count", "rsh_default_options", "rsh_context_create", "rsh_context_destroy",
     "rsh_context_synchronize", "rsh_system_create", "rsh_system_destroy", "rsh_system_set_values", "rsh_system_set_values_device",
-    "rsh_system_setup", "rsh_solve", "rsh_solve_device", "rsh_spmv", "rsh_precond_apply", "rsh_amg_apply", "rsh_cpr_weights",
+    "rsh_system_set_accumulation", "rsh_system_set_accumulation_device", "rsh_system_setup", "rsh_solve", "rsh_solve_device", "rsh_spmv", "rsh_precond_apply", "rsh_amg_apply", "rsh_cpr_weights",
     "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
     "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
@@ -101,6 +101,8 @@ def lib():
         L.rsh_precond_apply.argtypes = [_vp, _dp, _dp]
         L.rsh_amg_apply.argtypes = [_vp, _dp, _dp]
         L.rsh_cpr_weights.argtypes = [_vp, _dp]
+        L.rsh_system_set_accumulation.argtypes = [_vp, _dp]
+        L.rsh_system_set_accumulation_device.argtypes = [_vp, _vp]
         L.rsh_bilu0_factors.argtypes = [_vp, _dp]
         L.rsh_amg_levels.argtypes = [_vp, _ip, _ip, _ip, ctypes.c_int]
         L.rsh_system_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
@@ -272,6 +274,17 @@ class System:
         z = np.empty(self.n)
         _check(lib().rsh_amg_apply(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double)), "rsh_amg_apply")
         return z
+
+    def set_accumulation(self, acc):
+        """true-IMPES CPR weights from n accumulation blocks (None: quasi-IMPES again); used by the next setup()"""
+        if acc is None:
+            _check(lib().rsh_system_set_accumulation(self.h, None), "rsh_system_set_accumulation")
+            return self
+        a = np.ascontiguousarray(acc, np.float64).reshape(-1)
+        if a.size != self.n * self.nb * self.nb:
+            raise ValueError(f"accumulation blocks: {a.size} values, expected {self.n * self.nb * self.nb}")
+        _check(lib().rsh_system_set_accumulation(self.h, _p(a, ctypes.c_double)), "rsh_system_set_accumulation")
+        return self
 
     def cpr_weights(self):
         w = np.empty(self.N)

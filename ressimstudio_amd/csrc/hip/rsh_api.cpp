@@ -461,6 +461,27 @@ int rsh_system_set_values_device(rsh_system* sys, const double* val_dev) {
     });
 }
 
+// true-IMPES weights (BlockCPR::setup accBlocks, bsr.cpp:993-1030): n accumulation blocks (nb x nb, row-major as
+// the Jacobian's), kept until replaced or cleared (null); used by the next CPR set-ups
+static int setAccumulation(rsh_system* sys, const double* acc, hipMemcpyKind kind) {
+    return guarded([&] {
+        if (!sys) return fail(RSH_ERR_ARG, "null system");
+        RSH_HIP(hipSetDevice(sys->ctx->c.device));
+        if (!acc) {
+            sys->acc.release();
+            return RSH_OK;
+        }
+        sys->acc.alloc(size_t(sys->n) * sys->nb * sys->nb);
+        RSH_HIP(hipMemcpyAsync(sys->acc.p, acc, sys->acc.bytes(), kind, sys->s()));
+        RSH_HIP(hipStreamSynchronize(sys->s()));
+        return RSH_OK;
+    });
+}
+int rsh_system_set_accumulation(rsh_system* sys, const double* acc) { return setAccumulation(sys, acc, hipMemcpyHostToDevice); }
+int rsh_system_set_accumulation_device(rsh_system* sys, const double* acc_dev) {
+    return setAccumulation(sys, acc_dev, hipMemcpyDeviceToDevice);
+}
+
 int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
     return guarded([&] {
         if (!sys) return fail(RSH_ERR_ARG, "null system");
@@ -506,7 +527,8 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
                 S.colP32.alloc(S.colP.n);
                 launchToFloat(st, (long long)S.colP.n, S.colP.p, S.colP32.p);
             }
-            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, a32);
+            S.cpr->setup(S.pat, S.rp.p, S.ci.p, S.diag.p, S.val.p, o.reuse_amg != 0, st, a32, S.acc.n ? S.acc.p : nullptr);
+            RSH_HIP(hipMemcpyAsync(&S.impesFallbacks, S.cpr->fallbackDev.p, sizeof(int), hipMemcpyDeviceToHost, st));
         }
         if (!wantIlu) {
             // block-Jacobi second stage: D_b^-1 of the natural blocks (FP32 copy for the mixed precision)
@@ -651,7 +673,8 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
         std::ostringstream o;
         o << "{\"n\":" << S.n << ",\"nb\":" << S.nb << ",\"nnzb\":" << S.nnzb << ",\"spmv_slices\":" << S.spmvHost.nslices
           << ",\"spmv_slots\":" << S.spmvHost.nslots << ",\"spmv_long_rows\":" << S.spmvHost.lrow.size()
-          << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0);
+          << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0) << ",\"cpr_true_impes\":" << (S.acc.n ? 1 : 0)
+          << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks;
         if (S.ilu) {
             o << ",\"ilu_levels\":" << S.ilu->sL.levels() << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
               << ",\"ilu_long_rows_L\":" << S.ilu->hL.lrow.size() << ",\"ilu_long_rows_U\":" << S.ilu->hU.lrow.size() << ",\"ilu_bytes\":" << S.ilu->bytes();

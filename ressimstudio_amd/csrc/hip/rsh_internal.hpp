@@ -236,7 +236,9 @@ void launchIluGroupFactor(hipStream_t s, int nb, int g0, int g1, const int* gsta
 // x = double(float(x)) in place (the lean factorisation's rounded input)
 void launchRoundToFloat(hipStream_t s, long long n, double* x);
 // CPR kernels
-void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w);
+// quasi-IMPES weights, or true-IMPES from n accumulation blocks acc (ill-conditioned cells counted in fallbacks)
+void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w, const double* acc = nullptr,
+                      int* fallbacks = nullptr);
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap);
 void launchFixDiag(hipStream_t s, int n, const int* diag, double* v);
 // r_p = w^T r; with x1 given also level 0's first damped-Jacobi iterate x1 = omega D^-1 r_p (dinv FP32 when f32)

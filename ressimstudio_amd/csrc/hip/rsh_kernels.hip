@@ -1184,9 +1184,13 @@ __global__ __launch_bounds__(kWave) void kBilu0LongRows(const int* __restrict__ 
 }
 
 // ---------------------------------------------------------------- CPR (bsr.cpp:993-1084)
-// quasi-IMPES weights: w_i = first row of D_ii^-1; singular -> 1 / sum_e D[e][0]; non-finite or zero -> 1
+// quasi-IMPES weights (bsr.cpp:1000-1030): w_i = first row of D_ii^-1; singular -> 1 / sum_e D[e][0]; non-finite or
+// zero -> 1.  With acc (n accumulation blocks B_i), true-IMPES: B_i is inverted and, if its 1-norm condition estimate
+// |B|_1 |B^-1|_1 is at most 1e10 (or nb = 1), the first row of B_i^-1 is the weight; a singular scalar block is used
+// as it is; otherwise the cell falls back to the quasi-IMPES weight and is counted in fallbacks.
 template <int NB>
-__global__ void kCprWeights(int n, const int* __restrict__ diag, const double* __restrict__ val, double* __restrict__ w) {
+__global__ void kCprWeights(int n, const int* __restrict__ diag, const double* __restrict__ val, double* __restrict__ w,
+                            const double* __restrict__ acc, int* __restrict__ fallbacks) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1194,8 +1198,37 @@ __global__ void kCprWeights(int n, const int* __restrict__ diag, const double* _
     double D[NB * NB];
 #pragma unroll
     for (int e = 0; e < NB * NB; ++e) D[e] = D0[e];
+    bool inverted = false;
+    if (acc) {
+        const double* B = acc + (long long)i * NB * NB;
+        double inv[NB * NB];
+#pragma unroll
+        for (int e = 0; e < NB * NB; ++e) inv[e] = B[e];
+        if (invertBlockDev<NB>(inv)) {
+            double n1 = 0, ni = 0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                double c = 0, d = 0;
+#pragma unroll
+                for (int r = 0; r < NB; ++r) { c += fabs(B[r * NB + j]); d += fabs(inv[r * NB + j]); }
+                n1 = fmax(n1, c);
+                ni = fmax(ni, d);
+            }
+            if (NB == 1 || n1 * ni <= 1e10) {
+#pragma unroll
+                for (int e = 0; e < NB * NB; ++e) D[e] = inv[e];
+                inverted = true;
+            } else if (fallbacks) {
+                atomicAdd(fallbacks, 1);
+            }
+        } else if (NB == 1) {
+            D[0] = B[0];
+        } else if (fallbacks) {
+            atomicAdd(fallbacks, 1);
+        }
+    }
     double wi[NB];
-    if (invertBlockDev<NB>(D)) {
+    if (inverted || invertBlockDev<NB>(D)) {
 #pragma unroll
         for (int e = 0; e < NB; ++e) wi[e] = D[e];
     } else {
@@ -2032,8 +2065,8 @@ void iluGroupFactorT(hipStream_t s, int g0, int g1, const int* gstart, const int
                        rp, lean);
 }
 template <int NB>
-void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w) {
-    hipLaunchKernelGGL(kCprWeights<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, val, w);
+void cprWeightsT(hipStream_t s, int n, const int* diag, const double* val, double* w, const double* acc, int* fallbacks) {
+    hipLaunchKernelGGL(kCprWeights<NB>, dim3((n + 255) / 256), dim3(256), 0, s, n, diag, val, w, acc, fallbacks);
 }
 template <int NB>
 void cprPressureT(hipStream_t s, int n, const int* rp, const double* val, const double* w, double* ap) {
@@ -2292,8 +2325,8 @@ void launchRoundToFloat(hipStream_t s, long long n, double* x) {
     hipLaunchKernelGGL(kRoundToFloat, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, x);
     checkLaunch();
 }
-void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w) {
-    RSH_NB_DISPATCH(nb, (cprWeightsT<NB>(s, n, diag, val, w)));
+void launchCprWeights(hipStream_t s, int nb, int n, const int* diag, const double* val, double* w, const double* acc, int* fallbacks) {
+    RSH_NB_DISPATCH(nb, (cprWeightsT<NB>(s, n, diag, val, w, acc, fallbacks)));
     checkLaunch();
 }
 void launchCprPressure(hipStream_t s, int nb, int n, const int* rp, const double* val, const double* w, double* ap) {

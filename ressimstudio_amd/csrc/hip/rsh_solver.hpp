@@ -688,14 +688,19 @@ struct CprDev {
         Minv = a;
     }
     // numeric set-up on the device values of S: weights, pressure matrix, hierarchy (fresh or refreshed)
+    // acc: n accumulation blocks for true-IMPES weights (nullptr: quasi-IMPES); the cells that fell back to
+    // quasi-IMPES are counted in fallbackDev
+    DBuf<int> fallbackDev;
     void setup(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* val, bool reuse, hipStream_t s,
-               bool f32 = false) {
+               bool f32 = false, const double* acc = nullptr) {
         n = S.n;
         nb = S.nb;
         w.alloc(size_t(n) * nb);
         xp.alloc(size_t(n));
         t.alloc(size_t(n) * nb);
-        launchCprWeights(s, nb, n, ddiag, val, w.p);
+        fallbackDev.alloc(1);
+        fallbackDev.zero(s);
+        launchCprWeights(s, nb, n, ddiag, val, w.p, acc, fallbackDev.p);
         const bool refresh = reuse && built && patRp == S.rp && patCi == S.ci;
         DBuf<double> apTmp;
         double* ap0;
@@ -856,6 +861,8 @@ struct rsh_system {
     SliceDev<double> spmv64;
     SliceDev<float> spmv32;
     bool have32 = false;
+    DBuf<double> acc;                   // accumulation blocks for true-IMPES CPR weights (n*nb*nb; empty: quasi-IMPES)
+    int impesFallbacks = 0;             // cells of the last set-up whose accumulation block fell back to quasi-IMPES
     DBuf<double> colP;                  // pressure column of S in the SpMV slice layout (CPR second stage)
     DBuf<float> colP32;                 // its FP32 copy (mixed precision)
     DBuf<double> bjInv;                 // block-Jacobi second stage: D_b^-1 per block row

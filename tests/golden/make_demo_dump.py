@@ -1,6 +1,8 @@
 """Writes tests/golden/demo_jacobian.npz: the Jacobian dump (RESSIM_DUMP) of the first Newton iteration of the
 reference engine's demo waterflood (makeDemoCase, case.cpp:427-480), produced by the engine compiled from the
-reference sources (oracle/_ref/newton_harness, built by oracle/Makefile where the reference is mounted).
+reference sources (oracle/_ref/newton_harness, built by oracle/Makefile where the reference is mounted).  The run
+uses true-IMPES CPR weights (Tuning cprWeights = 1), so the engine also writes the accumulation blocks next to the
+dump (simulator.cpp:3299-3316); they are kept as "acc" (the Jacobian itself does not depend on the weights).
 
     python tests/golden/make_demo_dump.py
 """
@@ -24,11 +26,13 @@ def main():
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "demo.bin")
         env = dict(os.environ, RESSIM_DUMP=path)
-        subprocess.run([HARNESS, "1", "cpr", "gs"], env=env, check=True, capture_output=True, cwd=tmp, timeout=600)
+        subprocess.run([HARNESS, "1", "cpr", "gs", "cpu-true-impes"], env=env, check=True, capture_output=True, cwd=tmp, timeout=600)
         d = dump.read_dump(path)
+        if d["acc"] is None:
+            sys.exit("the engine wrote no accumulation blocks")
     out = os.path.join(ROOT, "tests", "golden", "demo_jacobian.npz")
     np.savez_compressed(out, n=d["n"], na=d["na"], nv=d["nv"], nw=d["nw"], row_ptr=d["row_ptr"], col=d["col"], val=d["val"],
-                        rhs=d["rhs"])
+                        rhs=d["rhs"], acc=d["acc"])
     print(f"{out}: n={d['n']} na={d['na']} nv={d['nv']} nw={d['nw']} nnz={int(d['row_ptr'][-1])}")
 
 

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 @pytest.fixture(scope="module")
 def demo():
     z = np.load(os.path.join(HERE, "golden", "demo_jacobian.npz"))
-    return {k: (int(z[k]) if z[k].ndim == 0 else z[k]) for k in z.files} | {"acc": None}
+    return {k: (int(z[k]) if z[k].ndim == 0 else z[k]) for k in z.files}
 
 
 def _csr(d):
@@ -28,7 +28,7 @@ def test_round_trip(demo, tmp_path):
     d = dump.read_dump(p)
     for k in ("n", "na", "nv", "nw"):
         assert d[k] == demo[k]
-    for k in ("row_ptr", "col", "val", "rhs"):
+    for k in ("row_ptr", "col", "val", "rhs", "acc"):
         assert np.array_equal(d[k], demo[k])
     assert os.path.getsize(p) == 20 + 4 * (demo["n"] + 1) + 12 * demo["row_ptr"][-1] + 8 * demo["n"]
 
@@ -67,3 +67,28 @@ def test_well_schur_of_the_engine_jacobian(demo, orc):
     assert ok
     x = s.recover(demo["rhs"], xc)
     assert np.linalg.norm(J @ x - demo["rhs"]) <= 1e-9 * np.linalg.norm(demo["rhs"])
+
+
+def _ill_conditioned(acc):
+    """the engine's accumulation blocks with cells made ill-conditioned (two nearly equal columns) or singular"""
+    a = acc.copy()
+    a[::7, :, 1] = a[::7, :, 0] * (1 + 1e-13)
+    a[3::11] = 0.0
+    return a
+
+
+def test_true_impes_weights_of_the_engine_dump(demo, orc, ref):
+    # true-IMPES CPR weights (BlockCPR::setup accBlocks, bsr.cpp:993-1030) from the engine's own accumulation blocks
+    # (the demo run with cprWeights = 1): the oracle's restatement equals the compiled reference bit for bit,
+    # including the cells that fall back to quasi-IMPES; and the weights are not the quasi-IMPES ones
+    s = dump.to_system(demo)
+    acc = demo["acc"]
+    assert acc.shape == (s.n, s.nb, s.nb)
+    for a in (acc, _ill_conditioned(acc)):
+        co, cr = orc.cpr(s, a), ref.cpr(s, a)
+        assert np.array_equal(co.weights(), cr.weights())
+        assert co.fallbacks() == cr.fallbacks()
+        r = np.cos(0.3 * np.arange(s.n * s.nb))
+        assert np.array_equal(co.apply(r), cr.apply(r))
+    assert orc.cpr(s, _ill_conditioned(acc)).fallbacks() > 0
+    assert not np.array_equal(orc.cpr(s, acc).weights(), orc.cpr(s).weights())

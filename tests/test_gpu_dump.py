@@ -18,9 +18,7 @@ TOL = 1e-8
 @pytest.fixture(scope="module")
 def demo():
     z = np.load(os.path.join(HERE, "golden", "demo_jacobian.npz"))
-    d = {k: (int(z[k]) if z[k].ndim == 0 else z[k]) for k in z.files}
-    d["acc"] = None
-    return d
+    return {k: (int(z[k]) if z[k].ndim == 0 else z[k]) for k in z.files}
 
 
 @pytest.mark.parametrize("opts", [dict(precond="cpr", smoother=rs.SMOOTH_GS), dict(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2),
@@ -39,3 +37,26 @@ def test_engine_jacobian(demo, orc, opts):
     J = sp.csr_matrix((demo["val"], demo["col"], demo["row_ptr"]), shape=(demo["n"], demo["n"]))
     xf = s.recover(demo["rhs"], x)
     assert np.linalg.norm(J @ xf - demo["rhs"]) <= 1e-6 * np.linalg.norm(demo["rhs"])
+
+
+def test_true_impes_weights(demo, orc):
+    # rsh_system_set_accumulation: the device's true-IMPES weights from the engine's accumulation blocks equal the
+    # oracle's bit for bit (also with cells forced to fall back to quasi-IMPES, counted alike); the CPR built on them
+    # solves the engine's system; clearing the blocks restores the quasi-IMPES weights
+    s = dump.to_system(demo)
+    acc = demo["acc"]
+    ill = acc.copy()
+    ill[::7, :, 1] = ill[::7, :, 0] * (1 + 1e-13)
+    ill[3::11] = 0.0
+    d = rs.System(s)
+    for a in (acc, ill):
+        d.set_accumulation(a).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+        o = orc.cpr(s, a)
+        assert np.array_equal(d.cpr_weights(), o.weights())
+        info = d.info()
+        assert info["cpr_true_impes"] == 1 and info["cpr_impes_fallbacks"] == o.fallbacks()
+        x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
+        assert ok and np.linalg.norm(s.rhs - orc.spmv(s, x)) <= 1.01 * TOL * np.linalg.norm(s.rhs)
+    d.set_accumulation(None).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+    assert np.array_equal(d.cpr_weights(), orc.cpr(s).weights())
+    assert d.info()["cpr_true_impes"] == 0
