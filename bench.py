@@ -334,15 +334,21 @@ def main():
         print(json.dumps(serving_leg(rs, gen, args)))
         return
     t = time.perf_counter()
+    acc = None
     if args.dump:
         from ressimstudio_amd import dump
-        s = dump.to_system(dump.read_dump(args.dump))
+        dd = dump.read_dump(args.dump)
+        s = dump.to_system(dd)
+        acc = dd["acc"]           # the engine wrote accumulation blocks: true-IMPES weights, as the engine used them
         args.scale = 0.0          # not the 1.1M-cell synthetic workload: no traffic file, no batched / serving legs
     else:
         s = gen.large(scale=args.scale)
     log(f"workload: {s.n} cells x {s.nb}, {s.nnzb} blocks ({s.fill_blocks} well fill), generated in {time.perf_counter() - t:.1f}s")
     ctx = rs.default_context()
     sysd = rs.System(s, ctx)
+    if acc is not None:
+        sysd.set_accumulation(acc)
+        log("CPR weights: true-IMPES from the dump's accumulation blocks")
     dev = DeviceInputs(rs, ctx, s)
 
     J, GS, FP64, MIXED, BI, FG = rs.SMOOTH_JACOBI, rs.SMOOTH_GS, rs.FP64, rs.MIXED, rs.KRYLOV_BICGSTAB, rs.KRYLOV_FGMRES
@@ -476,7 +482,7 @@ def main():
         "data": "synthetic (seeded generator, ressimstudio_amd/csrc/gen)",
         "config": {"workload": (f"engine Jacobian dump {os.path.basename(args.dump)}, {s.n} cells x nb {s.nb}" if args.dump else
                                 f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells") + f"; variant {head}: BiCGSTAB to 1e-8 "
-                               f"preconditioned by CPR (quasi-IMPES weights, aggregation AMG V-cycle with two Jacobi steps (weight 1) per level) + "
+                               f"preconditioned by CPR ({'true' if acc is not None else 'quasi'}-IMPES weights, aggregation AMG V-cycle with two Jacobi steps (weight 1) per level) + "
                                f"multicolour BILU0; numeric setup + solve per step",
                    "cells": s.n, "block": s.nb, "blocks": s.nnzb, "tol": args.tol, "scale": args.scale},
         "iterations": main_r["iterations"],
