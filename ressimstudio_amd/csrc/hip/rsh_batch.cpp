@@ -10,6 +10,7 @@
 // others iterate.  One BiCGSTAB iteration over all systems is captured once as a HIP graph and replayed; the host
 // reads the per-system states back once per iteration.
 #include <functional>
+#include <thread>
 
 #include "rsh_solver.hpp"
 
@@ -58,9 +59,12 @@ struct rsh_batch {
     bool lastFallback = false;    // the last refinement fell back to the FP64 preconditioner
 
     hipStream_t s() const { return ctx->c.stream; }
+    // one context (stream) per group for the numeric set-ups, which run concurrently (one host thread each)
+    std::vector<rsh_context*> setupCtx;
     ~rsh_batch() {
         if (iter) (void)hipGraphExecDestroy(iter);
         for (auto& g : groups) delete g.S;
+        for (auto* c : setupCtx) rsh_context_destroy(c);
         if (hks) (void)hipHostFree(hks);
         if (hnorm) (void)hipHostFree(hnorm);
     }
@@ -553,10 +557,31 @@ int rsh_batch_setup(rsh_batch* batch, const rsh_options* opt) {
         if (opt->krylov != RSH_KRYLOV_BICGSTAB) return fail(RSH_ERR_UNSUPPORTED, "batched solves use BiCGSTAB");
         RSH_HIP(hipSetDevice(batch->ctx->c.device));
         const double t0 = nowMs();
-        for (auto& g : batch->groups) {
-            const int rc = rsh_system_setup(g.S, opt);
+        // the groups are independent systems: their set-ups (host aggregation and short latency-bound kernels)
+        // run concurrently, each on a stream of its own, after the batch stream's earlier work (new values)
+        const size_t G = batch->groups.size();
+        RSH_HIP(hipStreamSynchronize(batch->s()));
+        while (batch->setupCtx.size() < G) {
+            rsh_context* c = nullptr;
+            const int rc = rsh_context_create(batch->ctx->c.device, &c);
             if (rc != RSH_OK) return rc;
+            batch->setupCtx.push_back(c);
         }
+        std::vector<int> rcs(G, RSH_OK);
+        std::vector<std::string> errs(G);
+        std::vector<std::thread> th;
+        th.reserve(G);
+        for (size_t gi = 0; gi < G; ++gi)
+            th.emplace_back([&, gi] {
+                rsh_system* S = batch->groups[gi].S;
+                S->ctx = batch->setupCtx[gi];
+                rcs[gi] = rsh_system_setup(S, opt);         // ends with a synchronisation of its stream
+                if (rcs[gi] != RSH_OK) errs[gi] = rsh_last_error();
+                S->ctx = batch->ctx;
+            });
+        for (auto& t : th) t.join();
+        for (size_t gi = 0; gi < G; ++gi)
+            if (rcs[gi] != RSH_OK) return fail(rcs[gi], errs[gi]);
         batch->opt = *opt;
         if (opt->precision == RSH_MIXED)
             for (auto* bf : {&batch->xo, &batch->rt, &batch->ax, &batch->xPrev, &batch->rPrev}) bf->alloc(size_t(batch->N));
