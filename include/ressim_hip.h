@@ -91,7 +91,8 @@ typedef struct rsh_options {
                            engine's default from 500 000 cells, simulator.cpp:3318): the matrix rounded to float32,
                            every row eliminated in double against the already-rounded earlier rows and stored
                            rounded; the sweeps read float32 factors with FP64 vectors and arithmetic.  Default 0 */
-    int reserved[3];
+    int coarse_smooth_steps; /* RSH_SMOOTH_JACOBI: steps on the levels below the finest (<= 0: smooth_steps) */
+    int reserved[2];
 } rsh_options;
 
 /* ---- library / context */

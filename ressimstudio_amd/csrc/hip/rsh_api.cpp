@@ -517,6 +517,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             if (!S.cpr) S.cpr = std::make_unique<CprDev>();
             S.cpr->smoother = o.smoother;
             S.cpr->steps = o.smooth_steps > 0 ? o.smooth_steps : 1;
+            S.cpr->coarseSteps = o.coarse_smooth_steps > 0 ? o.coarse_smooth_steps : 0;
             S.cpr->omega = o.jacobi_omega > 0 ? o.jacobi_omega : CprDev::kOmegaDefault;
             S.cpr->plan = &S.spmv64;
             if (S.colP.n == 0) {

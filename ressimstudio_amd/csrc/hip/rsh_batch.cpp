@@ -119,7 +119,8 @@ struct rsh_batch {
     }
     std::vector<int> currentKey() const {
         // every option a captured kernel bakes in (the Jacobi weight by its bit pattern)
-        std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0, opt.ilu_lean, opt.smoother, opt.smooth_steps};
+        std::vector<int> k{chunk, opt.precision, opt.precond, prec32 ? 1 : 0, opt.ilu_lean, opt.smoother, opt.smooth_steps,
+                           opt.coarse_smooth_steps};
         int w[2];
         std::memcpy(w, &opt.jacobi_omega, sizeof w);
         k.push_back(w[0]);

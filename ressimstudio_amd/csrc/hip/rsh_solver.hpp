@@ -594,6 +594,7 @@ struct CprDev {
     double* Minv = nullptr;
     int smoother = RSH_SMOOTH_GS;
     int steps = 1;                        // Jacobi pre- and post-smoothing steps
+    int coarseSteps = 0;                  // ... on the levels below the finest (0: steps)
     bool built = false;
     // the SpMV row-slice layout of S (same pattern as level 0), for level 0's streamed Jacobi steps
     const SliceDev<double>* plan = nullptr;
@@ -784,8 +785,9 @@ struct CprDev {
             const void* lvv = h ? static_cast<const void*>(L.pL32.p) : static_cast<const void*>(L.pL.p);
             const void* vv = h ? static_cast<const void*>(L.v32.p) : static_cast<const void*>(L.v.p);
             const void* dv = dinvOf(l, f32);
+            const int nst = l > 0 && coarseSteps > 0 ? coarseSteps : steps;
             double *xa = L.x1.p, *xb = L.x3.p;
-            for (int k = 1; k < steps; ++k) {
+            for (int k = 1; k < nst; ++k) {
                 launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, xa, omega, xb, mean);
                 std::swap(xa, xb);
             }
@@ -795,8 +797,8 @@ struct CprDev {
                            h, omega, smoothedC ? C.x1.p : nullptr);
             if (!cycle(l + 1, s, f32, xa)) launchProlongAdd(s, L.n, L.agg.p, C.result, xa);
             double *pa = xa, *pb = L.x2.p;
-            for (int k = 0; k < steps; ++k) {
-                const bool last = k + 1 == steps;
+            for (int k = 0; k < nst; ++k) {
+                const bool last = k + 1 == nst;
                 launchJacobiFused(s, 1, h, sl, sv, lvv, L.n, L.rp, L.ci, vv, dv, L.b.p, pa, omega, pb, mean, last ? prp : nullptr,
                                   last ? pci : nullptr, last ? pfine : nullptr);
                 std::swap(pa, pb);

@@ -424,6 +424,26 @@ def test_lean_solves(sysd, orc, name, precond, precision):
         assert abs(its - its_o) <= max(3, its_o // 5), (its, its_o)
 
 
+@pytest.mark.parametrize("name", ["layered", "medium"])
+def test_coarse_smoothing_steps_option(sysd, orc, name):
+    # rsh_options.coarse_smooth_steps: Jacobi steps on the levels below the finest (0: smooth_steps); another count
+    # changes the apply and still converges to the FP64 oracle's solution
+    s = sysd[name]
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-12, 3000)
+    assert ok_o
+    d = rs.System(s)
+    r = np.cos(0.3 * np.arange(s.n * s.nb))
+    zs = []
+    for cs in (0, 2, 1):
+        d.setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0, coarse_smooth_steps=cs)
+        zs.append(d.precond(r))
+        x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+        assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert np.array_equal(zs[0], zs[1])          # 0 means smooth_steps
+    assert not np.array_equal(zs[0], zs[2])
+
+
 @pytest.mark.parametrize("name", ["layered", "deepwell", "medium"])
 def test_jacobi_weight_option(sysd, orc, name):
     # rsh_options.jacobi_omega: the V-cycle's damping weight (default DeviceCprAmg's 0.6667); weight 1 converges to the
