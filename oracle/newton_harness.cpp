@@ -52,9 +52,11 @@ int main(int argc, char** argv) {
     const int steps = argc > 1 ? std::atoi(argv[1]) : 6;
     const std::string pc = argc > 2 ? argv[2] : "cpr";
     const std::string sm = argc > 3 ? argv[3] : "gs";
-    // "cpu-true-impes": the CPU run only, with true-IMPES weights (the engine then also writes the accumulation
-    // blocks next to a RESSIM_DUMP file, simulator.cpp:3313-3315)
-    const bool cpuTrueImpes = argc > 4 && std::string(argv[4]) == "cpu-true-impes";
+    // 4th argument: "cpu-true-impes": the CPU run only, with true-IMPES weights (the engine then also writes the
+    // accumulation blocks next to a RESSIM_DUMP file, simulator.cpp:3313-3315); "lean": the device run with the lean
+    // BILU0 factors; "store32": the device run with the FP32-stored preconditioner (FP64 arithmetic)
+    const std::string form = argc > 4 ? argv[4] : "";
+    const bool cpuTrueImpes = form == "cpu-true-impes";
     Case c = makeDemoCase();
     if (steps > 0 && int(c.schedule.size()) > steps) c.schedule.resize(size_t(steps));
     c.tuning.blockSolver = true;
@@ -71,6 +73,8 @@ int main(int argc, char** argv) {
     opt.precond = pc == "bilu0" ? RSH_PREC_BILU0 : pc == "cpr_mc" ? RSH_PREC_CPR_MC : RSH_PREC_CPR;
     opt.smoother = sm == "jacobi" ? RSH_SMOOTH_JACOBI : RSH_SMOOTH_GS;
     opt.reuse_amg = 0;                                 // a fresh hierarchy per solve, as the CPU path builds it
+    if (form == "lean") opt.ilu_lean = 1;
+    if (form == "store32") opt.precision = RSH_MIXED_STORE;
     ressim_hip::HipBlockSolver gpu(&opt, 0);
     setDistributedSolver(&gpu);
     std::fprintf(stderr, "== run device\n");

@@ -94,9 +94,11 @@ def test_cpu_run_matches_the_recorded_trajectory():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precond,smoother", [("cpr", "gs"), ("cpr_mc", "jacobi"), ("bilu0", "gs")])
-def test_demo_waterflood_with_device_solver(precond, smoother):
-    r, traj = run_harness(6, precond, smoother)
+@pytest.mark.parametrize("precond,smoother,form", [("cpr", "gs", ""), ("cpr_mc", "jacobi", ""), ("bilu0", "gs", ""),
+                                                  ("cpr_mc", "jacobi", "lean"), ("cpr_mc", "jacobi", "store32")])
+def test_demo_waterflood_with_device_solver(precond, smoother, form):
+    # form: the device's lean BILU0 factors, or its FP32-stored preconditioner with FP64 arithmetic
+    r, traj = run_harness(6, precond, smoother, *([form] if form else []))
     cpu, dev = r["cpu"], r["device"]
     print(json.dumps({"cpu": {k: cpu[k] for k in ("newton", "linear")}, "device": {k: dev[k] for k in ("newton", "linear", "dd_solves")},
                       "device_seconds": r["meta"]["device_seconds"]}))
