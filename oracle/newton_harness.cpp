@@ -5,7 +5,10 @@
 // test runs the binary and compares per-report-step Newton/linear counts, FOPT and FPR (the pattern of the reference's
 // own plug-in tests, run_tests.cpp:3755-3773, c29_tests.inc:280-296).
 //
-// usage: newton_harness [steps] [precond: cpr|cpr_mc|bilu0] [smoother: gs|jacobi]
+// usage: newton_harness [steps] [precond: cpr|cpr_mc|bilu0] [smoother: gs|jacobi] [form: lean|store32|hookA|cpu-true-impes]
+// hookA registers the solver in the engine's mixed hook instead (setGpuBlockSolver with Tuning gpu = 1: the engine's
+// FP64 refinement around the device solve, refineWithDevice, simulator.cpp:3368-3378), with FP32 preconditioner
+// arithmetic (RSH_MIXED).
 // With RESSIM_DEBUG set the engine prints one "it N cnv X mb Y" line per Newton iteration to stderr
 // (simulator.cpp:3207-3214); the harness marks where each run starts ("== run cpu" / "== run device").
 #include <cstdio>
@@ -36,8 +39,9 @@ Run run(const Case& c) {
 }
 
 void dump(const char* name, const Run& r) {
-    std::printf("{\"run\":\"%s\",\"newton\":%d,\"linear\":%d,\"dd_solves\":%d,\"dd_fallbacks\":%d,\"timesteps\":%d,\"cuts\":%d,\"steps\":[", name,
-                r.stats.newton, r.stats.linear, r.stats.ddSolves, r.stats.ddFallbacks, r.stats.timesteps, r.stats.cuts);
+    std::printf("{\"run\":\"%s\",\"newton\":%d,\"linear\":%d,\"dd_solves\":%d,\"dd_fallbacks\":%d,\"gpu_solves\":%d,\"gpu_fallbacks\":%d,"
+                "\"timesteps\":%d,\"cuts\":%d,\"steps\":[", name, r.stats.newton, r.stats.linear, r.stats.ddSolves, r.stats.ddFallbacks,
+                r.stats.gpuSolves, r.stats.gpuFallbacks, r.stats.timesteps, r.stats.cuts);
     for (size_t i = 0; i < r.summary.size(); ++i) {
         const SummaryPoint& p = r.summary[i];
         std::printf("%s{\"t\":%.10g,\"newton\":%d,\"linear\":%d,\"fopt\":%.12g,\"fpr\":%.12g,\"fwpt\":%.12g}", i ? "," : "", p.time, p.newton,
@@ -75,12 +79,20 @@ int main(int argc, char** argv) {
     opt.reuse_amg = 0;                                 // a fresh hierarchy per solve, as the CPU path builds it
     if (form == "lean") opt.ilu_lean = 1;
     if (form == "store32") opt.precision = RSH_MIXED_STORE;
+    const bool hookA = form == "hookA";
+    if (hookA) opt.precision = RSH_MIXED;
     ressim_hip::HipBlockSolver gpu(&opt, 0);
-    setDistributedSolver(&gpu);
+    if (hookA) {
+        c.tuning.gpu = 1;
+        setGpuBlockSolver(&gpu);
+    } else {
+        setDistributedSolver(&gpu);
+    }
     std::fprintf(stderr, "== run device\n");
     std::fflush(stderr);
     const Run dev = run(c);
     setDistributedSolver(nullptr);
+    setGpuBlockSolver(nullptr);
     dump("device", dev);
     std::printf("{\"device_seconds\":%.6f,\"last_error\":\"%s\"}\n", gpu.deviceSeconds(), gpu.lastError());
     return 0;

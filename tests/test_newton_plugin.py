@@ -95,14 +95,20 @@ def test_cpu_run_matches_the_recorded_trajectory():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precond,smoother,form", [("cpr", "gs", ""), ("cpr_mc", "jacobi", ""), ("bilu0", "gs", ""),
-                                                  ("cpr_mc", "jacobi", "lean"), ("cpr_mc", "jacobi", "store32")])
+                                                  ("cpr_mc", "jacobi", "lean"), ("cpr_mc", "jacobi", "store32"),
+                                                  ("cpr_mc", "jacobi", "hookA")])
 def test_demo_waterflood_with_device_solver(precond, smoother, form):
-    # form: the device's lean BILU0 factors, or its FP32-stored preconditioner with FP64 arithmetic
+    # form: the device's lean BILU0 factors, or its FP32-stored preconditioner with FP64 arithmetic (hook B, FP64
+    # drop-in); hookA: the engine's mixed hook (setGpuBlockSolver, its FP64 refinement around the device solve, here
+    # with FP32 preconditioner arithmetic)
     r, traj = run_harness(6, precond, smoother, *([form] if form else []))
     cpu, dev = r["cpu"], r["device"]
-    print(json.dumps({"cpu": {k: cpu[k] for k in ("newton", "linear")}, "device": {k: dev[k] for k in ("newton", "linear", "dd_solves")},
-                      "device_seconds": r["meta"]["device_seconds"]}))
-    assert dev["dd_solves"] > 0 and dev["dd_fallbacks"] == 0, r["meta"]
+    print(json.dumps({"cpu": {k: cpu[k] for k in ("newton", "linear")},
+                      "device": {k: dev[k] for k in ("newton", "linear", "dd_solves", "gpu_solves")}, "device_seconds": r["meta"]["device_seconds"]}))
+    if form == "hookA":
+        assert dev["gpu_solves"] > 0 and dev["gpu_fallbacks"] == 0 and dev["dd_solves"] == 0, r["meta"]
+    else:
+        assert dev["dd_solves"] > 0 and dev["dd_fallbacks"] == 0, r["meta"]
     # identical Newton convergence: the same number of Newton iterations in every report step
     assert [s["newton"] for s in dev["steps"]] == [s["newton"] for s in cpu["steps"]]
     _compare_steps(cpu, dev, 1e-5)
