@@ -98,8 +98,9 @@ def test_cpu_run_matches_the_recorded_trajectory():
                                                   ("cpr_mc", "jacobi", "lean"), ("cpr_mc", "jacobi", "store32"),
                                                   ("cpr_mc", "jacobi", "hookA")])
 def test_demo_waterflood_with_device_solver(precond, smoother, form):
-    # form: the device's lean BILU0 factors, or its FP32-stored preconditioner with FP64 arithmetic (hook B, FP64
-    # drop-in); hookA: the engine's mixed hook (setGpuBlockSolver, its FP64 refinement around the device solve, here
+    # form: the device's lean BILU0 factors (the reference's own lean check, testC45LeanSolver c45_tests.inc:14-37,
+    # asks FOPT within 1e-5 of the default run on this case), or its FP32-stored preconditioner with FP64 arithmetic
+    # (hook B, FP64 drop-in); hookA: the engine's mixed hook (setGpuBlockSolver, its FP64 refinement around the device solve, here
     # with FP32 preconditioner arithmetic)
     r, traj = run_harness(6, precond, smoother, *([form] if form else []))
     cpu, dev = r["cpu"], r["device"]
