@@ -424,6 +424,58 @@ def test_lean_solves(sysd, orc, name, precond, precision):
         assert abs(its - its_o) <= max(3, its_o // 5), (its, its_o)
 
 
+def _selftest_matrix(nx, ny, nz, nb, seed):
+    """The structure of the reference app's device self-test matrix (testMatrix, app/src/gpu_compute.cpp:720-749):
+    7-point couplings sqrt(k_c k_o) with log-uniform permeabilities (kz 0.1 x), diagonal 1.05 x the coupling sum
+    scaled 1 + 0.2 e on its diagonal and 0.03 off it, off-diagonal blocks -t on the diagonal (0.7 for e > 0).  The
+    reference draws with std::uniform_real_distribution, whose values are implementation-defined, so this draws the
+    same distributions from numpy: the check is the property (convergence, error < 1e-8), not the matrix."""
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    kx = 10.0 ** rng.uniform(-1, 1, n)
+    kz = 0.1 * 10.0 ** rng.uniform(-1, 1, n)
+    rp, cols, vals = [0], [], []
+    eye = np.eye(nb)
+    offd = -np.diag([1.0] + [0.7] * (nb - 1))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = (k * ny + j) * nx + i
+                row = []
+                if k > 0: row.append((c - nx * ny, np.sqrt(kz[c] * kz[c - nx * ny])))
+                if j > 0: row.append((c - nx, np.sqrt(kx[c] * kx[c - nx])))
+                if i > 0: row.append((c - 1, np.sqrt(kx[c] * kx[c - 1])))
+                if i < nx - 1: row.append((c + 1, np.sqrt(kx[c] * kx[c + 1])))
+                if j < ny - 1: row.append((c + nx, np.sqrt(kx[c] * kx[c + nx])))
+                if k < nz - 1: row.append((c + nx * ny, np.sqrt(kz[c] * kz[c + nx * ny])))
+                row.append((c, 1.05 * sum(t for _, t in row)))
+                row.sort()
+                for o, t in row:
+                    cols.append(o)
+                    if o == c:
+                        vals.append(np.where(eye > 0, t * (1 + 0.2 * np.arange(nb))[:, None] * eye, 0.03 * t))
+                    else:
+                        vals.append(t * offd)
+                rp.append(len(cols))
+    return gen.BlockSystem(n=n, nb=nb, row_ptr=np.array(rp, np.int32), col=np.array(cols, np.int32),
+                           val=np.array(vals, np.float64), rhs=np.zeros(n * nb))
+
+
+@pytest.mark.parametrize("precond", ["mcbilu0", "cpr_bj", "cpr_mc"])
+def test_device_selftest_known_answer(orc, precond):
+    # the reference app's device self-test (selfTest, app/src/gpu_compute.cpp:759-797): a 120 x 100 x 3 system with a
+    # known solution x_t = cos(0.003 e) + 0.2 (e mod 3), solved through the FP64 refinement around the FP32 device
+    # solve (refineWithDevice to 1e-10, 8 outer steps, 2000 inner iterations): converged, max-norm error < 1e-8
+    s = _selftest_matrix(120, 100, 1, 3, 5)
+    e = np.arange(s.n * s.nb)
+    xt = np.cos(0.003 * e) + 0.2 * (e % 3)
+    b = orc.spmv(s, xt)
+    d = rs.System(s).setup(precond=precond, precision=rs.MIXED_IR, max_outer=8, smoother=rs.SMOOTH_JACOBI)
+    x, its, rel, ok = d.solve(b, 1e-10, 2000)
+    assert ok
+    assert np.max(np.abs(x - xt)) / np.max(np.abs(xt)) < 1e-8
+
+
 @pytest.mark.parametrize("name", ["layered", "medium"])
 def test_coarse_smoothing_steps_option(sysd, orc, name):
     # rsh_options.coarse_smooth_steps: Jacobi steps on the levels below the finest (0: smooth_steps); another count
