@@ -476,6 +476,75 @@ def test_device_selftest_known_answer(orc, precond):
     assert np.max(np.abs(x - xt)) / np.max(np.abs(xt)) < 1e-8
 
 
+def _c38_matrix():
+    """The reference's own C38 test system (testC38FloatIlu, c38_tests.inc:159-186), built with its portable 64-bit
+    LCG (so the same values as the reference draws): a 30 x 30 x 8 grid, nb 3, weaker couplings across layers."""
+    nx, ny, nz, nb = 30, 30, 8, 3
+    state = [12345]
+
+    def rnd():
+        state[0] = (state[0] * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return float((state[0] >> 11) & 0xFFFFF) / float(0xFFFFF)
+
+    def cid(i, j, k):
+        return (k * ny + j) * nx + i
+
+    n = nx * ny * nz
+    rp, cols, vals = [0], [], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                row = []
+                if k > 0: row.append(cid(i, j, k - 1))
+                if j > 0: row.append(cid(i, j - 1, k))
+                if i > 0: row.append(cid(i - 1, j, k))
+                row.append(cid(i, j, k))
+                if i < nx - 1: row.append(cid(i + 1, j, k))
+                if j < ny - 1: row.append(cid(i, j + 1, k))
+                if k < nz - 1: row.append(cid(i, j, k + 1))
+                r = cid(i, j, k)
+                for c in row:
+                    blk = []
+                    for e in range(9):
+                        if c == r:
+                            blk.append(0.3 * (rnd() - 0.5))
+                        else:
+                            v = 1.0 + rnd() if e % 4 == 0 else 0.2 * (rnd() - 0.5)
+                            blk.append(-v * (0.1 if c // (nx * ny) != r // (nx * ny) else 1.0))
+                    if c == r:
+                        for e in range(3):
+                            blk[e * 4] += 7.5 + rnd()
+                    cols.append(c)
+                    vals.append(np.array(blk).reshape(3, 3))
+                rp.append(len(cols))
+    rhs = np.array([rnd() - 0.5 for _ in range(n * nb)])
+    return gen.BlockSystem(n=n, nb=nb, row_ptr=np.array(rp, np.int32), col=np.array(cols, np.int32), val=np.array(vals),
+                           rhs=rhs)
+
+
+def test_c38_float_factors(orc):
+    # the reference's C38 check (c38_tests.inc:186-211) on its own matrix: the BILU0 apply from float32-stored factors
+    # (FP64 vectors and arithmetic) equals the FP64 one to float rounding (|dz|/|z| < 1e-5), also after a
+    # re-factorisation of new values on the same pattern (values x 2); beyond it, both applies equal the oracle's
+    # (FP64, and C38's float factors) to FP64 rounding
+    s = _c38_matrix()
+    d64 = rs.System(s).setup(precond="bilu0")
+    d32 = rs.System(s).setup(precond="bilu0", precision=rs.MIXED_STORE)
+    for scale in (1.0, 2.0):
+        if scale != 1.0:
+            v = s.val * scale
+            d64.set_values(v)
+            d64.setup(precond="bilu0")
+            d32.set_values(v)
+            d32.setup(precond="bilu0", precision=rs.MIXED_STORE)
+            s = gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=v, rhs=s.rhs)
+        z1, z2 = d64.precond(s.rhs), d32.precond(s.rhs)
+        assert np.linalg.norm(z1 - z2) / np.linalg.norm(z1) < 1e-5
+        zo1, zo2 = orc.bilu0(s, 1).apply(s.rhs), orc.bilu0(s, 2).apply(s.rhs)
+        assert np.max(np.abs(z1 - zo1)) <= 1e-12 * np.max(np.abs(zo1))
+        assert np.max(np.abs(z2 - zo2)) <= 1e-12 * np.max(np.abs(zo2))
+
+
 @pytest.mark.parametrize("name", ["layered", "medium"])
 def test_coarse_smoothing_steps_option(sysd, orc, name):
     # rsh_options.coarse_smooth_steps: Jacobi steps on the levels below the finest (0: smooth_steps); another count
