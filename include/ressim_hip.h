@@ -170,8 +170,9 @@ int rsh_wells_reduced_values(rsh_wells* w, double* out);                        
  * never span two systems, so the preconditioner stays block-diagonal (a system's result does not depend on the
  * others) while the group's coarsest level holds every system's coarsest aggregates.  BiCGSTAB only (FGMRES and
  * RSH_MIXED_IR are rejected).  One BiCGSTAB iteration over all systems is one captured HIP graph.  The groups'
- * numeric set-ups run concurrently (a stream and a host thread each).  The level chunking of the sweeps is tuned
- * per group (cached), the entries per workgroup of the vector kernels at the first setup. */
+ * numeric set-ups run concurrently (a stream and a host thread each).  The level chunking of the sweeps and the
+ * LDS tile of a fused run (its row limit) are tuned per group (cached), the entries per workgroup of the vector
+ * kernels at the first setup. */
 int rsh_batch_create(rsh_context* ctx, int count, const int* n, const int* nb, const int* const* row_ptr, const int* const* col,
                      rsh_batch** out);
 int rsh_batch_destroy(rsh_batch* batch);

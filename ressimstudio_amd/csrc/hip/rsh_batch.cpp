@@ -332,7 +332,7 @@ namespace {
 // Autotuner results per (block size, log2 of the group's block rows) and per log2 of the batch's vector length:
 // each candidate is timed once per process with HIP events on the context stream, then reused.
 std::mutex g_tuneMu;
-std::map<std::pair<int, int>, int> g_fuseCache;
+std::map<std::pair<int, int>, std::pair<int, int>> g_fuseCache;   // -> (fuseMax, run rows)
 std::map<int, int> g_chunkCache;
 
 int log2Bucket(long long v) {
@@ -344,7 +344,8 @@ int log2Bucket(long long v) {
 }  // namespace
 
 // Tunes (1) the level chunking of each group's ILU sweeps: the largest level (in tasks) swept inside a run of small
-// levels, i.e. how many levels one workgroup or one dense-group pass covers; (2) the entries per workgroup of the
+// levels, i.e. how many levels one workgroup or one dense-group pass covers, and the rows a run may hold (the LDS
+// tile of its working vector, 512 or 2048 rows); (2) the entries per workgroup of the
 // batched vector kernels and reductions (rows per wavefront = entries / 256 threads).  Candidates are timed on the
 // resident data; the fastest wins.  Runs at every set-up: the choices are cached per (block size, ordering, size
 // bucket), so a set-up that builds new factors (another preconditioner) only looks its choice up; groups whose
@@ -368,22 +369,23 @@ void rsh_batch::tune() {
         IluDev& I = *g.S->ilu;
         const auto key = std::make_pair(g.nb * 1000 + (I.mc ? 1 : 0), log2Bucket(g.S->n));
         auto it = g_fuseCache.find(key);
-        int best = 16;
+        std::pair<int, int> best{16, kRunRowsMax};
         if (it != g_fuseCache.end()) {
             best = it->second;
         } else {
             arm();
             double bestMs = 1e300;
             launchFill(st, p.p + g.off, 1.0, g.N);
-            for (int cand : {4, 16, 64}) {
-                I.retune(cand, st);
-                const double ms = timeMs([&] { g.S->precond(p.p + g.off, ph.p + g.off, f32()); }, 3);
-                if (ms < bestMs) { bestMs = ms; best = cand; }
-            }
+            for (int rows : {kRunRowsMax / 4, kRunRowsMax})
+                for (int cand : {4, 16, 64}) {
+                    I.retune(cand, st, rows);
+                    const double ms = timeMs([&] { g.S->precond(p.p + g.off, ph.p + g.off, f32()); }, 3);
+                    if (ms < bestMs) { bestMs = ms; best = {cand, rows}; }
+                }
             g_fuseCache[key] = best;
         }
-        if (I.fuseMax != best) I.retune(best, st);
-        fuseChosen[gi] = best;
+        if (I.fuseMax != best.first || I.runRowsMax != best.second) I.retune(best.first, st, best.second);
+        fuseChosen[gi] = best.first;
     }
     if (!tuned) {
         const int ck = log2Bucket(N);
@@ -669,7 +671,8 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
             const auto& g = batch->groups[gi];
             o << (gi ? "," : "") << "{\"nb\":" << g.nb << ",\"systems\":" << g.members.size() << ",\"rows\":" << g.S->n << ",\"blocks\":" << g.S->nnzb;
             if (g.S->ilu) {
-                o << ",\"ilu_levels\":" << g.S->ilu->sL.levels() << ",\"fuse_max\":" << g.S->ilu->fuseMax << ",\"dense_groups\":["
+                o << ",\"ilu_levels\":" << g.S->ilu->sL.levels() << ",\"fuse_max\":" << g.S->ilu->fuseMax << ",\"run_rows\":"
+                  << g.S->ilu->runRowsMax << ",\"dense_groups\":["
                   << g.S->ilu->denseL.ngroups << "," << g.S->ilu->denseU.ngroups << "]";
             }
             if (g.S->cpr && g.S->cpr->built)

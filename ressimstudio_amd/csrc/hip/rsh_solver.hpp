@@ -261,8 +261,10 @@ struct IluDev {
     bool tailCopy = false;
     DBuf<double> tail64;
     DBuf<float> tail32;
-    // level chunking: levels with at most fuseMax tasks are swept in runs (one workgroup or dense groups)
+    // level chunking: levels with at most fuseMax tasks are swept in runs (one workgroup or dense groups) of at most
+    // runRowsMax rows (the LDS tile of a run's working vector)
     int fuseMax = 16;
+    int runRowsMax = kRunRowsMax;
     // batched systems: original row -> system, and the systems' BiCGSTAB states (slices of stopped systems skipped)
     const std::vector<int>* rowSys = nullptr;
     const KScalars* gate = nullptr;
@@ -336,8 +338,8 @@ struct IluDev {
             }
             std::vector<int> rowsL(static_cast<size_t>(nl));
             for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
-            runsL = levelRuns(wL, fuseMax, &rowsL, kRunRowsMax);
-            runsU = levelRuns(wU, fuseMax, &rowsL, kRunRowsMax);
+            runsL = levelRuns(wL, fuseMax, &rowsL, runRowsMax);
+            runsU = levelRuns(wU, fuseMax, &rowsL, runRowsMax);
             denseL.build(runsL, EL, lstart, nb, false, s, rowSys);
             denseU.build(runsU, EU, lstart, nb, true, s, rowSys);
             buildFactorGroups(s);
@@ -440,11 +442,13 @@ struct IluDev {
                        denseU.runTask0[b] >= 0 && denseU.noOutside[b];
         }
     }
-    void retune(int newFuseMax, hipStream_t s) {
+    // newRunRows > 0: also the rows a fused run may hold (its working vector's LDS tile, at most kRunRowsMax)
+    void retune(int newFuseMax, hipStream_t s, int newRunRows = 0) {
         if (!keepEntries) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a schedule built with keepEntries");
         fuseMax = newFuseMax;
-        runsL = levelRuns(workL, fuseMax, &rowsPerLevel, kRunRowsMax);
-        runsU = levelRuns(workU, fuseMax, &rowsPerLevel, kRunRowsMax);
+        if (newRunRows > 0) runRowsMax = std::min(newRunRows, kRunRowsMax);
+        runsL = levelRuns(workL, fuseMax, &rowsPerLevel, runRowsMax);
+        runsU = levelRuns(workU, fuseMax, &rowsPerLevel, runRowsMax);
         denseL.build(runsL, keptL, lstart, nb, false, s, rowSys);
         denseU.build(runsU, keptU, lstart, nb, true, s, rowSys);
         buildFactorGroups(s);
