@@ -293,7 +293,38 @@ def serving_leg(rs, gen, args):
         st = srv.stats()
         srv.close()
         out[name] = {"rates": rows, "peak_running": st["peak_running"], "hbm_used_GB": st["hbm_used"] / 1e9, "evictions": st["evictions"]}
+    if not args.no_cpu:
+        out["cpu_baseline"] = serving_cpu_sample(models)
     return out
+
+
+def serving_cpu_sample(models, count=8):
+    """The reference's OpenMP BlockCPR + BiCGSTAB (bsr.cpp:993-1084, linsolve.cpp:159-216) on this host's cores, on a
+    bounded sample of the served models (every 8th by size, so the sample spans 5k-500k cells): one request = the
+    model's set-up + solve to 1e-8, one request at a time with every thread on it (about 10-20 s of CPU work)."""
+    import oracle
+    ref = oracle.reference(fast=True)
+    kind = "reference"
+    if ref is None:
+        ref = oracle.oracle()
+        kind = "port"
+    threads = ref.set_threads(0)
+    order = sorted(range(len(models)), key=lambda i: models[i].n)
+    pick = order[len(order) // (2 * count)::max(1, len(order) // count)][:count]
+    ms, cells = [], []
+    for i in pick:
+        m = models[i]
+        t0 = time.perf_counter()
+        _, its, rel, conv = ref.krylov(m, m.rhs, "cpr", "bicgstab", 1e-8, 3000)
+        ms.append((time.perf_counter() - t0) * 1e3)
+        cells.append(int(m.n))
+    lat = np.array(ms)
+    log(f"serving cpu baseline: {len(pick)} requests, mean {lat.mean():.0f} ms, p50 {np.percentile(lat, 50):.0f} ms, max {lat.max():.0f} ms "
+        f"on {threads} threads ({kind})")
+    return {"requests": len(pick), "cells": cells, "latency_ms": [round(x, 1) for x in ms], "mean_ms": float(lat.mean()),
+            "p50_ms": float(np.percentile(lat, 50)), "max_ms": float(lat.max()), "throughput_req_s": float(1e3 / lat.mean()),
+            "cores": int(threads), "kind": kind,
+            "sample": "every 8th served model by size, its base matrix: set-up + CPR-BiCGSTAB to 1e-8 one request at a time"}
 
 
 def main():
