@@ -233,6 +233,8 @@ def batched_leg(rs, gen, args):
     out["speedup_fp64_vs_loop"] = out["batched_fp64"]["solves_per_s"] / out["loop_fp64"]["solves_per_s"]
     out["speedup_cpr_fp64_vs_loop_cpr"] = out["batched_cpr_fp64"]["solves_per_s"] / out["loop_cpr_fp64"]["solves_per_s"]
     out["speedup_cpr_store32_vs_loop_cpr"] = out["batched_cpr_store32"]["solves_per_s"] / out["loop_cpr_fp64"]["solves_per_s"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_sample(systems, 16, "batched")
     for p in ptrs:
         L.rsh_device_free(ctx.h, p)
     return out
@@ -294,14 +296,15 @@ def serving_leg(rs, gen, args):
         srv.close()
         out[name] = {"rates": rows, "peak_running": st["peak_running"], "hbm_used_GB": st["hbm_used"] / 1e9, "evictions": st["evictions"]}
     if not args.no_cpu:
-        out["cpu_baseline"] = serving_cpu_sample(models)
+        out["cpu_baseline"] = cpu_sample(models, 8, "serving")
     return out
 
 
-def serving_cpu_sample(models, count=8):
+def cpu_sample(models, count, what):
     """The reference's OpenMP BlockCPR + BiCGSTAB (bsr.cpp:993-1084, linsolve.cpp:159-216) on this host's cores, on a
-    bounded sample of the served models (every 8th by size, so the sample spans 5k-500k cells): one request = the
-    model's set-up + solve to 1e-8, one request at a time with every thread on it (about 10-20 s of CPU work)."""
+    bounded, size-stratified sample of the systems of a leg (count of them, evenly spaced by size): one solve = the
+    system's set-up + solve to 1e-8, one at a time with every thread on it (seconds of CPU work for the batched
+    systems, 10-20 s for the served models)."""
     import oracle
     ref = oracle.reference(fast=True)
     kind = "reference"
@@ -319,12 +322,13 @@ def serving_cpu_sample(models, count=8):
         ms.append((time.perf_counter() - t0) * 1e3)
         cells.append(int(m.n))
     lat = np.array(ms)
-    log(f"serving cpu baseline: {len(pick)} requests, mean {lat.mean():.0f} ms, p50 {np.percentile(lat, 50):.0f} ms, max {lat.max():.0f} ms "
+    log(f"{what} cpu baseline: {len(pick)} solves, mean {lat.mean():.1f} ms, p50 {np.percentile(lat, 50):.1f} ms, max {lat.max():.1f} ms "
         f"on {threads} threads ({kind})")
-    return {"requests": len(pick), "cells": cells, "latency_ms": [round(x, 1) for x in ms], "mean_ms": float(lat.mean()),
-            "p50_ms": float(np.percentile(lat, 50)), "max_ms": float(lat.max()), "throughput_req_s": float(1e3 / lat.mean()),
+    return {"solves": len(pick), "cells": cells, "latency_ms": [round(x, 2) for x in ms], "mean_ms": float(lat.mean()),
+            "p50_ms": float(np.percentile(lat, 50)), "max_ms": float(lat.max()), "solves_per_s": float(1e3 / lat.mean()),
             "cores": int(threads), "kind": kind,
-            "sample": "every 8th served model by size, its base matrix: set-up + CPR-BiCGSTAB to 1e-8 one request at a time"}
+            "sample": f"{len(pick)} of the {len(models)} systems, evenly spaced by size, base values: set-up + CPR-BiCGSTAB to 1e-8, "
+                      f"one at a time"}
 
 
 def main():
