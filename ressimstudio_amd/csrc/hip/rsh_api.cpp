@@ -43,6 +43,7 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     its = 0;
     rel = 0;
     if (c.hks->bn == 0) { S.lastStop = "zero rhs"; return RSH_OK; }
+    if (!std::isfinite(c.hks->bn)) { rel = c.hks->bn; S.lastStop = "non-finite right-hand side"; return RSH_ERR_NONFINITE; }
     // The stopping tests run on the device (the epilogues freeze the state), so the host reads the scalars only
     // every few iterations while the residual is far from the tolerance; iterations launched after a stop change
     // nothing.
@@ -102,6 +103,7 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     its = 0;
     rel = 0;
     if (bn == 0) { launchFill(st, x, 0.0, N); return RSH_OK; }
+    if (!std::isfinite(bn)) { rel = bn; return RSH_ERR_NONFINITE; }
     std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), g(static_cast<size_t>(m) + 1), y(static_cast<size_t>(m));
     auto h = [&](int i, int j) -> double& { return H[size_t(i) * m + j]; };
     int total = 0;
@@ -193,6 +195,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     outer = 0;
     rel = 0;
     if (bn == 0) return RSH_OK;
+    if (!std::isfinite(bn)) { rel = bn; return RSH_ERR_NONFINITE; }
     const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-4;
     const int maxOuter = S.opt.max_outer > 0 ? S.opt.max_outer : 10;
     double rn = bn;

@@ -258,6 +258,31 @@ def test_zero_rhs_and_errors(sysd):
         rs.System(rs.BSR(2, 3, np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.zeros(18)))
 
 
+@pytest.mark.parametrize("precond,precision", [("bilu0", rs.FP64), ("cpr_mc", rs.FP64), ("cpr_mc", rs.MIXED), ("cpr_mc", rs.MIXED_IR)])
+def test_failures_are_reported_not_hidden(sysd, precond, precision):
+    # the boundary's error behaviour (GpuBlockSolver::solve returns false, and the engine re-solves on its CPU path,
+    # simulator.cpp:3366, 3376-3377): a non-finite right-hand side gives RSH_ERR_NONFINITE, too few iterations
+    # RSH_ERR_NOT_CONVERGED; the Python hook mirror returns ok = False for both; the system keeps working afterwards
+    s = sysd["small"]
+    d = rs.System(s).setup(precond=precond, precision=precision, smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+    bad = s.rhs.copy()
+    bad[5] = np.nan
+    with pytest.raises(rs.RshError) as e:
+        d.solve(bad, TOL, 200)
+    assert e.value.code == rs.ERR_NONFINITE
+    x, its, rel, ok = d.solve(s.rhs, 1e-14, 2, raise_on_fail=False)
+    assert not ok and its <= 2
+    x, its, rel, ok = d.solve(s.rhs, TOL, 500)
+    assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    hook = rs.HipBlockSolver(precond)
+    ok, x, its = hook.solve(s, bad, TOL, 200)
+    assert not ok
+    ok, x, its = hook.solve(s, s.rhs, 1e-14, 2)
+    assert not ok
+    ok, x, its = hook.solve(s, s.rhs, TOL, 500)
+    assert ok
+
+
 def test_hook_adapter(sysd, orc):
     s = sysd["small"]
     hook = rs.HipBlockSolver("cpr")
