@@ -360,7 +360,7 @@ class Batch:
         st = np.zeros(m, np.int32)
         rc = lib().rsh_batch_solve(self.h, _ptrs(bs), _ptrs(xs), _p(tols, ctypes.c_double), max_it, _p(its, ctypes.c_int), _p(rel, ctypes.c_double),
                                    _p(st, ctypes.c_int))
-        if rc < 0 and (raise_on_fail or rc not in (ERR_NOT_CONVERGED, ERR_BREAKDOWN)):
+        if rc < 0 and (raise_on_fail or rc not in (ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_NONFINITE)):
             _check(rc, "rsh_batch_solve")
         return xs, its, rel, st
 

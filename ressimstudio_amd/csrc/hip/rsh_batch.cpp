@@ -222,7 +222,8 @@ struct rsh_batch {
                 lastIts[size_t(i)] = k.it;
                 const bool onS = k.state == kConvS || k.state == kDoneS;
                 lastRel[size_t(i)] = k.bn > 0 ? (onS ? k.sn : k.rn) / k.bn : 0.0;
-                if (k.state == kConv || onS) lastStatus[size_t(i)] = RSH_OK;
+                if (!std::isfinite(k.bn) || !std::isfinite(k.rn)) lastStatus[size_t(i)] = RSH_ERR_NONFINITE;
+                else if (k.state == kConv || onS) lastStatus[size_t(i)] = RSH_OK;
                 else if (k.state == kBreak) lastStatus[size_t(i)] = RSH_ERR_BREAKDOWN;
                 else lastStatus[size_t(i)] = RSH_ERR_NOT_CONVERGED;
             }
@@ -293,7 +294,8 @@ struct rsh_batch {
         for (int i = 0; i < ns; ++i) {
             if (lastDoneMs[size_t(i)] == 0.0) lastDoneMs[size_t(i)] = now;
             lastRel[size_t(i)] = bn[size_t(i)] > 0 ? rn[size_t(i)] / bn[size_t(i)] : 0.0;
-            if (lastRel[size_t(i)] <= tolV[size_t(i)]) lastStatus[size_t(i)] = RSH_OK;
+            if (!std::isfinite(bn[size_t(i)]) || !std::isfinite(rn[size_t(i)])) lastStatus[size_t(i)] = RSH_ERR_NONFINITE;
+            else if (lastRel[size_t(i)] <= tolV[size_t(i)]) lastStatus[size_t(i)] = RSH_OK;
             else if (lastStatus[size_t(i)] == RSH_OK) lastStatus[size_t(i)] = RSH_ERR_NOT_CONVERGED;
         }
         launchCopy(st, x.p, xo.p, N);

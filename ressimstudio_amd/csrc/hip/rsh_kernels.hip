@@ -1436,8 +1436,13 @@ __global__ __launch_bounds__(kBlock) void kDenseGemv(int m, int ld, const double
     const int rowi = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (rowi >= m) return;
+    // exact zeros of the inverse are skipped: a batch's coarsest matrix is block-diagonal over its systems, and a
+    // non-finite value of one system must not reach the others through 0 * x
     double s = 0;
-    for (int j = lane; j < m; j += kWave) s += M[(long long)rowi * ld + j] * x[j];
+    for (int j = lane; j < m; j += kWave) {
+        const double a = M[(long long)rowi * ld + j];
+        if (a != 0.0) s += a * x[j];
+    }
     s = waveSum(s);                            // every lane holds the sum
     if (lane == 0) y[rowi] = s;
     if (pfine)

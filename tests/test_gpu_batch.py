@@ -104,6 +104,25 @@ def test_batch_systems_do_not_interact(systems, precond):
         assert np.array_equal(x1[k], x2[k]), k
 
 
+@pytest.mark.parametrize("precond", ["mcbilu0", "cpr_mc"])
+def test_batch_nonfinite_system_stays_alone(systems, precond):
+    """A system with a non-finite right-hand side fails on its own: its status is not OK, and every other system's
+    solution, iteration count and residual are bit-identical to a batch where it had a finite one (with CPR the
+    coarsest inverse couples nothing across systems, its exact zeros are skipped)."""
+    B = rs.Batch(systems).setup(precond=precond, smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0)
+    bs = [s.rhs.copy() for s in systems]
+    x1, i1, r1, s1 = B.solve(bs, TOL, 300, raise_on_fail=False)
+    bs[2][3] = np.nan
+    x2, i2, r2, s2 = B.solve(bs, TOL, 300, raise_on_fail=False)
+    assert s2[2] == rs.ERR_NONFINITE
+    for k in range(len(systems)):
+        if k == 2:
+            continue
+        assert s1[k] == rs.OK and s2[k] == rs.OK, k
+        assert i1[k] == i2[k] and r1[k] == r2[k], k
+        assert np.array_equal(x1[k], x2[k]), k
+
+
 def test_batch_equals_single_system_path(systems):
     B = rs.Batch(systems).setup(precond="mcbilu0")
     xs, its, _, _ = B.solve([s.rhs for s in systems], TOL, 3000)
