@@ -11,6 +11,10 @@
 //   BiCGSTAB/GMRES  iteration counts and residuals equal the compiled reference; solution vs scipy spsolve
 //   BlockCPR / AMG  weights closed form (first row of D^-1); pressure matrix closed form; aggregates, level sizes and
 //                   V-cycle output equal the compiled reference; refresh equals a fresh setup
+//   lean BILU0      factors are float32 values and differ from rounding the FP64 factors; apply, BiCGSTAB iterations,
+//                   residuals and solutions equal the compiled reference (setLean) bit for bit
+//   true-IMPES      weights from accumulation blocks (the engine's own, dumped with the demo waterflood, and
+//                   ill-conditioned variants) and the fallback count equal the compiled reference bit for bit
 // Built with -ffp-contract=off (the reference's MinGW flags, build_mingw.sh:18) so products are not fused.
 #include "rs_check.h"
 
