@@ -64,10 +64,12 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
         syncScalars(S);
         if (c.hks->state != kRun) break;
         const double relNow = c.hks->rn / c.hks->bn;
+        if (!std::isfinite(relNow)) break;       // a non-finite residual (non-finite matrix values): stop here
         nextCheck = it + (relNow > 30.0 * tol ? kCheckEvery : 1);
     }
     const KScalars& k = *c.hks;
     its = k.it;
+    if (!std::isfinite(k.rn)) { rel = k.rn; S.lastStop = "non-finite residual"; return RSH_ERR_NONFINITE; }
     if (k.state == kConvS || k.state == kDoneS) { rel = k.sn / k.bn; S.lastStop = "converged (s)"; return RSH_OK; }
     rel = k.bn > 0 ? k.rn / k.bn : 0;
     if (k.state == kConv) { S.lastStop = "converged"; return RSH_OK; }

@@ -274,6 +274,15 @@ def test_failures_are_reported_not_hidden(sysd, precond, precision):
     assert not ok and its <= 2
     x, its, rel, ok = d.solve(s.rhs, TOL, 500)
     assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+    if precision == rs.FP64:
+        # a non-finite matrix value: reported as such, and the solve stops at its next residual check
+        v = s.val.copy()
+        v[7, 0, 0] = np.inf
+        d2 = rs.System(gen.BlockSystem(n=s.n, nb=s.nb, row_ptr=s.row_ptr, col=s.col, val=v, rhs=s.rhs))
+        d2.setup(precond=precond, smoother=rs.SMOOTH_JACOBI, smooth_steps=2)
+        with pytest.raises(rs.RshError) as e:
+            d2.solve(s.rhs, TOL, 200)
+        assert e.value.code in (rs.ERR_NONFINITE, rs.ERR_BREAKDOWN)
     hook = rs.HipBlockSolver(precond)
     ok, x, its = hook.solve(s, bad, TOL, 200)
     assert not ok
