@@ -214,7 +214,8 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
                            ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, precF32, opF32, false, inIts, inRel)
                            : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, precF32, opF32);
         its += inIts;
-        if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN) return rc;
+        // a diverged inner solve (non-finite residual) is a step without progress: the check below discards it
+        if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN && rc != RSH_ERR_NONFINITE) return rc;
         launchCopy(st, S.irX.p, x, N);
         launchAxpy(st, N, 1.0, S.irD.p, x);
         S.spmv(x, S.irR.p, false);
@@ -268,7 +269,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     S.lastRel = rel;
     if (iterations) *iterations = its;
     if (relRes) *relRes = rel;
-    if (!std::isfinite(rel)) return fail(RSH_ERR_NONFINITE, "non-finite residual");
+    if (!std::isfinite(rel) || rc == RSH_ERR_NONFINITE) return fail(RSH_ERR_NONFINITE, "non-finite residual");
     if (rc == RSH_ERR_NOT_CONVERGED) return fail(rc, "not converged: " + S.lastStop);
     if (rc == RSH_ERR_BREAKDOWN) return fail(rc, "Krylov breakdown");
     return rc;
