@@ -105,7 +105,7 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     its = 0;
     rel = 0;
     if (bn == 0) { launchFill(st, x, 0.0, N); return RSH_OK; }
-    if (!std::isfinite(bn)) { rel = bn; return RSH_ERR_NONFINITE; }
+    if (!std::isfinite(bn)) { rel = bn; S.lastStop = "non-finite right-hand side"; return RSH_ERR_NONFINITE; }
     std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), g(static_cast<size_t>(m) + 1), y(static_cast<size_t>(m));
     auto h = [&](int i, int j) -> double& { return H[size_t(i) * m + j]; };
     int total = 0;
@@ -197,7 +197,7 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     outer = 0;
     rel = 0;
     if (bn == 0) return RSH_OK;
-    if (!std::isfinite(bn)) { rel = bn; return RSH_ERR_NONFINITE; }
+    if (!std::isfinite(bn)) { rel = bn; S.lastStop = "non-finite right-hand side"; return RSH_ERR_NONFINITE; }
     const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-4;
     const int maxOuter = S.opt.max_outer > 0 ? S.opt.max_outer : 10;
     double rn = bn;
@@ -253,6 +253,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     int its = 0, outer = 0, rc;
     double rel = 0;
     S.lastFallback = false;
+    S.lastStop = "non-finite residual";
     // RSH_MIXED: FGMRES with the FP32 preconditioner (its restarts recompute the FP64 true residual), or BiCGSTAB
     // with the FP32 preconditioner restarted on the FP64 true residual (refinement with the FP64 operator), which
     // keeps it from stalling on the preconditioner's FP32 rounding; RSH_MIXED_IR: refinement around FP32-operator
@@ -269,7 +270,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     S.lastRel = rel;
     if (iterations) *iterations = its;
     if (relRes) *relRes = rel;
-    if (!std::isfinite(rel) || rc == RSH_ERR_NONFINITE) return fail(RSH_ERR_NONFINITE, "non-finite residual");
+    if (!std::isfinite(rel) || rc == RSH_ERR_NONFINITE) return fail(RSH_ERR_NONFINITE, "rsh_solve: " + S.lastStop);
     if (rc == RSH_ERR_NOT_CONVERGED) return fail(rc, "not converged: " + S.lastStop);
     if (rc == RSH_ERR_BREAKDOWN) return fail(rc, "Krylov breakdown");
     return rc;

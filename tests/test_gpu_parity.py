@@ -269,7 +269,7 @@ def test_failures_are_reported_not_hidden(sysd, precond, precision):
     bad[5] = np.nan
     with pytest.raises(rs.RshError) as e:
         d.solve(bad, TOL, 200)
-    assert e.value.code == rs.ERR_NONFINITE
+    assert e.value.code == rs.ERR_NONFINITE and "non-finite right-hand side" in str(e.value)
     x, its, rel, ok = d.solve(s.rhs, 1e-14, 2, raise_on_fail=False)
     assert not ok and its <= 2
     x, its, rel, ok = d.solve(s.rhs, TOL, 500)
