@@ -617,3 +617,32 @@ def test_jacobi_weight_option(sysd, orc, name):
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
         print(f"{name} weight {w or 0.6667}: {its} its")
     assert not np.array_equal(zs[0], zs[1])
+
+
+def test_wide_layers_use_32bit_columns(orc):
+    # 200 x 200 x 2 cells: the vertical neighbour is 40,000 block rows away, beyond the slice layout's 16-bit relative
+    # columns, so the SpMV, the sliced pressure smoother and the CPR second stage read 32-bit columns: the SpMV against
+    # the oracle's, the MC-BILU0 apply to 1e-12, and CPR-BiCGSTAB (FP64 and FP32-stored) against the FP64 oracle
+    # solution (<= 1e-6) with the true relative residual <= 1e-8
+    s = gen.reservoir(200, 200, 2, 3, model=0, sigma=1.0, wells=2, seed=77)
+    d = rs.System(s)
+    info = d.info()
+    assert info["spmv_col16"] == 0, info
+    x = np.random.default_rng(3).standard_normal(s.n * s.nb)
+    y, yo = d.spmv(x), orc.spmv(s, x)
+    if info["spmv_long_rows"] == 0:
+        assert np.array_equal(y, yo)
+    assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
+    d.setup(precond="mcbilu0")
+    r = np.sin(0.37 * np.arange(s.n * s.nb)) + 0.5
+    z, zo = d.precond(r), orc.mcbilu0(s).apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+    xo, _, _, ok_o = orc.krylov(s, s.rhs, "cpr", "bicgstab", 1e-10, 3000)
+    assert ok_o
+    for precision in (rs.FP64, rs.MIXED_STORE):
+        dc = rs.System(s).setup(precond="cpr_mc", precision=precision, smoother=rs.SMOOTH_JACOBI, smooth_steps=2,
+                                jacobi_omega=1.0)
+        assert dc.info()["spmv_col16"] == 0
+        x, its, rel, ok = dc.solve(s.rhs, TOL, 1000)
+        assert ok and _true_rel(s, x, s.rhs) <= 1.01 * TOL
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
