@@ -255,29 +255,31 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
 
 // ---------------------------------------------------------------- BILU0 level sweeps
 // One lane's slots of a sweep: acc_e -= sum_f a_ef y_col,f per block in slot (column) order, s_e formed first as
-// in the block products of bsr.cpp:14-56.  Two slots per round: both slots' columns and blocks, then both
+// in the block products of bsr.cpp:14-56.  kSweepSlots slots per round: their columns and blocks, then their
 // gathers of y, are in flight before the first product; the arithmetic is the same slot by slot.
+constexpr int kSweepSlots = 3;
 template <int NB, typename TA, typename TV>
 __device__ inline void sliceRowSweep(const int* __restrict__ col, const TA* __restrict__ val, int2 inf, int lane, const TV* y, TV (&acc)[NB]) {
 #pragma clang fp contract(off)
+    constexpr int U = kSweepSlots;
     int k = 0;
-    for (; k + 2 <= inf.x; k += 2) {
+    for (; k + U <= inf.x; k += U) {
         const long long slot = (long long)inf.y + k;
-        TV av[2][NB * NB], yk[2][NB];
-        int c[2];
+        TV av[U][NB * NB], yk[U][NB];
+        int c[U];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < U; ++u) {
             c[u] = __builtin_nontemporal_load(col + (slot + u) * kWave + lane);
             const TA* a = val + (slot + u) * (NB * NB * kWave) + lane;
 #pragma unroll
             for (int ef = 0; ef < NB * NB; ++ef) av[u][ef] = TV(__builtin_nontemporal_load(a + ef * kWave));
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int f = 0; f < NB; ++f) yk[u][f] = y[(long long)c[u] * NB + f];
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
                 TV s = 0;
