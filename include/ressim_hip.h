@@ -78,7 +78,8 @@ typedef struct rsh_options {
     int krylov;         /* RSH_KRYLOV_* (default BiCGSTAB; with RSH_MIXED_IR the method of the inner solves) */
     int precision;      /* RSH_FP64, RSH_MIXED, RSH_MIXED_IR or RSH_MIXED_STORE */
     int smoother;       /* RSH_SMOOTH_* */
-    int restart;        /* FGMRES restart length (default 30) */
+    int restart;        /* FGMRES restart length (default 30; the engine's fallback GMRES uses 60,
+                           simulator.cpp:3395); 1..256, larger values are refused with RSH_ERR_UNSUPPORTED */
     int max_outer;      /* refinement steps of RSH_MIXED (BiCGSTAB) and RSH_MIXED_IR (default 10) */
     double inner_tol;   /* their inner solves' relative tolerance (default 1e-4, refineWithDevice's, bsr.cpp:1383);
                            each inner solve runs at most max_it / max_outer (>= 50) iterations */
@@ -192,8 +193,10 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap);
  * study.cpp:326-364, 1254, call the global hook from many Simulator threads, one Newton iteration per call).
  * Models (one pattern each) stay resident with their slice layouts, ILU schedule and AMG hierarchy; requests queue
  * in arrival order and are admitted as soon as a worker slot (one HIP stream each) is free, the model is idle and
- * the model fits the HBM budget (idle models are evicted least-recently-used).  Pointers in a request must stay
- * valid until rsh_server_wait returns for it. */
+ * the model fits the HBM budget (idle models are evicted least-recently-used, and only when that makes room).  A
+ * model larger than the whole budget is served alone once nothing else is resident.  Submit and wait may be called
+ * from many threads at once; a failed request's message is rsh_last_error() of the thread that waits for it.
+ * Pointers in a request must stay valid until rsh_server_wait returns for it. */
 typedef struct rsh_request {
     int model;
     const double* values;          /* nnzb*nb*nb new values, or NULL for the model's base values */

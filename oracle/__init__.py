@@ -82,6 +82,13 @@ class Checker:
             lib.chk_bilu0_factors.argtypes = [_vp, _dp]
         if hasattr(lib, "chk_set_threads"):
             lib.chk_set_threads.argtypes = [ctypes.c_int]
+        if hasattr(lib, "chk_wellschur_new"):
+            lib.chk_wellschur_new.restype = _vp
+            lib.chk_wellschur_new.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip]
+            lib.chk_wellschur_reduced.argtypes = [_vp, _ip, _ip, _dp]
+            lib.chk_wellschur_reduce_rhs.argtypes = [_vp, _dp, _dp]
+            lib.chk_wellschur_recover.argtypes = [_vp, _dp, _dp, _dp]
+            lib.chk_wellschur_free.argtypes = [_vp]
         self.lib = lib
 
     # --- kernels
@@ -118,6 +125,13 @@ class Checker:
     def cpr(self, s, acc=None):
         """BlockCPR (bsr.cpp:993-1084); acc: n accumulation blocks for true-IMPES weights"""
         return _Cpr(self, s, acc)
+
+    def wellschur(self, row_ptr, col, val, na, nv, nw):
+        """WellSchur (bsr.cpp:733-990) of a scalar Jacobian (compiled reference only); None when it refuses."""
+        if not hasattr(self.lib, "chk_wellschur_new"):
+            raise NotImplementedError("this checker has no well Schur reduction")
+        w = _WellSchur(self, row_ptr, col, val, na, nv, nw)
+        return w if w.h else None
 
     def devcpr(self, s):
         """DeviceCprAmg (bsr.cpp:1235-1371): float32 CPR with damped-Jacobi AMG and block-Jacobi second stage."""
@@ -197,6 +211,41 @@ class _DevCpr(_Handle):
         z = np.empty(self.N)
         self.c.lib.chk_devcpr_apply(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double))
         return z
+
+
+class _WellSchur(_Handle):
+    _free = "chk_wellschur_free"
+
+    def __init__(self, c, row_ptr, col, val, na, nv, nw):
+        self.c = c
+        self._rp = np.ascontiguousarray(row_ptr, np.int32)
+        self._ci = np.ascontiguousarray(col, np.int32)
+        self._v = np.ascontiguousarray(val, np.float64)
+        self.n_full, self.nc, self.nv = len(self._rp) - 1, na * nv, nv
+        sizes = np.zeros(3, np.int32)
+        self.h = c.lib.chk_wellschur_new(self.n_full, _p(self._rp, ctypes.c_int), _p(self._ci, ctypes.c_int), _p(self._v, ctypes.c_double),
+                                         na, nv, nw, _p(sizes, ctypes.c_int))
+        self.n, self.nnzb, self.fill = int(sizes[0]), int(sizes[1]), int(sizes[2])
+
+    def reduced(self):
+        rp = np.empty(self.n + 1, np.int32)
+        ci = np.empty(self.nnzb, np.int32)
+        v = np.empty(self.nnzb * self.nv * self.nv)
+        self.c.lib.chk_wellschur_reduced(self.h, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double))
+        return rp, ci, v
+
+    def reduce_rhs(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        bc = np.empty(self.nc)
+        self.c.lib.chk_wellschur_reduce_rhs(self.h, _p(b, ctypes.c_double), _p(bc, ctypes.c_double))
+        return bc
+
+    def recover(self, b, xc):
+        b = np.ascontiguousarray(b, np.float64)
+        xc = np.ascontiguousarray(xc, np.float64)
+        x = np.empty(self.n_full)
+        self.c.lib.chk_wellschur_recover(self.h, _p(b, ctypes.c_double), _p(xc, ctypes.c_double), _p(x, ctypes.c_double))
+        return x
 
 
 class _Cpr(_Handle):

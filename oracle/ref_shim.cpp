@@ -165,6 +165,30 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
     return r.converged ? 1 : 0;
 }
 
+// WellSchur::build / reduceRhs / recover (bsr.cpp:733-990) on a scalar CSR Jacobian: returns a handle (null when the
+// reference refuses the system); sizes[0] = reduced block rows, sizes[1] = reduced blocks, sizes[2] = fill blocks
+void* chk_wellschur_new(int n, const int* rp, const int* ci, const double* v, int na, int nv, int nw, int* sizes) {
+    CSR J; J.n = n;
+    J.rowPtr.assign(rp, rp + n + 1);
+    J.col.assign(ci, ci + rp[n]);
+    J.val.assign(v, v + rp[n]);
+    J.finalizeDiag();
+    auto* w = new WellSchur;
+    if (!w->build(J, na, nv, nw)) { delete w; return nullptr; }
+    sizes[0] = w->reduced().n; sizes[1] = int(w->reduced().col.size()); sizes[2] = w->fillBlocks();
+    return w;
+}
+int chk_wellschur_reduced(void* h, int* rp, int* ci, double* v) {
+    const BSR& S = static_cast<WellSchur*>(h)->reduced();
+    for (size_t i = 0; i < S.rowPtr.size(); ++i) rp[i] = S.rowPtr[i];
+    for (size_t i = 0; i < S.col.size(); ++i) ci[i] = S.col[i];
+    for (size_t i = 0; i < S.val.size(); ++i) v[i] = S.val[i];
+    return 0;
+}
+int chk_wellschur_reduce_rhs(void* h, const double* b, double* bc) { static_cast<WellSchur*>(h)->reduceRhs(b, bc); return 0; }
+int chk_wellschur_recover(void* h, const double* b, const double* xc, double* x) { static_cast<WellSchur*>(h)->recover(b, xc, x); return 0; }
+void chk_wellschur_free(void* h) { delete static_cast<WellSchur*>(h); }
+
 // number of OpenMP threads the reference will use (setNumThreads(0) = physical cores unless OMP_NUM_THREADS)
 int chk_set_threads(int n) { setNumThreads(n); return numThreads(); }
 

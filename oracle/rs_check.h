@@ -52,6 +52,14 @@ void* chk_devcpr_new(int n, int nb, const int* rp, const int* ci, const double* 
 int chk_devcpr_apply(void* h, const double* r, double* z);
 void chk_devcpr_free(void* h);
 
+/* reference only: the engine's well Schur reduction of a scalar Jacobian (WellSchur, bsr.cpp:733-990); null when
+ * refused.  sizes = {reduced block rows, reduced blocks, fill blocks} */
+void* chk_wellschur_new(int n, const int* rp, const int* ci, const double* v, int na, int nv, int nw, int* sizes);
+int chk_wellschur_reduced(void* h, int* rp, int* ci, double* v);
+int chk_wellschur_reduce_rhs(void* h, const double* b, double* bc);
+int chk_wellschur_recover(void* h, const double* b, const double* xc, double* x);
+void chk_wellschur_free(void* h);
+
 int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond,
                int method, int restart, double tol, int maxit, int* its, double* relres);
 

@@ -3,9 +3,9 @@
 // This is the workload side of the linear-solve path: it assembles the scalar CSR Jacobian a fully-implicit
 // black-oil engine hands to its linear solver (cells in blocks of nv unknowns, pressure first, followed by one
 // BHP unknown per well), then eliminates the BHP unknowns exactly (S = Acc - Acw Dww^-1 Awc) to obtain the
-// block (BSR) system the solver sees.  The reduction follows ressim::WellSchur::build / reduceRhs / recover
-// (engine/src/bsr.cpp:733-840, 973-990); the algebra is the definition of the Schur complement, with the same
-// order of operations per entry so that the reduced system matches the engine's bit for bit.
+// block (BSR) system the solver sees, as the engine does before it calls its solver (ressim::WellSchur,
+// engine/src/bsr.cpp:733-990).  The reduction is the rank-1-per-well plan of csrc/host/well_plan.hpp, whose per-entry
+// order of operations makes the reduced system match the engine's bit for bit.
 //
 // All random numbers come from a counter-based SplitMix64 hash (seed, stream, index), so every value is
 // independent of generation order and identical on every platform.  Two reference-test generators that use
@@ -18,6 +18,8 @@
 #include <random>
 #include <string>
 #include <vector>
+
+#include "../host/well_plan.hpp"
 
 namespace {
 
@@ -51,123 +53,43 @@ void finalizeDiag(Bsr& S) {
 }
 
 // ------------------------------------------------------------------ well Schur complement (engine side)
-struct WellSchurData {
-    int na = 0, nv = 1, nw = 0, fill = 0;
+// The reduction itself (pattern, values, right-hand side, recovery) is csrc/host/well_plan.hpp: one rank-1 update
+// per well, in the engine's per-entry order.  Here it is applied to host values, with the engine's value rule for
+// the pattern: a cell -> well entry that is exactly zero couples nothing (bsr.cpp:755-757).
+struct WellReduction {
+    wellplan::Plan plan;
     std::vector<double> dInv;
-    std::vector<int> wcPtr, wcCol; std::vector<double> wcVal;    // well rows: cell columns
-    std::vector<int> cwPtr, cwWell; std::vector<double> cwVal;   // cell rows: well columns
+    std::vector<double> jval;      // J's values (the right-hand-side reduction and recovery read them)
+    std::vector<int> jcol;
 };
 
-// J: scalar CSR (sorted columns); rows 0..na*nv-1 cells, na*nv.. one BHP row per well
-bool wellSchurBuild(const Csr& J, int na, int nv, int nw, Bsr& S, WellSchurData& W) {
-    W.na = na; W.nv = nv; W.nw = nw;
-    const int nc = na * nv;
-    W.dInv.assign(size_t(nw), 0.0);
-    W.wcPtr.assign(size_t(nw) + 1, 0); W.wcCol.clear(); W.wcVal.clear();
-    for (int w = 0; w < nw; ++w) {
-        const int r = nc + w;
-        double d = 0, rowMax = 0;
-        for (int p = J.rowPtr[size_t(r)]; p < J.rowPtr[size_t(r) + 1]; ++p) {
-            const int c = J.col[size_t(p)];
-            const double v = J.val[size_t(p)];
-            if (c == r) d = v;
-            else if (c < nc) { W.wcCol.push_back(c); W.wcVal.push_back(v); rowMax = std::max(rowMax, std::fabs(v)); }
-            else if (v != 0.0) { g_err = "well-well coupling is not supported by the well Schur reduction"; return false; }
-        }
-        if (!(std::fabs(d) > 1e-12 * std::max(rowMax, 1e-300)) || !std::isfinite(d)) { g_err = "well row without usable BHP derivative"; return false; }
-        W.dInv[size_t(w)] = 1.0 / d;
-        W.wcPtr[size_t(w) + 1] = int(W.wcCol.size());
-    }
-    W.cwPtr.assign(size_t(nc) + 1, 0); W.cwWell.clear(); W.cwVal.clear();
-    for (int r = 0; r < nc; ++r) {
-        for (int p = J.rowPtr[size_t(r)]; p < J.rowPtr[size_t(r) + 1]; ++p)
-            if (J.col[size_t(p)] >= nc && J.val[size_t(p)] != 0.0) { W.cwWell.push_back(J.col[size_t(p)] - nc); W.cwVal.push_back(J.val[size_t(p)]); }
-        W.cwPtr[size_t(r) + 1] = int(W.cwWell.size());
-    }
-    // cells of each well (unique, ascending)
-    std::vector<std::vector<int>> wellCells(static_cast<size_t>(nw));
-    for (int w = 0; w < nw; ++w) {
-        auto& v = wellCells[size_t(w)];
-        for (int p = W.wcPtr[size_t(w)]; p < W.wcPtr[size_t(w) + 1]; ++p) v.push_back(W.wcCol[size_t(p)] / nv);
-        std::sort(v.begin(), v.end()); v.erase(std::unique(v.begin(), v.end()), v.end());
-    }
-    // block pattern: the cell pattern plus all cells of every well the cell is coupled to
-    S = Bsr{}; S.n = na; S.nb = nv; S.rowPtr.assign(size_t(na) + 1, 0);
-    std::vector<int> seen(size_t(na), -1), cols, wl;
-    int base = 0;
-    for (int i = 0; i < na; ++i) {
-        cols.clear(); wl.clear();
-        for (int e = 0; e < nv; ++e) {
-            const int r = i * nv + e;
-            for (int p = J.rowPtr[size_t(r)]; p < J.rowPtr[size_t(r) + 1]; ++p) {
-                const int c = J.col[size_t(p)];
-                if (c < nc && seen[size_t(c / nv)] != i) { seen[size_t(c / nv)] = i; cols.push_back(c / nv); }
-            }
-            for (int p = W.cwPtr[size_t(r)]; p < W.cwPtr[size_t(r) + 1]; ++p) wl.push_back(W.cwWell[size_t(p)]);
-        }
-        if (seen[size_t(i)] != i) { seen[size_t(i)] = i; cols.push_back(i); }
-        base += int(cols.size());
-        std::sort(wl.begin(), wl.end()); wl.erase(std::unique(wl.begin(), wl.end()), wl.end());
-        for (int w : wl) for (int j : wellCells[size_t(w)]) if (seen[size_t(j)] != i) { seen[size_t(j)] = i; cols.push_back(j); }
-        std::sort(cols.begin(), cols.end());
-        S.col.insert(S.col.end(), cols.begin(), cols.end());
-        S.rowPtr[size_t(i) + 1] = int(S.col.size());
-    }
-    W.fill = int(S.col.size()) - base;
-    const size_t b2 = size_t(nv) * nv;
-    S.val.assign(S.col.size() * b2, 0.0);
-    // values: S_ij = A_ij - sum_w A_iw Dww^-1 A_wj, accumulated per scalar row in CSR order
-    #pragma omp parallel
-    {
-        std::vector<int> pos(size_t(na), -1);
-        #pragma omp for schedule(dynamic, 256)
-        for (int i = 0; i < na; ++i) {
-            for (int p = S.rowPtr[size_t(i)]; p < S.rowPtr[size_t(i) + 1]; ++p) pos[size_t(S.col[size_t(p)])] = p;
-            for (int e = 0; e < nv; ++e) {
-                const int r = i * nv + e;
-                for (int p = J.rowPtr[size_t(r)]; p < J.rowPtr[size_t(r) + 1]; ++p) {
-                    const int c = J.col[size_t(p)];
-                    if (c < nc) S.val[size_t(pos[size_t(c / nv)]) * b2 + size_t(e) * nv + size_t(c % nv)] += J.val[size_t(p)];
-                }
-                for (int p = W.cwPtr[size_t(r)]; p < W.cwPtr[size_t(r) + 1]; ++p) {
-                    const int w = W.cwWell[size_t(p)];
-                    const double f = W.cwVal[size_t(p)] * W.dInv[size_t(w)];
-                    for (int q = W.wcPtr[size_t(w)]; q < W.wcPtr[size_t(w) + 1]; ++q) {
-                        const int c = W.wcCol[size_t(q)];
-                        S.val[size_t(pos[size_t(c / nv)]) * b2 + size_t(e) * nv + size_t(c % nv)] -= f * W.wcVal[size_t(q)];
-                    }
-                }
-            }
-        }
-    }
+bool reduceWells(const Csr& J, int na, int nv, int nw, Bsr& S, WellReduction& W) {
+    const double* v = J.val.data();
+    std::string why = wellplan::split(J.n, J.rowPtr.data(), J.col.data(), na, nv, nw, [v](int p) { return v[p] != 0.0; }, W.plan);
+    if (why.empty()) why = wellplan::wellInverses(W.plan, v, W.dInv);
+    if (!why.empty()) { g_err = why; return false; }
+    wellplan::pattern(J.n, J.rowPtr.data(), J.col.data(), W.plan);
+    S = Bsr{};
+    S.n = na; S.nb = nv;
+    S.rowPtr = W.plan.sPtr;
+    S.col = W.plan.sCol;
+    S.val.assign(S.col.size() * size_t(nv) * nv, 0.0);
+    wellplan::reduceValues(J.rowPtr.data(), J.col.data(), v, W.plan, W.dInv, S.val.data());
     finalizeDiag(S);
+    W.jval = J.val;
+    W.jcol = J.col;
     return true;
 }
 
-void wellSchurReduceRhs(const WellSchurData& W, const double* b, double* bc) {
-    const int nc = W.na * W.nv;
-    for (int r = 0; r < nc; ++r) {
-        double s = b[r];
-        for (int p = W.cwPtr[size_t(r)]; p < W.cwPtr[size_t(r) + 1]; ++p) s -= W.cwVal[size_t(p)] * W.dInv[size_t(W.cwWell[size_t(p)])] * b[nc + W.cwWell[size_t(p)]];
-        bc[r] = s;
-    }
+void reduceWellRhs(const WellReduction& W, const double* b, double* bc) { wellplan::reduceRhs(W.plan, W.jval.data(), W.dInv, b, bc); }
+void recoverWells(const WellReduction& W, const double* b, const double* xc, double* x) {
+    wellplan::recover(W.plan, W.jcol.data(), W.jval.data(), W.dInv, b, xc, x);
 }
 
-void wellSchurRecover(const WellSchurData& W, const double* b, const double* xc, double* x) {
-    const int nc = W.na * W.nv;
-    if (x != xc) std::copy(xc, xc + nc, x);
-    for (int w = 0; w < W.nw; ++w) {
-        double s = b[nc + w];
-        for (int p = W.wcPtr[size_t(w)]; p < W.wcPtr[size_t(w) + 1]; ++p) s -= W.wcVal[size_t(p)] * xc[W.wcCol[size_t(p)]];
-        x[nc + w] = s * W.dInv[size_t(w)];
-    }
-}
-
-// ------------------------------------------------------------------ generated system
 struct System {
     Bsr S; std::vector<double> bc;          // reduced block system and right-hand side
     Csr J; std::vector<double> b;           // full scalar system (cells + BHP rows)
-    WellSchurData W; bool hasFull = false;
+    WellReduction W; bool hasFull = false;
     std::vector<double> perm;               // cell permeability (mD), for diagnostics
 };
 
@@ -340,9 +262,9 @@ System* makeReservoir(int nx, int ny, int nz, int nb, int model, double sigma, d
     sys->b.resize(size_t(n));
     for (int r = 0; r < n; ++r) sys->b[size_t(r)] = 2.0 * u01(seed, kRhs, uint64_t(r)) - 1.0;
     sys->hasFull = true;
-    if (!wellSchurBuild(J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
+    if (!reduceWells(J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
     sys->bc.resize(size_t(nc));
-    wellSchurReduceRhs(sys->W, sys->b.data(), sys->bc.data());
+    reduceWellRhs(sys->W, sys->b.data(), sys->bc.data());
     return sys;
 }
 
@@ -418,9 +340,9 @@ System* makeChain() {
     sys->b.assign(size_t(n), 0.0);
     for (int r = 0; r < n; ++r) { double s = 0; for (int p = J.rowPtr[size_t(r)]; p < J.rowPtr[size_t(r) + 1]; ++p) s += J.val[size_t(p)] * xt[size_t(J.col[size_t(p)])]; sys->b[size_t(r)] = s; }
     sys->hasFull = true;
-    if (!wellSchurBuild(J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
+    if (!reduceWells(J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
     sys->bc.resize(size_t(nc));
-    wellSchurReduceRhs(sys->W, sys->b.data(), sys->bc.data());
+    reduceWellRhs(sys->W, sys->b.data(), sys->bc.data());
     return sys;
 }
 
@@ -445,7 +367,7 @@ int rsg_dims(void* h, long long* out) {
     const System* s = static_cast<const System*>(h);
     out[0] = s->S.n; out[1] = s->S.nb; out[2] = s->S.rowPtr.empty() ? 0 : s->S.rowPtr[size_t(s->S.n)];
     out[3] = s->hasFull; out[4] = s->J.n; out[5] = s->J.rowPtr.empty() ? 0 : s->J.rowPtr[size_t(s->J.n)];
-    out[6] = s->W.na; out[7] = s->W.nv; out[8] = s->W.nw; out[9] = s->W.fill;
+    out[6] = s->W.plan.na; out[7] = s->W.plan.nv; out[8] = s->W.plan.nw; out[9] = s->W.plan.fill();
     return 0;
 }
 int rsg_get_bsr(void* h, int* rowPtr, int* col, double* val, double* rhs) {
@@ -473,13 +395,13 @@ int rsg_get_perm(void* h, double* k) {
 int rsg_reduce_rhs(void* h, const double* b, double* bc) {
     const System* s = static_cast<const System*>(h);
     if (!s->hasFull) { g_err = "no well Schur data"; return -1; }
-    wellSchurReduceRhs(s->W, b, bc);
+    reduceWellRhs(s->W, b, bc);
     return 0;
 }
 int rsg_recover(void* h, const double* b, const double* xc, double* x) {
     const System* s = static_cast<const System*>(h);
     if (!s->hasFull) { g_err = "no well Schur data"; return -1; }
-    wellSchurRecover(s->W, b, xc, x);
+    recoverWells(s->W, b, xc, x);
     return 0;
 }
 // Reduce an arbitrary scalar CSR (cells + BHP rows) into a handle (no generation): the engine-side step above the
@@ -491,9 +413,9 @@ void* rsg_from_csr(int n, const int* rowPtr, const int* col, const double* val, 
     sys->J.col.assign(col, col + rowPtr[n]); sys->J.val.assign(val, val + rowPtr[n]);
     sys->b.assign(b, b + n);
     sys->hasFull = true;
-    if (!wellSchurBuild(sys->J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
+    if (!reduceWells(sys->J, na, nv, nw, sys->S, sys->W)) { delete sys; return nullptr; }
     sys->bc.resize(size_t(na) * nv);
-    wellSchurReduceRhs(sys->W, sys->b.data(), sys->bc.data());
+    reduceWellRhs(sys->W, sys->b.data(), sys->bc.data());
     return sys;
 }
 

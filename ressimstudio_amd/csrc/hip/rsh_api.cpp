@@ -87,7 +87,7 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     hipStream_t st = c.stream;
     const long long N = S.N;
     if (m < 1) m = 1;
-    if (m > 31) m = 31;
+    if (m > kMaxRestart) throw Error(RSH_ERR_UNSUPPORTED, "FGMRES restart above " + std::to_string(kMaxRestart));
     if (S.restartAlloc < m) {
         S.V.alloc(size_t(m + 1) * size_t(N));
         S.Z.alloc(size_t(m) * size_t(N));
@@ -95,8 +95,10 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     }
     S.ensureVec(S.r);
     S.ensureVec(S.w);
-    S.hcol.alloc(64);
-    S.yv.alloc(64);
+    // projections of one Arnoldi step: first CGS pass [0, m], second pass [m+1, 2m+1], ||w||^2 at 2m+2
+    const int hOff2 = m + 1, hNorm = 2 * m + 2;
+    S.hcol.alloc(size_t(2 * m + 3));
+    S.yv.alloc(size_t(m));
     if (!warm) launchFill(st, x, 0.0, N);
     launchDots(st, c, N, 1, b, b, nullptr, nullptr, c.scal.p, kEpNone, nullptr);
     RSH_HIP(hipMemcpyAsync(c.hscal, c.scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -133,14 +135,14 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
             // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2
             launchMultiDot(st, c, N, j + 1, S.V.p, N, S.w.p, S.hcol.p);
             launchMultiAxpy(st, N, j + 1, S.V.p, N, S.hcol.p, S.w.p);
-            launchMultiDot(st, c, N, j + 1, S.V.p, N, S.w.p, S.hcol.p + 32);
-            launchMultiAxpy(st, N, j + 1, S.V.p, N, S.hcol.p + 32, S.w.p);
-            launchDots(st, c, N, 1, S.w.p, S.w.p, nullptr, nullptr, S.hcol.p + 63, kEpNone, nullptr);
-            std::vector<double> hh(64);
-            RSH_HIP(hipMemcpyAsync(hh.data(), S.hcol.p, 64 * sizeof(double), hipMemcpyDeviceToHost, st));
+            launchMultiDot(st, c, N, j + 1, S.V.p, N, S.w.p, S.hcol.p + hOff2);
+            launchMultiAxpy(st, N, j + 1, S.V.p, N, S.hcol.p + hOff2, S.w.p);
+            launchDots(st, c, N, 1, S.w.p, S.w.p, nullptr, nullptr, S.hcol.p + hNorm, kEpNone, nullptr);
+            std::vector<double> hh(size_t(2 * m + 3));
+            RSH_HIP(hipMemcpyAsync(hh.data(), S.hcol.p, hh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
             RSH_HIP(hipStreamSynchronize(st));
-            for (int i = 0; i <= j; ++i) h(i, j) = hh[size_t(i)] + hh[size_t(32 + i)];
-            const double hn = std::sqrt(std::max(hh[63], 0.0));
+            for (int i = 0; i <= j; ++i) h(i, j) = hh[size_t(i)] + hh[size_t(hOff2 + i)];
+            const double hn = std::sqrt(std::max(hh[size_t(hNorm)], 0.0));
             h(j + 1, j) = hn;
             c.hscal[2] = hn;
             RSH_HIP(hipMemcpyAsync(c.scal.p + 2, c.hscal + 2, sizeof(double), hipMemcpyHostToDevice, st));
@@ -499,6 +501,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         if (o.precond < RSH_PREC_BILU0 || o.precond > RSH_PREC_CPR_BJ) return fail(RSH_ERR_ARG, "unknown preconditioner");
         if (o.precision != RSH_FP64 && o.precision != RSH_MIXED && o.precision != RSH_MIXED_IR && o.precision != RSH_MIXED_STORE)
             return fail(RSH_ERR_ARG, "unknown precision");
+        if (o.restart > kMaxRestart) return fail(RSH_ERR_UNSUPPORTED, "FGMRES restart above " + std::to_string(kMaxRestart) + " is not built");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
         rsh_system& S = *sys;
         hipStream_t st = S.s();

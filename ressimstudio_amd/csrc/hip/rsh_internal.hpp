@@ -16,6 +16,7 @@ constexpr int kWave = 64;             // CDNA wavefront
 constexpr int kSliceRows = 64;        // rows per slice: one row per lane of a wavefront
 constexpr int kRedBlocks = 1024;      // partial sums of the deterministic two-pass reductions
 constexpr int kMaxNb = 4;             // block sizes built: 1..4
+constexpr int kMaxRestart = 256;      // FGMRES restart lengths accepted (2m+1 vectors of n*nb doubles)
 
 struct Error : std::runtime_error {
     int code;
@@ -207,7 +208,8 @@ void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRo
                        const double* lu, double* raw, long long rawElems, double* inv);
 void launchToFloat(hipStream_t s, long long n, const double* a, float* b);
 // well Schur reduction of a scalar Jacobian on the device (see rsh_wells.cpp for the maps)
-void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const double* jval, double* dinv, int* bad);
+void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const int* wwPtr, const int* wwSrc,
+                    const double* jval, double* dinv, int* bad);
 void launchSchurValues(hipStream_t s, long long nvals, const int* direct, const int* termPtr, const int4* terms, const double* jval,
                        const double* dinv, double* sval);
 void launchSchurRhs(hipStream_t s, int nc, const int* cwPtr, const int* cwWell, const int* cwSrc, const double* jval, const double* dinv,

@@ -1022,18 +1022,22 @@ __global__ void kBlockJacobi(int n, const TD* __restrict__ dinv, const double* _
 // ---------------------------------------------------------------- well Schur reduction on the device (WellSchur, bsr.cpp:733-990)
 // The scalar Jacobian J: cells in blocks of nv unknowns (rows / columns < nc = na nv), then one BHP row per well.
 // dinv[w] = 1 / J_ww (refused: |J_ww| <= 1e-12 max |J_w,cells| or non-finite -> bad = 1)
+// 1 / d_w per well; bad[0]: a well without a usable BHP derivative, bad[1]: a nonzero well-well entry
 __global__ void kWellDinv(int nw, const int* __restrict__ wdiag, const int* __restrict__ wcPtr, const int* __restrict__ wcSrc,
-                          const double* __restrict__ jval, double* __restrict__ dinv, int* __restrict__ bad) {
+                          const int* __restrict__ wwPtr, const int* __restrict__ wwSrc, const double* __restrict__ jval,
+                          double* __restrict__ dinv, int* __restrict__ bad) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
+    for (int q = wwPtr[w]; q < wwPtr[w + 1]; ++q)
+        if (jval[wwSrc[q]] != 0.0) bad[1] = 1;
     const double d = wdiag[w] >= 0 ? jval[wdiag[w]] : 0.0;
     double rowMax = 0;
     for (int q = wcPtr[w]; q < wcPtr[w + 1]; ++q) rowMax = fmax(rowMax, fabs(jval[wcSrc[q]]));
     if (!(fabs(d) > 1e-12 * fmax(rowMax, 1e-300)) || !isfinite(d)) { bad[0] = 1; dinv[w] = 0.0; return; }
     dinv[w] = 1.0 / d;
 }
-// S value t = (J cell entry mapped to t, if any) - sum over its fill terms of (J_cw dinv_w) J_wc, in the reference's
-// order (bsr.cpp:889-907: the cell entries first, then the wells coupled to the row in row order)
+// S value t = (J cell entry mapped to t, if any) - its rank-1 terms (J_cw dinv_w) J_wc in ascending well order, the
+// engine's per-entry order (bsr.cpp:810-825: the cell entry, then the wells of the scalar row in column order)
 __global__ void kSchurValues(long long nvals, const int* __restrict__ direct, const int* __restrict__ termPtr, const int4* __restrict__ terms,
                              const double* __restrict__ jval, const double* __restrict__ dinv, double* __restrict__ sval) {
 #pragma clang fp contract(off)
@@ -2263,9 +2267,10 @@ void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32,
     RSH_NB_DISPATCH(nb, (blockJacobiT<NB>(s, n, dinv, f32, t, xp, z)))
     checkLaunch();
 }
-void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const double* jval, double* dinv, int* bad) {
+void launchWellDinv(hipStream_t s, int nw, const int* wdiag, const int* wcPtr, const int* wcSrc, const int* wwPtr, const int* wwSrc, const double* jval,
+                    double* dinv, int* bad) {
     if (nw <= 0) return;
-    hipLaunchKernelGGL(kWellDinv, dim3((nw + 63) / 64), dim3(64), 0, s, nw, wdiag, wcPtr, wcSrc, jval, dinv, bad);
+    hipLaunchKernelGGL(kWellDinv, dim3((nw + 63) / 64), dim3(64), 0, s, nw, wdiag, wcPtr, wcSrc, wwPtr, wwSrc, jval, dinv, bad);
     checkLaunch();
 }
 void launchSchurValues(hipStream_t s, long long nvals, const int* direct, const int* termPtr, const int4* terms, const double* jval,
@@ -2549,19 +2554,25 @@ void launchSub(hipStream_t s, long long n, const double* b, const double* ax, do
     hipLaunchKernelGGL(kSub, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, b, ax, r);
     checkLaunch();
 }
+// k > 32 vectors: consecutive chunks of 32 (the reduction buffer holds 32 partial rows)
 void launchMultiDot(hipStream_t s, Ctx& c, long long n, int k, const double* V, long long ld, const double* w, double* out) {
-    if (k < 1 || k > 32) throw Error(-1, "multi-dot width out of range");
+    if (k < 1) throw Error(-1, "multi-dot width out of range");
     const int g = redGrid(n);
-    hipLaunchKernelGGL(kMultiDot, dim3(g), dim3(kBlock), 0, s, n, k, V, ld, w, c.red.p);
-    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, k, g, c.red.p, out, int(kEpNone), nullptr);
+    for (int q0 = 0; q0 < k; q0 += 32) {
+        const int kc = std::min(32, k - q0);
+        hipLaunchKernelGGL(kMultiDot, dim3(g), dim3(kBlock), 0, s, n, kc, V + (long long)q0 * ld, ld, w, c.red.p);
+        hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, kc, g, c.red.p, out + q0, int(kEpNone), nullptr);
+    }
     checkLaunch();
 }
 void launchMultiAxpy(hipStream_t s, long long n, int k, const double* V, long long ld, const double* h, double* w) {
-    hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, k, V, ld, h, w, -1.0);
+    for (int q0 = 0; q0 < k; q0 += 32)
+        hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, std::min(32, k - q0), V + (long long)q0 * ld, ld, h + q0, w, -1.0);
     checkLaunch();
 }
 void launchMultiAxpyAdd(hipStream_t s, long long n, int k, const double* Z, long long ld, const double* y, double* x) {
-    hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, k, Z, ld, y, x, 1.0);
+    for (int q0 = 0; q0 < k; q0 += 32)
+        hipLaunchKernelGGL(kMultiAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, std::min(32, k - q0), Z + (long long)q0 * ld, ld, y + q0, x, 1.0);
     checkLaunch();
 }
 void launchCopyKernel(hipStream_t s, const double* src, double* dst, long long n) {

@@ -38,6 +38,7 @@ struct rsh_server {
         rsh_request r{};
         double tSubmit = 0, tStart = 0, tEnd = 0;
         rsh_result res{};
+        std::string msg;                     // the worker's error message, handed to the waiting thread
         bool done = false;
     };
     int device = 0;
@@ -70,16 +71,27 @@ struct rsh_server {
             b += v->bytes();
         return b + m.base.bytes() + m.x.bytes() + m.b.bytes();
     }
-    // make room for `need` bytes by evicting idle resident models, least recently used first (lock held)
+    // Make room for `need` bytes (lock held).  Nothing is evicted unless the eviction of idle resident models
+    // (least recently used first) is enough; a model larger than the whole budget (its measured footprint, which
+    // can exceed the estimate it was admitted with) is loaded once nothing else is resident, so its requests never
+    // wait forever.
     bool makeRoom(size_t need, int except) {
-        while (used + need > budget) {
+        size_t idle = 0;
+        for (size_t i = 0; i < models.size(); ++i) {
+            const Model& m = *models[i];
+            if (int(i) != except && m.S && !m.busy) idle += m.footprint;
+        }
+        const size_t pinned = used - std::min(used, idle);    // busy models stay
+        const bool alone = need > budget && pinned == 0;
+        if (pinned + need > budget && !alone) return false;
+        while (used + need > budget && used > 0) {
             int victim = -1;
             for (size_t i = 0; i < models.size(); ++i) {
                 const Model& m = *models[i];
                 if (int(i) == except || !m.S || m.busy) continue;
                 if (victim < 0 || m.lastUse < models[size_t(victim)]->lastUse) victim = int(i);
             }
-            if (victim < 0) return false;
+            if (victim < 0) break;
             Model& v = *models[size_t(victim)];
             rsh_system_destroy(v.S);
             v.S = nullptr;
@@ -91,13 +103,25 @@ struct rsh_server {
         }
         return true;
     }
-    // next admissible request in arrival order (lock held): its model idle and resident or fitting the budget
+    // next admissible request in arrival order (lock held): its model idle and resident or fitting the budget.
+    // Once a request waits for memory, later requests that would also load a model are held back behind it (and
+    // behind a model larger than the budget, every later request), so the waiting one is not starved.
     long long pick() {
+        bool loadBlocked = false, allBlocked = false;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
-            Req& q = *reqs[*it];
+            Req& q = *reqs.at(*it);
             Model& m = *models[size_t(q.r.model)];
             if (m.busy) continue;
-            if (!m.S && !makeRoom(m.footprint, q.r.model)) continue;
+            if (!m.S) {
+                if (allBlocked || loadBlocked) continue;
+                if (!makeRoom(m.footprint, q.r.model)) {
+                    loadBlocked = true;
+                    allBlocked = m.footprint > budget;
+                    continue;
+                }
+            } else if (allBlocked) {
+                continue;
+            }
             const long long id = *it;
             queue.erase(it);
             return id;
@@ -145,6 +169,7 @@ struct rsh_server {
         if (!q.r.x_on_device && q.r.x) RSH_HIP(hipMemcpyAsync(q.r.x, m.x.p, N * sizeof(double), hipMemcpyDeviceToHost, st));
         RSH_HIP(hipStreamSynchronize(st));
         q.res.status = rc;
+        if (rc != RSH_OK) q.msg = g_err;
         q.res.iterations = its;
         q.res.rel_res = rel;
     }
@@ -159,7 +184,7 @@ struct rsh_server {
                 std::unique_lock<std::mutex> lk(mu);
                 cvWork.wait(lk, [&] { return stop || (id = pick()) != 0; });
                 if (!id) return;                  // stop with nothing admissible
-                qp = reqs[id].get();
+                qp = reqs.at(id).get();
                 mp = models[size_t(qp->r.model)].get();
                 mp->busy = true;
                 wasResident = mp->S != nullptr;
@@ -173,8 +198,10 @@ struct rsh_server {
                 serve(q, m, ctx);
             } catch (const Error& e) {
                 q.res.status = e.code;
-            } catch (const std::exception&) {
+                q.msg = e.what();
+            } catch (const std::exception& e) {
                 q.res.status = RSH_ERR_DEVICE;
+                q.msg = e.what();
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -264,8 +291,8 @@ int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, con
         m->baseHost.assign(values, values + size_t(m->nnzb) * nb * nb);
         m->opt = *opt;
         m->footprint = rsh_server::estimate(*m);
+        // a model larger than the budget is accepted: it is served alone (makeRoom), so its requests cannot starve
         std::lock_guard<std::mutex> lk(srv->mu);
-        if (m->footprint > srv->budget) return fail(RSH_ERR_ARG, "the model alone exceeds the HBM budget");
         *model = int(srv->models.size());
         srv->models.push_back(std::move(m));
         return RSH_OK;
@@ -290,17 +317,19 @@ int rsh_server_submit(rsh_server* srv, const rsh_request* req, long long* ticket
     });
 }
 
-// blocks until the request is done; the request's record is released
+// blocks until the request is done; the request's record is released.  The record is looked up again by its
+// ticket after the wait: other threads submit (and insert into the map) while this one waits.
 int rsh_server_wait(rsh_server* srv, long long ticket, rsh_result* res) {
     return guarded([&] {
         if (!srv || !res) return fail(RSH_ERR_ARG, "null argument");
         std::unique_lock<std::mutex> lk(srv->mu);
-        auto it = srv->reqs.find(ticket);
-        if (it == srv->reqs.end()) return fail(RSH_ERR_ARG, "unknown ticket");
-        rsh_server::Req* q = it->second.get();
+        if (srv->reqs.find(ticket) == srv->reqs.end()) return fail(RSH_ERR_ARG, "unknown ticket");
+        rsh_server::Req* q = srv->reqs.at(ticket).get();   // the Req object itself never moves
         srv->cvDone.wait(lk, [&] { return q->done; });
         *res = q->res;
-        srv->reqs.erase(it);
+        const std::string msg = q->msg;
+        srv->reqs.erase(ticket);
+        if (res->status != RSH_OK && !msg.empty()) g_err = msg;   // rsh_last_error() in the waiting thread
         return RSH_OK;
     });
 }

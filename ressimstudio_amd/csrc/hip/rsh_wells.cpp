@@ -1,9 +1,16 @@
 // Well Schur reduction on the device (include/ressim_hip.h, rsh_wells_*): the engine's scalar Jacobian (cells in
-// blocks of nv unknowns, then one BHP row per well, the layout of WellSchur::build, bsr.cpp:733-840) is handed over as
-// is; the block pattern of S = A_cc - A_cw D_ww^-1 A_wc and the maps from J's entries to S's values are built once per
-// pattern on the host, then every Newton iteration moves only J's values: S's values, the reduced right-hand side and
-// the recovered BHPs are computed in HBM (kWellDinv, kSchurValues, kSchurRhs, kSchurRecover) in the reference's order
-// of operations.  The reduced system is an ordinary resident rsh_system (setup / solve as usual).
+// blocks of nv unknowns, then one BHP row per well, the layout the engine reduces with WellSchur, bsr.cpp:733-990) is
+// handed over as it is assembled.  The host plan of csrc/host/well_plan.hpp (S = A - sum_w (B[:, w] / d_w) C[w, :],
+// one rank-1 update per well) gives the reduced block pattern and, per value of S, its cell entry and its ordered
+// rank-1 terms, once per pattern; every Newton iteration then moves only J's values: 1 / d_w, S's values, the
+// reduced right-hand side and the recovered BHPs are computed in HBM (kWellDinv, kSchurValues, kSchurRhs,
+// kSchurRecover) with the engine's per-entry order of operations.  The reduced system is an ordinary resident
+// rsh_system (setup / solve as usual).
+//
+// The pattern is fixed before any values exist, so every structural cell -> well entry couples its cell to the well
+// (the engine drops exactly-zero couplings from its pattern, bsr.cpp:755-757, and would then build a smaller S).
+// Structural well-well entries are accepted and must hold zeros (checked on every set of values).
+#include "../host/well_plan.hpp"
 #include "rsh_solver.hpp"
 
 using namespace rsh;
@@ -12,7 +19,7 @@ struct rsh_wells {
     rsh_context* ctx = nullptr;
     int n = 0, na = 0, nv = 1, nw = 0, nnzJ = 0;
     rsh_system* S = nullptr;
-    DBuf<int> wdiag, wcPtr, wcCol, wcSrc, cwPtr, cwWell, cwSrc, direct, termPtr, bad;
+    DBuf<int> wdiag, wcPtr, wcCol, wcSrc, wwPtr, wwSrc, cwPtr, cwWell, cwSrc, direct, termPtr, bad;
     DBuf<int4> terms;
     DBuf<double> jval, dinv, bfull, bc, xc, xfull;
     int* hbad = nullptr;
@@ -31,122 +38,51 @@ int rsh_wells_create(rsh_context* ctx, int n, const int* row_ptr, const int* col
     return guarded([&] {
         if (!ctx || !row_ptr || !col || !out) return fail(RSH_ERR_ARG, "null argument");
         *out = nullptr;
-        if (na <= 0 || nv < 1 || nv > kMaxNb || nw < 0 || n != na * nv + nw) return fail(RSH_ERR_ARG, "n must equal na * nv + nw");
-        const int nc = na * nv, nnz = row_ptr[n];
-        for (int r = 0; r < n; ++r) {
-            if (row_ptr[r + 1] < row_ptr[r]) return fail(RSH_ERR_ARG, "row_ptr not monotone");
-            for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
-                if (col[p] < 0 || col[p] >= n) return fail(RSH_ERR_ARG, "column index out of range");
-        }
+        if (nv > kMaxNb) return fail(RSH_ERR_UNSUPPORTED, "block size above 4");
+        wellplan::Plan P;
+        std::string why = wellplan::split(n, row_ptr, col, na, nv, nw, [](int) { return true; }, P);
+        if (!why.empty()) return fail(RSH_ERR_ARG, why);
+        wellplan::pattern(n, row_ptr, col, P);
+        wellplan::ValueMap M;
+        why = wellplan::valueMap(row_ptr, col, P, M);
+        if (!why.empty()) return fail(RSH_ERR_ARG, why);
+        std::vector<int> byRowStart, byRowWell, byRowEntry;
+        wellplan::couplingsByRow(P, byRowStart, byRowWell, byRowEntry);
+        std::vector<int> toCellCol(P.toCellEntry.size());
+        for (size_t q = 0; q < toCellCol.size(); ++q) toCellCol[q] = col[P.toCellEntry[q]];
+        std::vector<int4> terms(M.termWell.size());
+        for (size_t k = 0; k < terms.size(); ++k) terms[k] = make_int4(M.termCoupling[k], M.termWell[k], M.termEntry[k], 0);
+
         auto W = std::make_unique<rsh_wells>();
         W->ctx = ctx;
-        W->n = n; W->na = na; W->nv = nv; W->nw = nw; W->nnzJ = nnz;
-        // well rows: diagonal and cell couplings (a well-well coupling is not supported, bsr.cpp:745)
-        std::vector<int> wdiag(size_t(nw), -1), wcPtr(1, 0), wcCol, wcSrc;
-        for (int w = 0; w < nw; ++w) {
-            const int r = nc + w;
-            for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
-                const int c = col[p];
-                if (c == r) wdiag[size_t(w)] = p;
-                else if (c < nc) { wcCol.push_back(c); wcSrc.push_back(p); }
-                else return fail(RSH_ERR_UNSUPPORTED, "well-well coupling (well groups) is not supported");
-            }
-            wcPtr.push_back(int(wcCol.size()));
-        }
-        // cell rows: well columns
-        std::vector<int> cwPtr(1, 0), cwWell, cwSrc;
-        for (int r = 0; r < nc; ++r) {
-            for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
-                if (col[p] >= nc) { cwWell.push_back(col[p] - nc); cwSrc.push_back(p); }
-            cwPtr.push_back(int(cwWell.size()));
-        }
-        // block pattern: the cell pattern plus the couplings through wells (bsr.cpp:775-807)
-        std::vector<std::vector<int>> wellCells(static_cast<size_t>(nw));
-        for (int w = 0; w < nw; ++w) {
-            auto& v = wellCells[size_t(w)];
-            for (int q = wcPtr[size_t(w)]; q < wcPtr[size_t(w) + 1]; ++q) v.push_back(wcCol[size_t(q)] / nv);
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-        }
-        std::vector<int> srp(1, 0), sci, mark(static_cast<size_t>(na), -1), cols, wl;
-        for (int i = 0; i < na; ++i) {
-            cols.clear();
-            wl.clear();
-            for (int e = 0; e < nv; ++e) {
-                const int r = i * nv + e;
-                for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
-                    const int c = col[p];
-                    if (c < nc && mark[size_t(c / nv)] != i) { mark[size_t(c / nv)] = i; cols.push_back(c / nv); }
-                }
-                for (int p = cwPtr[size_t(r)]; p < cwPtr[size_t(r) + 1]; ++p) wl.push_back(cwWell[size_t(p)]);
-            }
-            if (mark[size_t(i)] != i) { mark[size_t(i)] = i; cols.push_back(i); }
-            std::sort(wl.begin(), wl.end());
-            wl.erase(std::unique(wl.begin(), wl.end()), wl.end());
-            for (int w : wl)
-                for (int j : wellCells[size_t(w)])
-                    if (mark[size_t(j)] != i) { mark[size_t(j)] = i; cols.push_back(j); }
-            std::sort(cols.begin(), cols.end());
-            sci.insert(sci.end(), cols.begin(), cols.end());
-            srp.push_back(int(sci.size()));
-        }
-        // maps: per S value (block p, entry e f) the J cell entry landing there and the ordered fill terms
-        const int nv2 = nv * nv;
-        const long long nvals = (long long)sci.size() * nv2;
-        std::vector<int> direct(static_cast<size_t>(nvals), -1), pos(static_cast<size_t>(na), -1);
-        std::vector<std::vector<int4>> tl(static_cast<size_t>(nvals));
-        for (int i = 0; i < na; ++i) {
-            for (int p = srp[size_t(i)]; p < srp[size_t(i) + 1]; ++p) pos[size_t(sci[size_t(p)])] = p;
-            for (int e = 0; e < nv; ++e) {
-                const int r = i * nv + e;
-                for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
-                    const int c = col[p];
-                    if (c >= nc) continue;
-                    const long long t = (long long)pos[size_t(c / nv)] * nv2 + e * nv + c % nv;
-                    if (direct[size_t(t)] >= 0) return fail(RSH_ERR_ARG, "duplicate entries in a row of the Jacobian");
-                    direct[size_t(t)] = p;
-                }
-                for (int p = cwPtr[size_t(r)]; p < cwPtr[size_t(r) + 1]; ++p) {
-                    const int w = cwWell[size_t(p)];
-                    for (int q = wcPtr[size_t(w)]; q < wcPtr[size_t(w) + 1]; ++q) {
-                        const int c = wcCol[size_t(q)];
-                        const long long t = (long long)pos[size_t(c / nv)] * nv2 + e * nv + c % nv;
-                        tl[size_t(t)].push_back(make_int4(cwSrc[size_t(p)], w, wcSrc[size_t(q)], 0));
-                    }
-                }
-            }
-        }
-        std::vector<int> termPtr(1, 0);
-        std::vector<int4> terms;
-        for (auto& v : tl) {
-            terms.insert(terms.end(), v.begin(), v.end());
-            termPtr.push_back(int(terms.size()));
-        }
-        W->nvals = nvals;
+        W->n = n; W->na = na; W->nv = nv; W->nw = nw; W->nnzJ = row_ptr[n];
+        W->nvals = (long long)M.cellSource.size();
         W->nterms = (long long)terms.size();
         rsh_system* S = nullptr;
-        const int rc = rsh_system_create(ctx, na, nv, srp.data(), sci.data(), &S);
+        const int rc = rsh_system_create(ctx, na, nv, P.sPtr.data(), P.sCol.data(), &S);
         if (rc != RSH_OK) return rc;
         W->S = S;
         hipStream_t st = ctx->c.stream;
-        W->wdiag.upload(wdiag, st);
-        W->wcPtr.upload(wcPtr, st);
-        W->wcCol.upload(wcCol, st);
-        W->wcSrc.upload(wcSrc, st);
-        W->cwPtr.upload(cwPtr, st);
-        W->cwWell.upload(cwWell, st);
-        W->cwSrc.upload(cwSrc, st);
-        W->direct.upload(direct, st);
-        W->termPtr.upload(termPtr, st);
+        W->wdiag.upload(P.bhpDiag, st);
+        W->wcPtr.upload(P.toCellStart, st);
+        W->wcCol.upload(toCellCol, st);
+        W->wcSrc.upload(P.toCellEntry, st);
+        W->wwPtr.upload(P.toWellStart, st);
+        W->wwSrc.upload(P.toWellEntry, st);
+        W->cwPtr.upload(byRowStart, st);
+        W->cwWell.upload(byRowWell, st);
+        W->cwSrc.upload(byRowEntry, st);
+        W->direct.upload(M.cellSource, st);
+        W->termPtr.upload(M.termStart, st);
         W->terms.upload(terms, st);
-        W->jval.alloc(size_t(nnz));
+        W->jval.alloc(size_t(std::max(W->nnzJ, 1)));
         W->dinv.alloc(size_t(std::max(nw, 1)));
         W->bfull.alloc(size_t(n));
         W->xfull.alloc(size_t(n));
-        W->bc.alloc(size_t(nc));
-        W->xc.alloc(size_t(nc));
-        W->bad.alloc(1);
-        RSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&W->hbad), sizeof(int), hipHostMallocDefault));
+        W->bc.alloc(size_t(na) * nv);
+        W->xc.alloc(size_t(na) * nv);
+        W->bad.alloc(2);
+        RSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&W->hbad), 2 * sizeof(int), hipHostMallocDefault));
         RSH_HIP(hipStreamSynchronize(st));
         *out = W.release();
         return RSH_OK;
@@ -177,13 +113,14 @@ int rsh_wells_set_values(rsh_wells* w, const double* jval, int on_device) {
         hipStream_t st = w->s();
         w->S->ctx = w->ctx;
         RSH_HIP(hipMemcpyAsync(w->jval.p, jval, sizeof(double) * size_t(w->nnzJ), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-        RSH_HIP(hipMemsetAsync(w->bad.p, 0, sizeof(int), st));
-        launchWellDinv(st, w->nw, w->wdiag.p, w->wcPtr.p, w->wcSrc.p, w->jval.p, w->dinv.p, w->bad.p);
-        RSH_HIP(hipMemcpyAsync(w->hbad, w->bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        RSH_HIP(hipMemsetAsync(w->bad.p, 0, 2 * sizeof(int), st));
+        launchWellDinv(st, w->nw, w->wdiag.p, w->wcPtr.p, w->wcSrc.p, w->wwPtr.p, w->wwSrc.p, w->jval.p, w->dinv.p, w->bad.p);
+        RSH_HIP(hipMemcpyAsync(w->hbad, w->bad.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         launchSchurValues(st, w->nvals, w->direct.p, w->termPtr.p, w->terms.p, w->jval.p, w->dinv.p, w->S->val.p);
         w->S->valuesChanged();
         RSH_HIP(hipStreamSynchronize(st));
-        if (*w->hbad) return fail(RSH_ERR_UNSUPPORTED, "a well row has no usable diagonal (|J_ww| <= 1e-12 max |J_w,cells|)");
+        if (w->hbad[1]) return fail(RSH_ERR_UNSUPPORTED, "well-well coupling (a nonzero entry between two BHP unknowns) is not supported");
+        if (w->hbad[0]) return fail(RSH_ERR_UNSUPPORTED, "a well row has no usable diagonal (|J_ww| <= 1e-12 max |J_w,cells|)");
         w->valuesSet = true;
         return RSH_OK;
     });

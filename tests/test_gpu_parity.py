@@ -132,16 +132,27 @@ def test_bicgstab_matches_oracle(sysd, orc, name, prec):
     assert _true_rel(s, x, s.rhs) <= 10 * TOL
 
 
+@pytest.mark.parametrize("restart", [30, 60, 8])
 @pytest.mark.parametrize("prec", ["bilu0", "cpr"])
 @pytest.mark.parametrize("name", ["small", "layered"])
-def test_fgmres_matches_oracle_solution(sysd, orc, name, prec):
+def test_fgmres_matches_oracle_solution(sysd, orc, name, prec, restart):
+    # 60: the engine's fallback GMRES(60) (simulator.cpp:3395, 3402), orthogonalised in chunks of 32 vectors
     s = sysd[name]
-    d = rs.System(s).setup(precond=prec, krylov=rs.KRYLOV_FGMRES, restart=30)
+    d = rs.System(s).setup(precond=prec, krylov=rs.KRYLOV_FGMRES, restart=restart)
     x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
-    xo, its_o, _, ok_o = orc.krylov(s, s.rhs, prec, "gmres", TOL, 2000, restart=30)
+    xo, its_o, _, ok_o = orc.krylov(s, s.rhs, prec, "gmres", TOL, 2000, restart=restart)
     assert ok and ok_o
     assert abs(its - its_o) <= max(2, its_o // 10)
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert _true_rel(s, x, s.rhs) <= 10 * TOL
+
+
+def test_fgmres_restart_limit(sysd):
+    # restarts up to 256 are built; a longer one is refused, never silently shortened
+    s = sysd["small"]
+    with pytest.raises(rs.RshError) as e:
+        rs.System(s).setup(precond="bilu0", krylov=rs.KRYLOV_FGMRES, restart=257)
+    assert e.value.code == rs.ERR_UNSUPPORTED
 
 
 MIXED_MODES = {"fp32prec_bicgstab": (rs.MIXED, rs.KRYLOV_BICGSTAB), "fp32prec_fgmres": (rs.MIXED, rs.KRYLOV_FGMRES),

@@ -97,3 +97,66 @@ def test_perturbation_matches_host_formula(mods):
     x, its, rel, ok = d.solve(s.rhs, TOL, 2000)
     assert ok and np.array_equal(x, r["x"]) and its == r["iterations"]
     srv.close()
+
+
+def test_concurrent_callers(mods):
+    # the reference's ensemble callers (runBatch, study.cpp:345-361) are threads that each submit their own Newton
+    # iterations and wait for them; four Python threads do so against one server (ctypes releases the GIL), and
+    # every result must equal the same request served alone
+    import threading
+    srv = rs.Server(slots=4)
+    ids = [srv.add_model(s, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI) for s in mods]
+    _run(srv, ids, mods, [(m, 1) for m in range(len(mods))], together=False)
+    per_thread = [[(t, 3000 + 10 * t + k) for k in range(4)] for t in range(4)]
+    alone = {(m, sd): _run(srv, ids, mods, [(m, sd)], together=False)[0] for reqs in per_thread for (m, sd) in reqs}
+    got, errors = {}, []
+
+    def caller(reqs):
+        try:
+            for m, sd in reqs:
+                got[(m, sd)] = srv.wait(srv.submit(ids[m], mods[m].rhs, perturb=0.05, seed=sd, tol=TOL, max_it=2000))
+        except Exception as e:       # surfaced below
+            errors.append(e)
+
+    th = [threading.Thread(target=caller, args=(r,)) for r in per_thread]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors and not any(t.is_alive() for t in th), errors
+    for key, a in alone.items():
+        b = got[key]
+        assert a["status"] == rs.OK and b["status"] == rs.OK, key
+        assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"], key
+    srv.close()
+
+
+def test_model_larger_than_the_budget_is_served_alone(mods):
+    # FGMRES(30) keeps 61 Krylov vectors: the measured footprint exceeds the estimate the model was admitted with and
+    # the budget itself.  Its requests must still be served (alone, once nothing else is resident), never left queued
+    ref = rs.Server(slots=1)
+    mid = ref.add_model(mods[1], precond="bilu0", krylov=rs.KRYLOV_FGMRES, restart=30)
+    base = ref.wait(ref.submit(mid, mods[1].rhs, perturb=0.05, seed=5, tol=TOL, max_it=2000))
+    foot = ref.stats()["footprints"][mid]
+    ref.close()
+    srv = rs.Server(slots=2, hbm_budget=0.8 * foot)
+    i_big = srv.add_model(mods[1], precond="bilu0", krylov=rs.KRYLOV_FGMRES, restart=30)
+    i_small = srv.add_model(mods[2], precond="bilu0")
+    t = [srv.submit(i_big, mods[1].rhs, perturb=0.05, seed=5, tol=TOL, max_it=2000),
+         srv.submit(i_small, mods[2].rhs, perturb=0.05, seed=6, tol=TOL, max_it=2000),
+         srv.submit(i_big, mods[1].rhs, perturb=0.05, seed=5, tol=TOL, max_it=2000)]
+    res = [srv.wait(k) for k in t]
+    assert all(r["status"] == rs.OK for r in res), res
+    assert np.array_equal(res[0]["x"], base["x"]) and np.array_equal(res[2]["x"], base["x"])
+    srv.close()
+
+
+def test_worker_error_reaches_the_waiting_thread(mods):
+    # a request that fails inside a worker (here: an option refused at setup) reports its status, and
+    # rsh_last_error() in the waiting thread carries the worker's message
+    srv = rs.Server(slots=1)
+    mid = srv.add_model(mods[2], precond="bilu0", krylov=rs.KRYLOV_FGMRES, restart=300)
+    r = srv.wait(srv.submit(mid, mods[2].rhs, tol=TOL, max_it=100))
+    assert r["status"] == rs.ERR_UNSUPPORTED
+    assert "restart" in rs.lib().rsh_last_error().decode()
+    srv.close()

@@ -70,3 +70,45 @@ def test_rejects_a_well_row_without_diagonal():
     W = rs.Wells(d["row_ptr"], d["col"], d["na"], d["nv"], d["nw"])
     with pytest.raises(rs.RshError):
         W.set_values(v)
+
+
+def _well_cases():
+    import sys
+    g = os.path.join(HERE, "golden")
+    if g not in sys.path:
+        sys.path.insert(0, g)
+    import well_cases
+    return well_cases
+
+
+def test_device_reduction_of_planted_zeros():
+    # the device's pattern is fixed before any values: a structural cell -> well entry couples its cell even while its
+    # value is zero (the engine drops such couplings from its pattern), so the device's S has the pattern of the
+    # structural couplings; its values equal the engine-rule reduction bit for bit on the shared blocks and are exactly
+    # zero on the extra ones.  A zero well-well entry is accepted.
+    J = _well_cases().cases()["zeros"]
+    na, nv, nw = J["na"], J["nv"], J["nw"]
+    nc = na * nv
+    host = gen.from_csr(J["row_ptr"], J["col"], J["val"], J["rhs"], na, nv, nw)
+    v1 = J["val"].copy()
+    rows = np.repeat(np.arange(len(J["row_ptr"]) - 1), np.diff(J["row_ptr"]))
+    v1[(rows < nc) & (J["col"] >= nc)] = 1.0          # every structural coupling nonzero: the device's pattern
+    sup = gen.from_csr(J["row_ptr"], J["col"], v1, J["rhs"], na, nv, nw)
+    assert sup.nnzb > host.nnzb
+    W = rs.Wells(J["row_ptr"], J["col"], na, nv, nw)
+    W.set_values(J["val"])
+    sv = W.reduced_values(sup.nnzb, nv).reshape(sup.nnzb, nv, nv)
+    expect = np.zeros_like(sv)
+    for i in range(na):
+        cols = list(sup.col[sup.row_ptr[i]:sup.row_ptr[i + 1]])
+        for p in range(host.row_ptr[i], host.row_ptr[i + 1]):
+            expect[sup.row_ptr[i] + cols.index(host.col[p])] = host.val[p]
+    assert np.array_equal(sv, expect)
+
+
+def test_device_refuses_a_nonzero_well_well_coupling():
+    R = _well_cases().refused()["well_well"]
+    W = rs.Wells(R["row_ptr"], R["col"], R["na"], R["nv"], R["nw"])     # structural well-well entries are accepted ...
+    with pytest.raises(rs.RshError) as e:
+        W.set_values(R["val"])                                           # ... but their values must be zero
+    assert e.value.code == rs.ERR_UNSUPPORTED and "well-well" in str(e.value)

@@ -334,3 +334,76 @@ def test_devcpr_block_jacobi_second_stage(orc):
     r1, r2 = np.sin(np.arange(s.n * s.nb)), np.cos(0.3 * np.arange(s.n * s.nb))
     z12 = d.apply(r1 + r2)
     assert np.max(np.abs(z12 - d.apply(r1) - d.apply(r2))) <= 1e-5 * np.max(np.abs(z12))
+
+
+# ------------------------------------------------------------------ well Schur reduction (engine side)
+def _well_cases():
+    import sys
+    if GOLD not in sys.path:
+        sys.path.insert(0, GOLD)
+    import well_cases
+    return well_cases
+
+
+def _digest(a):
+    import hashlib
+    a = np.ascontiguousarray(a)
+    a = a.astype(np.int32) if a.dtype.kind == "i" else a.astype(np.float64)
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", ["small", "layered", "nb2", "nb4", "nb1", "chain", "zeros"])
+def test_well_reduction_equals_engine_bit_for_bit(name):
+    # the rank-1-per-well reduction (csrc/host/well_plan.hpp) against the engine's WellSchur (bsr.cpp:733-990) as
+    # compiled from its sources: pattern, values, reduced right-hand side and recovery bit for bit (digests of the
+    # reference's outputs in tests/golden/well_schur.json).  "zeros" plants the engine's value rules: zero cell -> well
+    # entries couple nothing, zero well -> cell entries still mark perforations, a zero well-well entry is accepted
+    with open(os.path.join(GOLD, "well_schur.json")) as f:
+        gold = json.load(f)[name]
+    J = _well_cases().cases()[name]
+    s = gen.from_csr(J["row_ptr"], J["col"], J["val"], J["rhs"], J["na"], J["nv"], J["nw"])
+    xc = np.cos(0.3 * np.arange(J["na"] * J["nv"]))
+    got = {"row_ptr": s.row_ptr, "col": s.col, "val": s.val, "bc": s.rhs, "x": s.recover(J["rhs"], xc)}
+    assert (s.n, s.nnzb, s.fill_blocks) == (gold["n"], gold["nnzb"], gold["fill"])
+    for k, a in got.items():
+        assert _digest(a) == gold["sha256"][k], k
+
+
+@pytest.mark.parametrize("name", ["small", "zeros", "chain"])
+def test_well_reduction_equals_compiled_engine(ref, name):
+    J = _well_cases().cases()[name]
+    w = ref.wellschur(J["row_ptr"], J["col"], J["val"], J["na"], J["nv"], J["nw"])
+    rp, ci, v = w.reduced()
+    s = gen.from_csr(J["row_ptr"], J["col"], J["val"], J["rhs"], J["na"], J["nv"], J["nw"])
+    assert np.array_equal(s.row_ptr, rp) and np.array_equal(s.col, ci) and np.array_equal(s.val.reshape(-1), v)
+    assert np.array_equal(s.rhs, w.reduce_rhs(J["rhs"]))
+    xc = np.sin(0.2 * np.arange(J["na"] * J["nv"]))
+    assert np.array_equal(s.recover(J["rhs"], xc), w.recover(J["rhs"], xc))
+
+
+@pytest.mark.parametrize("name", ["well_well", "zero_bhp"])
+def test_well_reduction_refuses_what_the_engine_refuses(name):
+    J = _well_cases().refused()[name]
+    with pytest.raises(RuntimeError) as e:
+        gen.from_csr(J["row_ptr"], J["col"], J["val"], J["rhs"], J["na"], J["nv"], J["nw"])
+    assert ("well-well" in str(e.value)) == (name == "well_well")
+
+
+def test_well_reduction_planted_mistake_is_caught():
+    # the digests see the order of operations: scaling the well row instead of the coupling, (J_rw) (d_w^-1 J_wc)
+    # instead of (J_rw d_w^-1) J_wc, gives the same matrix mathematically but not bit for bit; and a pattern built
+    # without the zero-coupling rule keeps fill blocks the engine drops (the "zeros" case: 12 instead of 8)
+    J = _well_cases().cases()["layered"]
+    f = sp.csr_matrix((J["val"], J["col"], J["row_ptr"])).toarray()
+    nc = J["na"] * J["nv"]
+    A, B, C, d = f[:nc, :nc].copy(), f[:nc, nc:], f[nc:, :nc], np.diag(f[nc:, nc:])
+    S_alt, S_ok = A.copy(), A.copy()
+    for w in range(J["nw"]):
+        S_alt -= np.outer(B[:, w], (1.0 / d[w]) * C[w])
+        S_ok -= np.outer(B[:, w] * (1.0 / d[w]), C[w])
+    S = _dense(gen.from_csr(J["row_ptr"], J["col"], J["val"], J["rhs"], J["na"], J["nv"], J["nw"]))
+    assert np.array_equal(S, S_ok)                 # one well term per entry here: the dense rank-1 form is exact
+    assert np.allclose(S, S_alt, rtol=1e-12, atol=1e-12 * np.abs(S).max()) and not np.array_equal(S, S_alt)
+    with open(os.path.join(GOLD, "well_schur.json")) as fh:
+        gold = json.load(fh)
+    assert gold["zeros"]["fill"] < gold["small"]["fill"]
