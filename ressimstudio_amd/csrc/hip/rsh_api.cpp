@@ -175,6 +175,7 @@ int fgmres(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
         if (done) { its = total; return RSH_OK; }
     }
     its = total;
+    S.lastStop = "max iterations";
     return RSH_ERR_NOT_CONVERGED;
 }
 
