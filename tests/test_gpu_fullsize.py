@@ -61,12 +61,23 @@ def test_full_size_mcbilu0_apply(big, orc):
     assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
 
 
-@pytest.mark.parametrize("form", ["fp64", "store32", "lean", "mixed", "mixed_ir"])
+def test_full_size_bilu0_apply(big, orc):
+    # the natural-order factors and level-scheduled sweeps against the oracle's natural-order BILU0
+    print("full-size natural-order BILU0 apply", flush=True)
+    d = rs.System(big).setup(precond="bilu0")
+    r = np.cos(0.7 * np.arange(big.n * big.nb))
+    z, zo = d.precond(r), orc.bilu0(big, 1).apply(r)
+    assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("form", ["fp64", "store32", "lean", "mixed", "mixed_ir", "mixed_fgmres30"])
 def test_full_size_cpr_solve(big, orc, x_ref, form):
+    # mixed_fgmres30: the FP32 preconditioner inside FP64 FGMRES(30), whose restarts recompute the FP64 true residual
     precision = {"fp64": rs.FP64, "store32": rs.MIXED_STORE, "lean": rs.FP64, "mixed": rs.MIXED,
-                 "mixed_ir": rs.MIXED_IR}[form]
-    print(f"full-size CPR-BiCGSTAB, form {form}", flush=True)
-    d = rs.System(big).setup(precond="cpr_mc", precision=precision, smoother=rs.SMOOTH_JACOBI, smooth_steps=2,
+                 "mixed_ir": rs.MIXED_IR, "mixed_fgmres30": rs.MIXED}[form]
+    krylov = rs.KRYLOV_FGMRES if form == "mixed_fgmres30" else rs.KRYLOV_BICGSTAB
+    print(f"full-size CPR, form {form}", flush=True)
+    d = rs.System(big).setup(precond="cpr_mc", precision=precision, krylov=krylov, restart=30, smoother=rs.SMOOTH_JACOBI, smooth_steps=2,
                              jacobi_omega=1.0, ilu_lean=int(form == "lean"))
     x, its, rel, ok = d.solve(big.rhs, TOL, 1000)
     true_rel = np.linalg.norm(big.rhs - orc.spmv(big, x)) / np.linalg.norm(big.rhs)
