@@ -82,6 +82,9 @@ class Checker:
             lib.chk_bilu0_factors.argtypes = [_vp, _dp]
         if hasattr(lib, "chk_set_threads"):
             lib.chk_set_threads.argtypes = [ctypes.c_int]
+        if hasattr(lib, "chk_float_solve"):
+            lib.chk_float_solve.argtypes = bsr + [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip]
+            lib.chk_refine_float.argtypes = bsr + [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
         if hasattr(lib, "chk_wellschur_new"):
             lib.chk_wellschur_new.restype = _vp
             lib.chk_wellschur_new.argtypes = [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip]
@@ -112,6 +115,34 @@ class Checker:
         if conv < 0:
             raise RuntimeError("checker krylov failed")
         return x, int(its.value), float(rel.value), bool(conv)
+
+    FLOAT_KIND = {"bilu0f": 0, "mcbilu0": 1, "devcpr": 2}
+
+    def float_solve(self, s, b, kind="bilu0f", tol=1e-4, maxit=1000):
+        """The float32 device solve (CpuFloatBlockSolver::solve, bsr.cpp:1130-1215): (x, iterations, finite)."""
+        n, nb, rp, ci, v = _bsr_args(s)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n * nb)
+        its = ctypes.c_int(0)
+        rc = self.lib.chk_float_solve(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double), _p(b, ctypes.c_double),
+                                      _p(x, ctypes.c_double), self.FLOAT_KIND[kind], tol, maxit, ctypes.byref(its))
+        if rc < 0:
+            raise RuntimeError("float solve set-up failed")
+        return x, int(its.value), bool(rc)
+
+    def refine_float(self, s, b, kind="bilu0f", tol=1e-8, max_outer=10, inner_it=500):
+        """FP64 refinement around the float device solve (refineWithDevice, bsr.cpp:1373-1396):
+        (x, total inner iterations, relative residual, converged)."""
+        n, nb, rp, ci, v = _bsr_args(s)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n * nb)
+        its = ctypes.c_int(0)
+        rel = ctypes.c_double(0)
+        rc = self.lib.chk_refine_float(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double), _p(b, ctypes.c_double),
+                                       _p(x, ctypes.c_double), self.FLOAT_KIND[kind], tol, max_outer, inner_it, ctypes.byref(its), ctypes.byref(rel))
+        if rc < 0:
+            raise RuntimeError("float solve set-up failed")
+        return x, int(its.value), float(rel.value), bool(rc)
 
     def set_threads(self, n: int) -> int:
         return int(self.lib.chk_set_threads(n)) if hasattr(self.lib, "chk_set_threads") else 1

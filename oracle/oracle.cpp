@@ -869,6 +869,162 @@ static void gmres(const Bsr& A, const Prec& M, const std::vector<double>& b, std
 
 using namespace oracle;
 
+// ---- the float32 device solve (CpuFloatBlockSolver::solve, bsr.cpp:1130-1215) and the FP64 refinement around it
+// (refineWithDevice, bsr.cpp:1373-1396).  Float32 matrix, vectors and preconditioner: the SpMV accumulates each
+// block row in float (entry by entry), the preconditioner is BILU0f (double factorisation, float factors and
+// sweeps), the multicolour factors of P A P^T rounded to float and swept colour by colour in float, or DeviceCprAmg;
+// dot products accumulate double products in double; every vector update is formed in double and rounded to float.
+struct FloatDeviceSolve {
+    enum Kind { kBilu0f = 0, kMcBilu0 = 1, kDevCpr = 2 };
+    int kind = kBilu0f;
+    Bsr A;
+    std::vector<float> Af;
+    Bilu0 ilu;                    // kBilu0f: mode 3
+    McBilu0 mc;                   // kMcBilu0: double factors of P A P^T ...
+    std::vector<float> mcF;       // ... and their float copy
+    DevCprAmg dev;
+    bool ready = false;
+    bool setup(const Bsr& M, int k) {
+        kind = k;
+        A = M;
+        Af.assign(A.v.begin(), A.v.end());
+        if (kind == kBilu0f) { ilu.mode = 3; ilu.factor(A); }
+        else if (kind == kMcBilu0) { mc.factor(A); mcF.assign(mc.ilu.LU.v.begin(), mc.ilu.LU.v.end()); }
+        else if (!dev.build(A)) return false;
+        ready = true;
+        return true;
+    }
+    void spmv(const std::vector<float>& x, std::vector<float>& y) const {
+        const int b = A.nb;
+        const size_t b2 = size_t(b) * b;
+        for (int i = 0; i < A.n; ++i) {
+            float acc[kMaxB] = {};
+            for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p) {
+                const float* B = Af.data() + size_t(p) * b2;
+                const float* xk = x.data() + size_t(A.ci[size_t(p)]) * b;
+                for (int e = 0; e < b; ++e)
+                    for (int f = 0; f < b; ++f) acc[e] += B[e * b + f] * xk[f];
+            }
+            for (int e = 0; e < b; ++e) y[size_t(i) * b + e] = acc[e];
+        }
+    }
+    void precond(const std::vector<float>& r, std::vector<float>& z) const {
+        const size_t N = r.size();
+        if (kind == kDevCpr) { dev.apply(r.data(), z.data()); return; }
+        if (kind == kBilu0f) {
+            std::vector<double> rd(r.begin(), r.end()), zd(N);
+            ilu.apply(rd.data(), zd.data());
+            for (size_t e = 0; e < N; ++e) z[e] = float(zd[e]);
+            return;
+        }
+        // colour order: gather, the float sweeps over the permuted factors (colours in order, rows ascending), scatter
+        const Bsr& F = mc.ilu.LU;
+        const int n = F.n, b = F.nb;
+        const size_t b2 = size_t(b) * b;
+        std::vector<float> y(N);
+        for (int k = 0; k < n; ++k)
+            for (int e = 0; e < b; ++e) y[size_t(k) * b + e] = r[size_t(mc.perm[size_t(k)]) * b + e];
+        for (int i = 0; i < n; ++i) {
+            float* yi = y.data() + size_t(i) * b;
+            for (int p = F.rp[size_t(i)]; p < F.diag[size_t(i)]; ++p) {
+                const float* a = mcF.data() + size_t(p) * b2;
+                const float* yk = y.data() + size_t(F.ci[size_t(p)]) * b;
+                for (int e = 0; e < b; ++e) { float t = 0; for (int f = 0; f < b; ++f) t += a[e * b + f] * yk[f]; yi[e] -= t; }
+            }
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            float* yi = y.data() + size_t(i) * b;
+            for (int p = F.diag[size_t(i)] + 1; p < F.rp[size_t(i) + 1]; ++p) {
+                const float* a = mcF.data() + size_t(p) * b2;
+                const float* yj = y.data() + size_t(F.ci[size_t(p)]) * b;
+                for (int e = 0; e < b; ++e) { float t = 0; for (int f = 0; f < b; ++f) t += a[e * b + f] * yj[f]; yi[e] -= t; }
+            }
+            const float* d = mcF.data() + size_t(F.diag[size_t(i)]) * b2;
+            float u[kMaxB];
+            for (int e = 0; e < b; ++e) { float t = 0; for (int f = 0; f < b; ++f) t += d[e * b + f] * yi[f]; u[e] = t; }
+            for (int e = 0; e < b; ++e) yi[e] = u[e];
+        }
+        for (int k = 0; k < n; ++k)
+            for (int e = 0; e < b; ++e) z[size_t(mc.perm[size_t(k)]) * b + e] = y[size_t(k) * b + e];
+    }
+    static double fdot(const std::vector<float>& u, const std::vector<float>& v) {
+        double acc = 0;
+        for (size_t e = 0; e < u.size(); ++e) acc += double(u[e]) * v[e];
+        return acc;
+    }
+    // false: a non-finite solution
+    bool solve(const double* bd, double* xd, double tol, int maxIt, int& its) const {
+        const size_t N = size_t(A.n) * A.nb;
+        its = 0;
+        std::vector<float> x(N, 0.f), r(N), rh(N), p(N, 0.f), v(N, 0.f), ph(N), sv(N), sh(N), t(N);
+        for (size_t e = 0; e < N; ++e) r[e] = float(bd[e]);
+        rh = r;
+        const double bn = std::sqrt(fdot(r, r));
+        std::fill(xd, xd + N, 0.0);
+        if (!(bn > 0)) return true;
+        double rho = 1, alpha = 1, omega = 1;
+        for (int it = 1; it <= maxIt; ++it) {
+            its = it;
+            const double rhoN = fdot(rh, r);
+            if (rhoN == 0 || !std::isfinite(rhoN)) break;
+            const double beta = (rhoN / rho) * (alpha / omega);
+            for (size_t e = 0; e < N; ++e) p[e] = float(r[e] + beta * (p[e] - omega * v[e]));
+            precond(p, ph);
+            spmv(ph, v);
+            const double rv = fdot(rh, v);
+            if (rv == 0 || !std::isfinite(rv)) break;
+            alpha = rhoN / rv;
+            for (size_t e = 0; e < N; ++e) sv[e] = float(r[e] - alpha * v[e]);
+            if (std::sqrt(fdot(sv, sv)) <= tol * bn) {
+                for (size_t e = 0; e < N; ++e) x[e] += float(alpha * ph[e]);
+                break;
+            }
+            precond(sv, sh);
+            spmv(sh, t);
+            const double tt = fdot(t, t);
+            omega = tt > 0 ? fdot(t, sv) / tt : 0;
+            for (size_t e = 0; e < N; ++e) {
+                x[e] += float(alpha * ph[e] + omega * sh[e]);
+                r[e] = float(sv[e] - omega * t[e]);
+            }
+            rho = rhoN;
+            if (std::sqrt(fdot(r, r)) <= tol * bn || omega == 0) break;
+        }
+        bool finite = true;
+        for (size_t e = 0; e < N; ++e) { xd[e] = x[e]; finite = finite && std::isfinite(xd[e]); }
+        return finite;
+    }
+    // FP64 refinement: x = sum of float corrections, r = b - A x in FP64, inner tolerance max(1e-4, min(1e-2, 10 tol))
+    bool refine(const double* b, double* x, double tol, int maxOuter, int innerIt, int& its, double& rel) const {
+        const size_t N = size_t(A.n) * A.nb;
+        std::vector<double> r(b, b + N), d(N), Ax(N);
+        std::fill(x, x + N, 0.0);
+        its = 0;
+        rel = 0;
+        double bn = 0;
+        for (size_t e = 0; e < N; ++e) bn += b[e] * b[e];
+        bn = std::sqrt(bn);
+        if (!(bn > 0)) return true;
+        const double innerTol = std::max(1e-4, std::min(1e-2, tol * 10));
+        for (int outer = 0; outer < maxOuter; ++outer) {
+            double rn = 0;
+            for (double q : r) rn += q * q;
+            rel = std::sqrt(rn) / bn;
+            if (rel <= tol) return true;
+            int k = 0;
+            if (!solve(r.data(), d.data(), innerTol, innerIt, k)) return false;
+            its += k;
+            for (size_t e = 0; e < N; ++e) x[e] += d[e];
+            oracle::spmv(A, x, Ax.data());
+            for (size_t e = 0; e < N; ++e) r[e] = b[e] - Ax[e];
+        }
+        double rn = 0;
+        for (double q : r) rn += q * q;
+        rel = std::sqrt(rn) / bn;
+        return rel <= tol;
+    }
+};
+
 extern "C" {
 
 int chk_spmv(int n, int nb, const int* rp, const int* ci, const double* v, const double* x, double* y) {
@@ -984,5 +1140,18 @@ int chk_devcpr_apply(void* h, const double* r, double* z) {
     return 0;
 }
 void chk_devcpr_free(void* h) { delete static_cast<DevCprAmg*>(h); }
+
+int chk_float_solve(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int maxit,
+                    int* its) {
+    FloatDeviceSolve F;
+    if (!F.setup(makeBsr(n, nb, rp, ci, v), kind)) return -1;
+    return F.solve(b, x, tol, maxit, *its) ? 1 : 0;
+}
+int chk_refine_float(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int max_outer,
+                     int inner_it, int* its, double* relres) {
+    FloatDeviceSolve F;
+    if (!F.setup(makeBsr(n, nb, rp, ci, v), kind)) return -1;
+    return F.refine(b, x, tol, max_outer, inner_it, *its, *relres) ? 1 : 0;
+}
 
 }  // extern "C"

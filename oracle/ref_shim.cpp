@@ -189,6 +189,31 @@ int chk_wellschur_reduce_rhs(void* h, const double* b, double* bc) { static_cast
 int chk_wellschur_recover(void* h, const double* b, const double* xc, double* x) { static_cast<WellSchur*>(h)->recover(b, xc, x); return 0; }
 void chk_wellschur_free(void* h) { delete static_cast<WellSchur*>(h); }
 
+int chk_float_solve(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int maxit,
+                    int* its) {
+    BSR A = makeBsr(n, nb, rp, ci, v);
+    CpuFloatBlockSolver dev;
+    dev.multicolour = kind == 1;
+    dev.cprAmg = kind == 2;
+    std::vector<double> bb(b, b + size_t(n) * nb), xx;
+    const bool ok = dev.solve(A, bb, xx, tol, maxit, *its);
+    for (size_t i = 0; i < xx.size(); ++i) x[i] = xx[i];
+    return ok ? 1 : 0;
+}
+int chk_refine_float(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int max_outer,
+                     int inner_it, int* its, double* relres) {
+    BSR A = makeBsr(n, nb, rp, ci, v);
+    CpuFloatBlockSolver dev;
+    dev.multicolour = kind == 1;
+    dev.cprAmg = kind == 2;
+    std::vector<double> bb(b, b + size_t(n) * nb), xx;
+    const LinResult r = refineWithDevice(dev, A, bb, xx, tol, max_outer, inner_it);
+    for (size_t i = 0; i < xx.size(); ++i) x[i] = xx[i];
+    *its = r.iterations;
+    *relres = r.relResidual;
+    return r.converged ? 1 : 0;
+}
+
 // number of OpenMP threads the reference will use (setNumThreads(0) = physical cores unless OMP_NUM_THREADS)
 int chk_set_threads(int n) { setNumThreads(n); return numThreads(); }
 

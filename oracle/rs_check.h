@@ -60,6 +60,14 @@ int chk_wellschur_reduce_rhs(void* h, const double* b, double* bc);
 int chk_wellschur_recover(void* h, const double* b, const double* xc, double* x);
 void chk_wellschur_free(void* h);
 
+/* the float32 device solve (CpuFloatBlockSolver::solve, bsr.cpp:1130-1215): kind 0 BILU0f, 1 MC-BILU0 (multicolour),
+ * 2 DeviceCprAmg; returns 1 (finite solution) / 0 / -1 (set-up failed).  chk_refine_float: refineWithDevice
+ * (bsr.cpp:1373-1396) around it; returns 1 when converged */
+int chk_float_solve(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int maxit,
+                    int* its);
+int chk_refine_float(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int kind, double tol, int max_outer,
+                     int inner_it, int* its, double* relres);
+
 int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond,
                int method, int restart, double tol, int maxit, int* its, double* relres);
 

@@ -407,3 +407,53 @@ def test_well_reduction_planted_mistake_is_caught():
     with open(os.path.join(GOLD, "well_schur.json")) as fh:
         gold = json.load(fh)
     assert gold["zeros"]["fill"] < gold["small"]["fill"]
+
+
+# ------------------------------------------------------------------ the float32 device solve and refinement
+def _float_cases():
+    import sys
+    if GOLD not in sys.path:
+        sys.path.insert(0, GOLD)
+    import make_float_golden
+    return make_float_golden.cases()
+
+
+@pytest.fixture(scope="module")
+def float_cases():
+    return _float_cases()
+
+
+@pytest.mark.parametrize("kind", ["bilu0f", "mcbilu0", "devcpr"])
+@pytest.mark.parametrize("name", ["small", "nb2", "nb4", "layered"])
+def test_float_device_solve_equals_reference_fixtures(orc, float_cases, name, kind):
+    # CpuFloatBlockSolver::solve (bsr.cpp:1130-1215) and refineWithDevice (bsr.cpp:1373-1396) restated in the oracle:
+    # solutions bit for bit (digests), iteration counts and refinement residuals equal the compiled reference's
+    with open(os.path.join(GOLD, "float_solve.json")) as f:
+        gold = json.load(f)[f"{name}/{kind}"]
+    s = float_cases[name]
+    x, its, fin = orc.float_solve(s, s.rhs, kind, 1e-4, 500)
+    assert (its, fin) == (gold["inner"]["its"], gold["inner"]["finite"]) and _digest(x) == gold["inner"]["x"]
+    xr, its_r, rel_r, conv = orc.refine_float(s, s.rhs, kind, 1e-8, 10, 500)
+    g = gold["refined"]
+    assert (its_r, rel_r, conv) == (g["its"], g["rel"], g["converged"]) and _digest(xr) == g["x"]
+
+
+@pytest.mark.parametrize("kind", ["bilu0f", "mcbilu0", "devcpr"])
+def test_float_device_solve_equals_compiled_reference(ref, orc, kind):
+    s = gen.chain()
+    a, b = orc.float_solve(s, s.rhs, kind, 1e-4, 300), ref.float_solve(s, s.rhs, kind, 1e-4, 300)
+    assert a[1:] == b[1:] and np.array_equal(a[0], b[0])
+
+
+def test_float_device_solve_is_float32_arithmetic(orc, float_cases):
+    # the inner solve is a float32 computation: its solution is made of float32 values, it reaches the requested
+    # 1e-4 but not FP64 accuracy, and the refinement around it reaches 1e-8 (scipy's direct solve as the exact answer)
+    s = float_cases["small"]
+    x, its, fin = orc.float_solve(s, s.rhs, "bilu0f", 1e-4, 500)
+    assert fin and np.array_equal(x, x.astype(np.float32).astype(np.float64))
+    A = sp.bsr_matrix((np.asarray(s.val).reshape(-1, s.nb, s.nb), s.col, s.row_ptr)).tocsc()
+    xe = spla.spsolve(A, s.rhs)
+    e_inner = np.linalg.norm(x - xe) / np.linalg.norm(xe)
+    assert 1e-9 < e_inner < 1e-2
+    xr, _, rel, conv = orc.refine_float(s, s.rhs, "bilu0f", 1e-8, 10, 500)
+    assert conv and rel <= 1e-8 and np.linalg.norm(xr - xe) <= 1e-6 * np.linalg.norm(xe)
