@@ -120,6 +120,22 @@ def cpu_baseline(s, tol, lean=False):
     return out
 
 
+def error_vs_reference(s, solutions):
+    """Each variant's solution against the reference's own FP64 CPR-BiCGSTAB solution of the same system to 1e-10
+    (compiled from its sources, oracle/_ref; the oracle's restatement where it was not built): relative 2-norm error.
+    The checker's solve is not timed."""
+    import oracle
+    ref = oracle.reference(fast=True) or oracle.oracle()
+    ref.set_threads(0)
+    t0 = time.perf_counter()
+    xr, its, rel, conv = ref.krylov(s, s.rhs, "cpr", "bicgstab", 1e-10, 3000)
+    if not conv:
+        raise RuntimeError("the reference solve did not reach 1e-10")
+    nr = float(np.linalg.norm(xr))
+    return {"sample": f"{its} iterations to rel.res {rel:.1e} in {time.perf_counter() - t0:.1f} s",
+            "errors": {k: float(np.linalg.norm(x - xr) / nr) for k, x in solutions.items()}}
+
+
 def _upload(rs, ctx, a, ptrs):
     a = np.ascontiguousarray(a)
     p = ctypes.c_void_p()
@@ -425,6 +441,7 @@ def main():
             log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {by / ms / 1e6:.0f} GB/s" if by > 0 else ""))
         return
     results = {}
+    solutions = {}
     x_fp64 = None
     for name, o in cfgs.items():
         # symbolic set-up (once per pattern) outside the timed region
@@ -432,6 +449,7 @@ def main():
         steps = args.steps if name == names[0] else 1
         dt, its, rel, info = time_config(rs, sysd, dev, o, args.tol, args.max_it, steps, args.warmup if name == names[0] else 0)
         x = dev.x_host()
+        solutions[name] = x
         results[name] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "iterations": its, "rel_res": rel, "steps": steps,
                          "setup_ms": info["last_setup_ms"], "solve_ms": info["last_solve_ms"], "outer": info["last_outer"]}
         if x_fp64 is None and o.precision == FP64:
@@ -492,6 +510,14 @@ def main():
                 log(f"cpu baseline, lean BILU0: {cpu['lean']['value']:.4f} solves/s, {cpu['lean']['iterations']} iterations ({cpu['lean']['sample']})")
         except Exception as e:  # noqa: BLE001
             log(f"cpu baseline unavailable: {e}")
+        try:
+            ref_err = error_vs_reference(s, solutions)
+            for name, e in ref_err["errors"].items():
+                results[name]["err_vs_reference"] = e
+            log(f"error against the reference's FP64 solution ({ref_err['sample']}): " +
+                ", ".join(f"{k} {v:.1e}" for k, v in ref_err["errors"].items()))
+        except Exception as e:  # noqa: BLE001
+            log(f"reference solution unavailable: {e}")
 
     main_r = results[head]
     # the mixed-precision forms against the plain FP64 GPU ILU0-BiCGSTAB and the FP64 head
@@ -499,7 +525,8 @@ def main():
     for name in ("cpr_mc_store32", "cpr_mc_lean", "cpr_mc_mixed", "cpr_mc_mixed_ir", "cpr_mc_mixed_fgmres"):
         if name in results:
             r = results[name]
-            mixed[name] = {"solves_per_s": r["solves_per_s"], "err_vs_fp64": r.get("err_vs_fp64"), "outer": r.get("outer"),
+            mixed[name] = {"solves_per_s": r["solves_per_s"], "err_vs_fp64": r.get("err_vs_fp64"), "err_vs_reference": r.get("err_vs_reference"),
+                           "outer": r.get("outer"),
                            "vs_fp64_head": r["solves_per_s"] / main_r["solves_per_s"]}
             if "ilu0" in results:
                 mixed[name]["vs_plain_fp64_ilu0"] = r["solves_per_s"] / results["ilu0"]["solves_per_s"]

@@ -104,6 +104,10 @@ void rsh_default_options(rsh_options* opt);
 int rsh_context_create(int device, rsh_context** out);
 int rsh_context_destroy(rsh_context* ctx);
 int rsh_context_synchronize(rsh_context* ctx);
+/* device seconds of the set-ups and solves run on the context (GpuBlockSolver::deviceSeconds, bsr.h:149-150: device
+ * busy time from GPU timestamps): HIP events on the context's stream, from the first rsh_system_setup or rsh_solve*
+ * of a span to the end of the solve that closes it */
+int rsh_context_device_seconds(rsh_context* ctx, double* seconds);
 
 /* ---- a resident system: pattern, values and preconditioner stay on the device across Newton iterations */
 int rsh_system_create(rsh_context* ctx, int n, int nb, const int* row_ptr, const int* col, rsh_system** out);

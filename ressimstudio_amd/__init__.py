@@ -42,7 +42,7 @@ EXPORTED = [
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
     "rsh_batch_info", "rsh_server_create", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
     "rsh_server_stats", "rsh_wells_create", "rsh_wells_destroy", "rsh_wells_system", "rsh_wells_set_values", "rsh_wells_solve",
-    "rsh_wells_reduced_values",
+    "rsh_wells_reduced_values", "rsh_context_device_seconds",
 ]
 
 
@@ -90,6 +90,7 @@ def lib():
         L.rsh_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
         L.rsh_context_destroy.argtypes = [_vp]
         L.rsh_context_synchronize.argtypes = [_vp]
+        L.rsh_context_device_seconds.argtypes = [_vp, _dp]
         L.rsh_system_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.POINTER(_vp)]
         L.rsh_system_destroy.argtypes = [_vp]
         L.rsh_system_set_values.argtypes = [_vp, _dp]

@@ -13,6 +13,22 @@ namespace {
 // iterations between host reads of the Krylov scalars while the residual is far from the tolerance
 constexpr int kCheckEvery = 4;
 
+// device-time spans of a context (rsh_context_device_seconds)
+void spanBegin(Ctx& c) {
+    if (c.spanOpen) return;
+    RSH_HIP(hipEventRecord(c.ev0, c.stream));
+    c.spanOpen = true;
+}
+void spanEnd(Ctx& c) {
+    if (!c.spanOpen) return;
+    c.spanOpen = false;
+    RSH_HIP(hipEventRecord(c.ev1, c.stream));
+    RSH_HIP(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    RSH_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.deviceMs += ms;
+}
+
 void syncScalars(rsh_system& S) {
     Ctx& c = S.ctx->c;
     RSH_HIP(hipMemcpyAsync(c.hks, c.ks.p, sizeof(KScalars), hipMemcpyDeviceToHost, c.stream));
@@ -328,10 +344,18 @@ int rsh_context_create(int device, rsh_context** out) {
         ctx->c.scal.alloc(64);
         RSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->c.hks), sizeof(KScalars), hipHostMallocDefault));
         RSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->c.hscal), 64 * sizeof(double), hipHostMallocDefault));
+        RSH_HIP(hipEventCreate(&ctx->c.ev0));
+        RSH_HIP(hipEventCreate(&ctx->c.ev1));
         RSH_HIP(hipStreamSynchronize(ctx->c.stream));
         *out = ctx.release();
         return RSH_OK;
     });
+}
+
+int rsh_context_device_seconds(rsh_context* ctx, double* seconds) {
+    if (!ctx || !seconds) return fail(RSH_ERR_ARG, "null argument");
+    *seconds = ctx->c.deviceMs * 1e-3;
+    return RSH_OK;
 }
 
 int rsh_context_destroy(rsh_context* ctx) {
@@ -345,6 +369,8 @@ int rsh_context_destroy(rsh_context* ctx) {
         ctx->c.scal.release();
         if (ctx->c.hks) (void)hipHostFree(ctx->c.hks);
         if (ctx->c.hscal) (void)hipHostFree(ctx->c.hscal);
+        if (ctx->c.ev0) (void)hipEventDestroy(ctx->c.ev0);
+        if (ctx->c.ev1) (void)hipEventDestroy(ctx->c.ev1);
         (void)hipStreamDestroy(ctx->c.stream);
         delete ctx;
         return RSH_OK;
@@ -504,6 +530,7 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
             return fail(RSH_ERR_ARG, "unknown precision");
         if (o.restart > kMaxRestart) return fail(RSH_ERR_UNSUPPORTED, "FGMRES restart above " + std::to_string(kMaxRestart) + " is not built");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
+        spanBegin(sys->ctx->c);
         rsh_system& S = *sys;
         hipStream_t st = S.s();
         const double t0 = nowMs();
@@ -569,7 +596,10 @@ int rsh_solve_device(rsh_system* sys, const double* b_dev, double* x_dev, double
     return guarded([&] {
         if (!sys || !b_dev || !x_dev) return fail(RSH_ERR_ARG, "null argument");
         RSH_HIP(hipSetDevice(sys->ctx->c.device));
-        return solveDevice(*sys, b_dev, x_dev, tol, max_it, iterations, rel_res);
+        spanBegin(sys->ctx->c);
+        const int rc = solveDevice(*sys, b_dev, x_dev, tol, max_it, iterations, rel_res);
+        spanEnd(sys->ctx->c);
+        return rc;
     });
 }
 
@@ -580,10 +610,11 @@ int rsh_solve(rsh_system* sys, const double* b, double* x, double tol, int max_i
         rsh_system& S = *sys;
         S.ensureVec(S.vb);
         S.ensureVec(S.vx);
+        spanBegin(S.ctx->c);
         RSH_HIP(hipMemcpyAsync(S.vb.p, b, size_t(S.N) * sizeof(double), hipMemcpyHostToDevice, S.s()));
         const int rc = solveDevice(S, S.vb.p, S.vx.p, tol, max_it, iterations, rel_res);
         RSH_HIP(hipMemcpyAsync(x, S.vx.p, size_t(S.N) * sizeof(double), hipMemcpyDeviceToHost, S.s()));
-        RSH_HIP(hipStreamSynchronize(S.s()));
+        spanEnd(S.ctx->c);
         if (rc == RSH_OK) {
             for (long long i = 0; i < S.N; ++i)
                 if (!std::isfinite(x[i])) return fail(RSH_ERR_NONFINITE, "non-finite solution");
@@ -686,7 +717,7 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
         o << "{\"n\":" << S.n << ",\"nb\":" << S.nb << ",\"nnzb\":" << S.nnzb << ",\"spmv_slices\":" << S.spmvHost.nslices
           << ",\"spmv_slots\":" << S.spmvHost.nslots << ",\"spmv_long_rows\":" << S.spmvHost.lrow.size()
           << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0) << ",\"cpr_true_impes\":" << (S.acc.n ? 1 : 0)
-          << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks;
+          << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks << ",\"device_bytes\":" << S.deviceBytes();
         if (S.ilu) {
             o << ",\"ilu_levels\":" << S.ilu->sL.levels() << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
               << ",\"ilu_long_rows_L\":" << S.ilu->hL.lrow.size() << ",\"ilu_long_rows_U\":" << S.ilu->hU.lrow.size() << ",\"ilu_bytes\":" << S.ilu->bytes();

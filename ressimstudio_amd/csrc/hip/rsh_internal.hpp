@@ -77,6 +77,11 @@ struct Ctx {
     DBuf<double> scal;         // general device scalars (dot results, 64 slots)
     KScalars* hks = nullptr;   // pinned host mirror
     double* hscal = nullptr;
+    // device time of set-up + solve spans on this stream (HIP events): a span opens at the first set-up or solve call
+    // and closes at the end of the solve
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool spanOpen = false;
+    double deviceMs = 0;
 };
 
 // ------------------------------------------------------------------ row-sliced block layout

@@ -61,16 +61,7 @@ struct rsh_server {
         // natural values, SpMV slices, ILU factors + sweep slices, pressure column, base values; ~20 vectors
         return vals * 6 + vec * 20 + size_t(m.nnzb) * 16;
     }
-    size_t measure(const Model& m) const {
-        const rsh_system& S = *m.S;
-        size_t b = S.val.bytes() + S.spmv64.bytes() + S.spmv32.bytes() + S.colP.bytes() + S.colP32.bytes() + S.bjInv.bytes() + S.bjInv32.bytes() +
-                   S.rp.bytes() + S.ci.bytes() + S.diag.bytes();
-        if (S.ilu) b += S.ilu->bytes();
-        if (S.cpr) b += S.cpr->bytes();
-        for (const DBuf<double>* v : {&S.vb, &S.vx, &S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh, &S.w, &S.irR, &S.irD, &S.irX, &S.V, &S.Z})
-            b += v->bytes();
-        return b + m.base.bytes() + m.x.bytes() + m.b.bytes();
-    }
+    size_t measure(const Model& m) const { return m.S->deviceBytes() + m.base.bytes() + m.x.bytes() + m.b.bytes(); }
     // Make room for `need` bytes (lock held).  Nothing is evicted unless the eviction of idle resident models
     // (least recently used first) is enough; a model larger than the whole budget (its measured footprint, which
     // can exceed the estimate it was admitted with) is loaded once nothing else is resident, so its requests never

@@ -907,6 +907,15 @@ struct rsh_system {
     bool iluInv64(bool f32) const { return !f32 && opt.ilu_lean && opt.precision == RSH_FP64; }
 
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
+    // device memory the system holds: pattern, values and their slice layouts, preconditioner, Krylov vectors
+    size_t deviceBytes() const {
+        size_t b = val.bytes() + spmv64.bytes() + spmv32.bytes() + colP.bytes() + colP32.bytes() + bjInv.bytes() + bjInv32.bytes() + rp.bytes() +
+                   ci.bytes() + diag.bytes();
+        if (ilu) b += ilu->bytes();
+        if (cpr) b += cpr->bytes();
+        for (const DBuf<double>* v : {&vb, &vx, &r, &rh, &p, &this->v, &sv, &t, &ph, &sh, &w, &irR, &irD, &irX, &V, &Z}) b += v->bytes();
+        return b;
+    }
 
     // new values are in val: repack the SpMV layouts; the preconditioner needs a new setup
     void valuesChanged() {
