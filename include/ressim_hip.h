@@ -131,6 +131,17 @@ int rsh_solve(rsh_system* sys, const double* b, double* x, double tol, int max_i
 int rsh_solve_device(rsh_system* sys, const double* b_dev, double* x_dev, double tol, int max_it, int* iterations,
                      double* rel_res);
 
+/* ---- a solve that allocates nothing (no counterpart in the reference): for callers that cannot call hipMalloc -
+ * tight serving loops and harnesses that forbid allocation.  The same BiCGSTAB as rsh_solve_device on a system already
+ * set up (RSH_FP64 or RSH_MIXED_STORE, RSH_KRYLOV_BICGSTAB; other forms: RSH_ERR_UNSUPPORTED), with its Krylov
+ * vectors, scalars and partial sums in the library's static device scratch: 64 MiB per GPU, held until the process
+ * ends, i.e. at most 1 047 552 unknowns (n * nb; larger systems: RSH_ERR_ARG).  Results are bit-identical to
+ * rsh_solve_device.  Every call on one GPU shares that scratch: calls that overlap on one GPU (from several threads
+ * or contexts) race, so the caller serialises them.  The call synchronises its context's stream (it is not meant for
+ * stream capture). */
+int rsh_solve_device_noalloc(rsh_system* sys, const double* b_dev, double* x_dev, double tol, int max_it, int* iterations,
+                             double* rel_res);
+
 /* ---- kernel-level entry points (host vectors), for parity checks against BSR::spmv / Preconditioner::apply */
 int rsh_spmv(rsh_system* sys, const double* x, double* y);
 int rsh_precond_apply(rsh_system* sys, const double* r, double* z);

@@ -42,7 +42,7 @@ EXPORTED = [
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
     "rsh_batch_info", "rsh_server_create", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
     "rsh_server_stats", "rsh_wells_create", "rsh_wells_destroy", "rsh_wells_system", "rsh_wells_set_values", "rsh_wells_solve",
-    "rsh_wells_reduced_values", "rsh_context_device_seconds",
+    "rsh_wells_reduced_values", "rsh_context_device_seconds", "rsh_solve_device_noalloc",
 ]
 
 
@@ -98,6 +98,7 @@ def lib():
         L.rsh_system_setup.argtypes = [_vp, ctypes.POINTER(Options)]
         L.rsh_solve.argtypes = [_vp, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.rsh_solve_device.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.rsh_solve_device_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.rsh_spmv.argtypes = [_vp, _dp, _dp]
         L.rsh_precond_apply.argtypes = [_vp, _dp, _dp]
         L.rsh_amg_apply.argtypes = [_vp, _dp, _dp]

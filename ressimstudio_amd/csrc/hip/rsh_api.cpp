@@ -29,61 +29,80 @@ void spanEnd(Ctx& c) {
     c.deviceMs += ms;
 }
 
-void syncScalars(rsh_system& S) {
+// The buffers one BiCGSTAB solve works in: the Krylov vectors (N doubles each), the device Krylov scalars and their
+// pinned host mirror, the reductions' partial sums (2 * kRedBlocks doubles) and a dot-product output.  The resident
+// solve takes them from the system and its context; rsh_solve_device_noalloc from a static device array.
+struct BicgWork {
+    double *r, *rh, *p, *v, *sv, *t, *ph, *sh;
+    KScalars* ks;
+    KScalars* hks;
+    double* red;
+    double* scal;
+    bool graphs;     // the preconditioner through captured graphs (instantiated on first use)
+};
+
+BicgWork residentWork(rsh_system& S) {
+    for (auto* bf : {&S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh}) S.ensureVec(*bf);
     Ctx& c = S.ctx->c;
-    RSH_HIP(hipMemcpyAsync(c.hks, c.ks.p, sizeof(KScalars), hipMemcpyDeviceToHost, c.stream));
-    RSH_HIP(hipStreamSynchronize(c.stream));
+    return BicgWork{S.r.p, S.rh.p, S.p.p, S.v.p, S.sv.p, S.t.p, S.ph.p, S.sh.p, c.ks.p, c.hks, c.red.p, c.scal.p, true};
+}
+
+void readScalars(hipStream_t st, const BicgWork& W) {
+    RSH_HIP(hipMemcpyAsync(W.hks, W.ks, sizeof(KScalars), hipMemcpyDeviceToHost, st));
+    RSH_HIP(hipStreamSynchronize(st));
 }
 
 // Right-preconditioned BiCGSTAB from x0 = 0 (linsolve.cpp:159-216); the loop structure, breakdown tests and
 // the two convergence tests (on ||s|| and on the recursive ||r||) are the reference's.
 // precF32: the preconditioner's factors, hierarchy and sweeps in FP32; opF32: the operator too (inner solves of the
 // refinement loop).  Vectors, dot products and the Krylov recurrences stay FP64.
-int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool precF32 = false,
-             bool opF32 = false) {
-    Ctx& c = S.ctx->c;
-    hipStream_t st = c.stream;
+int bicgstabIn(rsh_system& S, const BicgWork& W, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool precF32,
+               bool opF32) {
+    hipStream_t st = S.ctx->c.stream;
     const long long N = S.N;
-    for (auto* bf : {&S.r, &S.rh, &S.p, &S.v, &S.sv, &S.t, &S.ph, &S.sh}) S.ensureVec(*bf);
     launchFill(st, x, 0.0, N);
-    launchCopy(st, S.r.p, b, N);
-    launchCopy(st, S.rh.p, b, N);
-    launchFill(st, S.p.p, 0.0, N);
-    launchFill(st, S.v.p, 0.0, N);
+    launchCopy(st, W.r, b, N);
+    launchCopy(st, W.rh, b, N);
+    launchFill(st, W.p, 0.0, N);
+    launchFill(st, W.v, 0.0, N);
     KScalars init{};
     init.rho = 1; init.alpha = 1; init.omega = 1; init.tol = tol; init.state = kRun; init.it = 0;
-    *c.hks = init;
-    RSH_HIP(hipMemcpyAsync(c.ks.p, c.hks, sizeof(KScalars), hipMemcpyHostToDevice, st));
-    launchDots(st, c, N, 1, b, b, nullptr, nullptr, c.scal.p, kEpBn, c.ks.p);
-    syncScalars(S);
+    *W.hks = init;
+    RSH_HIP(hipMemcpyAsync(W.ks, W.hks, sizeof(KScalars), hipMemcpyHostToDevice, st));
+    launchDots(st, W.red, N, 1, b, b, nullptr, nullptr, W.scal, kEpBn, W.ks);
+    readScalars(st, W);
     its = 0;
     rel = 0;
-    if (c.hks->bn == 0) { S.lastStop = "zero rhs"; return RSH_OK; }
-    if (!std::isfinite(c.hks->bn)) { rel = c.hks->bn; S.lastStop = "non-finite right-hand side"; return RSH_ERR_NONFINITE; }
+    if (W.hks->bn == 0) { S.lastStop = "zero rhs"; return RSH_OK; }
+    if (!std::isfinite(W.hks->bn)) { rel = W.hks->bn; S.lastStop = "non-finite right-hand side"; return RSH_ERR_NONFINITE; }
+    auto prec = [&](const double* in, double* out) {
+        if (W.graphs) S.precondG(in, out, precF32);
+        else S.precond(in, out, precF32);
+    };
     // The stopping tests run on the device (the epilogues freeze the state), so the host reads the scalars only
     // every few iterations while the residual is far from the tolerance; iterations launched after a stop change
     // nothing.
     // rho = (rh, r) of the first iteration; the later ones come with the closing reduction of the previous iteration
-    launchDots(st, c, N, 1, S.rh.p, S.r.p, nullptr, nullptr, c.scal.p, kEpRho, c.ks.p);
+    launchDots(st, W.red, N, 1, W.rh, W.r, nullptr, nullptr, W.scal, kEpRho, W.ks);
     int nextCheck = 1;
     for (int it = 1; it <= maxIt; ++it) {
-        launchBicgP(st, N, c.ks.p, S.r.p, S.v.p, S.p.p);
-        S.precondG(S.p.p, S.ph.p, precF32);
-        S.spmv(S.ph.p, S.v.p, opF32);
-        launchDots(st, c, N, 1, S.rh.p, S.v.p, nullptr, nullptr, c.scal.p, kEpAlpha, c.ks.p);
-        launchBicgS(st, c, N, c.ks.p, S.r.p, S.v.p, S.sv.p);
-        S.precondG(S.sv.p, S.sh.p, precF32);
-        S.spmv(S.sh.p, S.t.p, opF32);
-        launchDots(st, c, N, 2, S.t.p, S.t.p, S.t.p, S.sv.p, c.scal.p, kEpOmega, c.ks.p);
-        launchBicgXR(st, c, N, c.ks.p, S.ph.p, S.sh.p, S.sv.p, S.t.p, S.rh.p, x, S.r.p);
+        launchBicgP(st, N, W.ks, W.r, W.v, W.p);
+        prec(W.p, W.ph);
+        S.spmv(W.ph, W.v, opF32);
+        launchDots(st, W.red, N, 1, W.rh, W.v, nullptr, nullptr, W.scal, kEpAlpha, W.ks);
+        launchBicgS(st, W.red, N, W.ks, W.r, W.v, W.sv);
+        prec(W.sv, W.sh);
+        S.spmv(W.sh, W.t, opF32);
+        launchDots(st, W.red, N, 2, W.t, W.t, W.t, W.sv, W.scal, kEpOmega, W.ks);
+        launchBicgXR(st, W.red, N, W.ks, W.ph, W.sh, W.sv, W.t, W.rh, x, W.r);
         if (it < nextCheck && it < maxIt) continue;
-        syncScalars(S);
-        if (c.hks->state != kRun) break;
-        const double relNow = c.hks->rn / c.hks->bn;
+        readScalars(st, W);
+        if (W.hks->state != kRun) break;
+        const double relNow = W.hks->rn / W.hks->bn;
         if (!std::isfinite(relNow)) break;       // a non-finite residual (non-finite matrix values): stop here
         nextCheck = it + (relNow > 30.0 * tol ? kCheckEvery : 1);
     }
-    const KScalars& k = *c.hks;
+    const KScalars& k = *W.hks;
     its = k.it;
     if (!std::isfinite(k.rn)) { rel = k.rn; S.lastStop = "non-finite residual"; return RSH_ERR_NONFINITE; }
     if (k.state == kConvS || k.state == kDoneS) { rel = k.sn / k.bn; S.lastStop = "converged (s)"; return RSH_OK; }
@@ -92,6 +111,11 @@ int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, i
     if (k.state == kBreak) { S.lastStop = "breakdown"; return RSH_ERR_BREAKDOWN; }
     S.lastStop = k.state == kOmegaZero ? "omega = 0" : "max iterations";
     return RSH_ERR_NOT_CONVERGED;
+}
+
+int bicgstab(rsh_system& S, const double* b, double* x, double tol, int maxIt, int& its, double& rel, bool precF32 = false,
+             bool opF32 = false) {
+    return bicgstabIn(S, residentWork(S), b, x, tol, maxIt, its, rel, precF32, opF32);
 }
 
 // Flexible restarted GMRES (linsolve.cpp:218-273): x0 = 0 (or the given x when warm), Z_j = M V_j stored, the
@@ -266,7 +290,8 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     return rel <= tol ? RSH_OK : RSH_ERR_NOT_CONVERGED;
 }
 
-int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt, int* iterations, double* relRes) {
+// W: the BiCGSTAB workspace of a solve that must not allocate (null: the system's own buffers)
+int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt, int* iterations, double* relRes, const BicgWork* W = nullptr) {
     if (!S.ready) return fail(RSH_ERR_ARG, "rsh_solve: call rsh_system_setup first");
     const double t0 = nowMs();
     int its = 0, outer = 0, rc;
@@ -281,7 +306,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     else if (S.opt.precision == RSH_MIXED && S.opt.krylov == RSH_KRYLOV_BICGSTAB) rc = refine(S, b, x, tol, maxIt, its, rel, outer, false);
     else if (S.opt.krylov == RSH_KRYLOV_FGMRES)
         rc = fgmres(S, b, x, tol, maxIt, S.opt.restart > 0 ? S.opt.restart : 30, S.opt.precision == RSH_MIXED, false, false, its, rel);
-    else rc = bicgstab(S, b, x, tol, maxIt, its, rel, false, false);
+    else rc = W ? bicgstabIn(S, *W, b, x, tol, maxIt, its, rel, false, false) : bicgstab(S, b, x, tol, maxIt, its, rel, false, false);
     RSH_HIP(hipStreamSynchronize(S.s()));
     S.lastSolveMs = nowMs() - t0;
     S.lastIts = its;
@@ -297,6 +322,33 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
 
 std::once_flag g_devOnce;
 int g_devCount = -1;
+
+// ------------------------------------------------------------------ scratch of rsh_solve_device_noalloc
+// A file-scope device array (each GPU the library runs on holds its own copy until the process ends): the BiCGSTAB
+// workspace of solves that must not allocate.  Host code never passes the array's name to a kernel or a copy (on the
+// host it is only a stand-in for the symbol): the per-device address comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;   // 64 MiB
+constexpr size_t kNoAllocHead = size_t(64) << 10;    // Krylov scalars, dot output and partial sums first
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+std::mutex g_scratchMu;
+void* g_scratchPtr[kNoAllocDevices] = {};
+
+void* scratchOn(int device) {
+    if (device < 0 || device >= kNoAllocDevices) throw Error(RSH_ERR_ARG, "device index out of range for the no-alloc scratch");
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) throw Error(RSH_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchPtr[device] = p;
+    }
+    return g_scratchPtr[device];
+}
+
+// vectors of N doubles rounded up to 256 bytes; the largest N whose eight vectors fit behind the head
+size_t noAllocVecBytes(long long N) { return (size_t(N) * sizeof(double) + 255) / 256 * 256; }
 
 }  // namespace
 
@@ -619,6 +671,30 @@ int rsh_solve(rsh_system* sys, const double* b, double* x, double tol, int max_i
             for (long long i = 0; i < S.N; ++i)
                 if (!std::isfinite(x[i])) return fail(RSH_ERR_NONFINITE, "non-finite solution");
         }
+        return rc;
+    });
+}
+
+int rsh_solve_device_noalloc(rsh_system* sys, const double* b_dev, double* x_dev, double tol, int max_it, int* iterations, double* rel_res) {
+    return guarded([&] {
+        if (!sys || !b_dev || !x_dev) return fail(RSH_ERR_ARG, "null argument");
+        rsh_system& S = *sys;
+        const size_t vb = noAllocVecBytes(S.N);
+        if (kNoAllocHead + 8 * vb > kNoAllocBytes)
+            return fail(RSH_ERR_ARG, "rsh_solve_device_noalloc: " + std::to_string(S.N) + " unknowns exceed the scratch of " +
+                                         std::to_string((kNoAllocBytes - kNoAllocHead) / 8 / sizeof(double)) + " (64 MiB for eight vectors)");
+        if (!S.ready) return fail(RSH_ERR_ARG, "rsh_solve_device_noalloc: call rsh_system_setup first");
+        if (S.opt.krylov != RSH_KRYLOV_BICGSTAB || (S.opt.precision != RSH_FP64 && S.opt.precision != RSH_MIXED_STORE))
+            return fail(RSH_ERR_UNSUPPORTED, "rsh_solve_device_noalloc: BiCGSTAB with RSH_FP64 or RSH_MIXED_STORE only");
+        RSH_HIP(hipSetDevice(S.ctx->c.device));
+        unsigned char* base = static_cast<unsigned char*>(scratchOn(S.ctx->c.device));
+        double* vec[8];
+        for (int k = 0; k < 8; ++k) vec[k] = reinterpret_cast<double*>(base + kNoAllocHead + size_t(k) * vb);
+        const BicgWork W{vec[0], vec[1], vec[2], vec[3], vec[4], vec[5], vec[6], vec[7], reinterpret_cast<KScalars*>(base), S.ctx->c.hks,
+                         reinterpret_cast<double*>(base + 1024), reinterpret_cast<double*>(base + 512), false};
+        spanBegin(S.ctx->c);
+        const int rc = solveDevice(S, b_dev, x_dev, tol, max_it, iterations, rel_res, &W);
+        spanEnd(S.ctx->c);
         return rc;
     });
 }

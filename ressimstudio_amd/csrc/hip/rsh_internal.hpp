@@ -289,6 +289,12 @@ void launchFill(hipStream_t s, double* x, double v, long long n);
 void launchCopy(hipStream_t s, double* dst, const double* src, long long n);
 void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1,
                 double* out, int epilogue, KScalars* ks);
+// the same with the partial sums in red (kRedBlocks * 2 doubles at least) instead of the context's buffer
+void launchDots(hipStream_t s, double* red, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1,
+                double* out, int epilogue, KScalars* ks);
+void launchBicgS(hipStream_t s, double* red, long long n, KScalars* ks, const double* r, const double* v, double* sv);
+void launchBicgXR(hipStream_t s, double* red, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv,
+                  const double* t, const double* rh, double* x, double* r);
 void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p);
 void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv);
 // x += alpha ph + omega sh, r = s - omega t, then ||r|| and the next iteration's rho = (rh, r) (epilogue kEpRnRho);

@@ -2513,30 +2513,41 @@ static int redGrid(long long n) {
     if (g > kRedBlocks) g = kRedBlocks;
     return int(g);
 }
-void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
+void launchDots(hipStream_t s, double* red, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
                 int ep, KScalars* ks) {
     const int g = redGrid(n);
-    if (k == 1) hipLaunchKernelGGL(kDots<1>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p);
-    else hipLaunchKernelGGL(kDots<2>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, c.red.p);
-    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, k, g, c.red.p, out, ep, ks);
+    if (k == 1) hipLaunchKernelGGL(kDots<1>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, red);
+    else hipLaunchKernelGGL(kDots<2>, dim3(g), dim3(kBlock), 0, s, n, a0, b0, a1, b1, red);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, k, g, red, out, ep, ks);
     checkLaunch();
+}
+void launchDots(hipStream_t s, Ctx& c, long long n, int k, const double* a0, const double* b0, const double* a1, const double* b1, double* out,
+                int ep, KScalars* ks) {
+    launchDots(s, c.red.p, n, k, a0, b0, a1, b1, out, ep, ks);
 }
 void launchBicgP(hipStream_t s, long long n, const KScalars* ks, const double* r, const double* v, double* p) {
     hipLaunchKernelGGL(kBicgP, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, ks, r, v, p);
     checkLaunch();
 }
-void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv) {
+void launchBicgS(hipStream_t s, double* red, long long n, KScalars* ks, const double* r, const double* v, double* sv) {
     const int g = redGrid(n);
-    hipLaunchKernelGGL(kBicgS, dim3(g), dim3(kBlock), 0, s, n, ks, r, v, sv, c.red.p);
-    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, c.red.p, nullptr, int(kEpSn), ks);
+    hipLaunchKernelGGL(kBicgS, dim3(g), dim3(kBlock), 0, s, n, ks, r, v, sv, red);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 1, g, red, nullptr, int(kEpSn), ks);
+    checkLaunch();
+}
+void launchBicgS(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* r, const double* v, double* sv) {
+    launchBicgS(s, c.red.p, n, ks, r, v, sv);
+}
+void launchBicgXR(hipStream_t s, double* red, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv, const double* t,
+                  const double* rh, double* x, double* r) {
+    const int g = redGrid(n);
+    hipLaunchKernelGGL(kBicgXR, dim3(g), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, rh, x, r, red);
+    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 2, g, red, nullptr, int(kEpRnRho), ks);
     checkLaunch();
 }
 void launchBicgXR(hipStream_t s, Ctx& c, long long n, KScalars* ks, const double* ph, const double* sh, const double* sv, const double* t,
                   const double* rh, double* x, double* r) {
-    const int g = redGrid(n);
-    hipLaunchKernelGGL(kBicgXR, dim3(g), dim3(kBlock), 0, s, n, ks, ph, sh, sv, t, rh, x, r, c.red.p);
-    hipLaunchKernelGGL(kReduceFinal, dim3(1), dim3(kBlock), 0, s, 2, g, c.red.p, nullptr, int(kEpRnRho), ks);
-    checkLaunch();
+    launchBicgXR(s, c.red.p, n, ks, ph, sh, sv, t, rh, x, r);
 }
 void launchAxpy(hipStream_t s, long long n, double a, const double* x, double* y) {
     hipLaunchKernelGGL(kAxpy, dim3(gridFor(n, kBlock * 4)), dim3(kBlock), 0, s, n, a, x, y);
