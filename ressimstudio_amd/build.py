@@ -2,6 +2,8 @@
 
   libressim_hip.so   the product: HIP kernels for gfx950 + host set-up + C-ABI (hipcc)
   libressim_gen.so   the seeded workload generator and engine-side well Schur reduction (g++, OpenMP)
+  host_check         the host set-up (level schedules, colouring, slices, AMG set-up, well Schur plan) under the host
+                     AddressSanitizer / UndefinedBehaviorSanitizer (g++; tests/test_host_sanitized.py runs it)
 """
 from __future__ import annotations
 
@@ -34,9 +36,20 @@ def build_gen():
     _run([cxx, "-std=c++17", "-O2", "-fopenmp", "-fPIC", "-shared", "-o", "libressim_gen.so", "csrc/gen/rs_gen.cpp"])
 
 
+# the host set-up alone (no device code) under the host sanitizers: g++, never hipcc
+HOST_CHECK_FLAGS = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include"]
+
+
+def build_host_check():
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, *HOST_CHECK_FLAGS, "-o", "host_check", "csrc/host/host_check.cpp", "csrc/hip/rsh_setup.cpp"])
+
+
 def build_all():
     build_gen()
     build_hip()
+    build_host_check()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,205 @@
+// The library's host-side set-up, built with g++ on its own (no device code) so that it runs under the host
+// sanitizers (AddressSanitizer, UndefinedBehaviorSanitizer): level schedules, colouring and the colour-permuted
+// pattern, row-slice layouts and level runs (csrc/hip/rsh_setup.cpp), the aggregation AMG set-up with its Galerkin maps,
+// and the well Schur plan (csrc/host/well_plan.hpp).  Every result is also checked for what it must satisfy.
+//
+// usage: host_check <systems file>
+// systems file: int32 count; per system int32 n, nb, row_ptr[n+1], col[nnzb], float64 val[nnzb nb nb], then int32
+// has_jacobian and, when 1, the scalar Jacobian it was reduced from: int32 rows, na, nv, nw, row_ptr[rows+1],
+// col[nnz], float64 val[nnz].  Prints one line per system and exits non-zero on the first failed check.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../hip/rsh_internal.hpp"
+#include "well_plan.hpp"
+
+using namespace rsh;
+
+namespace {
+struct Input {
+    int n = 0, nb = 1;
+    std::vector<int> rp, ci;
+    std::vector<double> val;
+    bool jac = false;
+    int rows = 0, na = 0, nv = 0, nw = 0;
+    std::vector<int> jrp, jci;
+    std::vector<double> jval;
+};
+
+FILE* g_in = nullptr;
+bool rd(void* p, size_t bytes) { return bytes == 0 || std::fread(p, 1, bytes, g_in) == bytes; }
+
+int fails = 0;
+void check(bool ok, const char* what, int sys) {
+    if (ok) return;
+    std::printf("system %d: FAILED %s\n", sys, what);
+    ++fails;
+}
+
+void run(const Input& in, int id) {
+    HostBsr A;
+    A.n = in.n;
+    A.nb = in.nb;
+    A.rp = in.rp;
+    A.ci = in.ci;
+    A.diag.assign(size_t(A.n), -1);
+    for (int i = 0; i < A.n; ++i)
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1]; ++p)
+            if (A.ci[size_t(p)] == i) A.diag[size_t(i)] = p;
+    // level schedule: a permutation in which every coupled pair keeps its order and sits on different levels
+    std::vector<int> ord, start;
+    levelOrder(A, ord, start);
+    std::vector<int> level(size_t(A.n), -1);
+    for (int l = 0; l + 1 < int(start.size()); ++l)
+        for (int w = start[size_t(l)]; w < start[size_t(l) + 1]; ++w) level[size_t(ord[size_t(w)])] = l;
+    bool okLev = int(ord.size()) == A.n;
+    for (int i = 0; i < A.n && okLev; ++i) {
+        okLev = level[size_t(i)] >= 0;
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1] && okLev; ++p) {
+            const int j = A.ci[size_t(p)];
+            if (j < i) okLev = level[size_t(j)] < level[size_t(i)];
+        }
+    }
+    check(okLev, "level order", id);
+    // colouring: neighbours differ; perm lists the rows colour by colour
+    std::vector<int> colour, perm, cstart;
+    const int ncol = colourGraph(A, colour, perm, cstart);
+    bool okCol = int(perm.size()) == A.n && int(cstart.size()) == ncol + 1 && cstart.back() == A.n;
+    for (int i = 0; i < A.n && okCol; ++i)
+        for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1] && okCol; ++p)
+            if (A.ci[size_t(p)] != i) okCol = colour[size_t(i)] != colour[size_t(A.ci[size_t(p)])];
+    for (int c = 0; c < ncol && okCol; ++c)
+        for (int k = cstart[size_t(c)]; k < cstart[size_t(c) + 1] && okCol; ++k) okCol = colour[size_t(perm[size_t(k)])] == c;
+    check(okCol, "colouring", id);
+    HostBsr P;
+    std::vector<int> bmap;
+    permutePattern(A, perm, P, bmap);
+    bool okPerm = P.n == A.n && P.rp.back() == A.rp.back() && int(bmap.size()) == A.rp.back();
+    for (int i = 0; i < P.n && okPerm; ++i)
+        for (int p = P.rp[size_t(i)] + 1; p < P.rp[size_t(i) + 1] && okPerm; ++p) okPerm = P.ci[size_t(p) - 1] < P.ci[size_t(p)];
+    check(okPerm, "permuted pattern", id);
+    // slices of the lower and upper entries in level order, as the sweeps build them
+    std::vector<int> iperm(size_t(A.n));
+    for (int w = 0; w < A.n; ++w) iperm[size_t(ord[size_t(w)])] = w;
+    RowEntries EL, EU;
+    EL.ptr.assign(1, 0);
+    EU.ptr.assign(1, 0);
+    std::vector<int> lenL, lenU;
+    for (int w = 0; w < A.n; ++w) {
+        const int i = ord[size_t(w)];
+        for (int p = A.rp[size_t(i)]; p < A.diag[size_t(i)]; ++p) { EL.col.push_back(iperm[size_t(A.ci[size_t(p)])]); EL.src.push_back(p); }
+        for (int p = A.diag[size_t(i)] + 1; p < A.rp[size_t(i) + 1]; ++p) { EU.col.push_back(iperm[size_t(A.ci[size_t(p)])]); EU.src.push_back(p); }
+        EL.ptr.push_back(int(EL.col.size()));
+        EU.ptr.push_back(int(EU.col.size()));
+        EU.dsrc.push_back(A.diag[size_t(i)]);
+        lenL.push_back(EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)]);
+        lenU.push_back(EU.ptr[size_t(w) + 1] - EU.ptr[size_t(w)]);
+    }
+    EL.orig = ord;
+    EU.orig = ord;
+    SliceHost hL, hU;
+    LevelSched sL, sU;
+    buildSlices(EL, start, rowCap(lenL), hL, sL);
+    buildSlices(EU, start, rowCap(lenU), hU, sU);
+    long long placed = 0;
+    for (int v : hL.src) placed += v >= 0;
+    for (int v : hL.lsrc) placed += v >= 0;
+    check(placed == (long long)EL.col.size() && sL.levels() == int(start.size()) - 1, "lower slices hold every entry once", id);
+    std::vector<int> work(size_t(sL.levels())), rows(size_t(sL.levels()));
+    for (int l = 0; l < sL.levels(); ++l) {
+        work[size_t(l)] = sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)] + sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)];
+        rows[size_t(l)] = start[size_t(l) + 1] - start[size_t(l)];
+    }
+    const std::vector<LevelRun> runs = levelRuns(work, 16, &rows, kRunRowsMax);
+    bool okRuns = !runs.empty() && runs.front().l0 == 0 && runs.back().l1 == sL.levels();
+    for (size_t k = 1; k < runs.size() && okRuns; ++k) okRuns = runs[k].l0 == runs[k - 1].l1;
+    check(okRuns, "level runs cover the levels in order", id);
+    // the pressure-pressure coefficients as a scalar matrix: AMG set-up (strength, aggregation, Galerkin maps)
+    HostCsr C;
+    C.n = A.n;
+    C.rp = A.rp;
+    C.ci = A.ci;
+    for (int p = 0; p < A.rp.back(); ++p) C.v.push_back(in.val[size_t(p) * A.nb * A.nb]);
+    std::vector<int> fo, fs, bo, bs;
+    csrLevels(C, true, fo, fs);
+    csrLevels(C, false, bo, bs);
+    check(int(fo.size()) == C.n && int(bo.size()) == C.n, "Gauss-Seidel level orders", id);
+    std::vector<AmgLevelHost> H;
+    amgSetup(C, H);
+    bool okAmg = !H.empty() && H.front().A.n == C.n;
+    for (size_t l = 0; l + 1 < H.size() && okAmg; ++l) {
+        okAmg = H[l].nc == H[l + 1].A.n && H[l + 1].A.n < H[l].A.n && int(H[l].agg.size()) == H[l].A.n;
+        for (int a : H[l].agg) okAmg = okAmg && a >= 0 && a < H[l].nc;
+        okAmg = okAmg && int(H[l].cPtr.size()) == H[l + 1].A.rp.back() + 1;
+    }
+    check(okAmg, "AMG hierarchy", id);
+    // the well Schur plan of the Jacobian the system was reduced from: the same pattern and values
+    int fill = 0;
+    if (in.jac) {
+        const double* jv = in.jval.data();
+        wellplan::Plan W;
+        std::string why = wellplan::split(in.rows, in.jrp.data(), in.jci.data(), in.na, in.nv, in.nw, [jv](int p) { return jv[p] != 0.0; }, W);
+        check(why.empty(), "well plan split", id);
+        wellplan::pattern(in.rows, in.jrp.data(), in.jci.data(), W);
+        check(W.sPtr == in.rp && W.sCol == in.ci, "well plan pattern equals the reduced system's", id);
+        std::vector<double> dInv;
+        why = wellplan::wellInverses(W, jv, dInv);
+        check(why.empty(), "well inverses", id);
+        std::vector<double> s(W.sCol.size() * size_t(in.nv) * in.nv);
+        wellplan::reduceValues(in.jrp.data(), in.jci.data(), jv, W, dInv, s.data());
+        check(s == in.val, "well plan values equal the reduced system's", id);
+        wellplan::ValueMap M;
+        check(wellplan::valueMap(in.jrp.data(), in.jci.data(), W, M).empty() && M.cellSource.size() == s.size(), "well value map", id);
+        std::vector<double> b(size_t(in.rows)), bc(size_t(W.nc)), x(size_t(in.rows));
+        for (int r = 0; r < in.rows; ++r) b[size_t(r)] = std::cos(0.1 * r);
+        wellplan::reduceRhs(W, jv, dInv, b.data(), bc.data());
+        wellplan::recover(W, in.jci.data(), jv, dInv, b.data(), bc.data(), x.data());
+        fill = W.fill();
+    }
+    std::printf("system %d: n %d nb %d levels %d colours %d L slices %d AMG levels %zu well fill %d\n", id, A.n, A.nb, sL.levels(), ncol, hL.nslices,
+                H.size(), fill);
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2 || !(g_in = std::fopen(argv[1], "rb"))) {
+        std::fprintf(stderr, "usage: host_check <systems file>\n");
+        return 2;
+    }
+    int count = 0;
+    if (!rd(&count, 4)) return 2;
+    for (int k = 0; k < count; ++k) {
+        Input in;
+        int hdr[2];
+        if (!rd(hdr, 8)) return 2;
+        in.n = hdr[0];
+        in.nb = hdr[1];
+        in.rp.resize(size_t(in.n) + 1);
+        if (!rd(in.rp.data(), 4 * in.rp.size())) return 2;
+        in.ci.resize(size_t(in.rp.back()));
+        in.val.resize(in.ci.size() * size_t(in.nb) * in.nb);
+        int has = 0;
+        if (!rd(in.ci.data(), 4 * in.ci.size()) || !rd(in.val.data(), 8 * in.val.size()) || !rd(&has, 4)) return 2;
+        if (has) {
+            int jh[4];
+            if (!rd(jh, 16)) return 2;
+            in.jac = true;
+            in.rows = jh[0];
+            in.na = jh[1];
+            in.nv = jh[2];
+            in.nw = jh[3];
+            in.jrp.resize(size_t(in.rows) + 1);
+            if (!rd(in.jrp.data(), 4 * in.jrp.size())) return 2;
+            in.jci.resize(size_t(in.jrp.back()));
+            in.jval.resize(in.jci.size());
+            if (!rd(in.jci.data(), 4 * in.jci.size()) || !rd(in.jval.data(), 8 * in.jval.size())) return 2;
+        }
+        run(in, k);
+    }
+    std::fclose(g_in);
+    std::printf("%d failed checks\n", fails);
+    return fails ? 1 : 0;
+}
