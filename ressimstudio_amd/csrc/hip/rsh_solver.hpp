@@ -927,7 +927,8 @@ struct rsh_system {
 
     // Preconditioner applies are captured once per (input, output, precision) into a HIP graph and replayed: a
     // level-scheduled apply is hundreds of small dependent launches, which the graph issues without host work.
-    std::map<std::tuple<const void*, const void*, int>, hipGraphExec_t> graphs;
+    // captured applies keyed by their vectors and every option their kernels bake in (weights and step counts too)
+    std::map<std::tuple<const void*, const void*, int, double, int>, hipGraphExec_t> graphs;
     std::pair<int, int> graphGen{-1, -1};
     bool useGraphs = [] { const char* e = std::getenv("RSH_GRAPHS"); return !(e && e[0] == '0'); }();
     void clearGraphs() {
@@ -941,7 +942,8 @@ struct rsh_system {
         if (gen != graphGen) { clearGraphs(); graphGen = gen; }
         const auto key = std::make_tuple(static_cast<const void*>(rr), static_cast<const void*>(z),
                                          (f32 ? 1 : 0) | (opt.precond << 1) | (opt.smoother << 5) | (amgF32(f32) ? 1 << 6 : 0) |
-                                             (opt.smooth_steps << 7));
+                                             (opt.smooth_steps << 7),
+                                         cpr ? cpr->omega : 0.0, cpr ? cpr->coarseSteps : 0);
         auto it = graphs.find(key);
         if (it == graphs.end()) {
             hipGraph_t g = nullptr;
