@@ -463,8 +463,16 @@ def main():
     head = names[0]
     sysd.setup(cfgs[head])
     k = {}
-    for kind, name in [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps"), (3, "amg_vcycle"), (1, "cpr_apply")]:
-        ms, by = sysd.time_kernel(kind, 10 if kind in (1, 2, 3) else 50)
+    kinds = [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps"), (3, "amg_vcycle"), (1, "cpr_apply")]
+    if cfgs[head].smoother == J:
+        kinds += [(5, "smoother_l0"), (6, "smoother_l1")]   # one damped-Jacobi step of AMG level 0 / level 1
+    for kind, name in kinds:
+        try:
+            ms, by = sysd.time_kernel(kind, 10 if kind in (1, 2, 3) else 50)
+        except rs.RshError:
+            if kind in (0, 2, 4):
+                raise
+            continue   # no AMG (or no Jacobi-smoothed AMG) in this preconditioner
         k[name] = {"ms": ms, "bytes": by, "GBs": (by / (ms * 1e-3) / 1e9) if by > 0 else None}
         log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {k[name]['GBs']:.0f} GB/s" if by > 0 else ""))
     its_cpr = results[head]["iterations"]
@@ -476,6 +484,10 @@ def main():
             "algorithmic_bytes": k[dom]["bytes"], "ilu_apply_GBs": k["ilu_sweeps"]["GBs"],
             "ilu_apply_frac": k["ilu_sweeps"]["GBs"] / HBM_PEAK_GBS, "ilu_apply_bytes": k["ilu_sweeps"]["bytes"],
             "copy_GBs": k["copy"]["GBs"]}
+    for name in ("smoother_l0", "smoother_l1"):
+        if name in k:
+            roof[name] = {"us": k[name]["ms"] * 1e3, "algorithmic_bytes": k[name]["bytes"], "GBs": k[name]["GBs"],
+                          "frac": k[name]["GBs"] / HBM_PEAK_GBS}
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this workload (profiles/)
     tfile = os.path.join(ROOT, "profiles", "traffic_cpr_mc_1.1M.json")
     if args.scale == 1.0 and head == "cpr_mc" and os.path.exists(tfile):

@@ -159,7 +159,8 @@ int rsh_memcpy_d2h(rsh_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- measurement: average duration (ms) of `reps` back-to-back launches of one kernel class on the context's
  * stream, bracketed by HIP events.  kind: 0 BSR SpMV (precision of the system setup), 1 preconditioner apply,
- * 2 ILU forward+backward sweeps, 3 AMG V-cycle, 4 copy kernel of `bytes` (bandwidth calibration). */
+ * 2 ILU forward+backward sweeps, 3 AMG V-cycle, 4 copy kernel of `bytes` (bandwidth calibration), 5 / 6 one
+ * damped-Jacobi smoothing step of AMG level 0 / level 1. */
 int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double* bytes_per_launch);
 
 /* ---- well Schur reduction on the device (replaces the host-side WellSchur::build / reduceRhs / recover,

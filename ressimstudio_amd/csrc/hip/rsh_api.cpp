@@ -471,44 +471,8 @@ int rsh_system_create(rsh_context* ctx, int n, int nb, const int* row_ptr, const
         S->ci.upload(S->pat.ci, st);
         S->diag.upload(S->pat.diag, st);
         S->val.alloc(size_t(nnzb) * nb * nb);
-        // SpMV slices over all rows in natural order; padding points at the lane's own row
-        RowEntries E;
-        E.ptr.assign(S->pat.rp.begin(), S->pat.rp.end());
-        E.col = S->pat.ci;
-        E.src.resize(size_t(nnzb));
-        for (int p = 0; p < nnzb; ++p) E.src[size_t(p)] = p;
-        E.orig.resize(size_t(n));
-        for (int i = 0; i < n; ++i) E.orig[size_t(i)] = i;
-        std::vector<int> len(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i) len[size_t(i)] = row_ptr[i + 1] - row_ptr[i];
-        LevelSched one;
-        buildSlices(E, {0, n}, rowCap(len), S->spmvHost, one);
-        for (int sl = 0; sl < S->spmvHost.nslices; ++sl)
-            for (int k = 0; k < S->spmvHost.width[size_t(sl)]; ++k)
-                for (int lane = 0; lane < kSliceRows; ++lane) {
-                    const size_t at = (size_t(S->spmvHost.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
-                    if (S->spmvHost.src[at] < 0) {
-                        const int rr = S->spmvHost.row[size_t(sl) * kSliceRows + size_t(lane)];
-                        S->spmvHost.col[at] = rr >= 0 ? rr : 0;
-                    }
-                }
-        {
-            // the columns relative to the lane's row fit 16 bits when no slice row reaches further than 32767 block
-            // rows (the 7-point stencil of grids up to nx * ny = 32767 cells per layer; far couplings are long rows)
-            SliceHost& h = S->spmvHost;
-            std::vector<short> d(h.col.size());
-            bool fits = true;
-            for (int sl = 0; sl < h.nslices && fits; ++sl)
-                for (int k = 0; k < h.width[size_t(sl)] && fits; ++k)
-                    for (int lane = 0; lane < kSliceRows; ++lane) {
-                        const size_t at = (size_t(h.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
-                        const int rr = h.row[size_t(sl) * kSliceRows + size_t(lane)];
-                        const long long delta = (long long)h.col[at] - (rr >= 0 ? rr : -1);
-                        if (delta < -32768 || delta > 32767) { fits = false; break; }
-                        d[at] = short(delta);
-                    }
-            if (fits) h.dcol = std::move(d);
-        }
+        // SpMV slices over all rows in natural order
+        rowSliceLayout(n, S->pat.rp, S->pat.ci, S->spmvHost);
         uploadSlices(S->spmvHost, S->spmv64, nb, st);
         RSH_HIP(hipStreamSynchronize(st));
         *out = S.release();
@@ -813,6 +777,11 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << (S.cpr->lv[l]->gsFs.size() - 1);
             o << "],\"amg_sliced\":[";
             for (size_t l = 0; l < S.cpr->lv.size(); ++l) o << (l ? "," : "") << (S.cpr->lv[l]->plan ? 1 : 0);
+            o << "],\"amg_slices\":[";
+            for (size_t l = 0; l < S.cpr->lv.size(); ++l) {
+                const SliceDev<double>* sl = S.cpr->lv[l]->plan;
+                o << (l ? "," : "") << "[" << (sl ? sl->nslices : 0) << "," << (sl ? sl->nlong : 0) << "," << (sl && sl->dcol.n ? 1 : 0) << "]";
+            }
             o << "],\"coarse_dense\":" << (S.cpr->dense ? 1 : 0) << ",\"cpr_bytes\":" << S.cpr->bytes();
         }
         o << ",\"last_fallback_fp64\":" << (S.lastFallback ? 1 : 0) << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
@@ -861,7 +830,9 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
         if (!sys || !avg_ms || reps < 1) return fail(RSH_ERR_ARG, "bad argument");
         rsh_system& S = *sys;
         if ((kind == 1 || kind == 2 || kind == 3) && !S.ready) return fail(RSH_ERR_ARG, "call rsh_system_setup first");
-        if (kind == 3 && !S.cpr) return fail(RSH_ERR_ARG, "no AMG");
+        if ((kind == 3 || kind == 5 || kind == 6) && !S.cpr) return fail(RSH_ERR_ARG, "no AMG");
+        if ((kind == 5 || kind == 6) && (!S.ready || S.cpr->smoother != RSH_SMOOTH_JACOBI || S.cpr->lv.size() < size_t(kind - 3)))
+            return fail(RSH_ERR_ARG, "no Jacobi-smoothed AMG level " + std::to_string(kind - 5));
         RSH_HIP(hipSetDevice(S.ctx->c.device));
         hipStream_t st = S.s();
         S.ensureVec(S.vb);
@@ -884,6 +855,8 @@ int rsh_time_kernel(rsh_system* sys, int kind, int reps, double* avg_ms, double*
                     break;
                 case 3: S.cpr->vcycle(st, S.amgF32(f32)); break;
                 case 4: launchCopyKernel(st, S.vb.p, S.vx.p, S.N); break;
+                case 5:
+                case 6: bytes = S.cpr->smoothStep(size_t(kind - 5), st, S.amgF32(f32)); break;
                 default: throw Error(RSH_ERR_ARG, "unknown kernel kind");
             }
         };

@@ -116,6 +116,8 @@ struct SliceDev {
     DBuf<int> lrow, lorig, lptr, lcol, lsrc, ldsrc;
     DBuf<TA> lval, ldval;
     DBuf<int> owner, lowner;
+    // coarse AMG levels: the rows' sums are formed in G chains folded like the G-lane CSR step (0: one sum per row)
+    int chains = 0;
     size_t bytes() const {
         return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + dcol.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
                lrow.bytes() + lorig.bytes() + lptr.bytes() + lcol.bytes() + lsrc.bytes() + ldsrc.bytes() + lval.bytes() + ldval.bytes();
@@ -161,7 +163,14 @@ struct RowEntries {
     std::vector<int> orig;   // working row -> original row
     std::vector<int> dsrc;   // working row -> diagonal block (or empty)
 };
-void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L);
+// longRow (optional, per working row): rows taken as long rows whatever their length
+void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L,
+                 const std::vector<char>* longRow = nullptr);
+// one-level row slices of a CSR / BSR pattern in natural row order: rows longer than the bulk are long rows, padding
+// gathers the lane's own row, and the 16-bit relative columns are kept when every slice row fits them.  coarse (an
+// AMG level): the bulk ends a quarter above the 99th percentile of the row lengths, and rows with a column beyond
+// 16-bit reach are long rows too
+void rowSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, SliceHost& h, bool coarse = false);
 
 // AMG set-up on the host (AMG::setup, amg.cpp:137-252): returns the hierarchy
 struct AmgLevelHost {
@@ -270,6 +279,8 @@ void launchJacobiInit(hipStream_t s, int n, bool f32, const void* dinv, const do
 // sl != null: level 0 through the SpMV slices (values sval / lval); else CSR (rp, ci, v) with meanRow entries per row
 // f32: level values (sval / lval / v) and dinv stored in FP32 (vectors and sums FP64).  pfine (CSR levels): also
 // pfine[j] += out0[a] for the members j of aggregate a (prp / pci), the prolongation into the finer level's iterate
+// lanes per CSR row of the coarse-level kernels for a mean row length (a power of two from 4 to 64)
+int groupLanes(double mean);
 void launchJacobiFused(hipStream_t s, int mode, bool f32, const SliceDev<double>* sl, const void* sval, const void* lval, int n, const int* rp,
                        const int* ci, const void* v, const void* dinv, const double* b, const double* x1, double omega, double* out0,
                        double meanRow, const int* prp = nullptr, const int* pci = nullptr, double* pfine = nullptr);

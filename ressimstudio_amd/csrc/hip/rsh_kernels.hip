@@ -1658,6 +1658,65 @@ __global__ __launch_bounds__(kBlock) void kJacobiSliced(int nslices, const int2*
     acc = waveSum(acc);
     if (lane == 0) out0[i] = jacValue<MODE>(acc, omega, self);
 }
+// coarse levels in row slices (one row per lane), with the arithmetic of the G-lane CSR step below: entry k of a row
+// goes to chain k mod G (the sum lane k mod G forms there), and the G chains are folded in the order of the
+// groupSum butterfly (offsets G/2, G/4, ..., 1), so the result is bit-identical to kJacobiCsr<MODE, G>.  Padding
+// slots add 0 * x_own.  Four slots per round are in flight before their products; long rows take G lanes each as
+// in kJacobiCsr (64 / G of them per wavefront).
+template <int MODE, int G, typename TA, typename CT>
+__global__ __launch_bounds__(kBlock) void kJacobiSlicedG(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
+                                                         const CT* __restrict__ col, const TA* __restrict__ val, int nlong,
+                                                         const int* __restrict__ lrow, const int* __restrict__ lptr, const int* __restrict__ lcol,
+                                                         const TA* __restrict__ lval, const TA* __restrict__ dinv,
+                                                         const double* __restrict__ b, const double* __restrict__ x1, double omega,
+                                                         double* __restrict__ out0) {
+    static_assert(G % 4 == 0 && G <= kWave, "chains in rounds of four slots");
+    const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid < nslices) {
+        const int i = row[(long long)wid * kWave + lane];
+        if (i < 0) return;
+        const JacSelf self = jacSelf<MODE>(i, dinv, b, x1);
+        const int2 inf = info[wid];
+        double a[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) a[j] = 0;
+        for (int kb = 0; kb < inf.x; kb += G) {
+#pragma unroll
+            for (int r = 0; r < G; r += 4) {
+                if (kb + r >= inf.x) break;
+                const long long at = ((long long)inf.y + kb + r) * kWave + lane;
+                double v[4], xv[4];
+                int c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool in = kb + r + u < inf.x;
+                    v[u] = in ? double(val[at + u * kWave]) : 0.0;
+                    c[u] = in ? slotCol<CT, false>(col, at + u * kWave, i) : i;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xv[u] = x1[c[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (kb + r + u < inf.x) a[r + u] += v[u] * xv[u];
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int j = 0; j < o; ++j) a[j] = a[j] + a[j + o];
+        out0[i] = jacValue<MODE>(a[0], omega, self);
+        return;
+    }
+    const int q = (wid - nslices) * (kWave / G) + lane / G, sub = lane % G;
+    if (q >= nlong) return;                  // whole groups leave together
+    const int i = lrow[q];
+    const JacSelf self = jacSelf<MODE>(i, dinv, b, x1);
+    double acc = 0;
+    for (int p = lptr[q] + sub; p < lptr[q + 1]; p += G) acc += double(lval[p]) * x1[lcol[p]];
+    acc = groupSum<G>(acc);
+    if (sub == 0) out0[i] = jacValue<MODE>(acc, omega, self);
+}
 // coarse levels (CSR): G lanes per row (G a power of two near the mean row length), the row's entries strided over
 // the group so their loads are issued together, then a shuffle reduction in a fixed order.  With pfine given (the
 // last post-smoothing step of a coarse level) the group also adds the new value to the finer level's iterate at
@@ -2465,12 +2524,38 @@ void jacobiSlicedT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, co
         hipLaunchKernelGGL((kJacobiSliced<MODE, TA, int>), grid, dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
                            sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
 }
+template <int MODE, int G, typename TA>
+void jacobiSlicedGT(hipStream_t s, const SliceDev<double>* sl, const TA* sval, const TA* lval, const TA* dinv, const double* b,
+                    const double* x1, double omega, double* out0) {
+    const int waves = sl->nslices + (sl->nlong + kWave / G - 1) / (kWave / G);
+    if (!waves) return;
+    const dim3 grid((waves + 3) / 4);
+    if (sl->dcol.n)
+        hipLaunchKernelGGL((kJacobiSlicedG<MODE, G, TA, short>), grid, dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->dcol.p, sval,
+                           sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
+    else
+        hipLaunchKernelGGL((kJacobiSlicedG<MODE, G, TA, int>), grid, dim3(kBlock), 0, s, sl->nslices, sl->info.p, sl->row.p, sl->col.p, sval,
+                           sl->nlong, sl->lrow.p, sl->lptr.p, sl->lcol.p, lval, dinv, b, x1, omega, out0);
+}
 template <typename TA>
 void jacobiFusedT(hipStream_t s, int mode, const SliceDev<double>* sl, const TA* sval, const TA* lval, int n, const int* rp, const int* ci,
                   const TA* v, const TA* dinv, const double* b, const double* x1, double omega, double* out0, double meanRow,
                   const int* prp, const int* pci, double* pfine) {
     if (sl) {
-        if (pfine) throw Error(-1, "the sliced level-0 Jacobi step has no coarser level to prolong from");
+        if (pfine) throw Error(-1, "a sliced Jacobi step does not prolong: its caller adds P x");
+        if (sl->chains) {
+            if (sl->chains != 8 && sl->chains != 16 && sl->chains != 32) throw Error(-1, "sliced coarse level with an unsupported lane count");
+            if (mode == 0) {
+                if (sl->chains == 8) jacobiSlicedGT<0, 8, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+                else if (sl->chains == 16) jacobiSlicedGT<0, 16, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+                else jacobiSlicedGT<0, 32, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+            } else {
+                if (sl->chains == 8) jacobiSlicedGT<1, 8, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+                else if (sl->chains == 16) jacobiSlicedGT<1, 16, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+                else jacobiSlicedGT<1, 32, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
+            }
+            return;
+        }
         if (mode == 0) jacobiSlicedT<0, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
         else jacobiSlicedT<1, TA>(s, sl, sval, lval, dinv, b, x1, omega, out0);
         return;

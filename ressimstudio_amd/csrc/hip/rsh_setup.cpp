@@ -159,7 +159,7 @@ int rowCap(const std::vector<int>& lengths) {
     return mx <= cap + cap / 2 + 1 ? mx : cap;   // no long rows unless some rows are well beyond the bulk
 }
 
-void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L) {
+void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L, const std::vector<char>* longRow) {
     const int nl = int(levelStart.size()) - 1;
     const int n = int(E.ptr.size()) - 1;
     const bool withDiag = !E.dsrc.empty();
@@ -172,7 +172,7 @@ void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int ca
         shortRows.clear();
         for (int w = levelStart[size_t(l)]; w < levelStart[size_t(l) + 1]; ++w) {
             const int len = E.ptr[size_t(w) + 1] - E.ptr[size_t(w)];
-            if (len > cap) {
+            if (len > cap || (longRow && (*longRow)[size_t(w)])) {
                 S.lrow.push_back(w);
                 S.lorig.push_back(E.orig[size_t(w)]);
                 for (int p = E.ptr[size_t(w)]; p < E.ptr[size_t(w) + 1]; ++p) { S.lcol.push_back(E.col[size_t(p)]); S.lsrc.push_back(E.src[size_t(p)]); }
@@ -210,6 +210,57 @@ void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int ca
         L.sStart[size_t(l) + 1] = S.nslices;
         L.qStart[size_t(l) + 1] = int(S.lrow.size());
     }
+}
+
+void rowSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, SliceHost& h, bool coarse) {
+    std::vector<int> len(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) len[size_t(i)] = rp[size_t(i) + 1] - rp[size_t(i)];
+    int cap = rowCap(len);
+    std::vector<char> longRow;
+    if (coarse) {
+        // rows beyond the bulk (a quarter above the 99th percentile: the aggregates of perforated cells) and rows
+        // reaching further than 16-bit offsets are long rows
+        std::vector<int> srt(len);
+        std::sort(srt.begin(), srt.end());
+        const int p99 = n ? srt[size_t(std::floor(0.99 * double(n - 1)))] : 1;
+        cap = std::max(4, p99 + (p99 + 3) / 4);
+        longRow.assign(size_t(n), 0);
+        for (int i = 0; i < n; ++i)
+            for (int p = rp[size_t(i)]; p < rp[size_t(i) + 1]; ++p)
+                if (std::abs((long long)ci[size_t(p)] - i) > 32767) { longRow[size_t(i)] = 1; break; }
+    }
+    RowEntries E;
+    E.ptr.assign(rp.begin(), rp.end());
+    E.col = ci;
+    E.src.resize(ci.size());
+    for (size_t p = 0; p < ci.size(); ++p) E.src[p] = int(p);
+    E.orig.resize(size_t(n));
+    for (int i = 0; i < n; ++i) E.orig[size_t(i)] = i;
+    LevelSched one;
+    buildSlices(E, {0, n}, cap, h, one, longRow.empty() ? nullptr : &longRow);
+    // padding points at the lane's own row (idle lanes at row 0): every gather stays inside the vector
+    for (int sl = 0; sl < h.nslices; ++sl)
+        for (int k = 0; k < h.width[size_t(sl)]; ++k)
+            for (int lane = 0; lane < kSliceRows; ++lane) {
+                const size_t at = (size_t(h.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
+                if (h.src[at] < 0) {
+                    const int rr = h.row[size_t(sl) * kSliceRows + size_t(lane)];
+                    h.col[at] = rr >= 0 ? rr : 0;
+                }
+            }
+    // the columns relative to the lane's row fit 16 bits when no slice row reaches further than 32767 rows (the
+    // 7-point stencil of grids up to nx * ny = 32767 cells per layer; far couplings are long rows)
+    std::vector<short> d(h.col.size());
+    for (int sl = 0; sl < h.nslices; ++sl)
+        for (int k = 0; k < h.width[size_t(sl)]; ++k)
+            for (int lane = 0; lane < kSliceRows; ++lane) {
+                const size_t at = (size_t(h.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
+                const int rr = h.row[size_t(sl) * kSliceRows + size_t(lane)];
+                const long long delta = (long long)h.col[at] - (rr >= 0 ? rr : -1);
+                if (delta < -32768 || delta > 32767) return;
+                d[at] = short(delta);
+            }
+    h.dcol = std::move(d);
 }
 
 // ------------------------------------------------------------------ aggregation AMG set-up (amg.cpp:137-252)

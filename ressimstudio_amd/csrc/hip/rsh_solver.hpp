@@ -567,6 +567,8 @@ struct AmgLevelDev {
     // pattern); coarse levels (nullptr) stay CSR with G lanes per row, which keeps more loads in flight on a few
     // hundred thousand rows than one row per lane does
     const SliceDev<double>* plan = nullptr;
+    SliceHost planHost;                  // a coarse level's own row-slice layout (when it has one)
+    std::unique_ptr<SliceDev<double>> ownPlan;
     DBuf<double> pS, pL;
     DBuf<float> pS32, pL32;
     // repack the slice values from the CSR values v (after every numeric set-up)
@@ -604,6 +606,10 @@ struct CprDev {
     const SliceDev<double>* plan = nullptr;
     bool have32 = false;
     std::vector<int> patRp, patCi;        // pattern of the hierarchy (reuse check)
+    // coarse levels of at least this many rows are smoothed in row slices (one row per lane, which reads each
+    // slot's values and columns as whole cache lines; smaller levels have too few wavefronts for it).  The sums are
+    // the same either way; RSH_AMG_SLICE_ROWS sets another threshold (tests compare the two layouts)
+    int sliceLevelRows = [] { const char* e = std::getenv("RSH_AMG_SLICE_ROWS"); return e ? std::max(1, std::atoi(e)) : 100000; }();
 
     int generation = 0;
     void buildHierarchy(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* apHost, hipStream_t s) {
@@ -631,6 +637,15 @@ struct CprDev {
                 L->rp = L->rpB.p; L->ci = L->ciB.p; L->diag = L->diagB.p;
             }
             L->v.upload(h.A.v, s);
+            const int lanes = groupLanes(double(L->nnz) / std::max(L->n, 1));
+            if (l > 0 && !h.agg.empty() && h.A.n >= sliceLevelRows && lanes >= 8 && lanes <= 32) {
+                // same sums as the G-lane CSR step (bit-identical), read through row slices
+                rowSliceLayout(h.A.n, h.A.rp, h.A.ci, L->planHost, true);
+                L->ownPlan = std::make_unique<SliceDev<double>>();
+                uploadSlices(L->planHost, *L->ownPlan, 1, s);
+                L->ownPlan->chains = lanes;
+                L->plan = L->ownPlan.get();
+            }
             if (!h.agg.empty()) {
                 L->agg.upload(h.agg, s);
                 std::vector<int> rrp(size_t(h.nc) + 1, 0), rci(h.agg.size());
@@ -783,6 +798,7 @@ struct CprDev {
             // residual; the coarse level's last kernel adds P x_c to the pre-smoothed iterate; the post-smoothing
             // steps ping-pong between it and L.x2, the last one adding P x_l to the finer level's iterate
             const SliceDev<double>* sl = L.plan;
+            if (sl) pfine = nullptr;   // a sliced level's caller adds its prolongation (the same one add per fine row)
             const double mean = double(L.nnz) / std::max(L.n, 1);
             const bool h = use32(f32);
             const void* sv = h ? static_cast<const void*>(L.pS32.p) : static_cast<const void*>(L.pS.p);
@@ -823,6 +839,20 @@ struct CprDev {
     const void* dinvOf(size_t l, bool f32) const {
         return use32(f32) ? static_cast<const void*>(lv[l]->dinv32.p) : static_cast<const void*>(lv[l]->dinv.p);
     }
+    // one damped-Jacobi step of level l (x1 -> x3), the kernel the V-cycle smooths with; returns its algorithmic bytes:
+    // values and column indices per entry, the row pointers, and D^-1, b, the iterate and the new iterate per row
+    double smoothStep(size_t l, hipStream_t s, bool f32) {
+        if (l + 1 >= lv.size()) throw Error(RSH_ERR_ARG, "no smoothed level " + std::to_string(l));
+        AmgLevelDev& L = *lv[l];
+        const bool h = use32(f32);
+        const void* sv = h ? static_cast<const void*>(L.pS32.p) : static_cast<const void*>(L.pS.p);
+        const void* lvv = h ? static_cast<const void*>(L.pL32.p) : static_cast<const void*>(L.pL.p);
+        const void* vv = h ? static_cast<const void*>(L.v32.p) : static_cast<const void*>(L.v.p);
+        launchJacobiFused(s, 1, h, L.plan, sv, lvv, L.n, L.rp, L.ci, vv, dinvOf(l, f32), L.b.p, L.x1.p, omega, L.x3.p,
+                          double(L.nnz) / std::max(L.n, 1));
+        const double pa = h ? 4.0 : 8.0;
+        return double(L.nnz) * (pa + 4.0) + (L.n + 1) * 4.0 + double(L.n) * (pa + 3 * 8.0);
+    }
     // level 0 is smoothed by damped Jacobi: its first iterate is needed with r_p
     bool jacobi0() const { return smoother == RSH_SMOOTH_JACOBI && lv.size() > 1; }
     // one V-cycle on r_p (in lv[0].b) -> result pointer; x1Ready: level 0's first Jacobi iterate was written with r_p
@@ -837,7 +867,7 @@ struct CprDev {
             b += L->rpB.bytes() + L->ciB.bytes() + L->diagB.bytes() + L->v.bytes() + L->agg.bytes() + L->rrp.bytes() + L->rci.bytes() +
                  L->apPtr.bytes() + L->apSrc.bytes() + L->cPtr.bytes() + L->cSrc.bytes() + L->ap.bytes() + L->gsF.bytes() + L->gsB.bytes() +
                  L->x1.bytes() + L->x2.bytes() + L->x3.bytes() + L->b.bytes() + L->r.bytes() + L->v32.bytes() + L->dinv32.bytes() +
-                 L->pS.bytes() + L->pL.bytes() + L->pS32.bytes() + L->pL32.bytes();
+                 L->pS.bytes() + L->pL.bytes() + L->pS32.bytes() + L->pL32.bytes() + (L->ownPlan ? L->ownPlan->bytes() : 0);
         return b;
     }
 };

@@ -136,6 +136,48 @@ void run(const Input& in, int id) {
         okAmg = okAmg && int(H[l].cPtr.size()) == H[l + 1].A.rp.back() + 1;
     }
     check(okAmg, "AMG hierarchy", id);
+    // row slices of a coarse level as its Jacobi steps read them (far rows long): every row once, every entry once in
+    // its row's order, columns inside the vector, 16-bit offsets equal to the columns
+    for (size_t l = 1; l < H.size(); ++l) {
+        const HostCsr& M = H[l].A;
+        SliceHost sh;
+        rowSliceLayout(M.n, M.rp, M.ci, sh, true);
+        std::vector<int> seen(size_t(M.n), 0);
+        std::vector<double> xv(size_t(M.n)), ys(size_t(M.n), 0.0), yc(size_t(M.n), 0.0);
+        for (int i = 0; i < M.n; ++i) xv[size_t(i)] = 1.0 + 0.001 * (i % 97);
+        bool okS = true;
+        for (int sl = 0; sl < sh.nslices && okS; ++sl)
+            for (int lane = 0; lane < kSliceRows && okS; ++lane) {
+                const int i = sh.row[size_t(sl) * kSliceRows + size_t(lane)];
+                if (i < 0) continue;
+                okS = i < M.n && seen[size_t(i)]++ == 0;
+                double acc = 0;
+                int p = M.rp[size_t(i)];
+                for (int k = 0; k < sh.width[size_t(sl)] && okS; ++k) {
+                    const size_t at = (size_t(sh.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
+                    const int c = sh.col[at];
+                    okS = c >= 0 && c < M.n && (sh.dcol.empty() || i + sh.dcol[at] == c);
+                    if (sh.src[at] < 0) continue;
+                    okS = okS && sh.src[at] == p && c == M.ci[size_t(p)];
+                    acc += M.v[size_t(sh.src[at])] * xv[size_t(c)];
+                    ++p;
+                }
+                okS = okS && p == M.rp[size_t(i) + 1];
+                ys[size_t(i)] = acc;
+            }
+        for (size_t q = 0; q < sh.lrow.size() && okS; ++q) {
+            const int i = sh.lrow[q];
+            okS = i >= 0 && i < M.n && seen[size_t(i)]++ == 0 && sh.lptr[q + 1] - sh.lptr[q] == M.rp[size_t(i) + 1] - M.rp[size_t(i)];
+            double acc = 0;
+            for (int u = sh.lptr[q]; u < sh.lptr[q + 1] && okS; ++u) acc += M.v[size_t(sh.lsrc[size_t(u)])] * xv[size_t(sh.lcol[size_t(u)])];
+            ys[size_t(i)] = acc;
+        }
+        for (int i = 0; i < M.n && okS; ++i) {
+            okS = seen[size_t(i)] == 1;
+            for (int p = M.rp[size_t(i)]; p < M.rp[size_t(i) + 1]; ++p) yc[size_t(i)] += M.v[size_t(p)] * xv[size_t(M.ci[size_t(p)])];
+        }
+        check(okS && ys == yc, "coarse-level row slices", id);
+    }
     // the well Schur plan of the Jacobian the system was reduced from: the same pattern and values
     int fill = 0;
     if (in.jac) {

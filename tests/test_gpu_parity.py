@@ -690,3 +690,29 @@ def test_mixed_ir_against_the_float_device_solve(sysd, orc, name, kind):
     assert ok and conv and res(x) <= 1.01 * TOL and rel_r <= TOL
     assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
     print(f"{name}/{kind}: device outer {full.info()['last_outer']} inner its {its}; reference outer-loop inner its {its_r}")
+
+
+@pytest.mark.parametrize("precision", [rs.FP64, rs.MIXED_STORE])
+def test_sliced_coarse_levels_bit_identical(sysd, monkeypatch, precision):
+    # coarse AMG levels above a row threshold are smoothed through row slices (one row per lane); each row's sum is
+    # formed in the chains of the CSR step's G lanes and folded in its butterfly order, so V-cycles, CPR applies and
+    # whole solves equal those of the G-lanes-per-row CSR step bit for bit.  The threshold is lowered so that the
+    # 60,000-cell system's coarse levels (about 11,000 and 1,600 rows) take the slices, long rows included.
+    s = sysd["medium"]
+    r = np.cos(0.3 * np.arange(s.n * s.nb))
+    out = {}
+    for rows in ("1000000000", "1000"):
+        monkeypatch.setenv("RSH_AMG_SLICE_ROWS", rows)
+        d = rs.System(s).setup(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0, precision=precision)
+        info = d.info()
+        x, its, rel, ok = d.solve(s.rhs, TOL, 3000)
+        assert ok
+        out[rows] = (info, d.amg_apply(r[: s.n]), d.precond(r), x, its)
+    csr, sl = out["1000000000"], out["1000"]
+    assert all(v[0] == 0 for v in csr[0]["amg_slices"][1:])
+    sliced = [l for l, v in enumerate(sl[0]["amg_slices"]) if l > 0 and v[0] > 0]
+    assert len(sliced) >= 1 and any(sl[0]["amg_slices"][l][1] > 0 for l in sliced)   # a sliced level with long rows
+    assert np.array_equal(csr[1], sl[1])
+    assert np.array_equal(csr[2], sl[2])
+    assert np.array_equal(csr[3], sl[3]) and csr[4] == sl[4]
+    print(f"precision {precision}: levels {sl[0]['amg_levels']} sliced {sliced} slices {sl[0]['amg_slices']}, {sl[4]} its")
