@@ -93,7 +93,12 @@ typedef struct rsh_options {
                            every row eliminated in double against the already-rounded earlier rows and stored
                            rounded; the sweeps read float32 factors with FP64 vectors and arithmetic.  Default 0 */
     int coarse_smooth_steps; /* RSH_SMOOTH_JACOBI: steps on the levels below the finest (<= 0: smooth_steps) */
-    int reserved[2];
+    int ilu_schedule;   /* natural-order BILU0 sweeps (RSH_PREC_BILU0, RSH_PREC_CPR): 0 tiled where a cost model finds it
+                           faster than one launch per level, 1 one launch per level, 2 tiled whenever the pattern admits
+                           it.  Tiled: rows grouped into compact tiles along the pattern's dominant lower offsets (the
+                           grid axes of a natural-ordered grid), one launch per tile level, one workgroup per tile walking
+                           its own levels; the same factors, each bulk row's sweep in the same slot order.  Default 0 */
+    int ilu_tile_dims;  /* tile extents along those offsets, bx | by << 8 | bz << 16 (0: chosen by the cost model) */
 } rsh_options;
 
 /* ---- library / context */

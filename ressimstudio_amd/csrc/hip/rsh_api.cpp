@@ -559,8 +559,11 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         }
         const bool wantMc = o.precond == RSH_PREC_MCBILU0 || o.precond == RSH_PREC_CPR_MC;
         const bool wantIlu = o.precond != RSH_PREC_CPR_BJ;
-        if (wantIlu && (!S.ilu || S.ilu->mc != wantMc)) {
+        if (o.ilu_schedule < 0 || o.ilu_schedule > 2) return fail(RSH_ERR_ARG, "unknown ILU schedule");
+        if (wantIlu && (!S.ilu || S.ilu->mc != wantMc || (!wantMc && (S.ilu->schedule != o.ilu_schedule || S.ilu->tileDims != o.ilu_tile_dims)))) {
             S.ilu = std::make_unique<IluDev>();
+            S.ilu->schedule = o.ilu_schedule;
+            S.ilu->tileDims = o.ilu_tile_dims;
             S.ilu->keepEntries = S.keepIluEntries;
             S.ilu->fuseMax = S.iluFuseMax;
             S.ilu->rowSys = S.rowSys.empty() ? nullptr : &S.rowSys;
@@ -759,7 +762,13 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
           << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0) << ",\"cpr_true_impes\":" << (S.acc.n ? 1 : 0)
           << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks << ",\"device_bytes\":" << S.deviceBytes();
         if (S.ilu) {
-            o << ",\"ilu_levels\":" << S.ilu->sL.levels() << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
+            const IluDev& I = *S.ilu;
+            o << ",\"ilu_tiled\":" << (I.tiled ? 1 : 0);
+            if (I.tiled)
+                o << ",\"ilu_tiles\":{\"dims\":[" << I.tiles.dims[0] << "," << I.tiles.dims[1] << "," << I.tiles.dims[2] << "],\"tile_levels\":" << I.tiles.tileLevels()
+                  << ",\"units\":" << I.tiles.units() << ",\"max_rows\":" << I.tiles.maxRows << ",\"max_levels\":" << I.tiles.maxLevels
+                  << ",\"long_max\":" << I.tLongMax << ",\"long_inside_max\":" << I.tLongCap << "}";
+            o << ",\"ilu_levels\":" << (int(I.lstart.size()) - 1) << ",\"ilu_multicolour\":" << (S.ilu->mc ? 1 : 0) << ",\"ilu_regularised\":" << S.ilu->regularised
               << ",\"ilu_long_rows_L\":" << S.ilu->hL.lrow.size() << ",\"ilu_long_rows_U\":" << S.ilu->hU.lrow.size() << ",\"ilu_bytes\":" << S.ilu->bytes();
             int fusedL = 0, fusedU = 0, denseRunsL = 0, denseRunsU = 0;
             for (size_t i = 0; i < S.ilu->runsL.size(); ++i) { fusedL += S.ilu->runsL[i].fused; denseRunsL += S.ilu->denseL.runTask0[i] >= 0; }

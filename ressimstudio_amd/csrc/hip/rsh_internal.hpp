@@ -172,6 +172,28 @@ void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int ca
 // 16-bit reach are long rows too
 void rowSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, SliceHost& h, bool coarse = false);
 
+// Tiled schedule of the natural-order triangular sweeps.  Rows are grouped into units: the rows of one tile of the grid
+// (boxes along the pattern's dominant lower offsets) that share a tile level, where tileLevel(i) is the maximum over
+// the rows j < i coupled to i of tileLevel(j), plus one when j lies in another tile.  A unit therefore depends only on
+// units of earlier tile levels and on its own rows, whatever the tiles are (the tiles only decide how many launches
+// and local levels there are); inside a unit the rows are ordered by their local level.
+struct TileSched {
+    int dims[3] = {1, 1, 1};
+    std::vector<int> ord;          // working order: by tile level, tile, local level, row
+    std::vector<int> unitStart;    // first working row of each unit (units + 1 entries)
+    std::vector<int> tlStart;      // first unit of each tile level (tile levels + 1 entries)
+    std::vector<int> lvPtr;        // unit u's local level starts: lvStart[lvPtr[u] .. lvPtr[u + 1]), offsets from the
+    std::vector<int> lvStart;      // unit's first row, the last one its row count
+    int maxRows = 0, maxLevels = 0;
+    int tileLevels() const { return int(tlStart.size()) - 1; }
+    int units() const { return int(unitStart.size()) - 1; }
+};
+// up to three dominant lower offsets of the pattern (ascending; returns how many) and per row its chain coordinate
+// along each: the number of consecutive steps back by that offset present in the pattern (for a natural-ordered grid
+// the cell's grid indices)
+int gridCoords(const HostBsr& A, int off[3], std::vector<int> coord[3]);
+void tileOrder(const HostBsr& A, const std::vector<int> coord[3], int ndim, const int dims[3], TileSched& T);
+
 // AMG set-up on the host (AMG::setup, amg.cpp:137-252): returns the hierarchy
 struct AmgLevelHost {
     HostCsr A;
@@ -207,6 +229,15 @@ void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0
 constexpr int kRunRowsMax = 2048;
 void launchIluFused(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int l0, int l1, const int* sStart, const int* qStart,
                     int wbase, int wcount, const int* perm, const double* r, void* y, double* z, const double* xp);
+// tiled natural-order sweeps (TileSched): one launch per tile level [u0, u0 + nu) of units, one workgroup of kTileWaves
+// wavefronts per unit (one per slice of the unit, one for all of its long rows).  rowsCap / levelsCap: the largest
+// unit's rows and local levels; longMax: long rows per unit; longCap: inside slots staged per long row
+constexpr int kTileWaves = 12;
+constexpr int kTileLdsMax = 64 * 1024;
+int tileLdsBytes(int nb, int fp, int rowsCap, int levelsCap, int longMax, int longCap);
+void launchIluTile(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int u0, int nu, const int* unitStart, const int* sStart,
+                   const int* qStart, const int* lvPtr, const int* lvStart, int rowsCap, int levelsCap, int longMax, int longCap, const int* perm,
+                   const double* r, void* y, double* z, const double* xp);
 // dense groups of a run: largest group (block rows); the dense product runs one workgroup of kDenseThreads per
 // tile of kWave scalar rows of a group
 constexpr int kDenseMaxRows = 256;

@@ -770,6 +770,428 @@ __global__ __launch_bounds__(kFuseThreads) void kIluFused(int l0, int l1, const 
     }
 }
 
+// ---------------------------------------------------------------- tiled natural-order sweeps
+// The natural-order sweeps have one dependent level per grid diagonal (363 at 60 x 220 x 85).  Here the rows are
+// grouped into units (a compact tile of the grid at one tile level, rows ordered by local level): a unit depends only
+// on units of earlier tile levels and on its own rows, so one launch per tile level runs one workgroup per unit, which
+// walks the unit's local levels with workgroup barriers, the unit's values in LDS.
+// Bulk rows (one per lane of a slice wavefront): phase A loads the row's first TileCap slots into registers, gathers
+// the values of the slots outside the unit (final before the launch) and applies the leading outside slots; phase B, at
+// the row's local level, applies the remaining slots in slot order (inside values from LDS).  Every bulk row thus keeps
+// the slot order and arithmetic of the level-scheduled sweep.  Long rows (a well's perforated cells after the Schur
+// fill): one wavefront takes the unit's long rows one after another, lanes over slots; phase A sums the outside slots
+// and stages the inside slots (index and block) in LDS, phase B sums those at the row's local level.
+constexpr int kTileThreads = kTileWaves * kWave;
+constexpr int kTileLongWaves = 4;   // wavefronts that share a unit's long rows in phase A (when the slices leave them)
+template <int NB> struct TileCap { static constexpr int v = NB <= 2 ? 6 : (NB == 3 ? 3 : 2); };
+
+struct TileLds { int blk, ldv, ys, lpart, red, lidx, lv, llev, lnin, lwr, total; };
+__host__ __device__ inline int alignLds(int b) { return (b + 15) & ~15; }
+// dynamic LDS of a tile launch: staged long-row blocks and inverted pivots, the unit's values, long-row partial sums,
+// the long-row wavefront's per-lane partials, staged inside indices, local level starts, long-row levels / inside
+// counts / working rows
+__host__ __device__ inline TileLds tileLds(int rowsCap, int levelsCap, int longMax, int longCap, int nb, int tvBytes, int taBytes) {
+    TileLds L{};
+    int o = 0;
+    L.blk = o; o = alignLds(o + longMax * longCap * nb * nb * taBytes);
+    L.ldv = o; o = alignLds(o + longMax * nb * nb * taBytes);
+    L.ys = o; o = alignLds(o + rowsCap * nb * tvBytes);
+    L.lpart = o; o = alignLds(o + longMax * nb * tvBytes);
+    L.red = o; o = alignLds(o + kTileLongWaves * kWave * nb * tvBytes);
+    L.lidx = o; o = alignLds(o + longMax * longCap * 4);
+    L.lv = o; o = alignLds(o + (levelsCap + 1) * 4);
+    L.llev = o; o = alignLds(o + longMax * 4);
+    L.lnin = o; o = alignLds(o + longMax * 4);
+    L.lwr = o; o = alignLds(o + longMax * 4);
+    L.total = o;
+    return L;
+}
+
+// sum of red[j * NB + e] over j in [0, cnt) in ascending order (four loads in flight at a time)
+template <int NB, typename TV>
+__device__ inline void ldsOrderedSum(const TV* red, int cnt, TV (&sum)[NB]) {
+#pragma clang fp contract(off)
+    for (int j0 = 0; j0 < cnt; j0 += 4) {
+        TV t[4][NB];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + u < cnt) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) t[u][e] = red[(j0 + u) * NB + e];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + u < cnt) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) sum[e] += t[u][e];
+            }
+    }
+}
+
+template <int NB, typename TA, typename TV, bool FWD>
+__global__ __launch_bounds__(kTileThreads) void kIluTile(int u0, const int* __restrict__ unitStart, const int* __restrict__ sStart,
+                                                         const int* __restrict__ qStart, const int* __restrict__ lvPtr,
+                                                         const int* __restrict__ lvStart, SliceView<TA> A, int rowsCap, int levelsCap,
+                                                         int longMax, int longCap, const int* __restrict__ perm,
+                                                         const double* __restrict__ r, TV* y, double* __restrict__ z,
+                                                         const double* __restrict__ xp) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem[];
+    const TileLds Ls = tileLds(rowsCap, levelsCap, longMax, longCap, NB, int(sizeof(TV)), int(sizeof(TA)));
+    TA* lblk = reinterpret_cast<TA*>(smem + Ls.blk);
+    TA* ldv = reinterpret_cast<TA*>(smem + Ls.ldv);
+    TV* ys = reinterpret_cast<TV*>(smem + Ls.ys);
+    TV* lpart = reinterpret_cast<TV*>(smem + Ls.lpart);
+    TV* red = reinterpret_cast<TV*>(smem + Ls.red);
+    int* lidx = reinterpret_cast<int*>(smem + Ls.lidx);
+    int* lv = reinterpret_cast<int*>(smem + Ls.lv);
+    int* llev = reinterpret_cast<int*>(smem + Ls.llev);
+    int* lnin = reinterpret_cast<int*>(smem + Ls.lnin);
+    int* lwr = reinterpret_cast<int*>(smem + Ls.lwr);
+    constexpr int CAP = TileCap<NB>::v;
+    constexpr int NB2 = NB * NB;
+    const int u = u0 + blockIdx.x;
+    const int wb = unitStart[u], we = unitStart[u + 1];
+    const int s0 = sStart[u], ns = sStart[u + 1] - s0, q0 = qStart[u], nq = qStart[u + 1] - q0;
+    const int lp = lvPtr[u], nl = lvPtr[u + 1] - lp - 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the slice's row and shape are loaded together with the level starts (one round trip before the barrier)
+    int wr = -1, o = -1;
+    int2 inf = make_int2(0, 0);
+    if (wave < ns) {
+        wr = A.row[(long long)(s0 + wave) * kWave + lane];
+        o = A.orig[(long long)(s0 + wave) * kWave + lane];
+        inf = A.info[s0 + wave];
+    }
+    for (int t = threadIdx.x; t <= nl; t += kTileThreads) lv[t] = lvStart[lp + t];
+    __syncthreads();
+    // local level of the unit's row with local index li (li < the unit's row count = lv[nl])
+    auto levelOf = [&](int li) {
+        int l = 0;
+        while (lv[l + 1] <= li) ++l;
+        return l;
+    };
+    auto inside = [&](int c) { return c >= wb && c < we; };
+
+    // ---- phase A
+    int myLev = -1, width = 0, kfirst = 0;
+    long long slot0 = 0;
+    TV acc[NB];
+    int idx[CAP];       // LDS row of an inside slot, -1 otherwise
+    TA av[CAP][NB2];
+    TV yv[CAP][NB];     // outside slots: their block products (formed here); inside slots: the values read at the step
+    TA dv[NB2];
+#pragma unroll
+    for (int k = 0; k < CAP; ++k) {
+        idx[k] = -1;
+#pragma unroll
+        for (int ef = 0; ef < NB2; ++ef) av[k][ef] = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) yv[k][e] = 0;
+    }
+    if (wave < ns) {
+        const int sl = s0 + wave;
+        if (wr >= 0) {
+            width = inf.x;
+            slot0 = inf.y;
+            myLev = levelOf(wr - wb);
+            if (FWD) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] = TV(r[(long long)o * NB + e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] = y[(long long)wr * NB + e];
+            }
+            int c[CAP];
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k < width) {
+                    c[k] = __builtin_nontemporal_load(A.col + (slot0 + k) * kWave + lane);
+                    idx[k] = inside(c[k]) ? c[k] - wb : -1;
+                    const TA* a = A.val + (slot0 + k) * (NB2 * kWave) + lane;
+#pragma unroll
+                    for (int ef = 0; ef < NB2; ++ef) av[k][ef] = __builtin_nontemporal_load(a + ef * kWave);
+                }
+            if (!FWD) {
+                const TA* d = A.dval + (long long)sl * (NB2 * kWave) + lane;
+#pragma unroll
+                for (int ef = 0; ef < NB2; ++ef) dv[ef] = __builtin_nontemporal_load(d + ef * kWave);
+            }
+            TV yk[CAP][NB];
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k < width && idx[k] < 0) {
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) yk[k][f] = y[(long long)c[k] * NB + f];
+                }
+            // the block products of the outside slots (s_e = sum_f a_ef y_f, f ascending); the leading ones applied
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k < width && idx[k] < 0) {
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) yv[k][e] = 0;
+#pragma unroll
+                    for (int f = 0; f < NB; ++f)
+#pragma unroll
+                        for (int e = 0; e < NB; ++e) yv[k][e] += TV(av[k][e * NB + f]) * yk[k][f];
+                }
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+                if (k < width && kfirst == k && idx[k] < 0) {
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) acc[e] -= yv[k][e];
+                    kfirst = k + 1;
+                }
+            if (kfirst == CAP) {
+                for (int k = CAP; k < width; ++k) {
+                    const int ck = A.col[(slot0 + k) * kWave + lane];
+                    if (inside(ck)) break;
+                    const TA* a = A.val + (slot0 + k) * (NB2 * kWave) + lane;
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+                        TV sacc = 0;
+#pragma unroll
+                        for (int f = 0; f < NB; ++f) sacc += TV(a[(e * NB + f) * kWave]) * y[(long long)ck * NB + f];
+                        acc[e] -= sacc;
+                    }
+                    kfirst = k + 1;
+                }
+            }
+        }
+    } else if (wave >= ns && wave < ns + kTileLongWaves && nq > 0) {
+        // the unit's long rows, spread over the wavefronts left after the slices (up to kTileLongWaves), G lanes each:
+        // outside slots summed (per-lane partials added in lane order by the group's first lane), inside slots staged
+        const int lw = wave - ns;
+        const int nlw = min(kTileLongWaves, kTileWaves - ns);
+        const int mine = lw < nq ? (nq - lw + nlw - 1) / nlw : 0;   // long rows lw, lw + nlw, ... of this wavefront
+        int G = kWave;
+        while (G > 1 && G * mine > kWave) G >>= 1;
+        const int grp = lane / G, gl = lane % G;
+        const int qi = grp < mine ? lw + grp * nlw : -1;
+        TV* myred = red + lw * kWave * NB;
+        TV part[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) part[e] = 0;
+        int p0 = 0, p1 = 0;
+        if (qi >= 0) {
+            p0 = A.lptr[q0 + qi];
+            p1 = A.lptr[q0 + qi + 1];
+        }
+        const unsigned long long gmask = (G == kWave ? ~0ull : ((1ull << G) - 1ull)) << (grp * G);
+        int base = 0;
+        int trips = 0;
+        for (int j = 0; j < mine; ++j) {
+            const int qq = q0 + lw + j * nlw;
+            trips = max(trips, (A.lptr[qq + 1] - A.lptr[qq] + G - 1) / G);
+        }
+        constexpr int UT = 4;   // trips whose loads are in flight together
+        for (int t0 = 0; t0 < trips; t0 += UT) {
+            int cc[UT];
+            bool valid[UT], in[UT];
+            TA ab[UT][NB2];
+            TV yk[UT][NB];
+#pragma unroll
+            for (int v = 0; v < UT; ++v) {
+                const int p = p0 + (t0 + v) * G + gl;
+                valid[v] = qi >= 0 && t0 + v < trips && p < p1;
+                cc[v] = valid[v] ? A.lcol[p] : -1;
+            }
+#pragma unroll
+            for (int v = 0; v < UT; ++v) {
+                in[v] = valid[v] && inside(cc[v]);
+                if (valid[v]) {
+                    const TA* a = A.lval + (long long)(p0 + (t0 + v) * G + gl) * NB2;
+#pragma unroll
+                    for (int ef = 0; ef < NB2; ++ef) ab[v][ef] = a[ef];
+                }
+                if (valid[v] && !in[v]) {
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) yk[v][f] = y[(long long)cc[v] * NB + f];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < UT; ++v) {
+                if (valid[v] && !in[v]) {
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+                        TV sacc = 0;
+#pragma unroll
+                        for (int f = 0; f < NB; ++f) sacc += TV(ab[v][e * NB + f]) * yk[v][f];
+                        part[e] += sacc;
+                    }
+                }
+                const unsigned long long m = __ballot(in[v] ? 1 : 0) & gmask;
+                if (in[v]) {
+                    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+                    lidx[qi * longCap + pos] = cc[v] - wb;
+#pragma unroll
+                    for (int ef = 0; ef < NB2; ++ef) lblk[(qi * longCap + pos) * NB2 + ef] = ab[v][ef];
+                }
+                base += __popcll(m);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) myred[lane * NB + e] = part[e];
+        if (!FWD && qi >= 0)
+            for (int t = gl; t < NB2; t += G) ldv[qi * NB2 + t] = A.ldval[(long long)(q0 + qi) * NB2 + t];
+        __builtin_amdgcn_wave_barrier();
+        if (qi >= 0 && gl == 0) {
+            TV sum[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) sum[e] = 0;
+            ldsOrderedSum<NB, TV>(myred + grp * G * NB, G, sum);
+            const int q = q0 + qi;
+            const int w = A.lrow[q];
+            lwr[qi] = w;
+            llev[qi] = levelOf(w - wb);
+            lnin[qi] = base;
+            if (FWD) {
+                const int o = A.lorig[q];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) lpart[qi * NB + e] = TV(r[(long long)o * NB + e]) - sum[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) lpart[qi * NB + e] = y[(long long)w * NB + e] - sum[e];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase B: local level by level
+    // the long rows are in local-level order: the wavefront keeps the next one to take (FWD: ascending)
+    int qcur = FWD ? 0 : nq - 1;
+    int nextLev = -1;
+    if (wave == ns && nq > 0) nextLev = llev[qcur];
+    for (int li = 0; li < nl; ++li) {
+        const int l = FWD ? li : nl - 1 - li;
+        if (wr >= 0 && myLev == l) {
+            // the inside values first (their reads in flight together), their block products (independent chains side
+            // by side, each summed f ascending), then every remaining slot's product subtracted in slot order
+            // (straight-line: a slot that takes no part reads LDS row 0 and subtracts an exact zero, x - 0 = x)
+            TV yk[CAP][NB], sk[CAP][NB];
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) {
+                const int ii = idx[k] >= 0 ? idx[k] : 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) yk[k][f] = ys[ii * NB + f];
+            }
+#pragma unroll
+            for (int k = 0; k < CAP; ++k)
+#pragma unroll
+                for (int e = 0; e < NB; ++e) sk[k][e] = 0;
+#pragma unroll
+            for (int f = 0; f < NB; ++f)
+#pragma unroll
+                for (int k = 0; k < CAP; ++k)
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) sk[k][e] += TV(av[k][e * NB + f]) * yk[k][f];
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) {
+                const bool take = k >= kfirst && k < width;
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] -= take ? (idx[k] >= 0 ? sk[k][e] : yv[k][e]) : TV(0);
+            }
+            for (int k = kfirst > CAP ? kfirst : CAP; k < width; ++k) {
+                const int ck = A.col[(slot0 + k) * kWave + lane];
+                const TA* a = A.val + (slot0 + k) * (NB2 * kWave) + lane;
+                TV yk[NB];
+                if (inside(ck)) {
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) yk[f] = ys[(ck - wb) * NB + f];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) yk[f] = y[(long long)ck * NB + f];
+                }
+#pragma unroll
+                for (int e = 0; e < NB; ++e) {
+                    TV sacc = 0;
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) sacc += TV(a[(e * NB + f) * kWave]) * yk[f];
+                    acc[e] -= sacc;
+                }
+            }
+            if (!FWD) {
+                TV tv[NB];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) tv[e] = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f)
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) tv[e] += TV(dv[e * NB + f]) * acc[f];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] = tv[e];
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) ys[(wr - wb) * NB + e] = acc[e];
+        }
+        while (nextLev == l) {
+            // long row qcur: lanes over its staged inside slots, per-lane partials summed in lane order by lane 0
+            const int qi = qcur;
+            const int cnt = lnin[qi];
+            TV part[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) part[e] = 0;
+            for (int j = lane; j < cnt; j += kWave) {
+                const int li2 = lidx[qi * longCap + j];
+                const TA* a = lblk + (qi * longCap + j) * NB2;
+#pragma unroll
+                for (int e = 0; e < NB; ++e) {
+                    TV sacc = 0;
+#pragma unroll
+                    for (int f = 0; f < NB; ++f) sacc += TV(a[e * NB + f]) * ys[li2 * NB + f];
+                    part[e] += sacc;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) red[lane * NB + e] = part[e];
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                TV sum[NB];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) sum[e] = 0;
+                ldsOrderedSum<NB, TV>(red, cnt < kWave ? cnt : kWave, sum);
+                TV a2[NB];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) a2[e] = lpart[qi * NB + e] - sum[e];
+                if (!FWD) {
+                    TV tv[NB];
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+                        TV sacc = 0;
+#pragma unroll
+                        for (int f = 0; f < NB; ++f) sacc += TV(ldv[qi * NB2 + e * NB + f]) * a2[f];
+                        tv[e] = sacc;
+                    }
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) a2[e] = tv[e];
+                }
+#pragma unroll
+                for (int e = 0; e < NB; ++e) ys[(lwr[qi] - wb) * NB + e] = a2[e];
+            }
+            __builtin_amdgcn_wave_barrier();
+            qcur += FWD ? 1 : -1;
+            nextLev = (qcur >= 0 && qcur < nq) ? llev[qcur] : -1;
+        }
+        __syncthreads();
+    }
+
+    // ---- phase C: the unit's values to HBM (and z in the original numbering after the backward sweep)
+    const int cnt = we - wb;
+    for (int k = threadIdx.x; k < cnt; k += kTileThreads) {
+#pragma unroll
+        for (int e = 0; e < NB; ++e) y[(long long)(wb + k) * NB + e] = ys[k * NB + e];
+        if (!FWD) {
+            const int o = perm[wb + k];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                double v = double(ys[k * NB + e]);
+                if (e == 0 && xp) v += xp[o];
+                z[(long long)o * NB + e] = v;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dense groups inside a run of small levels
 // Rows of a run that are coupled to each other inside the run form groups (a well's perforated cells, coupled
 // all-to-all by the Schur-reduced bottom-hole-pressure row, make one group per well).  Per group the unit
@@ -2175,6 +2597,17 @@ void iluFusedT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int l0, int l1, c
     if (fwd) hipLaunchKernelGGL((kIluFused<NB, TA, TV, true>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
     else hipLaunchKernelGGL((kIluFused<NB, TA, TV, false>), dim3(1), dim3(kFuseThreads), 0, s, l0, l1, sStart, qStart, A, wbase, wcount, perm, r, y, z, xp);
 }
+template <int NB, typename TA, typename TV>
+void iluTileT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int u0, int nu, const int* unitStart, const int* sStart, const int* qStart,
+              const int* lvPtr, const int* lvStart, int rowsCap, int levelsCap, int longMax, int longCap, const int* perm, const double* r, TV* y,
+              double* z, const double* xp) {
+    const SliceView<TA> A{d.info.p, d.row.p, d.orig.p, d.col.p, d.val.p, d.dval.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, d.ldval.p, d.owner.p, d.lowner.p};
+    const int lds = tileLds(rowsCap, levelsCap, longMax, longCap, NB, int(sizeof(TV)), int(sizeof(TA))).total;
+    if (fwd) hipLaunchKernelGGL((kIluTile<NB, TA, TV, true>), dim3(nu), dim3(kTileThreads), lds, s, u0, unitStart, sStart, qStart, lvPtr, lvStart, A, rowsCap,
+                                levelsCap, longMax, longCap, perm, r, y, z, xp);
+    else hipLaunchKernelGGL((kIluTile<NB, TA, TV, false>), dim3(nu), dim3(kTileThreads), lds, s, u0, unitStart, sStart, qStart, lvPtr, lvStart, A, rowsCap,
+                            levelsCap, longMax, longCap, perm, r, y, z, xp);
+}
 template <int NB>
 void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
                 int lean) {
@@ -2364,6 +2797,20 @@ void launchIluFused(hipStream_t s, int nb, int fp, const void* slices, bool fwd,
     if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (float*)y, z, xp)))
     else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluFusedT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (double*)y, z, xp)))
     else RSH_NB_DISPATCH(nb, (iluFusedT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, l0, l1, sStart, qStart, wbase, wcount, perm, r, (double*)y, z, xp)))
+    checkLaunch();
+}
+int tileLdsBytes(int nb, int fp, int rowsCap, int levelsCap, int longMax, int longCap) {
+    const int tv = fp == kIluFp32 ? 4 : 8, ta = fp == kIluFp64 ? 8 : 4;
+    return tileLds(rowsCap, levelsCap, longMax, longCap, nb, tv, ta).total;
+}
+void launchIluTile(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int u0, int nu, const int* unitStart, const int* sStart,
+                   const int* qStart, const int* lvPtr, const int* lvStart, int rowsCap, int levelsCap, int longMax, int longCap, const int* perm,
+                   const double* r, void* y, double* z, const double* xp) {
+    if (nu <= 0) return;
+    if (tileLdsBytes(nb, fp, rowsCap, levelsCap, longMax, longCap) > kTileLdsMax) throw Error(-1, "tile sweep: LDS request above the limit");
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluTileT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, u0, nu, unitStart, sStart, qStart, lvPtr, lvStart, rowsCap, levelsCap, longMax, longCap, perm, r, (float*)y, z, xp)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluTileT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, u0, nu, unitStart, sStart, qStart, lvPtr, lvStart, rowsCap, levelsCap, longMax, longCap, perm, r, (double*)y, z, xp)))
+    else RSH_NB_DISPATCH(nb, (iluTileT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, u0, nu, unitStart, sStart, qStart, lvPtr, lvStart, rowsCap, levelsCap, longMax, longCap, perm, r, (double*)y, z, xp)))
     checkLaunch();
 }
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <unordered_map>
 
 #include "rsh_internal.hpp"
 
@@ -429,6 +430,113 @@ void amgSetup(HostCsr A0, std::vector<AmgLevelHost>& H) {
         galerkin(L, next.A);
         H.push_back(std::move(next));
     }
+}
+
+// ------------------------------------------------------------------ tiled schedule of the natural-order sweeps
+int gridCoords(const HostBsr& A, int off[3], std::vector<int> coord[3]) {
+    const int n = A.n;
+    // the most frequent offsets i - j of the lower entries; those carried by at least an eighth of the rows are the
+    // grid's steps (1, nx, nx ny for a natural-ordered grid)
+    std::unordered_map<int, long long> cnt;
+    for (int i = 0; i < n; ++i)
+        for (int p = A.rp[size_t(i)]; p < A.diag[size_t(i)]; ++p) cnt[i - A.ci[size_t(p)]]++;
+    std::vector<std::pair<long long, int>> v;
+    for (const auto& kv : cnt)
+        if (kv.second * 8 >= n) v.push_back({kv.second, kv.first});
+    std::sort(v.begin(), v.end(), [](const std::pair<long long, int>& a, const std::pair<long long, int>& b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    const int nd = int(std::min<size_t>(3, v.size()));
+    for (int a = 0; a < nd; ++a) off[a] = v[size_t(a)].second;
+    std::sort(off, off + nd);
+    for (int a = 0; a < 3; ++a) {
+        coord[a].assign(size_t(n), 0);
+        if (a >= nd) continue;
+        const int d = off[a];
+        for (int i = d; i < n; ++i) {
+            // columns are sorted: look for i - d among the lower entries
+            const int* b = A.ci.data() + A.rp[size_t(i)];
+            const int* e = A.ci.data() + A.diag[size_t(i)];
+            const int* f = std::lower_bound(b, e, i - d);
+            if (f != e && *f == i - d) coord[a][size_t(i)] = coord[a][size_t(i - d)] + 1;
+        }
+    }
+    return nd;
+}
+
+void tileOrder(const HostBsr& A, const std::vector<int> coord[3], int ndim, const int dims[3], TileSched& T) {
+    const int n = A.n;
+    T = TileSched{};
+    long long ext[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+        T.dims[a] = a < ndim ? std::max(1, dims[a]) : 1;
+        int mx = 0;
+        if (a < ndim)
+            for (int i = 0; i < n; ++i) mx = std::max(mx, coord[a][size_t(i)]);
+        ext[a] = mx / T.dims[a] + 1;
+    }
+    std::vector<long long> tile(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        long long t[3] = {0, 0, 0};
+        for (int a = 0; a < ndim; ++a) t[a] = coord[a][size_t(i)] / T.dims[a];
+        tile[size_t(i)] = t[0] + ext[0] * (t[1] + ext[1] * t[2]);
+    }
+    // tile level on the symmetrised lower graph: a coupled earlier row of the same tile may share the level, one of
+    // another tile forces the next level (pushed forward along the upper entries, as levelOrder does)
+    std::vector<int> tl(static_cast<size_t>(n), 0), ll(static_cast<size_t>(n), 0);
+    for (int j = 0; j < n; ++j) {
+        int l = tl[size_t(j)];
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c < j) l = std::max(l, tl[size_t(c)] + (tile[size_t(c)] != tile[size_t(j)] ? 1 : 0));
+        }
+        tl[size_t(j)] = l;
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c > j) tl[size_t(c)] = std::max(tl[size_t(c)], l + (tile[size_t(c)] != tile[size_t(j)] ? 1 : 0));
+        }
+    }
+    // local levels inside a unit (same tile and tile level)
+    auto same = [&](int a, int b) { return tile[size_t(a)] == tile[size_t(b)] && tl[size_t(a)] == tl[size_t(b)]; };
+    for (int j = 0; j < n; ++j) {
+        int l = ll[size_t(j)];
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c < j && same(c, j)) l = std::max(l, ll[size_t(c)] + 1);
+        }
+        ll[size_t(j)] = l;
+        for (int p = A.rp[size_t(j)]; p < A.rp[size_t(j) + 1]; ++p) {
+            const int c = A.ci[size_t(p)];
+            if (c > j && same(c, j)) ll[size_t(c)] = std::max(ll[size_t(c)], l + 1);
+        }
+    }
+    T.ord.resize(size_t(n));
+    std::iota(T.ord.begin(), T.ord.end(), 0);
+    std::sort(T.ord.begin(), T.ord.end(), [&](int a, int b) {
+        if (tl[size_t(a)] != tl[size_t(b)]) return tl[size_t(a)] < tl[size_t(b)];
+        if (tile[size_t(a)] != tile[size_t(b)]) return tile[size_t(a)] < tile[size_t(b)];
+        if (ll[size_t(a)] != ll[size_t(b)]) return ll[size_t(a)] < ll[size_t(b)];
+        return a < b;
+    });
+    T.tlStart.assign(1, 0);
+    T.lvPtr.assign(1, 0);
+    for (int w = 0; w < n;) {
+        const int i0 = T.ord[size_t(w)];
+        if (w > 0 && tl[size_t(T.ord[size_t(w - 1)])] != tl[size_t(i0)]) T.tlStart.push_back(int(T.unitStart.size()));
+        T.unitStart.push_back(w);
+        int e = w;
+        while (e < n && same(T.ord[size_t(e)], i0)) {
+            if (e == w || ll[size_t(T.ord[size_t(e)])] != ll[size_t(T.ord[size_t(e - 1)])]) T.lvStart.push_back(e - w);
+            ++e;
+        }
+        T.lvStart.push_back(e - w);
+        T.lvPtr.push_back(int(T.lvStart.size()));
+        T.maxRows = std::max(T.maxRows, e - w);
+        T.maxLevels = std::max(T.maxLevels, ll[size_t(T.ord[size_t(e - 1)])] + 1);
+        w = e;
+    }
+    T.unitStart.push_back(n);
+    T.tlStart.push_back(int(T.unitStart.size()) - 1);
 }
 
 }  // namespace rsh

@@ -268,6 +268,14 @@ struct IluDev {
     // batched systems: original row -> system, and the systems' BiCGSTAB states (slices of stopped systems skipped)
     const std::vector<int>* rowSys = nullptr;
     const KScalars* gate = nullptr;
+    // natural order: the sweeps tiled by TileSched (one launch per tile level) instead of one launch per level.
+    // schedule: 0 tiles where the cost model prefers them, 1 level launches, 2 tiles whenever the pattern admits them;
+    // tileDims: the tile extents along the pattern's three dominant lower offsets (bx | by << 8 | bz << 16, 0: chosen)
+    int schedule = 0, tileDims = 0;
+    bool tiled = false;
+    TileSched tiles;
+    DBuf<int> tUnit, tLs, tLq, tUs, tUq, tLvPtr, tLv;
+    int tLongMax = 0, tLongCap = 0;
     // kept when keepEntries is set (batched systems re-chunk the levels when tuning)
     bool keepEntries = false;
     RowEntries keptL, keptU;
@@ -281,19 +289,36 @@ struct IluDev {
         RowEntries EL, EU;
         EL.ptr.assign(1, 0);
         EU.ptr.assign(1, 0);
-        if (!mc) {
-            levelOrder(S, ord, lstart);
+        // sweep numbering of the natural order: level order, or the tile order when the sweeps are tiled
+        std::vector<int> sweepOrd;
+        auto naturalEntries = [&](const std::vector<int>& order) {
+            EL = RowEntries{};
+            EU = RowEntries{};
+            EL.ptr.assign(1, 0);
+            EU.ptr.assign(1, 0);
             std::vector<int> iperm(static_cast<size_t>(n));
-            for (int w = 0; w < n; ++w) iperm[size_t(ord[size_t(w)])] = w;
-            rp = drp; ci = dci; diag = ddiag;
+            for (int w = 0; w < n; ++w) iperm[size_t(order[size_t(w)])] = w;
             for (int w = 0; w < n; ++w) {
-                const int i = ord[size_t(w)];
+                const int i = order[size_t(w)];
                 for (int p = S.rp[size_t(i)]; p < S.diag[size_t(i)]; ++p) { EL.col.push_back(iperm[size_t(S.ci[size_t(p)])]); EL.src.push_back(p); }
                 for (int p = S.diag[size_t(i)] + 1; p < S.rp[size_t(i) + 1]; ++p) { EU.col.push_back(iperm[size_t(S.ci[size_t(p)])]); EU.src.push_back(p); }
                 EL.ptr.push_back(int(EL.col.size()));
                 EU.ptr.push_back(int(EU.col.size()));
                 EU.dsrc.push_back(S.diag[size_t(i)]);
             }
+            EL.orig = order;
+            EU.orig = order;
+        };
+        tiled = false;
+        if (!mc) {
+            levelOrder(S, ord, lstart);
+            rp = drp; ci = dci; diag = ddiag;
+            if (!rowSys && schedule != 1 && chooseTiles(S, int(lstart.size()) - 1)) {
+                naturalEntries(tiles.ord);
+                tiled = tileSlices(EL, EU);
+            }
+            if (!tiled) naturalEntries(ord);
+            sweepOrd = tiled ? tiles.ord : ord;
         } else {
             colours = colourGraph(S, colour, ord, lstart);
             std::vector<int> bmap;
@@ -311,18 +336,21 @@ struct IluDev {
                 EU.dsrc.push_back(fpat.diag[size_t(w)]);
             }
         }
-        EL.orig = ord;
-        EU.orig = ord;
-        perm.upload(ord, s);
+        if (mc) {
+            EL.orig = ord;
+            EU.orig = ord;
+            sweepOrd = ord;
+        }
+        perm.upload(sweepOrd, s);
         {
             std::vector<int> ip(static_cast<size_t>(n));
-            for (int w = 0; w < n; ++w) ip[size_t(ord[size_t(w)])] = w;
+            for (int w = 0; w < n; ++w) ip[size_t(sweepOrd[size_t(w)])] = w;
             iperm.upload(ip, s);
         }
-        std::vector<int> lenL(static_cast<size_t>(n)), lenU(static_cast<size_t>(n));
-        for (int w = 0; w < n; ++w) { lenL[size_t(w)] = EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)]; lenU[size_t(w)] = EU.ptr[size_t(w) + 1] - EU.ptr[size_t(w)]; }
-        buildSlices(EL, lstart, rowCap(lenL), hL, sL);
-        buildSlices(EU, lstart, rowCap(lenU), hU, sU);
+        if (!tiled) {
+            buildSlices(EL, lstart, entriesCap(EL), hL, sL);
+            buildSlices(EU, lstart, entriesCap(EU), hU, sU);
+        }
         if (rowSys) {
             sliceOwners(hL, *rowSys);
             sliceOwners(hU, *rowSys);
@@ -332,14 +360,18 @@ struct IluDev {
         {
             const int nl = int(lstart.size()) - 1;
             std::vector<int> wL(static_cast<size_t>(nl)), wU(static_cast<size_t>(nl));
-            for (int l = 0; l < nl; ++l) {
+            for (int l = 0; l < nl && !tiled; ++l) {
                 wL[size_t(l)] = (sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)]) + (sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)]);
                 wU[size_t(l)] = (sU.sStart[size_t(l) + 1] - sU.sStart[size_t(l)]) + (sU.qStart[size_t(l) + 1] - sU.qStart[size_t(l)]);
             }
             std::vector<int> rowsL(static_cast<size_t>(nl));
             for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
-            runsL = levelRuns(wL, fuseMax, &rowsL, runRowsMax);
-            runsU = levelRuns(wU, fuseMax, &rowsL, runRowsMax);
+            runsL.clear();
+            runsU.clear();
+            if (!tiled) {
+                runsL = levelRuns(wL, fuseMax, &rowsL, runRowsMax);
+                runsU = levelRuns(wU, fuseMax, &rowsL, runRowsMax);
+            }
             denseL.build(runsL, EL, lstart, nb, false, s, rowSys);
             denseU.build(runsU, EU, lstart, nb, true, s, rowSys);
             buildFactorGroups(s);
@@ -354,6 +386,11 @@ struct IluDev {
             dLq.upload(sL.qStart, s);
             dUs.upload(sU.sStart, s);
             dUq.upload(sU.qStart, s);
+            if (tiled) {
+                tUnit.upload(tiles.unitStart, s);
+                tLvPtr.upload(tiles.lvPtr, s);
+                tLv.upload(tiles.lvStart, s);
+            }
             updateFirstCopy();
             // factor rows (F numbering) per level, long rows apart
             const HostBsr& F = mc ? fpat : S;
@@ -378,6 +415,88 @@ struct IluDev {
         rt64.alloc(size_t(kRunRowsMax) * nb);
         tail64.alloc(size_t(kRunRowsMax) * nb);
         nreg.alloc(1);
+    }
+    static int entriesCap(const RowEntries& E) {
+        const int m = int(E.ptr.size()) - 1;
+        std::vector<int> len(static_cast<size_t>(m));
+        for (int w = 0; w < m; ++w) len[size_t(w)] = E.ptr[size_t(w) + 1] - E.ptr[size_t(w)];
+        return rowCap(len);
+    }
+    // Tile extents for the natural-order sweeps from a cost model of one sweep: per tile level a launch and the
+    // unit's loads, then one barrier step per local level (tileLaunchUs + tileStepUs * depth + tileRowUs * rows), against
+    // levelLaunchUs per level launch (constants measured on MI355X with in-kernel clock stamps at 1.1M cells: dispatch
+    // gap + prologue + write-back about 8 us per launch, phase A about 5 ns per row, 0.35-0.55 us per local level).
+    // Tiles are used when the model makes them at least a quarter cheaper (schedule 0) or whenever the pattern admits
+    // them (schedule 2).
+    static constexpr double tileLaunchUs = 8.0, tileStepUs = 0.45, tileRowUs = 0.005, levelLaunchUs = 4.85;
+    bool chooseTiles(const HostBsr& S, int nlevels) {
+        int off[3] = {0, 0, 0};
+        std::vector<int> coord[3];
+        const int nd = gridCoords(S, off, coord);
+        if (nd == 0) return false;
+        int ext[3] = {1, 1, 1};
+        for (int a = 0; a < nd; ++a)
+            for (int i = 0; i < S.n; ++i) ext[a] = std::max(ext[a], coord[a][size_t(i)] + 1);
+        int best[3] = {1, 1, 1};
+        double bestCost = 1e300;
+        if (tileDims > 0) {
+            best[0] = tileDims & 255;
+            best[1] = (tileDims >> 8) & 255;
+            best[2] = (tileDims >> 16) & 255;
+            bestCost = 0;
+        } else {
+            static const int cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 30, 32, 40, 48, 64};
+            for (int b0 : cand)
+                for (int b1 : cand)
+                    for (int b2 : cand) {
+                        const int b[3] = {b0, b1, b2};
+                        bool ok = true;
+                        long long rows = 1;
+                        int ntl = 1, depth = 1;
+                        for (int a = 0; a < 3; ++a) {
+                            if (a >= nd) { ok = ok && b[a] == 1; continue; }
+                            const int e = std::min(b[a], ext[a]);
+                            rows *= e;
+                            ntl += (ext[a] + b[a] - 1) / b[a] - 1;
+                            depth += e - 1;
+                        }
+                        if (!ok || rows > (kTileWaves - 1) * kWave) continue;
+                        const double cost = ntl * (tileLaunchUs + tileStepUs * depth + tileRowUs * double(rows));
+                        if (cost < bestCost) {
+                            bestCost = cost;
+                            best[0] = b0; best[1] = b1; best[2] = b2;
+                        }
+                    }
+            if (schedule != 2 && bestCost > 0.75 * levelLaunchUs * nlevels) return false;
+        }
+        tileOrder(S, coord, nd, best, tiles);
+        return tiles.tileLevels() > 0;
+    }
+    // slices of the tiled sweeps (the units as the slice layout's levels) and the launch limits they must meet; false
+    // (level launches instead) when a unit needs more wavefronts or LDS than a workgroup has
+    bool tileSlices(const RowEntries& EL, const RowEntries& EU) {
+        buildSlices(EL, tiles.unitStart, entriesCap(EL), hL, sL);
+        buildSlices(EU, tiles.unitStart, entriesCap(EU), hU, sU);
+        tLongMax = 0;
+        tLongCap = 0;
+        for (int u = 0; u < tiles.units(); ++u) {
+            const int w0 = tiles.unitStart[size_t(u)], w1 = tiles.unitStart[size_t(u) + 1];
+            for (int side = 0; side < 2; ++side) {
+                const SliceHost& h = side ? hU : hL;
+                const LevelSched& sc = side ? sU : sL;
+                const int ns = sc.sStart[size_t(u) + 1] - sc.sStart[size_t(u)], nq = sc.qStart[size_t(u) + 1] - sc.qStart[size_t(u)];
+                if (ns + (nq > 0 ? 1 : 0) > kTileWaves) return false;
+                tLongMax = std::max(tLongMax, nq);
+                for (int q = sc.qStart[size_t(u)]; q < sc.qStart[size_t(u) + 1]; ++q) {
+                    int in = 0;
+                    for (int p = h.lptr[size_t(q)]; p < h.lptr[size_t(q) + 1]; ++p) in += h.lcol[size_t(p)] >= w0 && h.lcol[size_t(p)] < w1;
+                    tLongCap = std::max(tLongCap, in);
+                }
+            }
+        }
+        tLongCap = std::max(tLongCap, 1);
+        if (tLongMax > kWave) return false;   // the long-row wavefront gives each long row of a unit at least one lane
+        return tileLdsBytes(nb, kIluFp64, tiles.maxRows, tiles.maxLevels, tLongMax, tLongCap) <= kTileLdsMax;
     }
     void ensure32(hipStream_t s) {
         if (have32) return;
@@ -469,7 +588,13 @@ struct IluDev {
         else RSH_HIP(hipMemcpyAsync(lu.p, val, nvals * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (lean) launchRoundToFloat(s, (long long)nvals, lu.p);
         nreg.zero(s);
-        for (size_t ri = 0; ri < runsL.size(); ++ri) {
+        if (!mc) {
+            for (int l = 0; l + 1 < int(fShortStart.size()); ++l) {
+                launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, -1, lean);
+                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, lean);
+            }
+        }
+        for (size_t ri = 0; mc && ri < runsL.size(); ++ri) {
             const LevelRun& run = runsL[ri];
             if (ri < fgRun0.size() && fgRun0[ri] >= 0) {
                 // the run's rows: the contributions of the rows before the run, then the groups right-looking
@@ -524,6 +649,16 @@ struct IluDev {
             }
             launchIluFused(s, nb, fp, sl, fwd, run.l0, run.l1, fwd ? dLs.p : dUs.p, fwd ? dLq.p : dUq.p, wb, we - wb, perm.p, r, y, z, xp);
         };
+        if (tiled) {
+            const int ntl = tiles.tileLevels();
+            for (int t = 0; t < ntl; ++t)
+                launchIluTile(s, nb, fp, Lp, true, tiles.tlStart[size_t(t)], tiles.tlStart[size_t(t) + 1] - tiles.tlStart[size_t(t)], tUnit.p, dLs.p,
+                              dLq.p, tLvPtr.p, tLv.p, tiles.maxRows, tiles.maxLevels, tLongMax, tLongCap, perm.p, r, y, z, xp);
+            for (int t = ntl; t-- > 0;)
+                launchIluTile(s, nb, fp, Up, false, tiles.tlStart[size_t(t)], tiles.tlStart[size_t(t) + 1] - tiles.tlStart[size_t(t)], tUnit.p, dUs.p,
+                              dUq.p, tLvPtr.p, tLv.p, tiles.maxRows, tiles.maxLevels, tLongMax, tLongCap, perm.p, r, y, z, xp);
+            return;
+        }
         for (size_t ri = 0; ri < runsL.size(); ++ri) {
             const LevelRun& run = runsL[ri];
             if (run.fused) { sweepRun(run, ri, true); continue; }

@@ -117,6 +117,45 @@ void run(const Input& in, int id) {
     bool okRuns = !runs.empty() && runs.front().l0 == 0 && runs.back().l1 == sL.levels();
     for (size_t k = 1; k < runs.size() && okRuns; ++k) okRuns = runs[k].l0 == runs[k - 1].l1;
     check(okRuns, "level runs cover the levels in order", id);
+    // tiled schedules of the natural-order sweeps (several tile shapes): a permutation, units contiguous, and every
+    // coupled pair j < i either in an earlier tile level or in the same unit on an earlier local level
+    int off[3] = {0, 0, 0};
+    std::vector<int> coord[3];
+    const int nd = gridCoords(A, off, coord);
+    int ntl = 0;
+    static const int shapes[4][3] = {{1, 1, 1}, {4, 4, 4}, {8, 3, 2}, {64, 64, 64}};
+    for (const int* dims : shapes) {
+        TileSched T;
+        tileOrder(A, coord, nd, dims, T);
+        std::vector<int> unitOf(size_t(A.n), -1), tlOf(size_t(A.n), -1), lvOf(size_t(A.n), -1);
+        bool okT = int(T.ord.size()) == A.n && T.unitStart.front() == 0 && T.unitStart.back() == A.n && T.tlStart.back() == T.units() &&
+                   int(T.lvPtr.size()) == T.units() + 1;
+        for (int t = 0; t < T.tileLevels() && okT; ++t)
+            for (int u = T.tlStart[size_t(t)]; u < T.tlStart[size_t(t) + 1] && okT; ++u) {
+                const int w0 = T.unitStart[size_t(u)], w1 = T.unitStart[size_t(u) + 1];
+                const int l0 = T.lvPtr[size_t(u)], l1 = T.lvPtr[size_t(u) + 1];
+                okT = w1 > w0 && w1 - w0 <= T.maxRows && l1 - l0 >= 2 && T.lvStart[size_t(l0)] == 0 && T.lvStart[size_t(l1) - 1] == w1 - w0 &&
+                      l1 - l0 - 1 <= T.maxLevels;
+                for (int l = l0; l + 1 < l1 && okT; ++l)
+                    for (int w = w0 + T.lvStart[size_t(l)]; w < w0 + T.lvStart[size_t(l) + 1] && okT; ++w) {
+                        const int i = T.ord[size_t(w)];
+                        okT = i >= 0 && i < A.n && unitOf[size_t(i)] < 0;
+                        if (!okT) break;
+                        unitOf[size_t(i)] = u;
+                        tlOf[size_t(i)] = t;
+                        lvOf[size_t(i)] = l - l0;
+                    }
+            }
+        for (int i = 0; i < A.n && okT; ++i)
+            for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1] && okT; ++p) {
+                const int j = A.ci[size_t(p)];
+                const int lo = std::min(i, j), hi = std::max(i, j);
+                if (lo == hi) continue;
+                okT = tlOf[size_t(lo)] < tlOf[size_t(hi)] || (unitOf[size_t(lo)] == unitOf[size_t(hi)] && lvOf[size_t(lo)] < lvOf[size_t(hi)]);
+            }
+        check(okT, "tiled sweep schedule", id);
+        if (dims[0] == 4) ntl = T.tileLevels();
+    }
     // the pressure-pressure coefficients as a scalar matrix: AMG set-up (strength, aggregation, Galerkin maps)
     HostCsr C;
     C.n = A.n;
@@ -201,8 +240,8 @@ void run(const Input& in, int id) {
         wellplan::recover(W, in.jci.data(), jv, dInv, b.data(), bc.data(), x.data());
         fill = W.fill();
     }
-    std::printf("system %d: n %d nb %d levels %d colours %d L slices %d AMG levels %zu well fill %d\n", id, A.n, A.nb, sL.levels(), ncol, hL.nslices,
-                H.size(), fill);
+    std::printf("system %d: n %d nb %d levels %d colours %d L slices %d AMG levels %zu well fill %d grid offsets %d tile levels (4^3) %d\n", id,
+                A.n, A.nb, sL.levels(), ncol, hL.nslices, H.size(), fill, nd, ntl);
 }
 }  // namespace
 

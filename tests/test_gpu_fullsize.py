@@ -61,12 +61,20 @@ def test_full_size_mcbilu0_apply(big, orc):
     assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
 
 
-def test_full_size_bilu0_apply(big, orc):
-    # the natural-order factors and level-scheduled sweeps against the oracle's natural-order BILU0
-    print("full-size natural-order BILU0 apply", flush=True)
-    d = rs.System(big).setup(precond="bilu0")
+@pytest.fixture(scope="module")
+def bilu0_ref(big, orc):
     r = np.cos(0.7 * np.arange(big.n * big.nb))
-    z, zo = d.precond(r), orc.bilu0(big, 1).apply(r)
+    return r, orc.bilu0(big, 1).apply(r)
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_full_size_bilu0_apply(big, bilu0_ref, schedule):
+    # the natural-order factors, swept level by level (1) or tiled (2), against the oracle's natural-order BILU0
+    print(f"full-size natural-order BILU0 apply, schedule {schedule}", flush=True)
+    d = rs.System(big).setup(precond="bilu0", ilu_schedule=schedule)
+    assert d.info()["ilu_tiled"] == (schedule == 2)
+    r, zo = bilu0_ref
+    z = d.precond(r)
     assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
 
 

@@ -61,16 +61,47 @@ def test_spmv_matches_oracle(sysd, orc, name):
     assert np.max(np.abs(y - yo)) <= 1e-13 * np.max(np.abs(yo))
 
 
+# natural-order sweeps: one launch per level, or tiled (one launch per tile level, one workgroup per tile walking its
+# own levels) with the cost model's tiles and with fixed tile shapes (bx | by << 8 | bz << 16)
+SCHEDULES = {"levels": (1, 0), "tiles": (2, 0), "tiles444": (2, 4 | 4 << 8 | 4 << 16), "tiles832": (2, 8 | 3 << 8 | 2 << 16),
+             "tiles1": (2, 1 | 1 << 8 | 1 << 16)}
+
+
+@pytest.mark.parametrize("sched", list(SCHEDULES))
 @pytest.mark.parametrize("name", ["small", "laplace", "chain", "nb2", "nb4", "nb1", "layered", "deepwell"])
-def test_bilu0_factors_and_apply(sysd, orc, name):
+def test_bilu0_factors_and_apply(sysd, orc, name, sched):
     s = sysd[name]
-    d = rs.System(s).setup(precond="bilu0")
+    schedule, dims = SCHEDULES[sched]
+    d = rs.System(s).setup(precond="bilu0", ilu_schedule=schedule, ilu_tile_dims=dims)
+    info = d.info()
+    assert info["ilu_tiled"] == (schedule == 2), info
     o = orc.bilu0(s, 1)
     lu, luo = d.bilu0_factors(), o.factors()
     assert np.array_equal(lu, luo), "level-scheduled factorisation reproduces the natural-order factors bit for bit"
     r = np.cos(0.7 * np.arange(s.n * s.nb))
     z, zo = d.precond(r), o.apply(r)
     assert np.max(np.abs(z - zo)) <= 1e-12 * np.max(np.abs(zo))
+    if schedule == 2 and info["ilu_long_rows_L"] == 0 and info["ilu_long_rows_U"] == 0:
+        # every row swept by one lane in its slot order: the oracle's arithmetic, bit for bit
+        assert np.array_equal(z, zo)
+
+
+@pytest.mark.parametrize("fp", ["mixed", "store32"])
+@pytest.mark.parametrize("name", ["layered", "deepwell", "nb2", "nb4"])
+def test_bilu0_tiled_fp32_forms(sysd, orc, name, fp):
+    # FP32 factors with FP32 vectors (the mixed inner preconditioner), and FP32-stored factors with FP64 arithmetic:
+    # the tiled sweeps against the level-scheduled ones in the same precision
+    s = sysd[name]
+    precision = rs.MIXED if fp == "mixed" else rs.MIXED_STORE
+    r = np.cos(0.7 * np.arange(s.n * s.nb))
+    zl = rs.System(s).setup(precond="bilu0", precision=precision, ilu_schedule=1).precond(r)
+    dt = rs.System(s).setup(precond="bilu0", precision=precision, ilu_schedule=2)
+    assert dt.info()["ilu_tiled"] == 1
+    zt = dt.precond(r)
+    zo = orc.bilu0(s, 1).apply(r)
+    tol = 1e-4 if fp == "mixed" else 1e-12
+    assert np.max(np.abs(zt - zl)) <= tol * np.max(np.abs(zl))
+    assert np.max(np.abs(zt - zo)) <= (1e-4 if fp == "mixed" else 1e-3) * np.max(np.abs(zo))
 
 
 @pytest.mark.parametrize("name", ["small", "laplace", "layered", "nb2", "deepwell", "deepwell_nb2", "deepwell_nb4"])
