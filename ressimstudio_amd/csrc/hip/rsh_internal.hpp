@@ -268,9 +268,13 @@ void launchBlockJacobi(hipStream_t s, int nb, int n, const void* dinv, bool f32,
 void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed);
 void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* rows, const int* rp, const int* ci, const int* diag, const double* v,
                    const double* b, const double* xin, double* xout, bool forward);
-// BILU0 elimination of long rows rows[r0..r1): one wavefront per row
+// BILU0 elimination of long rows rows[r0..r1): one wavefront per row.  With map (device arrays): the row held in LDS
+// and its updates taken from the precomputed pairs (for long row t = rows index, lower entry p of row i:
+// map[mptr[mrow[t] + p - rp[i]] ..] = {U block q of row ci[p], entry offset s in row i}); maxLen = the longest row
+struct LongRowMap { const int* mrow; const int* mptr; const int2* map; int maxLen; };
+constexpr int kLongRowLdsMax = 48 * 1024;
 void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
-                         bool lean = false);
+                         bool lean = false, const LongRowMap* map = nullptr);
 // BILU0 numeric factorisation of the rows of one level (in place, natural layout); stop >= 0: only the lower entries
 // in columns below stop, and the pivot is left for kIluGroupFactor
 // (lean: every finished row is stored rounded to float32, the C45 lean factorisation)

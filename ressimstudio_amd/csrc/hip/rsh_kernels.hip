@@ -1611,6 +1611,133 @@ __global__ __launch_bounds__(kWave) void kBilu0LongRows(const int* __restrict__ 
     }
 }
 
+// The same elimination through a precomputed update map, the row held in LDS (natural order, where the well rows are
+// long).  For long row t (rows[t] = i) and its lower entry p, the pairs map[mptr[mrow[t] + p - rp[i]] ..] = {q, s}
+// list the U blocks q of row k = ci[p] that meet the row, s being the entry they update (offset from rp[i]).  The rows
+// before i are final, so the next step's pivot and U blocks are loaded while the current step runs; the sequential
+// chain is the row itself, in LDS.  Every entry receives the same updates in the same order with the same block
+// arithmetic as kBilu0LongRows: the factors are bit-identical.
+template <int NB>
+__global__ __launch_bounds__(kWave) void kBilu0LongMapped(const int* __restrict__ rows, int r0, const int* __restrict__ rp,
+                                                          const int* __restrict__ ci, const int* __restrict__ diag, double* val,
+                                                          int* __restrict__ nreg, int lean, const int* __restrict__ mrow,
+                                                          const int* __restrict__ mptr, const int2* __restrict__ map) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smemRow[];
+    double* rowv = reinterpret_cast<double*>(smemRow);
+    constexpr int B2 = NB * NB;
+    const int t = r0 + blockIdx.x;
+    const int i = rows[t];
+    const int lane = threadIdx.x;
+    const int pb = rp[i], pe = rp[i + 1], pd = diag[i];
+    const int m0 = mrow[t];
+    for (int e = lane; e < (pe - pb) * B2; e += kWave) rowv[e] = val[(long long)pb * B2 + e];
+    // one step's operands: the pivot inverse of row k = ci[p] and this lane's first pair
+    auto fetch = [&](int p, double (&dk)[B2], double (&u)[B2], int& sl, int& cnt) {
+        const int k = ci[p];
+        const double* d = val + (long long)diag[k] * B2;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) dk[e] = d[e];
+        const int a = mptr[m0 + (p - pb)];
+        cnt = mptr[m0 + (p - pb) + 1] - a;
+        sl = -1;
+        if (lane < cnt) {
+            const int2 qs = map[a + lane];
+            sl = qs.y;
+            const double* uu = val + (long long)qs.x * B2;
+#pragma unroll
+            for (int e = 0; e < B2; ++e) u[e] = uu[e];
+        }
+    };
+    double dkC[B2], uC[B2];
+    int sC = -1, cntC = 0;
+#pragma unroll
+    for (int e = 0; e < B2; ++e) { dkC[e] = 0; uC[e] = 0; }
+    if (pb < pd) fetch(pb, dkC, uC, sC, cntC);
+    __syncthreads();
+    for (int p = pb; p < pd; ++p) {
+        double dkN[B2], uN[B2];
+        int sN = -1, cntN = 0;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) { dkN[e] = 0; uN[e] = 0; }
+        if (p + 1 < pd) fetch(p + 1, dkN, uN, sN, cntN);
+        // L_ik = A_ik U_kk^-1
+        double L[B2];
+        {
+            const double* ap = rowv + (p - pb) * B2;
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) rr[j] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = ap[ii * NB + kk];
+#pragma unroll
+                    for (int j = 0; j < NB; ++j) rr[j] += f * dkC[kk * NB + j];
+                }
+#pragma unroll
+                for (int j = 0; j < NB; ++j) L[ii * NB + j] = rr[j];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int e = 0; e < B2; ++e) rowv[(p - pb) * B2 + e] = L[e];
+        }
+        // A_is -= L_ik U_kj for the pairs of this step (distinct s > p)
+        auto update = [&](int sl, const double (&u)[B2]) {
+            double* as = rowv + sl * B2;
+#pragma unroll
+            for (int ii = 0; ii < NB; ++ii) {
+                double rr[NB];
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) rr[jj] = 0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double f = L[ii * NB + kk];
+#pragma unroll
+                    for (int jj = 0; jj < NB; ++jj) rr[jj] += f * u[kk * NB + jj];
+                }
+#pragma unroll
+                for (int jj = 0; jj < NB; ++jj) as[ii * NB + jj] -= rr[jj];
+            }
+        };
+        if (sC >= 0) update(sC, uC);
+        if (cntC > kWave) {
+            const int a = mptr[m0 + (p - pb)];
+            for (int j = lane + kWave; j < cntC; j += kWave) {
+                const int2 qs = map[a + j];
+                double u[B2];
+#pragma unroll
+                for (int e = 0; e < B2; ++e) u[e] = val[(long long)qs.x * B2 + e];
+                update(qs.y, u);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < B2; ++e) { dkC[e] = dkN[e]; uC[e] = uN[e]; }
+        sC = sN;
+        cntC = cntN;
+    }
+    if (lane == 0) {
+        double d[B2];
+        double* dp = rowv + (pd - pb) * B2;
+#pragma unroll
+        for (int e = 0; e < B2; ++e) d[e] = dp[e];
+        const int reg = invertPivotDev<NB>(d);
+#pragma unroll
+        for (int e = 0; e < B2; ++e) dp[e] = d[e];
+        if (reg && nreg) atomicAdd(nreg, 1);
+    }
+    __syncthreads();
+    // the finished row to HBM (lean, C45: stored rounded to float32)
+    for (int e = lane; e < (pe - pb) * B2; e += kWave) {
+        const double v = rowv[e];
+        val[(long long)pb * B2 + e] = lean ? double(float(v)) : v;
+    }
+}
+
 // ---------------------------------------------------------------- CPR (bsr.cpp:993-1084)
 // quasi-IMPES weights (bsr.cpp:1000-1030): w_i = first row of D_ii^-1; singular -> 1 / sum_e D[e][0]; non-finite or
 // zero -> 1.  With acc (n accumulation blocks B_i), true-IMPES: B_i is inverted and, if its 1-norm condition estimate
@@ -2609,6 +2736,11 @@ void iluTileT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int u0, int nu, co
                             levelsCap, longMax, longCap, perm, r, y, z, xp);
 }
 template <int NB>
+void bilu0MappedT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg, int lean,
+                  const LongRowMap& m, int lds) {
+    hipLaunchKernelGGL(kBilu0LongMapped<NB>, dim3(r1 - r0), dim3(kWave), lds, s, rows, r0, rp, ci, diag, val, nreg, lean, m.mrow, m.mptr, m.map);
+}
+template <int NB>
 void bilu0LongT(hipStream_t s, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
                 int lean) {
     hipLaunchKernelGGL(kBilu0LongRows<NB>, dim3(r1 - r0), dim3(kWave), 0, s, rows, r0, rp, ci, diag, val, nreg, lean);
@@ -2820,9 +2952,15 @@ void launchGsFused(hipStream_t s, int l0, int l1, const int* lstart, const int* 
     checkLaunch();
 }
 void launchBilu0LongRows(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,
-                         bool lean) {
+                         bool lean, const LongRowMap* map) {
     if (r1 <= r0) return;
-    RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, lean ? 1 : 0)));
+    if (map && map->maxLen > 0) {
+        const int lds = map->maxLen * nb * nb * int(sizeof(double));
+        if (lds > kLongRowLdsMax) throw Error(-1, "long-row elimination: row longer than the LDS buffer");
+        RSH_NB_DISPATCH(nb, (bilu0MappedT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, lean ? 1 : 0, *map, lds)));
+    } else {
+        RSH_NB_DISPATCH(nb, (bilu0LongT<NB>(s, rows, r0, r1, rp, ci, diag, val, nreg, lean ? 1 : 0)));
+    }
     checkLaunch();
 }
 void launchBilu0Level(hipStream_t s, int nb, const int* rows, int r0, int r1, const int* rp, const int* ci, const int* diag, double* val, int* nreg,

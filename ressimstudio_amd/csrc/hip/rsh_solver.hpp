@@ -276,6 +276,10 @@ struct IluDev {
     TileSched tiles;
     DBuf<int> tUnit, tLs, tLq, tUs, tUq, tLvPtr, tLv;
     int tLongMax = 0, tLongCap = 0;
+    // natural order: the long rows' update maps (maxLen 0: none, the long rows use kBilu0LongRows)
+    DBuf<int> lmRow, lmPtr;
+    DBuf<int2> lmMap;
+    LongRowMap longMap{nullptr, nullptr, nullptr, 0};
     // kept when keepEntries is set (batched systems re-chunk the levels when tuning)
     bool keepEntries = false;
     RowEntries keptL, keptU;
@@ -401,13 +405,51 @@ struct IluDev {
                 for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) {
                     const int i = mc ? w : ord[size_t(w)];
                     const int lower = F.diag[size_t(i)] - F.rp[size_t(i)];
-                    (lower > 24 ? lo : sh).push_back(i);
+                    // natural order: a row whose pivot rows carry long upper parts (the neighbours of a well's perforated
+                    // cells walk every perforation's U entries) is eliminated like a long row, through its update map
+                    int walk = 0;
+                    if (!mc)
+                        for (int p = F.rp[size_t(i)]; p < F.diag[size_t(i)]; ++p) {
+                            const int k = F.ci[size_t(p)];
+                            walk += F.rp[size_t(k) + 1] - F.diag[size_t(k)] - 1;
+                        }
+                    (lower > 24 || walk > 48 ? lo : sh).push_back(i);
                 }
                 fShortStart.push_back(int(sh.size()));
                 fLongStart.push_back(int(lo.size()));
             }
             fShort.upload(sh, s);
             fLong.upload(lo, s);
+            // natural order: the long rows' update maps (per lower entry, the pivot row's U blocks that meet the row and
+            // the entry each updates), so that their elimination runs from LDS (kBilu0LongMapped)
+            longMap = LongRowMap{nullptr, nullptr, nullptr, 0};
+            if (!mc && !lo.empty()) {
+                std::vector<int> mrow, mptr(1, 0);
+                std::vector<int2> mp;
+                int maxLen = 0;
+                for (size_t t = 0; t < lo.size(); ++t) {
+                    const int i = lo[t];
+                    const int pb = F.rp[size_t(i)], pe = F.rp[size_t(i) + 1], pd = F.diag[size_t(i)];
+                    mrow.push_back(int(mptr.size()) - 1);
+                    maxLen = std::max(maxLen, pe - pb);
+                    for (int p = pb; p < pd; ++p) {
+                        const int k = F.ci[size_t(p)];
+                        for (int q = F.diag[size_t(k)] + 1; q < F.rp[size_t(k) + 1]; ++q) {
+                            const int* b = F.ci.data() + p + 1;
+                            const int* e = F.ci.data() + pe;
+                            const int* f = std::lower_bound(b, e, F.ci[size_t(q)]);
+                            if (f != e && *f == F.ci[size_t(q)]) mp.push_back(make_int2(q, int(f - F.ci.data()) - pb));
+                        }
+                        mptr.push_back(int(mp.size()));
+                    }
+                }
+                if (size_t(maxLen) * size_t(nb) * size_t(nb) * sizeof(double) <= size_t(kLongRowLdsMax)) {
+                    lmRow.upload(mrow, s);
+                    lmPtr.upload(mptr, s);
+                    lmMap.upload(mp, s);
+                    longMap = LongRowMap{lmRow.p, lmPtr.p, lmMap.p, maxLen};
+                }
+            }
         }
         lu.alloc(size_t(S.rp[size_t(n)]) * nb * nb);
         y64.alloc(size_t(n + 1) * nb);
@@ -591,7 +633,8 @@ struct IluDev {
         if (!mc) {
             for (int l = 0; l + 1 < int(fShortStart.size()); ++l) {
                 launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, -1, lean);
-                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, lean);
+                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, lean,
+                                    longMap.maxLen > 0 ? &longMap : nullptr);
             }
         }
         for (size_t ri = 0; mc && ri < runsL.size(); ++ri) {
@@ -676,7 +719,8 @@ struct IluDev {
         }
     }
     size_t bytes() const {
-        return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes() + denseL.bytes() + denseU.bytes();
+        return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes() + denseL.bytes() + denseU.bytes() +
+               lmRow.bytes() + lmPtr.bytes() + lmMap.bytes() + tUnit.bytes() + tLvPtr.bytes() + tLv.bytes();
     }
 };
 
