@@ -163,9 +163,9 @@ struct RowEntries {
     std::vector<int> orig;   // working row -> original row
     std::vector<int> dsrc;   // working row -> diagonal block (or empty)
 };
-// longRow (optional, per working row): rows taken as long rows whatever their length
+// longRow (optional, per working row): rows taken as long rows whatever their length; skipRow (optional): rows left out
 void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L,
-                 const std::vector<char>* longRow = nullptr);
+                 const std::vector<char>* longRow = nullptr, const std::vector<char>* skipRow = nullptr);
 // one-level row slices of a CSR / BSR pattern in natural row order: rows longer than the bulk are long rows, padding
 // gathers the lane's own row, and the 16-bit relative columns are kept when every slice row fits them.  coarse (an
 // AMG level): the bulk ends a quarter above the 99th percentile of the row lengths, and rows with a column beyond
@@ -219,9 +219,10 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
 // precision of an ILU apply: FP64 factors and vectors; FP32 factors and FP32 working vector (FP32 arithmetic);
 // FP32-stored factors with the FP64 working vector and FP64 arithmetic (an exactly linear FP64 operator)
 enum IluFp { kIluFp64 = 0, kIluFp32 = 1, kIluStore32 = 2 };
-// ILU sweeps of one level (fp: IluFp)
+// ILU sweeps of one level (fp: IluFp).  zFused (forward): the rows whose slice lane carries a pivot block (dsrc >= 0:
+// no upper entries, read by no later row) finish their backward step here, z = D^-1 y into y and z (+ xp)
 void launchIluForward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
-                      const KScalars* gate = nullptr);
+                      const KScalars* gate = nullptr, double* zFused = nullptr, const double* xp = nullptr);
 void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,
                        const double* xp, const KScalars* gate = nullptr, bool writeY = true);
 // a run of small levels [l0, l1) swept by one workgroup (level starts on the device); its rows are the working rows

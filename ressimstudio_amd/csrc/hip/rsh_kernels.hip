@@ -315,7 +315,8 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
                                                       const int* __restrict__ lptr, const int* __restrict__ lcol,
                                                       const TA* __restrict__ lval, const double* __restrict__ r, TV* __restrict__ y,
                                                       const int* __restrict__ owner, const int* __restrict__ lowner,
-                                                      const KScalars* __restrict__ gate) {
+                                                      const KScalars* __restrict__ gate, const int* __restrict__ fdsrc,
+                                                      const TA* __restrict__ fdval, double* __restrict__ z, const double* __restrict__ xp) {
 #pragma clang fp contract(off)
     const int wid = blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -330,6 +331,26 @@ __global__ __launch_bounds__(kBlock) void kIluForward(int s0, int ns, int q0, in
 #pragma unroll
         for (int e = 0; e < NB; ++e) acc[e] = TV(r[(long long)o * NB + e]);
         sliceRowSweep<NB, TA, TV>(col, val, inf, lane, y, acc);
+        if (fdval && fdsrc[(long long)sl * kWave + lane] >= 0) {
+            // no upper entries and no later reader: the backward step z = D^-1 y now (kIluBackward's arithmetic)
+            const TA* d = fdval + (long long)sl * (NB * NB * kWave) + lane;
+            TV t[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                TV sacc = 0;
+#pragma unroll
+                for (int f = 0; f < NB; ++f) sacc += TV(d[(e * NB + f) * kWave]) * acc[f];
+                t[e] = sacc;
+            }
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                y[(long long)wr * NB + e] = t[e];
+                double v = double(t[e]);
+                if (e == 0 && xp) v += xp[o];
+                z[(long long)o * NB + e] = v;
+            }
+            return;
+        }
 #pragma unroll
         for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
         return;
@@ -2767,11 +2788,14 @@ void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<
 }
 
 template <int NB, typename TA, typename TV>
-void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, const double* r, TV* y, const KScalars* gate) {
+void iluFwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, const double* r, TV* y, const KScalars* gate, double* zFused,
+             const double* xp) {
     const int ns = s1 - s0, nq = q1 - q0, waves = ns + nq;
     if (!waves) return;
+    const bool fused = zFused && d.dsrc.n && d.dval.n;
     hipLaunchKernelGGL((kIluForward<NB, TA, TV>), dim3((waves + 3) / 4), dim3(kBlock), 0, s, s0, ns, q0, nq, d.info.p, d.row.p, d.orig.p,
-                       d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y, d.owner.p, d.lowner.p, gate);
+                       d.col.p, d.val.p, d.lrow.p, d.lorig.p, d.lptr.p, d.lcol.p, d.lval.p, r, y, d.owner.p, d.lowner.p, gate,
+                       fused ? d.dsrc.p : nullptr, fused ? d.dval.p : nullptr, zFused, xp);
 }
 template <int NB, typename TA, typename TV>
 void iluBwdT(hipStream_t s, const SliceDev<TA>& d, int s0, int s1, int q0, int q1, TV* y, double* z, const double* xp, const KScalars* gate,
@@ -2907,10 +2931,10 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
     checkLaunch();
 }
 void launchIluForward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
-                      const KScalars* gate) {
-    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y, gate)))
-    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (double*)y, gate)))
-    else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y, gate)))
+                      const KScalars* gate, double* zFused, const double* xp) {
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (float*)y, gate, zFused, xp)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluFwdT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), s0, s1, q0, q1, r, (double*)y, gate, zFused, xp)))
+    else RSH_NB_DISPATCH(nb, (iluFwdT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), s0, s1, q0, q1, r, (double*)y, gate, zFused, xp)))
     checkLaunch();
 }
 void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z, const double* xp,

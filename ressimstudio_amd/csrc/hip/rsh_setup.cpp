@@ -160,7 +160,8 @@ int rowCap(const std::vector<int>& lengths) {
     return mx <= cap + cap / 2 + 1 ? mx : cap;   // no long rows unless some rows are well beyond the bulk
 }
 
-void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L, const std::vector<char>* longRow) {
+void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int cap, SliceHost& S, LevelSched& L, const std::vector<char>* longRow,
+                 const std::vector<char>* skipRow) {
     const int nl = int(levelStart.size()) - 1;
     const int n = int(E.ptr.size()) - 1;
     const bool withDiag = !E.dsrc.empty();
@@ -172,6 +173,7 @@ void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int ca
     for (int l = 0; l < nl; ++l) {
         shortRows.clear();
         for (int w = levelStart[size_t(l)]; w < levelStart[size_t(l) + 1]; ++w) {
+            if (skipRow && (*skipRow)[size_t(w)]) continue;
             const int len = E.ptr[size_t(w) + 1] - E.ptr[size_t(w)];
             if (len > cap || (longRow && (*longRow)[size_t(w)])) {
                 S.lrow.push_back(w);

@@ -276,6 +276,9 @@ struct IluDev {
     TileSched tiles;
     DBuf<int> tUnit, tLs, tLq, tUs, tUq, tLvPtr, tLv;
     int tLongMax = 0, tLongCap = 0;
+    // multicolour order: some rows finish their backward step in the forward launch (their lower slice lanes carry the
+    // pivot block, dsrc >= 0) and are left out of the upper slices
+    bool fuseBwd = false;
     // the last run of well-clique colours swept in one launch (kRunTail): its forward and backward dense groups
     // coincide and its rows have no upper couplings outside it; per group M = Uinv Linv (tOffM, per forward dense
     // group), and the outside entries of the run's rows over the forward dense side's row list
@@ -362,9 +365,34 @@ struct IluDev {
             for (int w = 0; w < n; ++w) ip[size_t(sweepOrd[size_t(w)])] = w;
             iperm.upload(ip, s);
         }
+        fuseBwd = false;
         if (!tiled) {
-            buildSlices(EL, lstart, entriesCap(EL), hL, sL);
-            buildSlices(EU, lstart, entriesCap(EU), hU, sU);
+            // multicolour order: rows of a large level (beyond any fused run) with no upper entries that no later row
+            // reads finish their backward step in the forward launch (z = D^-1 y, the pivot block carried by the lower
+            // slices), and are left out of the upper slices
+            std::vector<char> skipU;
+            const int capL = entriesCap(EL);
+            if (mc && !rowSys) {
+                std::vector<char> read(static_cast<size_t>(n), 0);
+                for (int c : EL.col) if (c >= 0 && c < n) read[size_t(c)] = 1;
+                skipU.assign(size_t(n), 0);
+                EL.dsrc.assign(size_t(n), -1);
+                for (int l = 1; l + 1 < int(lstart.size()); ++l) {
+                    if (lstart[size_t(l) + 1] - lstart[size_t(l)] <= kRunRowsMax) continue;
+                    for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) {
+                        if (EU.ptr[size_t(w) + 1] != EU.ptr[size_t(w)] || read[size_t(w)] || EL.ptr[size_t(w) + 1] - EL.ptr[size_t(w)] > capL) continue;
+                        skipU[size_t(w)] = 1;
+                        EL.dsrc[size_t(w)] = EU.dsrc[size_t(w)];
+                        fuseBwd = true;
+                    }
+                }
+                if (!fuseBwd) {
+                    EL.dsrc.clear();
+                    skipU.clear();
+                }
+            }
+            buildSlices(EL, lstart, capL, hL, sL);
+            buildSlices(EU, lstart, entriesCap(EU), hU, sU, nullptr, skipU.empty() ? nullptr : &skipU);
         }
         if (rowSys) {
             sliceOwners(hL, *rowSys);
@@ -785,7 +813,7 @@ struct IluDev {
             const int l = run.l0;
             if (l == 0 && firstDone) continue;   // the caller wrote y of the first level
             launchIluForward(s, nb, fp, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
-                             gate);
+                             gate, fuseBwd ? z : nullptr, xp);
         }
         for (size_t k = runsU.size(); k-- > 0;) {
             const LevelRun& run = runsU[k];
