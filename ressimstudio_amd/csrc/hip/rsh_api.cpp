@@ -763,7 +763,7 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
           << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks << ",\"device_bytes\":" << S.deviceBytes();
         if (S.ilu) {
             const IluDev& I = *S.ilu;
-            o << ",\"ilu_tiled\":" << (I.tiled ? 1 : 0) << ",\"ilu_tail_fused\":" << (I.tailOn ? 1 : 0) << ",\"ilu_tail_copy\":" << (I.tailCopy ? 1 : 0) << ",\"ilu_bwd_fused\":" << (I.fuseBwd ? 1 : 0);
+            o << ",\"ilu_tiled\":" << (I.tiled ? 1 : 0) << ",\"ilu_tail_copy\":" << (I.tailCopy ? 1 : 0) << ",\"ilu_bwd_fused\":" << (I.fuseBwd ? 1 : 0);
             if (I.tiled)
                 o << ",\"ilu_tiles\":{\"dims\":[" << I.tiles.dims[0] << "," << I.tiles.dims[1] << "," << I.tiles.dims[2] << "],\"tile_levels\":" << I.tiles.tileLevels()
                   << ",\"units\":" << I.tiles.units() << ",\"max_rows\":" << I.tiles.maxRows << ",\"max_levels\":" << I.tiles.maxLevels

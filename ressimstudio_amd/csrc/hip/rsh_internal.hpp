@@ -249,15 +249,6 @@ void launchRunDense(hipStream_t s, int nb, int fp, const void* slices, bool fwd,
                     int ntasks, const int2* tasks, const int* start, const int* rows, const long long* off, const void* inv,
                     const int* perm, const double* r, void* rt, void* y, double* z, const double* xp, const int* gowner = nullptr,
                     const KScalars* gate = nullptr, void* ycopy = nullptr, bool inv64 = false);
-// the last run of well-clique colours in one launch (no upper couplings outside the run): M_G = Uinv_G Linv_G per group
-// (column-major K x K at offM), then per task {group, first scalar row} rt_G from the member rows' outside entries
-// (outPtr / outCol / outVal over the groups' rows in order, values FP64 or FP32 like the factor slices) and its rows of
-// M_G rt_G into y (working numbering) and z (original numbering, + xp on the pressure unknown)
-void launchDenseMul(hipStream_t s, int nb, int ngroups, int maxRows, const int* start, const long long* offU, const long long* offL,
-                    const long long* offM, const double* U, const double* L, double* M);
-void launchRunTail(hipStream_t s, int nb, int fp, bool inv64, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* offM,
-                   const void* M, const int* outPtr, const int* outCol, const void* outVal, const double* r, void* y, double* z, const int* perm,
-                   const double* xp, const int* gowner, const KScalars* gate);
 // numeric set-up of the dense groups of one sweep direction: gather the factor blocks, invert per group
 void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRows, const int* start, const long long* off, int nmap, const int4* map,
                        const double* lu, double* raw, long long rawElems, double* inv);

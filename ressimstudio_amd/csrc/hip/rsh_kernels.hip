@@ -1284,118 +1284,6 @@ __global__ __launch_bounds__(kDenseThreads) void kRunDense(const int2* __restric
     }
 }
 
-// The last run of well-clique colours in one launch.  When the multicolour order ends in a run swept through dense
-// groups whose rows have no upper couplings outside the run, its forward and backward products combine:
-// z_G = (I + D^-1 U_GG)^-1 D^-1 L_GG^-1 rt_G = M_G rt_G, rt_G being the right-hand side minus the couplings to the rows
-// before the run.  M_G is formed once per factorisation (kDenseMul); one workgroup per 64-row tile of a group then
-// forms rt_G (each member row's outside entries over the lanes of a wavefront, the per-lane partials added in lane
-// order) and its rows of M_G rt_G (the backward product's column split and fixed-order sum).
-// M = Uinv Linv per group, column-major K x K; grid (element blocks, groups)
-__global__ __launch_bounds__(256) void kDenseMul(const int* __restrict__ start, int nb, const long long* __restrict__ offU,
-                                                 const long long* __restrict__ offL, const long long* __restrict__ offM,
-                                                 const double* __restrict__ U, const double* __restrict__ L, double* __restrict__ M) {
-    const int g = blockIdx.y;
-    const int K = (start[g + 1] - start[g]) * nb;
-    const double* u = U + offU[g];
-    const double* l = L + offL[g];
-    double* m = M + offM[g];
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < (long long)K * K; e += (long long)gridDim.x * 256) {
-        const int i = int(e % K), j = int(e / K);
-        double acc = 0;
-        for (int k = 0; k < K; ++k) acc += u[(long long)k * K + i] * l[(long long)j * K + k];
-        m[e] = acc;
-    }
-}
-
-template <int NB, typename TA, typename TM, typename TV>
-__global__ __launch_bounds__(kDenseThreads) void kRunTail(const int2* __restrict__ tasks, const int* __restrict__ start,
-                                                         const int* __restrict__ rows, const long long* __restrict__ offM,
-                                                         const TM* __restrict__ M, const int* __restrict__ outPtr,
-                                                         const int* __restrict__ outCol, const TA* __restrict__ outVal,
-                                                         const double* __restrict__ r, TV* y, double* __restrict__ z,
-                                                         const int* __restrict__ perm, const double* __restrict__ xp,
-                                                         const int* __restrict__ gowner, const KScalars* __restrict__ gate) {
-#pragma clang fp contract(off)
-    constexpr int W = kDenseThreads / kWave;
-    constexpr int NB2 = NB * NB;
-    __shared__ TV v[kDenseMaxRows * NB];
-    __shared__ TV part[W * kWave * NB];
-    const int2 tk = tasks[blockIdx.x];   // x group, y first scalar row of the tile
-    if (gatedOff(gowner, tk.x, gate)) return;
-    const int g0 = start[tk.x], nr = start[tk.x + 1] - g0, K = nr * NB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // rt_G: 16 lanes per member row (every row at once), their partial sums added in lane order; the right-hand side
-    // is loaded first
-    constexpr int RL = 16;
-    for (int b0 = 0; b0 < nr; b0 += kDenseThreads / RL) {
-        const int bi = b0 + int(threadIdx.x) / RL, sub = int(threadIdx.x) % RL;
-        TV rv[NB];
-        TV pt[NB];
-#pragma unroll
-        for (int e = 0; e < NB; ++e) { pt[e] = 0; rv[e] = 0; }
-        if (bi < nr) {
-            if (sub == 0) {
-                const int o = perm[rows[g0 + bi]];
-#pragma unroll
-                for (int e = 0; e < NB; ++e) rv[e] = TV(r[(long long)o * NB + e]);
-            }
-            const int pa = outPtr[g0 + bi], pb = outPtr[g0 + bi + 1];
-            for (int p = pa + sub; p < pb; p += RL) {
-                const int c = outCol[p];
-                const TA* a = outVal + (long long)p * NB2;
-#pragma unroll
-                for (int e = 0; e < NB; ++e) {
-                    TV sacc = 0;
-#pragma unroll
-                    for (int f = 0; f < NB; ++f) sacc += TV(a[e * NB + f]) * y[(long long)c * NB + f];
-                    pt[e] += sacc;
-                }
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < NB; ++e) part[threadIdx.x * NB + e] = pt[e];
-        __syncthreads();
-        if (bi < nr && sub == 0) {
-            TV sum[NB];
-#pragma unroll
-            for (int e = 0; e < NB; ++e) sum[e] = 0;
-#pragma unroll
-            for (int j = 0; j < RL; ++j)
-#pragma unroll
-                for (int e = 0; e < NB; ++e) sum[e] += part[(threadIdx.x + j) * NB + e];
-#pragma unroll
-            for (int e = 0; e < NB; ++e) v[bi * NB + e] = rv[e] - sum[e];
-        }
-        __syncthreads();
-    }
-    // this tile's rows of M_G rt_G
-    const int a = tk.y + lane;
-    TV s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    if (a < K) {
-        const TM* Mg = M + offM[tk.x] + a;
-        int b = wave;
-        for (; b + 3 * W < K; b += 4 * W) {
-            s0 += TV(Mg[(long long)b * K]) * v[b];
-            s1 += TV(Mg[(long long)(b + W) * K]) * v[b + W];
-            s2 += TV(Mg[(long long)(b + 2 * W) * K]) * v[b + 2 * W];
-            s3 += TV(Mg[(long long)(b + 3 * W) * K]) * v[b + 3 * W];
-        }
-        for (; b < K; b += W) s0 += TV(Mg[(long long)b * K]) * v[b];
-    }
-    part[wave * kWave + lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (wave != 0 || a >= K) return;
-    TV out = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) out += part[w * kWave + lane];
-    const int blk = a / NB, wr = rows[g0 + blk], e = a - blk * NB;
-    y[(long long)wr * NB + e] = out;
-    const int o = perm[wr];
-    double val = double(out);
-    if (e == 0 && xp) val += xp[o];
-    z[(long long)o * NB + e] = val;
-}
-
 // gather: raw[group, local row i, local col j] = lu block src (map = {group, i, j, src}); raw zeroed beforehand
 __global__ void kDenseGather(int count, int nb, const int4* __restrict__ map, const int* __restrict__ start,
                              const long long* __restrict__ off, const double* __restrict__ lu, double* __restrict__ raw) {
@@ -3001,31 +2889,6 @@ void launchDenseGroups(hipStream_t s, int nb, bool upper, int ngroups, int maxRo
         checkLaunch();
     }
     RSH_NB_DISPATCH(nb, (denseInvertT<NB>(s, upper, ngroups, maxRows, start, off, raw, inv)))
-    checkLaunch();
-}
-void launchDenseMul(hipStream_t s, int nb, int ngroups, int maxRows, const int* start, const long long* offU, const long long* offL,
-                    const long long* offM, const double* U, const double* L, double* M) {
-    if (ngroups <= 0) return;
-    const long long K = (long long)maxRows * nb;
-    const unsigned bx = unsigned(std::max(1ll, std::min(1024ll, (K * K + 255) / 256)));
-    hipLaunchKernelGGL(kDenseMul, dim3(bx, unsigned(ngroups)), dim3(256), 0, s, start, nb, offU, offL, offM, U, L, M);
-    checkLaunch();
-}
-template <int NB, typename TA, typename TM, typename TV>
-void runTailT(hipStream_t s, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* offM, const void* M, const int* outPtr,
-              const int* outCol, const void* outVal, const double* r, void* y, double* z, const int* perm, const double* xp, const int* gowner,
-              const KScalars* gate) {
-    hipLaunchKernelGGL((kRunTail<NB, TA, TM, TV>), dim3(ntasks), dim3(kDenseThreads), 0, s, tasks, start, rows, offM, static_cast<const TM*>(M), outPtr,
-                       outCol, static_cast<const TA*>(outVal), r, static_cast<TV*>(y), z, perm, xp, gowner, gate);
-}
-void launchRunTail(hipStream_t s, int nb, int fp, bool inv64, int ntasks, const int2* tasks, const int* start, const int* rows, const long long* offM,
-                   const void* M, const int* outPtr, const int* outCol, const void* outVal, const double* r, void* y, double* z, const int* perm,
-                   const double* xp, const int* gowner, const KScalars* gate) {
-    if (ntasks <= 0) return;
-    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (runTailT<NB, float, float, float>(s, ntasks, tasks, start, rows, offM, M, outPtr, outCol, outVal, r, y, z, perm, xp, gowner, gate)))
-    else if (fp == kIluStore32 && !inv64) RSH_NB_DISPATCH(nb, (runTailT<NB, float, float, double>(s, ntasks, tasks, start, rows, offM, M, outPtr, outCol, outVal, r, y, z, perm, xp, gowner, gate)))
-    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (runTailT<NB, float, double, double>(s, ntasks, tasks, start, rows, offM, M, outPtr, outCol, outVal, r, y, z, perm, xp, gowner, gate)))
-    else RSH_NB_DISPATCH(nb, (runTailT<NB, double, double, double>(s, ntasks, tasks, start, rows, offM, M, outPtr, outCol, outVal, r, y, z, perm, xp, gowner, gate)))
     checkLaunch();
 }
 void launchPerturbRows(hipStream_t s, int n, const int* rp, int nb, const double* base, double* out, double amp, unsigned long long seed) {

@@ -279,17 +279,6 @@ struct IluDev {
     // multicolour order: some rows finish their backward step in the forward launch (their lower slice lanes carry the
     // pivot block, dsrc >= 0) and are left out of the upper slices
     bool fuseBwd = false;
-    // the last run of well-clique colours swept in one launch (kRunTail): its forward and backward dense groups
-    // coincide and its rows have no upper couplings outside it; per group M = Uinv Linv (tOffM, per forward dense
-    // group), and the outside entries of the run's rows over the forward dense side's row list
-    bool tailOn = false;
-    int tailL = -1, tailU = -1;
-    DBuf<long long> tOffM;
-    DBuf<double> tM, tOutVal;
-    DBuf<float> tM32, tOutVal32;
-    DBuf<int> tOutPtr, tOutCol, tOutSrc;
-    long long tElems = 0;
-    int tOutCount = 0;
     // natural order: the long rows' update maps (maxLen 0: none, the long rows use kBilu0LongRows)
     DBuf<int> lmRow, lmPtr;
     DBuf<int2> lmMap;
@@ -435,7 +424,6 @@ struct IluDev {
                 tLv.upload(tiles.lvStart, s);
             }
             updateFirstCopy();
-            buildTail(EL, s);
             // factor rows (F numbering) per level, long rows apart
             const HostBsr& F = mc ? fpat : S;
             std::vector<int> sh, lo;
@@ -497,67 +485,6 @@ struct IluDev {
         rt64.alloc(size_t(kRunRowsMax) * nb);
         tail64.alloc(size_t(kRunRowsMax) * nb);
         nreg.alloc(1);
-    }
-    void buildTail(const RowEntries& EL, hipStream_t s) {
-        tailOn = false;
-        if (!tailCopy) return;
-        tailL = int(runsL.size()) - 1;
-        tailU = int(runsU.size()) - 1;
-        const int gL0 = denseL.runGroup0[size_t(tailL)], gL1 = denseL.runGroup1[size_t(tailL)];
-        const int gU0 = denseU.runGroup0[size_t(tailU)], gU1 = denseU.runGroup1[size_t(tailU)];
-        if (gL0 < 0 || gU0 < 0 || gL1 - gL0 != gU1 - gU0) return;
-        for (int k = 0; k <= gL1 - gL0; ++k)
-            if (denseL.hStart[size_t(gL0 + k)] - denseL.hStart[size_t(gL0)] != denseU.hStart[size_t(gU0 + k)] - denseU.hStart[size_t(gU0)]) return;
-        const int r0 = denseL.hStart[size_t(gL0)], r1 = denseL.hStart[size_t(gL1)];
-        for (int k = 0; k < r1 - r0; ++k)
-            if (denseL.hRows[size_t(r0 + k)] != denseU.hRows[size_t(denseU.hStart[size_t(gU0)] + k)]) return;
-        const LevelRun& run = runsL[size_t(tailL)];
-        const int wb = lstart[size_t(run.l0)], we = lstart[size_t(run.l1)];
-        std::vector<long long> offM(size_t(denseL.ngroups), 0);
-        tElems = 0;
-        for (int g = gL0; g < gL1; ++g) {
-            const long long K = (long long)(denseL.hStart[size_t(g) + 1] - denseL.hStart[size_t(g)]) * nb;
-            offM[size_t(g)] = tElems;
-            tElems += K * K;
-        }
-        std::vector<int> ptr(denseL.hRows.size() + 1, 0), col, src;
-        for (size_t k = 0; k < denseL.hRows.size(); ++k) {
-            if (int(k) >= r0 && int(k) < r1) {
-                const int w = denseL.hRows[k];
-                for (int p = EL.ptr[size_t(w)]; p < EL.ptr[size_t(w) + 1]; ++p) {
-                    const int c = EL.col[size_t(p)];
-                    if (c >= wb && c < we) continue;
-                    col.push_back(c);
-                    src.push_back(EL.src[size_t(p)]);
-                }
-            }
-            ptr[k + 1] = int(col.size());
-        }
-        tOffM.upload(offM, s);
-        tOutPtr.upload(ptr, s);
-        tOutCol.upload(col, s);
-        tOutSrc.upload(src, s);
-        tOutCount = int(col.size());
-        tM.alloc(size_t(std::max(1ll, tElems)));
-        tOutVal.alloc(size_t(std::max(1, tOutCount)) * nb * nb);
-        tM32.release();
-        tOutVal32.release();
-        tailOn = true;
-    }
-    // the tail's combined matrices and outside blocks from the current factors (after the dense groups' inverses)
-    void factorTail(bool f32, hipStream_t s) {
-        if (!tailOn) return;
-        const int gL0 = denseL.runGroup0[size_t(tailL)], gL1 = denseL.runGroup1[size_t(tailL)];
-        const int gU0 = denseU.runGroup0[size_t(tailU)];
-        launchDenseMul(s, nb, gL1 - gL0, denseL.maxRows, denseL.start.p + gL0, denseU.off.p + gU0, denseL.off.p + gL0, tOffM.p + gL0, denseU.inv.p,
-                       denseL.inv.p, tM.p);
-        if (tOutCount > 0) launchGatherBlocks(s, nb, lu.p, tOutSrc.p, tOutCount, tOutVal.p);
-        if (f32) {
-            tM32.alloc(tM.n);
-            tOutVal32.alloc(tOutVal.n);
-            launchToFloat(s, (long long)tM.n, tM.p, tM32.p);
-            launchToFloat(s, (long long)tOutVal.n, tOutVal.p, tOutVal32.p);
-        }
     }
     static int entriesCap(const RowEntries& E) {
         const int m = int(E.ptr.size()) - 1;
@@ -715,10 +642,8 @@ struct IluDev {
         denseU.build(runsU, keptU, lstart, nb, true, s, rowSys);
         buildFactorGroups(s);
         updateFirstCopy();
-        buildTail(keptL, s);
         denseL.factor(nb, false, lu.p, have32, s);
         denseU.factor(nb, true, lu.p, have32, s);
-        factorTail(have32, s);
         ++generation;
     }
     // numeric factorisation from the system's values (natural layout), then the sweep layouts
@@ -763,7 +688,6 @@ struct IluDev {
         }
         denseL.factor(nb, false, lu.p, f32, s);
         denseU.factor(nb, true, lu.p, f32, s);
-        factorTail(f32, s);
         RSH_HIP(hipMemcpyAsync(&regularised, nreg.p, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     // fp: IluFp (kIluFp32 and kIluStore32 need the FP32 factors of a factor(..., true))
@@ -808,7 +732,6 @@ struct IluDev {
         }
         for (size_t ri = 0; ri < runsL.size(); ++ri) {
             const LevelRun& run = runsL[ri];
-            if (tailOn && int(ri) == tailL) continue;   // swept with the backward tail below
             if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
             if (l == 0 && firstDone) continue;   // the caller wrote y of the first level
@@ -817,15 +740,6 @@ struct IluDev {
         }
         for (size_t k = runsU.size(); k-- > 0;) {
             const LevelRun& run = runsU[k];
-            if (tailOn && int(k) == tailU) {
-                const bool i64 = !f32 || (inv64 && fp == kIluStore32);
-                const int t0 = denseL.runTask0[size_t(tailL)];
-                launchRunTail(s, nb, fp, i64, denseL.runTask1[size_t(tailL)] - t0, denseL.tasks.p + t0, denseL.start.p, denseL.rows.p, tOffM.p,
-                              i64 ? static_cast<const void*>(tM.p) : static_cast<const void*>(tM32.p), tOutPtr.p, tOutCol.p,
-                              f32 ? static_cast<const void*>(tOutVal32.p) : static_cast<const void*>(tOutVal.p), r, y, z, perm.p, xp,
-                              denseL.gowner.p, gate);
-                continue;
-            }
             if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
             launchIluBackward(s, nb, fp, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
@@ -834,8 +748,7 @@ struct IluDev {
     }
     size_t bytes() const {
         return lu.bytes() + L64.bytes() + U64.bytes() + L32.bytes() + U32.bytes() + y64.bytes() + y32.bytes() + denseL.bytes() + denseU.bytes() +
-               lmRow.bytes() + lmPtr.bytes() + lmMap.bytes() + tUnit.bytes() + tLvPtr.bytes() + tLv.bytes() + tM.bytes() + tM32.bytes() +
-               tOutVal.bytes() + tOutVal32.bytes() + tOffM.bytes() + tOutPtr.bytes() + tOutCol.bytes() + tOutSrc.bytes();
+               lmRow.bytes() + lmPtr.bytes() + lmMap.bytes() + tUnit.bytes() + tLvPtr.bytes() + tLv.bytes();
     }
 };
 
