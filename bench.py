@@ -436,7 +436,10 @@ def main():
         L = rs.lib()
         rs._check(L.rsh_system_set_values_device(sysd.h, dev.val), "set_values_device")
         rs._check(L.rsh_system_setup(sysd.h, ctypes.byref(cfgs[names[0]])), "setup")
-        for kind, name in [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps")]:
+        kinds = [(4, "copy"), (0, "spmv"), (2, "ilu_sweeps")]
+        if cfgs[names[0]].smoother == J and cfgs[names[0]].precond in (rs.PREC_CPR, rs.PREC_CPR_MC, rs.PREC_CPR_BJ):
+            kinds += [(5, "smoother_l0"), (6, "smoother_l1")]
+        for kind, name in kinds:
             ms, by = sysd.time_kernel(kind, 20)
             log(f"kernel {name}: {ms * 1e3:.1f} us" + (f", {by / ms / 1e6:.0f} GB/s" if by > 0 else ""))
         return
@@ -488,6 +491,15 @@ def main():
         if name in k:
             roof[name] = {"us": k[name]["ms"] * 1e3, "algorithmic_bytes": k[name]["bytes"], "GBs": k[name]["GBs"],
                           "frac": k[name]["GBs"] / HBM_PEAK_GBS}
+    # the natural-order BILU0 apply (tiled sweeps, the factors the reference computes) beside it, when that variant ran
+    if "cpr" in cfgs:
+        sysd.setup(cfgs["cpr"])
+        ms, by = sysd.time_kernel(2, 10)
+        inf = sysd.info()
+        roof["ilu_natural_apply"] = {"us": ms * 1e3, "algorithmic_bytes": by, "GBs": by / (ms * 1e-3) / 1e9, "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                     "tiled": inf.get("ilu_tiled", 0), "tiles": inf.get("ilu_tiles")}
+        log(f"kernel ilu_natural_apply: {ms * 1e3:.1f} us, {by / (ms * 1e-3) / 1e9:.0f} GB/s")
+        sysd.setup(cfgs[head])
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this workload (profiles/)
     tfile = os.path.join(ROOT, "profiles", "traffic_cpr_mc_1.1M.json")
     if args.scale == 1.0 and head == "cpr_mc" and os.path.exists(tfile):
