@@ -507,6 +507,11 @@ def main():
             tr = json.load(f)
         roof["traffic"] = tr["spmv"]["traffic_bytes"]
         roof["ilu_apply_traffic"] = tr["mc_bilu0_apply"]["traffic_bytes"]
+        for name in ("smoother_l0", "smoother_l1"):
+            if name in roof and name in tr:
+                roof[name]["traffic"] = tr[name]["traffic_bytes"]
+        if "ilu_natural_apply" in roof and "natural_bilu0_apply" in tr:
+            roof["ilu_natural_apply"]["traffic"] = tr["natural_bilu0_apply"]["traffic_bytes"]
         roof["traffic_source"] = "profiles/cpr_mc_fp64_1.1M_hbm_traffic.md"
 
     batched = None
