@@ -99,6 +99,11 @@ typedef struct rsh_options {
                            grid axes of a natural-ordered grid), one launch per tile level, one workgroup per tile walking
                            its own levels; the same factors, each bulk row's sweep in the same slot order.  Default 0 */
     int ilu_tile_dims;  /* tile extents along those offsets, bx | by << 8 | bz << 16 (0: chosen by the cost model) */
+    int retry_fp32;     /* RSH_MIXED (BiCGSTAB) and RSH_MIXED_IR: 0 (default) once a refinement on this system has stalled
+                           with the FP32 preconditioner and fallen back to the FP64 one, the system's later solves start
+                           with the FP64 preconditioner (a well-scaled system keeps FP32; one whose FP32 preconditioner
+                           loses too much accuracy pays the stall once, not once per Newton iteration); 1: every solve
+                           starts with FP32 again.  rsh_system_info reports the state as fp32_stalled */
 } rsh_options;
 
 /* ---- library / context */

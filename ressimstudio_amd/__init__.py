@@ -56,7 +56,8 @@ class Options(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int), ("krylov", ctypes.c_int), ("precision", ctypes.c_int), ("smoother", ctypes.c_int),
                 ("restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("inner_tol", ctypes.c_double), ("reuse_amg", ctypes.c_int),
                 ("smooth_steps", ctypes.c_int), ("jacobi_omega", ctypes.c_double), ("ilu_lean", ctypes.c_int),
-                ("coarse_smooth_steps", ctypes.c_int), ("ilu_schedule", ctypes.c_int), ("ilu_tile_dims", ctypes.c_int)]
+                ("coarse_smooth_steps", ctypes.c_int), ("ilu_schedule", ctypes.c_int), ("ilu_tile_dims", ctypes.c_int),
+                ("retry_fp32", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

@@ -244,19 +244,27 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
     const double innerTol = S.opt.inner_tol > 0 ? S.opt.inner_tol : 1e-4;
     const int maxOuter = S.opt.max_outer > 0 ? S.opt.max_outer : 10;
     double rn = bn;
-    bool precF32 = true;
+    // a system whose FP32 preconditioner stalled before starts with the FP64 one (unless the caller asks to retry)
+    bool precF32 = !(S.fp32Stalled && !S.opt.retry_fp32);
+    if (!precF32) opF32 = false;
+    int firstInner = 0;   // iterations of the first inner solve that reduced the residual with the FP32 preconditioner
     for (outer = 0; outer < maxOuter; ++outer) {
         rel = rn / bn;
         if (rel <= tol) return RSH_OK;
         int inIts = 0;
         double inRel = 0;
         const double itol = std::max(innerTol, tol * bn / rn * 0.5);
-        // an inner solve that stalls is restarted by the next refinement step: at most maxIt / maxOuter iterations
-        const int left = std::max(1, std::min(maxIt - its, std::max(50, maxIt / maxOuter)));
+        // an inner solve that stalls is restarted by the next refinement step: at most maxIt / maxOuter iterations,
+        // and with the FP32 preconditioner at most twice (plus 20) what its first productive solve took: every inner
+        // solve asks for the same relative reduction, so one running far longer has stalled (its correction is still
+        // kept when it reduces the residual, else the FP64 preconditioner takes over)
+        int left = std::max(1, std::min(maxIt - its, std::max(50, maxIt / maxOuter)));
+        if ((precF32 || opF32) && firstInner > 0) left = std::min(left, 2 * firstInner + 20);
         const int rc = S.opt.krylov == RSH_KRYLOV_FGMRES
                            ? fgmres(S, S.irR.p, S.irD.p, itol, left, S.opt.restart > 0 ? S.opt.restart : 30, precF32, opF32, false, inIts, inRel)
                            : bicgstab(S, S.irR.p, S.irD.p, itol, left, inIts, inRel, precF32, opF32);
         its += inIts;
+        S.lastInner.push_back(inIts);
         // a diverged inner solve (non-finite residual) is a step without progress: the check below discards it
         if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN && rc != RSH_ERR_NONFINITE) return rc;
         launchCopy(st, S.irX.p, x, N);
@@ -277,12 +285,14 @@ int refine(rsh_system& S, const double* b, double* x, double tol, int maxIt, int
                 precF32 = false;
                 opF32 = false;
                 S.lastFallback = true;
+                S.fp32Stalled = true;
                 if (its >= maxIt) break;
                 continue;
             }
             S.lastStop = "refinement stalled";
             break;
         }
+        if ((precF32 || opF32) && firstInner == 0) firstInner = inIts;
         rn = rnew;
         if (its >= maxIt) break;
     }
@@ -297,6 +307,7 @@ int solveDevice(rsh_system& S, const double* b, double* x, double tol, int maxIt
     int its = 0, outer = 0, rc;
     double rel = 0;
     S.lastFallback = false;
+    S.lastInner.clear();
     S.lastStop = "non-finite residual";
     // RSH_MIXED: FGMRES with the FP32 preconditioner (its restarts recompute the FP64 true residual), or BiCGSTAB
     // with the FP32 preconditioner restarted on the FP64 true residual (refinement with the FP64 operator), which
@@ -793,7 +804,9 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
             }
             o << "],\"coarse_dense\":" << (S.cpr->dense ? 1 : 0) << ",\"cpr_bytes\":" << S.cpr->bytes();
         }
-        o << ",\"last_fallback_fp64\":" << (S.lastFallback ? 1 : 0) << ",\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
+        o << ",\"last_fallback_fp64\":" << (S.lastFallback ? 1 : 0) << ",\"fp32_stalled\":" << (S.fp32Stalled ? 1 : 0) << ",\"last_inner\":[";
+        for (size_t q = 0; q < S.lastInner.size(); ++q) o << (q ? "," : "") << S.lastInner[q];
+        o << "],\"last_iterations\":" << S.lastIts << ",\"last_outer\":" << S.lastOuter << ",\"last_rel_res\":" << S.lastRel
  << ",\"last_setup_ms\":" << S.lastSetupMs << ",\"last_setup_ilu_ms\":" << S.lastIluMs << ",\"last_setup_other_ms\":" << S.lastCprMs << ",\"last_solve_ms\":" << S.lastSolveMs << ",\"last_stop\":\"" << S.lastStop << "\"}";
         const std::string str = o.str();
         if (int(str.size()) + 1 > cap) return fail(RSH_ERR_ARG, "info buffer too small");

@@ -1128,6 +1128,8 @@ struct rsh_system {
     double lastRel = 0, lastSetupMs = 0, lastSolveMs = 0, lastIluMs = 0, lastCprMs = 0;
     std::string lastStop;
     bool lastFallback = false;          // the last refinement fell back to the FP64 preconditioner
+    std::vector<int> lastInner;         // the last refinement's inner iterations per outer step
+    bool fp32Stalled = false;           // a refinement fell back to the FP64 preconditioner on this system
 
     hipStream_t s() const { return ctx->c.stream; }
     // the FP32 preconditioner of the mixed modes (asked for by the solve; FP32 ILU arithmetic)

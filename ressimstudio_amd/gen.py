@@ -162,6 +162,16 @@ def serving_models(count: int = 64, seed: int = 20240605) -> list:
     """Resident ensemble models for the serving loop: cells log-uniform in [5,000, 500,000] (nz uniform in 4..20,
     nx = ny from the cell count), nb 3, lognormal k with sigma ln k uniform in [0.5, 1.5], two to four wells; model i
     uses generator seed ``seed + i``; the draws come from numpy's PCG64 seeded with ``seed``."""
+    return [reservoir(*a, **k) for a, k in _serving_params(count, seed)]
+
+
+def serving_model(i: int, seed: int = 20240605):
+    """Model i of ``serving_models(count > i, seed)`` alone."""
+    a, k = _serving_params(i + 1, seed)[i]
+    return reservoir(*a, **k)
+
+
+def _serving_params(count: int, seed: int) -> list:
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
@@ -170,7 +180,7 @@ def serving_models(count: int = 64, seed: int = 20240605) -> list:
         nx = max(4, int(round(np.sqrt(cells / nz))))
         sigma = float(rng.uniform(0.5, 1.5))
         wells = int(rng.integers(2, 5))
-        out.append(reservoir(nx, nx, nz, 3, model=0, sigma=sigma, wells=wells, seed=seed + i))
+        out.append(((nx, nx, nz, 3), dict(model=0, sigma=sigma, wells=wells, seed=seed + i)))
     return out
 
 

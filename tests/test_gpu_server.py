@@ -160,3 +160,30 @@ def test_worker_error_reaches_the_waiting_thread(mods):
     assert r["status"] == rs.ERR_UNSUPPORTED
     assert "restart" in rs.lib().rsh_last_error().decode()
     srv.close()
+
+
+def test_fp32_stall_is_paid_once(orc):
+    # model 9 of the serving workload (155,526 cells): its FP32-preconditioned inner BiCGSTAB stalls (the refinement
+    # then falls back to the FP64 preconditioner: about 365 iterations instead of 65).  The system remembers the
+    # stall, so its next Newton iteration starts with the FP64 preconditioner; retry_fp32=1 starts with FP32 again.
+    m = gen.serving_model(9)
+    assert m.n == 155526
+    kw = dict(precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0, precision=rs.MIXED)
+    d = rs.System(m).setup(**kw)
+    x, its, rel, ok = d.solve(m.rhs, TOL, 3000)
+    inf = d.info()
+    assert ok and inf["last_fallback_fp64"] == 1 and inf["fp32_stalled"] == 1, inf["last_inner"]
+    for retry in (0, 1):
+        v = rs.perturbed_values(m.val, m.row_ptr, 0.05, 9002)
+        sp = gen.BlockSystem(n=m.n, nb=m.nb, row_ptr=m.row_ptr, col=m.col, val=v.reshape(m.val.shape), rhs=m.rhs)
+        d.set_values(v)
+        d.setup(retry_fp32=retry, **kw)
+        x2, its2, rel2, ok2 = d.solve(m.rhs, TOL, 3000)
+        inf2 = d.info()
+        true_rel = np.linalg.norm(m.rhs - orc.spmv(sp, x2)) / np.linalg.norm(m.rhs)
+        print(f"retry_fp32={retry}: {its2} its (first solve {its}), inner {inf2['last_inner']}, fallback {inf2['last_fallback_fp64']}")
+        assert ok2 and true_rel <= 1.01 * TOL
+        if retry == 0:
+            assert inf2["last_fallback_fp64"] == 0 and its2 < its / 3
+        else:
+            assert inf2["last_fallback_fp64"] == 1
