@@ -78,6 +78,7 @@ class Checker:
         lib.chk_devcpr_apply.argtypes = [_vp, _dp, _dp]
         lib.chk_devcpr_free.argtypes = [_vp]
         lib.chk_krylov.argtypes = bsr + [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        lib.chk_krylov_cpr_steady.argtypes = bsr + [_dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _dp]
         if hasattr(lib, "chk_bilu0_factors"):
             lib.chk_bilu0_factors.argtypes = [_vp, _dp]
         if hasattr(lib, "chk_set_threads"):
@@ -115,6 +116,20 @@ class Checker:
         if conv < 0:
             raise RuntimeError("checker krylov failed")
         return x, int(its.value), float(rel.value), bool(conv)
+
+    def krylov_cpr_steady(self, s, b, tol=1e-8, maxit=1000):
+        """CPR-BiCGSTAB as in the Newton loop after the first iteration: the CPR set-up with the AMG aggregates
+        reused, then the solve; returns x, iterations, rel.res, converged, set-up seconds, solve seconds."""
+        n, nb, rp, ci, v = _bsr_args(s)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n * nb)
+        its = ctypes.c_int(0)
+        rel = ctypes.c_double(0)
+        secs = np.zeros(2)
+        conv = self.lib.chk_krylov_cpr_steady(n, nb, _p(rp, ctypes.c_int), _p(ci, ctypes.c_int), _p(v, ctypes.c_double),
+                                              _p(b, ctypes.c_double), _p(x, ctypes.c_double), tol, maxit, ctypes.byref(its),
+                                              ctypes.byref(rel), _p(secs, ctypes.c_double))
+        return x, int(its.value), float(rel.value), bool(conv), float(secs[0]), float(secs[1])
 
     FLOAT_KIND = {"bilu0f": 0, "mcbilu0": 1, "devcpr": 2}
 

@@ -19,6 +19,7 @@
 #include "rs_check.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1125,6 +1126,26 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
     return conv ? 1 : 0;
 }
 
+// the steady state of the Newton loop: BlockCPR::setup with reuseAmg (bsr.cpp:993-1059) after a first set-up
+int chk_krylov_cpr_steady(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, double tol,
+                          int maxit, int* its, double* relres, double* secs) {
+    Bsr A = makeBsr(n, nb, rp, ci, v);
+    BlockCpr cpr;
+    cpr.setup(A, false);
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    cpr.setup(A, true);
+    const auto t1 = clk::now();
+    PrecOf<BlockCpr> M(cpr);
+    std::vector<double> bb(b, b + size_t(n) * nb), xx;
+    bool conv = false;
+    bicgstab(A, M, bb, xx, tol, maxit, *its, *relres, conv);
+    const auto t2 = clk::now();
+    std::copy(xx.begin(), xx.end(), x);
+    secs[0] = std::chrono::duration<double>(t1 - t0).count();
+    secs[1] = std::chrono::duration<double>(t2 - t1).count();
+    return conv ? 1 : 0;
+}
 
 void* chk_devcpr_new(int n, int nb, const int* rp, const int* ci, const double* v) {
     auto* d = new DevCprAmg;

@@ -4,6 +4,7 @@
 #include "rs_check.h"
 #include "ressim/bsr.h"
 
+#include <chrono>
 #include <cstdlib>
 #include <memory>
 #include <vector>
@@ -162,6 +163,26 @@ int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, con
     const LinResult r = method == 0 ? bicgstab(op, *M, bb, xx, tol, maxit) : gmres(op, *M, bb, xx, tol, maxit, restart);
     for (size_t i = 0; i < xx.size(); ++i) x[i] = xx[i];
     *its = r.iterations; *relres = r.relResidual;
+    return r.converged ? 1 : 0;
+}
+
+int chk_krylov_cpr_steady(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, double tol,
+                          int maxit, int* its, double* relres, double* secs) {
+    BSR A = makeBsr(n, nb, rp, ci, v);
+    BlockCPR P;
+    P.setup(A);
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    P.setup(A, nullptr, true);
+    const auto t1 = clk::now();
+    std::vector<double> bb(b, b + size_t(n) * nb), xx;
+    BsrOperator op(A);
+    const LinResult r = bicgstab(op, P, bb, xx, tol, maxit);
+    const auto t2 = clk::now();
+    for (size_t i = 0; i < xx.size(); ++i) x[i] = xx[i];
+    *its = r.iterations; *relres = r.relResidual;
+    secs[0] = std::chrono::duration<double>(t1 - t0).count();
+    secs[1] = std::chrono::duration<double>(t2 - t1).count();
     return r.converged ? 1 : 0;
 }
 

@@ -70,6 +70,11 @@ int chk_refine_float(int n, int nb, const int* rp, const int* ci, const double* 
 
 int chk_krylov(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, int precond,
                int method, int restart, double tol, int maxit, int* its, double* relres);
+/* CPR-BiCGSTAB in the Newton loop's steady state: a first CPR set-up on the matrix (not timed), then the set-up with the
+ * AMG aggregates reused (BlockCPR::setup(S, acc, reuseAmg = true), bsr.cpp:993-1059, AMG::refresh amg.cpp:257-271)
+ * and the solve, timed: secs[0] = refresh set-up seconds, secs[1] = solve seconds.  Returns 1 when converged */
+int chk_krylov_cpr_steady(int n, int nb, const int* rp, const int* ci, const double* v, const double* b, double* x, double tol,
+                          int maxit, int* its, double* relres, double* secs);
 
 #ifdef __cplusplus
 }
