@@ -2,6 +2,7 @@
 // level-scheduled / multicolour block ILU(0), the CPR-AMG hierarchy, the context and the resident system.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -216,6 +217,13 @@ struct DenseSide {
     size_t bytes() const { return raw.bytes() + inv.bytes() + inv32.bytes(); }
 };
 
+// a new generation number for a rebuilt schedule or hierarchy: unique over the process, so that an apply captured
+// against an object that has since been replaced (a new IluDev after an option change) is never replayed
+inline int nextGeneration() {
+    static std::atomic<int> g{0};
+    return ++g;
+}
+
 // ------------------------------------------------------------------ level-scheduled block ILU(0)
 struct IluDev {
     bool mc = false;
@@ -288,7 +296,7 @@ struct IluDev {
     RowEntries keptL, keptU;
     std::vector<int> workL, workU, rowsPerLevel;
     void build(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, bool multicolour, hipStream_t s) {
-        ++generation;
+        generation = nextGeneration();
         mc = multicolour;
         n = S.n;
         nb = S.nb;
@@ -644,7 +652,7 @@ struct IluDev {
         updateFirstCopy();
         denseL.factor(nb, false, lu.p, have32, s);
         denseU.factor(nb, true, lu.p, have32, s);
-        ++generation;
+        generation = nextGeneration();
     }
     // numeric factorisation from the system's values (natural layout), then the sweep layouts
     // lean: the C45 lean factorisation (input and every finished row rounded to float32; the factors are then
@@ -820,7 +828,7 @@ struct CprDev {
 
     int generation = 0;
     void buildHierarchy(const HostBsr& S, const int* drp, const int* dci, const int* ddiag, const double* apHost, hipStream_t s) {
-        ++generation;
+        generation = nextGeneration();
         HostCsr A0;
         A0.n = S.n;
         A0.rp = S.rp;
