@@ -272,13 +272,18 @@ def serving_leg(rs, gen, args):
     models = gen.serving_models(args.serve_models)
     cells = int(sum(m.n for m in models))
     log(f"serving workload: {len(models)} models, {cells} cells, generated in {time.perf_counter() - t:.1f}s")
-    rng = np.random.default_rng(20240606)
     lambdas = [float(x) for x in args.serve_lambdas.split(",") if x]
     out = {"workload": f"{len(models)} resident models, 5k-500k cells log-uniform ({cells} cells), nb 3, CPR(Jacobi AMG)+MC-BILU0 "
                        f"BiCGSTAB to 1e-8, block rows scaled by +-5 % per request", "models": len(models), "cells": cells,
            "slots": args.serve_slots, "requests_per_rate": args.serve_requests, "hbm_budget_GB": 200}
-    for name, prec in (("fp64", rs.FP64), ("mixed", rs.MIXED), ("store32", rs.MIXED_STORE)):
-        srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9)
+    forms = (("fp64", rs.FP64, False), ("mixed", rs.MIXED, False), ("store32", rs.MIXED_STORE, False), ("fp64_batched", rs.FP64, True),
+             ("store32_batched", rs.MIXED_STORE, True))
+    only = [f for f in args.serve_forms.split(",") if f] or ["fp64", "mixed", "store32", "fp64_batched"]
+    for name, prec, batched in forms:
+        if only and name not in only:
+            continue
+        # batched: the requests queued for one group of models (at most serve_group_cells cells) share each round
+        srv = rs.Server(slots=args.serve_slots, hbm_budget=200e9, batched=batched, group_cells=args.serve_group_cells, group_systems=16)
         ids = [srv.add_model(m, precond="cpr_mc", smoother=rs.SMOOTH_JACOBI, smooth_steps=2, jacobi_omega=1.0, precision=prec)
                for m in models]
         t = time.perf_counter()
@@ -288,6 +293,7 @@ def serving_leg(rs, gen, args):
         log(f"serving {name}: models loaded (symbolic set-up + first iteration) in {time.perf_counter() - t:.1f}s; {srv.stats()}")
         newton = [2] * len(models)
         rows = []
+        rng = np.random.default_rng(20240606)    # every form serves the same arrival times and models
         for lam in lambdas:
             gaps = rng.exponential(1.0 / lam, args.serve_requests)
             picks = rng.integers(0, len(models), args.serve_requests)
@@ -319,6 +325,8 @@ def serving_leg(rs, gen, args):
         st = srv.stats()
         srv.close()
         out[name] = {"rates": rows, "peak_running": st["peak_running"], "hbm_used_GB": st["hbm_used"] / 1e9, "evictions": st["evictions"]}
+        if batched:
+            out[name].update({"groups": st["groups"], "group_models": st["group_models"], "rounds": st["rounds"], "group_cells": args.serve_group_cells})
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_sample(models, 8, "serving")
     return out
@@ -379,6 +387,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-lambdas", default="10,25,50,200,2000")
+    ap.add_argument("--serve-forms", default="", help="comma list of serving forms (default: all)")
+    ap.add_argument("--serve-group-cells", type=int, default=150000, help="batched serving: cells per model group")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus})", file=sys.stderr)

@@ -244,6 +244,19 @@ typedef struct rsh_result {
     double queue_ms, service_ms, latency_ms;   /* submit -> start, start -> end, submit -> end */
 } rsh_result;
 int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server** out);
+/* The same loop serving requests in batched rounds through the batch engine (rsh_batch_*), so that several requests
+ * share each kernel launch (no counterpart in the reference; for ensembles of many small models, whose requests are
+ * bound by kernel launches rather than bytes).  At the first request the models are grouped (same block size and
+ * options, ascending size, a group holds at most group_cells cells — or one larger model — and at most group_systems
+ * models); every model must be added before it.  A round takes every queued request of one idle group (the earliest
+ * per model), applies their values, runs one numeric set-up of the whole group and one batched BiCGSTAB in which the
+ * group's other models solve a zero right-hand side (they stop at once).  A request's result does not depend on the
+ * other requests of its round (the batch's preconditioner is block-diagonal, its scalars per system), but it is the
+ * batch engine's solve (CPR: one pressure hierarchy per group), not the single-system one.  A round runs to the
+ * largest max_it among its requests.  Groups are not evicted: one that does not fit the budget waits until a
+ * resident group leaves room (none does; size the budget for every group).  RSH_MIXED_IR and FGMRES are refused by
+ * the batch engine. */
+int rsh_server_create_batched(int device, int slots, double hbm_budget_bytes, long long group_cells, int group_systems, rsh_server** out);
 int rsh_server_destroy(rsh_server* srv);   /* finishes the queued requests first */
 int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, const int* col, const double* values, const rsh_options* opt,
                          int* model);

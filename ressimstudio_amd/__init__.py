@@ -40,7 +40,7 @@ EXPORTED = [
     "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
     "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
-    "rsh_batch_info", "rsh_server_create", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
+    "rsh_batch_info", "rsh_server_create", "rsh_server_create_batched", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
     "rsh_server_stats", "rsh_wells_create", "rsh_wells_destroy", "rsh_wells_system", "rsh_wells_set_values", "rsh_wells_solve",
     "rsh_wells_reduced_values", "rsh_context_device_seconds", "rsh_solve_device_noalloc",
 ]
@@ -125,6 +125,7 @@ def lib():
         L.rsh_batch_latencies.argtypes = [_vp, _dp]
         L.rsh_batch_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
         L.rsh_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]
+        L.rsh_server_create_batched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp)]
         L.rsh_server_destroy.argtypes = [_vp]
         L.rsh_server_add_model.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(Options), _ip]
         L.rsh_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
@@ -399,9 +400,16 @@ class Server:
     """Serving loop (rsh_server_*): resident models, FIFO admission by worker slot, model availability and HBM
     budget, one HIP stream per slot."""
 
-    def __init__(self, slots: int = 4, hbm_budget: float = 200e9, device: int = 0):
+    def __init__(self, slots: int = 4, hbm_budget: float = 200e9, device: int = 0, batched: bool = False, group_cells: int = 200000,
+                 group_systems: int = 16):
+        """batched: requests served in batched rounds through the batch engine (rsh_server_create_batched): models grouped
+        at the first request (group_cells cells / group_systems models at most per group)."""
         h = _vp()
-        _check(lib().rsh_server_create(device, slots, float(hbm_budget), ctypes.byref(h)), "rsh_server_create")
+        if batched:
+            _check(lib().rsh_server_create_batched(device, slots, float(hbm_budget), int(group_cells), int(group_systems), ctypes.byref(h)),
+                   "rsh_server_create_batched")
+        else:
+            _check(lib().rsh_server_create(device, slots, float(hbm_budget), ctypes.byref(h)), "rsh_server_create")
         self.h = h
         self._keep = {}
         self._models = []

@@ -172,18 +172,31 @@ struct rsh_batch {
         RSH_HIP(hipStreamSynchronize(st));
         int running = 0;
         for (int i = 0; i < ns; ++i) running += hks[i].state == kRun;
-        int sweeps = 0;
+        // the host reads the states every sweep once some running system is near its tolerance, else every fourth
+        // sweep (a stopped system's state is frozen by the epilogues, so the sweeps in between change nothing for it;
+        // its latency is then taken at the next read)
+        constexpr int kBatchCheckEvery = 4;
+        int sweeps = 0, nextCheck = 1;
         while (running > 0 && sweeps < maxIt) {
             iterationGraphed();
-            RSH_HIP(hipStreamSynchronize(st));
             ++sweeps;
+            if (sweeps < nextCheck && sweeps < maxIt) continue;
+            RSH_HIP(hipStreamSynchronize(st));
             const double now = nowMs() - t0;
             running = 0;
+            double worst = 0;
             for (int i = 0; i < ns; ++i) {
-                if (hks[i].state == kRun) ++running;
-                else if (doneAt[size_t(i)] == 0.0) doneAt[size_t(i)] = now;
+                if (hks[i].state == kRun) {
+                    ++running;
+                    const double rel = hks[i].bn > 0 ? hks[i].rn / (hks[i].bn * hks[i].tol) : 0.0;
+                    worst = std::max(worst, std::isfinite(rel) ? rel : 1e300);
+                } else if (doneAt[size_t(i)] == 0.0) {
+                    doneAt[size_t(i)] = now;
+                }
             }
+            nextCheck = sweeps + (worst > 30.0 ? kBatchCheckEvery : 1);
         }
+        RSH_HIP(hipStreamSynchronize(st));
         return sweeps;
     }
     // per-system 2-norms of a flat vector (host)
@@ -692,3 +705,30 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ the serving loop's access to a batch's members
+namespace rsh {
+
+BatchMember batchMember(rsh_batch* b, int i) {
+    if (!b || i < 0 || i >= int(b->sys.size())) throw Error(RSH_ERR_ARG, "batch member out of range");
+    const auto& q = b->sys[size_t(i)];
+    rsh_system* S = b->groups[size_t(q.group)].S;
+    return BatchMember{S->val.p, S->rp.p + q.rowBase, q.valOff, q.inOff, q.n, q.nb, q.nnzb};
+}
+
+void batchValuesChanged(rsh_batch* b) {
+    for (auto& g : b->groups) g.S->valuesChanged();
+    b->ready = false;
+}
+
+hipStream_t batchStream(rsh_batch* b) { return b->s(); }
+
+size_t batchDeviceBytes(rsh_batch* b) {
+    size_t s = 0;
+    for (auto& g : b->groups) s += g.S->deviceBytes();
+    for (auto* bf : {&b->x, &b->b, &b->r, &b->rh, &b->p, &b->v, &b->sv, &b->t, &b->ph, &b->sh, &b->xo, &b->rt, &b->ax, &b->xPrev, &b->rPrev})
+        s += bf->bytes();
+    return s + b->partial.bytes() + b->task.bytes();
+}
+
+}  // namespace rsh

@@ -41,6 +41,26 @@ struct rsh_server {
         std::string msg;                     // the worker's error message, handed to the waiting thread
         bool done = false;
     };
+    // batched rounds (rsh_server_create_batched): the models are grouped once, at the first request, into rsh_batch
+    // groups (same block size and options, ascending size, at most groupCells cells or one model, at most
+    // groupSystems models); a round serves every queued request of one idle group (the earliest per model) with one
+    // numeric set-up and one batched BiCGSTAB of the group, the group's other members solving a zero right-hand side
+    struct Group {
+        std::vector<int> models;             // model ids, batch member order
+        rsh_context* ctx = nullptr;
+        rsh_batch* B = nullptr;
+        DBuf<double> base, b, x, tol;        // base values in the batch's group layout; rhs / solution in input order
+        std::vector<double> tolH;
+        size_t footprint = 0;
+        bool busy = false;
+        long long rounds = 0;
+    };
+    bool batched = false, grouped = false;
+    long long groupCells = 0;
+    int groupSystems = 0;
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<int> groupOf;                // model -> group
+    long long rounds = 0;
     int device = 0;
     size_t budget = 0, used = 0;
     std::vector<std::unique_ptr<Model>> models;
@@ -164,7 +184,235 @@ struct rsh_server {
         q.res.iterations = its;
         q.res.rel_res = rel;
     }
+    // models -> groups (lock held; at the first request)
+    void formGroups() {
+        std::vector<int> ord(models.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = int(i);
+        auto same = [&](int a, int b) {
+            const Model& A = *models[size_t(a)];
+            const Model& B = *models[size_t(b)];
+            return A.nb == B.nb && std::memcmp(&A.opt, &B.opt, sizeof A.opt) == 0;
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            const Model& A = *models[size_t(a)];
+            const Model& B = *models[size_t(b)];
+            if (A.nb != B.nb) return A.nb < B.nb;
+            const int c = std::memcmp(&A.opt, &B.opt, sizeof A.opt);
+            if (c != 0) return c < 0;
+            return A.n < B.n;
+        });
+        groupOf.assign(models.size(), -1);
+        for (size_t k = 0; k < ord.size();) {
+            auto g = std::make_unique<Group>();
+            long long cells = 0;
+            size_t e = k;
+            while (e < ord.size() && same(ord[k], ord[e]) && int(g->models.size()) < groupSystems &&
+                   (g->models.empty() || cells + models[size_t(ord[e])]->n <= groupCells)) {
+                cells += models[size_t(ord[e])]->n;
+                g->models.push_back(ord[e]);
+                ++e;
+            }
+            for (int m : g->models) {
+                groupOf[size_t(m)] = int(groups.size());
+                g->footprint += models[size_t(m)]->footprint;
+            }
+            groups.push_back(std::move(g));
+            k = e;
+        }
+        grouped = true;
+    }
+    // next round (lock held): the first queued request whose group is idle, with every queued request of that group
+    // (the earliest per model); a group that is not resident yet must fit the budget
+    int pickRound(std::vector<long long>& ids) {
+        ids.clear();
+        for (long long id : queue) {
+            const int g = groupOf[size_t(reqs.at(id)->r.model)];
+            Group& G = *groups[size_t(g)];
+            if (G.busy) continue;
+            const bool resident = G.models.size() == 1 ? models[size_t(G.models[0])]->S != nullptr : G.B != nullptr;
+            if (!resident && used + G.footprint > budget && used > 0) continue;
+            std::vector<char> taken(models.size(), 0);
+            for (auto it = queue.begin(); it != queue.end();) {
+                const int m = reqs.at(*it)->r.model;
+                if (groupOf[size_t(m)] == g && !taken[size_t(m)]) {
+                    taken[size_t(m)] = 1;
+                    ids.push_back(*it);
+                    it = queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            return g;
+        }
+        return -1;
+    }
+    void serveRound(Group& G, const std::vector<Req*>& qs) {
+        RSH_HIP(hipSetDevice(device));
+        const int cnt = int(G.models.size());
+        if (!G.B) {
+            int rc = rsh_context_create(device, &G.ctx);
+            if (rc != RSH_OK) throw Error(rc, g_err);
+            const size_t nc = static_cast<size_t>(cnt);
+            std::vector<int> n(nc, 0), nb(nc, 0);
+            std::vector<const int*> rp(nc, nullptr), ci(nc, nullptr);
+            for (int j = 0; j < cnt; ++j) {
+                const Model& m = *models[size_t(G.models[size_t(j)])];
+                n[size_t(j)] = m.n;
+                nb[size_t(j)] = m.nb;
+                rp[size_t(j)] = m.rp.data();
+                ci[size_t(j)] = m.ci.data();
+            }
+            rc = rsh_batch_create(G.ctx, cnt, n.data(), nb.data(), rp.data(), ci.data(), &G.B);
+            if (rc != RSH_OK) throw Error(rc, g_err);
+            hipStream_t st = batchStream(G.B);
+            // base values in the group layout, rhs / solution in input order
+            long long vals = 0, vec = 0;
+            for (int j = 0; j < cnt; ++j) {
+                const BatchMember q = batchMember(G.B, j);
+                vals = std::max(vals, q.valOff + (long long)q.nnzb * q.nb * q.nb);
+                vec = std::max(vec, q.inOff + (long long)q.n * q.nb);
+            }
+            G.base.alloc(size_t(vals));
+            for (int j = 0; j < cnt; ++j) {
+                const BatchMember q = batchMember(G.B, j);
+                const Model& m = *models[size_t(G.models[size_t(j)])];
+                RSH_HIP(hipMemcpyAsync(G.base.p + q.valOff, m.baseHost.data(), m.baseHost.size() * sizeof(double), hipMemcpyHostToDevice, st));
+                // every member starts from its base values (its factors must exist even when it has no request)
+                RSH_HIP(hipMemcpyAsync(q.groupVal + q.valOff, G.base.p + q.valOff, m.baseHost.size() * sizeof(double), hipMemcpyDeviceToDevice, st));
+            }
+            G.b.alloc(size_t(vec));
+            G.x.alloc(size_t(vec));
+            G.tolH.assign(size_t(cnt), 1.0);
+            RSH_HIP(hipStreamSynchronize(st));
+        }
+        hipStream_t st = batchStream(G.B);
+        launchFill(st, G.b.p, 0.0, (long long)G.b.n);
+        std::fill(G.tolH.begin(), G.tolH.end(), 1.0);
+        int maxIt = 1;
+        std::vector<int> member(qs.size());
+        for (size_t k = 0; k < qs.size(); ++k) {
+            const rsh_request& r = qs[k]->r;
+            int j = 0;
+            while (G.models[size_t(j)] != r.model) ++j;
+            member[k] = j;
+            const BatchMember q = batchMember(G.B, j);
+            const long long nv = (long long)q.nnzb * q.nb * q.nb;
+            double* dst = q.groupVal + q.valOff;
+            const double* src = G.base.p + q.valOff;
+            if (r.values) {
+                RSH_HIP(hipMemcpyAsync(dst, r.values, size_t(nv) * sizeof(double), r.values_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+                if (r.perturb > 0) launchPerturbRows(st, q.n, q.rp, q.nb, q.groupVal, q.groupVal, r.perturb, r.perturb_seed);
+            } else if (r.perturb > 0) {
+                launchPerturbRows(st, q.n, q.rp, q.nb, G.base.p, q.groupVal, r.perturb, r.perturb_seed);
+            } else {
+                RSH_HIP(hipMemcpyAsync(dst, src, size_t(nv) * sizeof(double), hipMemcpyDeviceToDevice, st));
+            }
+            RSH_HIP(hipMemcpyAsync(G.b.p + q.inOff, r.rhs, sizeof(double) * size_t(q.n) * q.nb, r.rhs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+            G.tolH[size_t(j)] = r.tol;
+            maxIt = std::max(maxIt, r.max_it);
+        }
+        batchValuesChanged(G.B);
+        const rsh_options& opt = models[size_t(G.models[0])]->opt;
+        int rc = rsh_batch_setup(G.B, &opt);
+        if (rc != RSH_OK) throw Error(rc, g_err);
+        const size_t nc = static_cast<size_t>(cnt);
+        std::vector<int> its(nc, 0), status(nc, 0);
+        std::vector<double> rel(nc, 0.0);
+        rc = rsh_batch_solve_device(G.B, G.b.p, G.x.p, G.tolH.data(), maxIt, its.data(), rel.data(), status.data());
+        if (rc != RSH_OK && rc != RSH_ERR_NOT_CONVERGED && rc != RSH_ERR_BREAKDOWN && rc != RSH_ERR_NONFINITE) throw Error(rc, g_err);
+        for (size_t k = 0; k < qs.size(); ++k) {
+            const int j = member[k];
+            const BatchMember q = batchMember(G.B, j);
+            const rsh_request& r = qs[k]->r;
+            if (r.x)
+                RSH_HIP(hipMemcpyAsync(r.x, G.x.p + q.inOff, sizeof(double) * size_t(q.n) * q.nb, r.x_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+            qs[k]->res.status = status[size_t(j)];
+            qs[k]->res.iterations = its[size_t(j)];
+            qs[k]->res.rel_res = rel[size_t(j)];
+            if (status[size_t(j)] != RSH_OK) qs[k]->msg = "the request's system did not converge in its batched round";
+        }
+        RSH_HIP(hipStreamSynchronize(st));
+    }
+    void batchedLoop(int w) {
+        for (;;) {
+            std::vector<long long> ids;
+            std::vector<Req*> qs;
+            int g = -1;
+            bool wasResident = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cvWork.wait(lk, [&] {
+                    if (stop) return true;
+                    if (!grouped && !queue.empty()) formGroups();
+                    return grouped && (g = pickRound(ids)) >= 0;
+                });
+                if (g < 0) return;                   // stop with nothing admissible
+                Group& G = *groups[size_t(g)];
+                G.busy = true;
+                wasResident = G.models.size() == 1 ? models[size_t(G.models[0])]->S != nullptr : G.B != nullptr;
+                if (!wasResident) used += G.footprint;
+                const double now = nowMs();
+                for (long long id : ids) {
+                    Req* q = reqs.at(id).get();
+                    q->tStart = now;
+                    qs.push_back(q);
+                    models[size_t(q->r.model)]->busy = true;
+                }
+                running += int(qs.size());
+                peakRunning = std::max(peakRunning, running);
+                ++rounds;
+            }
+            Group& G = *groups[size_t(g)];
+            const bool single = G.models.size() == 1;   // a model alone: the resident single-system path
+            int code = RSH_OK;
+            std::string msg;
+            try {
+                if (single) serve(*qs[0], *models[size_t(G.models[0])], ctxs[size_t(w)]);
+                else serveRound(G, qs);
+            } catch (const Error& e) {
+                code = e.code;
+                msg = e.what();
+            } catch (const std::exception& e) {
+                code = RSH_ERR_DEVICE;
+                msg = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                const Model* sm = single ? models[size_t(G.models[0])].get() : nullptr;
+                const bool loaded = single ? sm->S != nullptr : G.B != nullptr;
+                if (loaded && !wasResident) {
+                    used -= std::min(used, G.footprint);
+                    G.footprint = single ? measure(*sm) : batchDeviceBytes(G.B) + G.base.bytes() + G.b.bytes() + G.x.bytes();
+                    used += G.footprint;
+                } else if (!loaded && !wasResident) {
+                    used -= std::min(used, G.footprint);
+                }
+                ++G.rounds;
+                const double now = nowMs();
+                for (Req* q : qs) {
+                    if (code != RSH_OK) {
+                        q->res.status = code;
+                        q->msg = msg;
+                    }
+                    q->tEnd = now;
+                    q->res.queue_ms = q->tStart - q->tSubmit;
+                    q->res.service_ms = q->tEnd - q->tStart;
+                    q->res.latency_ms = q->tEnd - q->tSubmit;
+                    q->done = true;
+                    Model& m = *models[size_t(q->r.model)];
+                    m.busy = false;
+                    m.lastUse = now;
+                }
+                G.busy = false;
+                running -= int(qs.size());
+                completed += (long long)qs.size();
+            }
+            cvDone.notify_all();
+            cvWork.notify_all();
+        }
+    }
     void workerLoop(int w) {
+        if (batched) { batchedLoop(w); return; }
         rsh_context* ctx = ctxs[size_t(w)];
         for (;;) {
             long long id = 0;
@@ -224,7 +472,8 @@ struct rsh_server {
 
 extern "C" {
 
-int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server** out) {
+static int serverCreate(int device, int slots, double hbm_budget_bytes, bool batched, long long groupCells, int groupSystems,
+                        rsh_server** out) {
     return guarded([&] {
         if (!out) return fail(RSH_ERR_ARG, "null output pointer");
         *out = nullptr;
@@ -233,6 +482,9 @@ int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server
         auto S = std::make_unique<rsh_server>();
         S->device = device;
         S->budget = size_t(hbm_budget_bytes);
+        S->batched = batched;
+        S->groupCells = groupCells;
+        S->groupSystems = groupSystems;
         for (int w = 0; w < slots; ++w) {
             rsh_context* c = nullptr;
             const int rc = rsh_context_create(device, &c);
@@ -249,6 +501,16 @@ int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server
     });
 }
 
+int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server** out) {
+    return serverCreate(device, slots, hbm_budget_bytes, false, 0, 0, out);
+}
+
+int rsh_server_create_batched(int device, int slots, double hbm_budget_bytes, long long group_cells, int group_systems, rsh_server** out) {
+    if (group_cells < 1 || group_systems < 1 || group_systems > 1024)
+        return fail(RSH_ERR_ARG, "group_cells must be positive and group_systems 1..1024");
+    return serverCreate(device, slots, hbm_budget_bytes, true, group_cells, group_systems, out);
+}
+
 int rsh_server_destroy(rsh_server* srv) {
     if (!srv) return RSH_OK;
     return guarded([&] {
@@ -261,6 +523,14 @@ int rsh_server_destroy(rsh_server* srv) {
         for (auto& t : srv->workers) t.join();
         for (auto& m : srv->models)
             if (m->S) rsh_system_destroy(m->S);
+        for (auto& g : srv->groups) {
+            if (g->B) rsh_batch_destroy(g->B);
+            g->base.release();
+            g->b.release();
+            g->x.release();
+            if (g->ctx) rsh_context_destroy(g->ctx);
+        }
+        srv->groups.clear();
         srv->models.clear();
         for (auto* c : srv->ctxs) rsh_context_destroy(c);
         delete srv;
@@ -273,6 +543,10 @@ int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, con
     return guarded([&] {
         if (!srv || !row_ptr || !col || !values || !opt || !model) return fail(RSH_ERR_ARG, "null argument");
         if (n <= 0 || nb < 1 || nb > kMaxNb) return fail(RSH_ERR_ARG, "bad system size");
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            if (srv->grouped) return fail(RSH_ERR_ARG, "batched server: models are grouped at the first request; add every model before it");
+        }
         auto m = std::make_unique<rsh_server::Model>();
         m->n = n;
         m->nb = nb;
@@ -338,8 +612,13 @@ int rsh_server_stats(rsh_server* srv, char* json, int cap) {
         o << "{\"models\":" << srv->models.size() << ",\"resident\":" << resident << ",\"loads\":" << loads << ",\"evictions\":" << srv->evictions
           << ",\"hbm_used\":" << srv->used << ",\"hbm_budget\":" << srv->budget << ",\"slots\":" << srv->ctxs.size() << ",\"queued\":"
           << srv->queue.size() << ",\"running\":" << srv->running << ",\"peak_running\":" << srv->peakRunning << ",\"completed\":" << srv->completed
+          << ",\"batched\":" << (srv->batched ? 1 : 0) << ",\"groups\":" << srv->groups.size() << ",\"rounds\":" << srv->rounds
           << ",\"footprints\":[";
         for (size_t i = 0; i < srv->models.size(); ++i) o << (i ? "," : "") << srv->models[i]->footprint;
+        o << "],\"group_models\":[";
+        for (size_t i = 0; i < srv->groups.size(); ++i) o << (i ? "," : "") << srv->groups[i]->models.size();
+        o << "],\"group_rounds\":[";
+        for (size_t i = 0; i < srv->groups.size(); ++i) o << (i ? "," : "") << srv->groups[i]->rounds;
         o << "]}";
         const std::string s = o.str();
         if (int(s.size()) + 1 > cap) return fail(RSH_ERR_ARG, "buffer too small");

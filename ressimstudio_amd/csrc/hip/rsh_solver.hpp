@@ -1087,6 +1087,21 @@ struct CprDev {
     }
 };
 
+// a member system of a batch (rsh_batch.cpp), for the serving loop's batched rounds: its block-size group's value
+// array and row pointers from the member's first row (block indices into that array), its first value there, its first
+// entry in the input-order vectors, and its sizes
+struct BatchMember {
+    double* groupVal;
+    const int* rp;
+    long long valOff, inOff;
+    int n, nb, nnzb;
+};
+BatchMember batchMember(rsh_batch* b, int i);
+// after writing members' values: every group refactors at the next rsh_batch_setup
+void batchValuesChanged(rsh_batch* b);
+hipStream_t batchStream(rsh_batch* b);
+size_t batchDeviceBytes(rsh_batch* b);
+
 inline double nowMs() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
