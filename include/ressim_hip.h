@@ -253,9 +253,9 @@ int rsh_server_create(int device, int slots, double hbm_budget_bytes, rsh_server
  * group's other models solve a zero right-hand side (they stop at once).  A request's result does not depend on the
  * other requests of its round (the batch's preconditioner is block-diagonal, its scalars per system), but it is the
  * batch engine's solve (CPR: one pressure hierarchy per group), not the single-system one.  A round runs to the
- * largest max_it among its requests.  Groups are not evicted: one that does not fit the budget waits until a
- * resident group leaves room (none does; size the budget for every group).  RSH_MIXED_IR and FGMRES are refused by
- * the batch engine. */
+ * largest max_it among its requests.  Groups are not evicted: a request whose group is not resident and does not fit
+ * the budget left beside the resident ones fails with RSH_ERR_UNSUPPORTED (size the budget for every group; the first
+ * group is always admitted).  RSH_MIXED_IR and FGMRES are refused by the batch engine. */
 int rsh_server_create_batched(int device, int slots, double hbm_budget_bytes, long long group_cells, int group_systems, rsh_server** out);
 int rsh_server_destroy(rsh_server* srv);   /* finishes the queued requests first */
 int rsh_server_add_model(rsh_server* srv, int n, int nb, const int* row_ptr, const int* col, const double* values, const rsh_options* opt,

@@ -222,15 +222,18 @@ struct rsh_server {
         grouped = true;
     }
     // next round (lock held): the first queued request whose group is idle, with every queued request of that group
-    // (the earliest per model); a group that is not resident yet must fit the budget
-    int pickRound(std::vector<long long>& ids) {
+    // (the earliest per model).  Groups are not evicted: a group that is not resident and does not fit the budget left
+    // while others are resident gets its requests back as refused (refuse = true) instead of waiting for room that
+    // never comes
+    int pickRound(std::vector<long long>& ids, bool& refuse) {
         ids.clear();
+        refuse = false;
         for (long long id : queue) {
             const int g = groupOf[size_t(reqs.at(id)->r.model)];
             Group& G = *groups[size_t(g)];
             if (G.busy) continue;
             const bool resident = G.models.size() == 1 ? models[size_t(G.models[0])]->S != nullptr : G.B != nullptr;
-            if (!resident && used + G.footprint > budget && used > 0) continue;
+            refuse = !resident && used + G.footprint > budget && used > 0;
             std::vector<char> taken(models.size(), 0);
             for (auto it = queue.begin(); it != queue.end();) {
                 const int m = reqs.at(*it)->r.model;
@@ -250,8 +253,11 @@ struct rsh_server {
         RSH_HIP(hipSetDevice(device));
         const int cnt = int(G.models.size());
         if (!G.B) {
-            int rc = rsh_context_create(device, &G.ctx);
-            if (rc != RSH_OK) throw Error(rc, g_err);
+            int rc = G.ctx ? RSH_OK : rsh_context_create(device, &G.ctx);
+            if (rc != RSH_OK) {
+                G.ctx = nullptr;
+                throw Error(rc, g_err);
+            }
             const size_t nc = static_cast<size_t>(cnt);
             std::vector<int> n(nc, 0), nb(nc, 0);
             std::vector<const int*> rp(nc, nullptr), ci(nc, nullptr);
@@ -338,19 +344,19 @@ struct rsh_server {
             std::vector<long long> ids;
             std::vector<Req*> qs;
             int g = -1;
-            bool wasResident = false;
+            bool wasResident = false, refuse = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cvWork.wait(lk, [&] {
                     if (stop) return true;
                     if (!grouped && !queue.empty()) formGroups();
-                    return grouped && (g = pickRound(ids)) >= 0;
+                    return grouped && (g = pickRound(ids, refuse)) >= 0;
                 });
                 if (g < 0) return;                   // stop with nothing admissible
                 Group& G = *groups[size_t(g)];
                 G.busy = true;
                 wasResident = G.models.size() == 1 ? models[size_t(G.models[0])]->S != nullptr : G.B != nullptr;
-                if (!wasResident) used += G.footprint;
+                if (!wasResident && !refuse) used += G.footprint;
                 const double now = nowMs();
                 for (long long id : ids) {
                     Req* q = reqs.at(id).get();
@@ -367,6 +373,8 @@ struct rsh_server {
             int code = RSH_OK;
             std::string msg;
             try {
+                if (refuse) throw Error(RSH_ERR_UNSUPPORTED, "batched server: the request's model group does not fit the HBM budget left (groups are "
+                                                             "not evicted; size the budget for every group)");
                 if (single) serve(*qs[0], *models[size_t(G.models[0])], ctxs[size_t(w)]);
                 else serveRound(G, qs);
             } catch (const Error& e) {
@@ -384,7 +392,7 @@ struct rsh_server {
                     used -= std::min(used, G.footprint);
                     G.footprint = single ? measure(*sm) : batchDeviceBytes(G.B) + G.base.bytes() + G.b.bytes() + G.x.bytes();
                     used += G.footprint;
-                } else if (!loaded && !wasResident) {
+                } else if (!loaded && !wasResident && !refuse) {
                     used -= std::min(used, G.footprint);
                 }
                 ++G.rounds;

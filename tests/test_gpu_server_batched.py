@@ -78,3 +78,18 @@ def test_batched_server_argument_checks():
         rs.Server(slots=2, batched=True, group_cells=0)
     with pytest.raises(rs.RshError):
         rs.Server(slots=2, batched=True, group_systems=0)
+
+
+def test_batched_groups_beyond_the_budget_are_refused(mods):
+    # a one-byte budget: the first group is admitted (nothing else resident), every other group is refused at once
+    # (groups are not evicted) rather than left waiting; the admitted group keeps serving
+    srv = rs.Server(slots=1, batched=True, group_cells=3000, group_systems=16, hbm_budget=1.0)
+    ids = [srv.add_model(s, precond="mcbilu0") for s in mods]
+    res = [srv.wait(srv.submit(ids[m], mods[m].rhs, tol=TOL, max_it=2000)) for m in range(len(mods))]
+    st = srv.stats()
+    assert st["groups"] == 3 and sorted(st["group_models"]) == [1, 1, 2], st     # sizes 720 + 1920 | 4680 | 5760 cells
+    assert res[0]["status"] == rs.OK and res[2]["status"] == rs.OK                 # models 0 and 2: the first group
+    assert res[1]["status"] == rs.ERR_UNSUPPORTED and res[3]["status"] == rs.ERR_UNSUPPORTED
+    again = srv.wait(srv.submit(ids[0], mods[0].rhs, tol=TOL, max_it=2000))
+    assert again["status"] == rs.OK and np.array_equal(again["x"], res[0]["x"])
+    srv.close()
