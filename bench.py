@@ -431,6 +431,9 @@ def main():
         # DeviceCprAmg: Jacobi V-cycle (one step, weight 0.6667) + block-Jacobi second stage (no ILU)
         "cpr_bj": rs.default_options(precond="cpr_bj", precision=FP64, krylov=BI, smoother=J, smooth_steps=1),
         "cpr": rs.default_options(precond="cpr", precision=FP64, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0),
+        # the head's CPR with the BILU0 of red / black tiles of 10 x 10 x 6 cells instead of the cell colouring
+        "cpr_rbt": rs.default_options(precond="cpr_mc", precision=FP64, krylov=BI, smoother=J, smooth_steps=2, jacobi_omega=1.0,
+                                      ilu_rb_tiles=10 | (10 << 8) | (6 << 16)),
         "ilu0": rs.default_options(precond="bilu0", precision=FP64, krylov=BI),
         "mcilu0": rs.default_options(precond="mcbilu0", precision=FP64, krylov=BI),
         # FP32-stored preconditioner (factors, hierarchy, pressure column) with FP64 vectors and arithmetic, FP64 BiCGSTAB

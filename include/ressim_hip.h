@@ -104,6 +104,15 @@ typedef struct rsh_options {
                            with the FP64 preconditioner (a well-scaled system keeps FP32; one whose FP32 preconditioner
                            loses too much accuracy pays the stall once, not once per Newton iteration); 1: every solve
                            starts with FP32 again.  rsh_system_info reports the state as fp32_stalled */
+    int ilu_rb_tiles;   /* RSH_PREC_MCBILU0 / RSH_PREC_CPR_MC: instead of colouring single cells, colour tiles of extents
+                           bx | by << 8 | bz << 16 along the pattern's dominant lower offsets (the grid axes of a
+                           natural-ordered grid) red / black by the parity of their tile coordinates; rows by colour, tile
+                           and natural order inside the tile, the rows far longer than the bulk (a well's perforated cells)
+                           behind the tiles.  The BILU0 of that order is factorised and swept tile by tile (a tile of one
+                           colour couples only to tiles of the other): fewer iterations than the cell colouring, closer to
+                           the natural order's.  Not the reference's factors (neither is MC-BILU0's order).  0 (default):
+                           the cell colouring; a pattern without grid offsets, or whose tiles of one colour are coupled
+                           beyond a tenth of its rows, keeps the cell colouring */
 } rsh_options;
 
 /* ---- library / context */
@@ -158,6 +167,9 @@ int rsh_precond_apply(rsh_system* sys, const double* r, double* z);
 int rsh_amg_apply(rsh_system* sys, const double* r, double* z);          /* CPR only: one V-cycle, n doubles */
 int rsh_cpr_weights(rsh_system* sys, double* w);                         /* CPR only: n*nb quasi-IMPES weights */
 int rsh_bilu0_factors(rsh_system* sys, double* lu);                      /* natural-order factors, S's layout */
+/* the ILU's working order: perm[w] = the original row of working row w (level order, tile order, colour order or
+ * red / black tile order); the factors of rsh_bilu0_factors belong to P A P^T in this order in the multicolour forms */
+int rsh_ilu_order(rsh_system* sys, int* perm);
 int rsh_amg_levels(rsh_system* sys, int* nlevels, int* sizes, int* nnz, int cap);
 int rsh_system_info(rsh_system* sys, char* json, int cap);               /* levels, colours, bytes, last solve */
 

@@ -37,7 +37,7 @@ EXPORTED = [
     "rsh_version", "rsh_last_error", "rsh_device_count", "rsh_default_options", "rsh_context_create", "rsh_context_destroy",
     "rsh_context_synchronize", "rsh_system_create", "rsh_system_destroy", "rsh_system_set_values", "rsh_system_set_values_device",
     "rsh_system_set_accumulation", "rsh_system_set_accumulation_device", "rsh_system_setup", "rsh_solve", "rsh_solve_device", "rsh_spmv", "rsh_precond_apply", "rsh_amg_apply", "rsh_cpr_weights",
-    "rsh_bilu0_factors", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
+    "rsh_bilu0_factors", "rsh_ilu_order", "rsh_amg_levels", "rsh_system_info", "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_h2d",
     "rsh_memcpy_d2h", "rsh_time_kernel", "rsh_batch_create", "rsh_batch_destroy", "rsh_batch_set_values",
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
     "rsh_batch_info", "rsh_server_create", "rsh_server_create_batched", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
@@ -57,7 +57,7 @@ class Options(ctypes.Structure):
                 ("restart", ctypes.c_int), ("max_outer", ctypes.c_int), ("inner_tol", ctypes.c_double), ("reuse_amg", ctypes.c_int),
                 ("smooth_steps", ctypes.c_int), ("jacobi_omega", ctypes.c_double), ("ilu_lean", ctypes.c_int),
                 ("coarse_smooth_steps", ctypes.c_int), ("ilu_schedule", ctypes.c_int), ("ilu_tile_dims", ctypes.c_int),
-                ("retry_fp32", ctypes.c_int)]
+                ("retry_fp32", ctypes.c_int), ("ilu_rb_tiles", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -107,6 +107,7 @@ def lib():
         L.rsh_system_set_accumulation.argtypes = [_vp, _dp]
         L.rsh_system_set_accumulation_device.argtypes = [_vp, _vp]
         L.rsh_bilu0_factors.argtypes = [_vp, _dp]
+        L.rsh_ilu_order.argtypes = [_vp, _ip]
         L.rsh_amg_levels.argtypes = [_vp, _ip, _ip, _ip, ctypes.c_int]
         L.rsh_system_info.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
         L.rsh_device_alloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
@@ -299,6 +300,11 @@ class System:
         lu = np.empty(int(self.row_ptr[-1]) * self.nb * self.nb)
         _check(lib().rsh_bilu0_factors(self.h, _p(lu, ctypes.c_double)), "rsh_bilu0_factors")
         return lu
+
+    def ilu_order(self):
+        perm = np.empty(self.n, np.int32)
+        _check(lib().rsh_ilu_order(self.h, _p(perm, ctypes.c_int)), "rsh_ilu_order")
+        return perm
 
     def info(self) -> dict:
         buf = ctypes.create_string_buffer(1 << 16)

@@ -571,10 +571,14 @@ int rsh_system_setup(rsh_system* sys, const rsh_options* opt) {
         const bool wantMc = o.precond == RSH_PREC_MCBILU0 || o.precond == RSH_PREC_CPR_MC;
         const bool wantIlu = o.precond != RSH_PREC_CPR_BJ;
         if (o.ilu_schedule < 0 || o.ilu_schedule > 2) return fail(RSH_ERR_ARG, "unknown ILU schedule");
-        if (wantIlu && (!S.ilu || S.ilu->mc != wantMc || (!wantMc && (S.ilu->schedule != o.ilu_schedule || S.ilu->tileDims != o.ilu_tile_dims)))) {
+        if (o.ilu_rb_tiles < 0 || o.ilu_rb_tiles >= (1 << 24)) return fail(RSH_ERR_ARG, "ilu_rb_tiles: extents bx | by << 8 | bz << 16, each below 256");
+        const int rbDims = wantMc ? o.ilu_rb_tiles : 0;
+        if (wantIlu && (!S.ilu || S.ilu->mc != wantMc || (!wantMc && (S.ilu->schedule != o.ilu_schedule || S.ilu->tileDims != o.ilu_tile_dims)) ||
+                        (wantMc && S.ilu->rbDims != rbDims))) {
             S.ilu = std::make_unique<IluDev>();
             S.ilu->schedule = o.ilu_schedule;
             S.ilu->tileDims = o.ilu_tile_dims;
+            S.ilu->rbDims = rbDims;
             S.ilu->keepEntries = S.keepIluEntries;
             S.ilu->fuseMax = S.iluFuseMax;
             S.ilu->rowSys = S.rowSys.empty() ? nullptr : &S.rowSys;
@@ -749,6 +753,15 @@ int rsh_bilu0_factors(rsh_system* sys, double* lu) {
     });
 }
 
+int rsh_ilu_order(rsh_system* sys, int* perm) {
+    return guarded([&] {
+        if (!sys || !perm) return fail(RSH_ERR_ARG, "null argument");
+        if (!sys->ready || !sys->ilu) return fail(RSH_ERR_ARG, "call rsh_system_setup first");
+        RSH_HIP(hipMemcpy(perm, sys->ilu->perm.p, sizeof(int) * size_t(sys->n), hipMemcpyDeviceToHost));
+        return RSH_OK;
+    });
+}
+
 int rsh_amg_levels(rsh_system* sys, int* nlevels, int* sizes, int* nnz, int cap) {
     return guarded([&] {
         if (!sys || !nlevels) return fail(RSH_ERR_ARG, "null argument");
@@ -774,7 +787,11 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
           << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks << ",\"device_bytes\":" << S.deviceBytes();
         if (S.ilu) {
             const IluDev& I = *S.ilu;
-            o << ",\"ilu_tiled\":" << (I.tiled ? 1 : 0) << ",\"ilu_tail_copy\":" << (I.tailCopy ? 1 : 0) << ",\"ilu_bwd_fused\":" << (I.fuseBwd ? 1 : 0);
+            o << ",\"ilu_tiled\":" << (I.tiled ? 1 : 0) << ",\"ilu_tail_copy\":" << (I.tailCopy ? 1 : 0) << ",\"ilu_bwd_fused\":" << (I.fuseBwd ? 1 : 0) << ",\"ilu_rb_tiles\":" << (I.rbt ? 1 : 0);
+            if (I.rbt)
+                o << ",\"ilu_rb\":{\"tiles\":" << I.rbTiles << ",\"red_tiles\":" << I.rbRedTiles << ",\"units\":" << I.rb.unitStart.size() - 1
+                  << ",\"red_levels\":" << I.rb.redLevels << ",\"black_levels\":" << I.rb.blackLevels << ",\"far_rows\":" << I.n - I.rb.nTile
+                  << ",\"far_levels\":" << I.rb.farStart.size() - 1 << "}";
             if (I.tiled)
                 o << ",\"ilu_tiles\":{\"dims\":[" << I.tiles.dims[0] << "," << I.tiles.dims[1] << "," << I.tiles.dims[2] << "],\"tile_levels\":" << I.tiles.tileLevels()
                   << ",\"units\":" << I.tiles.units() << ",\"max_rows\":" << I.tiles.maxRows << ",\"max_levels\":" << I.tiles.maxLevels

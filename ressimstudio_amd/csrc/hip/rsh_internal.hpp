@@ -194,6 +194,23 @@ struct TileSched {
 int gridCoords(const HostBsr& A, int off[3], std::vector<int> coord[3]);
 void tileOrder(const HostBsr& A, const std::vector<int> coord[3], int ndim, const int dims[3], TileSched& T);
 
+// Red / black tiles: the grid cut into tiles of dims cells along the pattern's dominant lower offsets, the tiles
+// coloured by the parity of their tile coordinates; the bulk rows by colour, tile, local level (natural order inside a
+// tile) and row, then the rows far longer than the bulk (a well's perforated cells after the Schur fill) in the level
+// order of their own couplings.  A tile of one colour couples only to tiles of the other (face neighbours of a 7-point
+// stencil), so the tiles of a colour are independent and each is a natural-order triangular sweep of its own.
+struct RbTiles {
+    std::vector<int> ord;          // working row -> original row
+    int nRed = 0, nTile = 0;       // red bulk rows [0, nRed), black [nRed, nTile), far rows [nTile, n)
+    std::vector<int> unitStart;    // (tile, local level) units of the bulk rows: first rows (+ nTile)
+    std::vector<int> unitLevel;    // per unit: its factorisation level (red local levels, then black ones)
+    std::vector<int> tileUnit;     // per tile (working order): first unit (+ units); tiles [0, redTiles) are red
+    int redTiles = 0, redLevels = 0, blackLevels = 0;
+    std::vector<int> farStart;     // level starts of the far rows, relative to nTile
+};
+// false when the pattern does not admit it (no grid offsets, one colour empty, same-colour tiles coupled)
+bool rbTileOrder(const HostBsr& S, const int dims[3], RbTiles& T);
+
 // AMG set-up on the host (AMG::setup, amg.cpp:137-252): returns the hierarchy
 struct AmgLevelHost {
     HostCsr A;
@@ -221,6 +238,11 @@ void launchSpmvSliced(hipStream_t s, int nb, bool f32, const SliceDev<double>* d
 enum IluFp { kIluFp64 = 0, kIluFp32 = 1, kIluStore32 = 2 };
 // ILU sweeps of one level (fp: IluFp).  zFused (forward): the rows whose slice lane carries a pivot block (dsrc >= 0:
 // no upper entries, read by no later row) finish their backward step here, z = D^-1 y into y and z (+ xp)
+// red / black tile sweeps: tiles[t] = (first slice, end slice, first working row, end row) of tile t, one slice per
+// local level, at most kTileWaveCap slots per row; forward (fwd) or backward
+constexpr int kTileWaveCap = 8;
+void launchIluTileWave(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int ntiles, const int4* tiles, const double* r, void* y,
+                       double* z, const double* xp);
 void launchIluForward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, const double* r, void* y,
                       const KScalars* gate = nullptr, double* zFused = nullptr, const double* xp = nullptr);
 void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0, int s1, int q0, int q1, void* y, double* z,

@@ -464,6 +464,72 @@ __global__ __launch_bounds__(kBlock) void kIluBackward(int s0, int ns, int q0, i
 }
 
 
+// Red / black tile sweeps: one wavefront per tile of one colour (kTileWaveTiles tiles per workgroup) walks the tile's
+// slices - one per local level, natural order inside the tile - forward in order or backward in reverse, with a
+// workgroup barrier between levels, which makes a level's writes to the working vector visible to the next one.  Each
+// row's arithmetic is kIluForward's (y_i = r_i - sum L_ik y_k, slot by slot) or kIluBackward's (z_i = U_ii^-1 (y_i -
+// sum U_ik z_k)), with the values of the rows of other tiles (the other colour, the rows behind the tiles) read from
+// the working vector as those kernels do.  tiles[t] = (first slice, end slice, first working row, end row).
+// (Measured: holding the tile's own values in LDS and fetching every slot of a level in one round, with the next
+// level's columns prefetched, needed 246-256 registers, two wavefronts per SIMD, and was slower.)
+constexpr int kTileWaveTiles = 4;
+template <int NB, typename TA, typename TV, bool FWD>
+__global__ __launch_bounds__(kTileWaveTiles * kWave) void kIluTileWave(int ntiles, const int4* __restrict__ tiles, const int2* __restrict__ info,
+                                                                       const int* __restrict__ row, const int* __restrict__ orig,
+                                                                       const int* __restrict__ col, const TA* __restrict__ val,
+                                                                       const TA* __restrict__ dval, const double* __restrict__ r, TV* y,
+                                                                       double* __restrict__ z, const double* __restrict__ xp) {
+#pragma clang fp contract(off)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tb = blockIdx.x * kTileWaveTiles;
+    int steps = 0;
+    for (int k = 0; k < kTileWaveTiles; ++k)
+        if (tb + k < ntiles) {
+            const int4 Tk = tiles[tb + k];
+            steps = max(steps, Tk.y - Tk.x);
+        }
+    int4 T = make_int4(0, 0, 0, 0);
+    if (tb + wave < ntiles) T = tiles[tb + wave];
+    const int ns = T.y - T.x;
+    for (int q = 0; q < steps; ++q) {
+        if (q < ns) {
+            const int sl = FWD ? T.x + q : T.y - 1 - q;
+            const int wr = row[(long long)sl * kWave + lane];
+            if (wr >= 0) {
+                const int o = orig[(long long)sl * kWave + lane];
+                const int2 inf = info[sl];
+                TV acc[NB];
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] = FWD ? TV(r[(long long)o * NB + e]) : y[(long long)wr * NB + e];
+                sliceRowSweep<NB, TA, TV>(col, val, inf, lane, y, acc);
+                if (FWD) {
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) y[(long long)wr * NB + e] = acc[e];
+                } else {
+                    const TA* d = dval + (long long)sl * (NB * NB * kWave) + lane;
+                    TV t[NB];
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+                        TV sacc = 0;
+#pragma unroll
+                        for (int f = 0; f < NB; ++f) sacc += TV(d[(e * NB + f) * kWave]) * acc[f];
+                        t[e] = sacc;
+                    }
+#pragma unroll
+                    for (int e = 0; e < NB; ++e) {
+                        y[(long long)wr * NB + e] = t[e];
+                        double v = double(t[e]);
+                        if (e == 0 && xp) v += xp[o];
+                        z[(long long)o * NB + e] = v;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
 // ---------------------------------------------------------------- BILU0 factorisation, one level, in place
 // Row i (bsr.cpp:183-215): for each k < i in row order: L_ik = A_ik U_kk^-1; A_ij -= L_ik U_kj for every j > k
 // present in row i; then the pivot block is inverted (U_ii^-1 stored).  Rows of one level only read rows of
@@ -2833,6 +2899,26 @@ void launchIluBackward(hipStream_t s, int nb, int fp, const void* slices, int s0
     checkLaunch();
 }
 
+
+template <int NB, typename TA, typename TV>
+void iluTileWaveT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int ntiles, const int4* tiles, const double* r, TV* y, double* z, const double* xp) {
+    if (ntiles <= 0) return;
+    if (!fwd && !d.dval.n) throw Error(-1, "tile sweep: the upper slices carry no pivot blocks");
+    const dim3 grid((ntiles + kTileWaveTiles - 1) / kTileWaveTiles);
+    if (fwd)
+        hipLaunchKernelGGL((kIluTileWave<NB, TA, TV, true>), grid, dim3(kTileWaveTiles * kWave), 0, s, ntiles, tiles, d.info.p, d.row.p, d.orig.p,
+                           d.col.p, d.val.p, d.dval.p, r, y, z, xp);
+    else
+        hipLaunchKernelGGL((kIluTileWave<NB, TA, TV, false>), grid, dim3(kTileWaveTiles * kWave), 0, s, ntiles, tiles, d.info.p, d.row.p, d.orig.p,
+                           d.col.p, d.val.p, d.dval.p, r, y, z, xp);
+}
+void launchIluTileWave(hipStream_t s, int nb, int fp, const void* slices, bool fwd, int ntiles, const int4* tiles, const double* r, void* y,
+                       double* z, const double* xp) {
+    if (fp == kIluFp32) RSH_NB_DISPATCH(nb, (iluTileWaveT<NB, float, float>(s, *static_cast<const SliceDev<float>*>(slices), fwd, ntiles, tiles, r, (float*)y, z, xp)))
+    else if (fp == kIluStore32) RSH_NB_DISPATCH(nb, (iluTileWaveT<NB, float, double>(s, *static_cast<const SliceDev<float>*>(slices), fwd, ntiles, tiles, r, (double*)y, z, xp)))
+    else RSH_NB_DISPATCH(nb, (iluTileWaveT<NB, double, double>(s, *static_cast<const SliceDev<double>*>(slices), fwd, ntiles, tiles, r, (double*)y, z, xp)))
+    checkLaunch();
+}
 
 template <int NB, typename TA, typename TV>
 void runOutsideT(hipStream_t s, const SliceDev<TA>& d, bool fwd, int t0, int ns, int u0, int nq, int wbase, int wend, const double* r,

@@ -541,4 +541,116 @@ void tileOrder(const HostBsr& A, const std::vector<int> coord[3], int ndim, cons
     T.tlStart.push_back(int(T.unitStart.size()) - 1);
 }
 
+bool rbTileOrder(const HostBsr& S, const int dims[3], RbTiles& T) {
+    T = RbTiles{};
+    const int n = S.n;
+    int off[3] = {0, 0, 0};
+    std::vector<int> coord[3];
+    const int nd = gridCoords(S, off, coord);
+    if (nd == 0 || n == 0) return false;
+    std::vector<int> len(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) len[size_t(i)] = S.rp[size_t(i) + 1] - S.rp[size_t(i)];
+    const int cap = rowCap(len);
+    std::vector<char> far(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) far[size_t(i)] = len[size_t(i)] > cap || len[size_t(i)] - 1 > kTileWaveCap;   // the tile sweep's slots per row
+    // tile key (tile coordinates, the slowest axis first) and colour (parity of the tile coordinates' sum)
+    std::vector<long long> tile(static_cast<size_t>(n));
+    std::vector<char> colour(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        long long key = 0;
+        int par = 0;
+        for (int a = nd; a-- > 0;) {
+            const int t = coord[a][size_t(i)] / std::max(1, dims[a]);
+            key = key * 65536 + t;
+            par += t;
+        }
+        tile[size_t(i)] = key;
+        colour[size_t(i)] = char(par & 1);
+    }
+    // bulk rows of different tiles of one colour must not be coupled (a 7-point stencil couples face neighbours only):
+    // a row with such a coupling (a perforated cell coupled to the well's other perforations) goes behind the tiles too
+    std::vector<char> cross(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; ++i) {
+        if (far[size_t(i)]) continue;
+        for (int p = S.rp[size_t(i)]; p < S.rp[size_t(i) + 1]; ++p) {
+            const int j = S.ci[size_t(p)];
+            if (j != i && !far[size_t(j)] && tile[size_t(j)] != tile[size_t(i)] && colour[size_t(j)] == colour[size_t(i)]) cross[size_t(i)] = 1;
+        }
+    }
+    long long nfar = 0;
+    for (int i = 0; i < n; ++i) {
+        far[size_t(i)] = far[size_t(i)] || cross[size_t(i)];
+        nfar += far[size_t(i)];
+    }
+    if (10 * nfar > n) return false;   // not a grid with a few wells
+    // local level inside a tile: natural order, one more than the coupled earlier rows of the same tile
+    std::vector<int> lvl(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; ++i) {
+        if (far[size_t(i)]) continue;
+        int l = 0;
+        for (int p = S.rp[size_t(i)]; p < S.rp[size_t(i) + 1]; ++p) {
+            const int j = S.ci[size_t(p)];
+            if (j < i && !far[size_t(j)] && tile[size_t(j)] == tile[size_t(i)]) l = std::max(l, lvl[size_t(j)] + 1);
+        }
+        lvl[size_t(i)] = l;
+    }
+    std::vector<int> bulk;
+    std::vector<int> farRows;
+    for (int i = 0; i < n; ++i) (far[size_t(i)] ? farRows : bulk).push_back(i);
+    std::sort(bulk.begin(), bulk.end(), [&](int a, int b) {
+        if (colour[size_t(a)] != colour[size_t(b)]) return colour[size_t(a)] < colour[size_t(b)];
+        if (tile[size_t(a)] != tile[size_t(b)]) return tile[size_t(a)] < tile[size_t(b)];
+        if (lvl[size_t(a)] != lvl[size_t(b)]) return lvl[size_t(a)] < lvl[size_t(b)];
+        return a < b;
+    });
+    T.nTile = int(bulk.size());
+    T.nRed = 0;
+    while (T.nRed < T.nTile && colour[size_t(bulk[size_t(T.nRed)])] == 0) ++T.nRed;
+    if (T.nRed == 0 || T.nRed == T.nTile) return false;
+    // units (tile, local level) and tiles over the bulk rows; factor levels: red local levels, then black ones
+    int redLevels = 0;
+    for (int k = 0; k < T.nRed; ++k) redLevels = std::max(redLevels, lvl[size_t(bulk[size_t(k)])] + 1);
+    T.redLevels = redLevels;
+    int blackLevels = 0;
+    for (int k = T.nRed; k < T.nTile; ++k) blackLevels = std::max(blackLevels, lvl[size_t(bulk[size_t(k)])] + 1);
+    T.blackLevels = blackLevels;
+    for (int k = 0; k < T.nTile; ++k) {
+        const int i = bulk[size_t(k)];
+        const bool newTile = k == 0 || tile[size_t(bulk[size_t(k - 1)])] != tile[size_t(i)];
+        const bool newUnit = newTile || lvl[size_t(bulk[size_t(k - 1)])] != lvl[size_t(i)];
+        if (newTile) {
+            T.tileUnit.push_back(int(T.unitStart.size()));
+            if (k < T.nRed) T.redTiles = int(T.tileUnit.size());
+        }
+        if (newUnit) {
+            T.unitStart.push_back(k);
+            T.unitLevel.push_back(colour[size_t(i)] ? redLevels + lvl[size_t(i)] : lvl[size_t(i)]);
+        }
+    }
+    T.unitStart.push_back(T.nTile);
+    T.tileUnit.push_back(int(T.unitStart.size()) - 1);
+    // the far rows behind the tiles, in the level order of their own couplings
+    HostBsr W;
+    W.n = int(farRows.size());
+    W.nb = S.nb;
+    W.rp.assign(1, 0);
+    std::vector<int> at(static_cast<size_t>(n), -1);
+    for (size_t k = 0; k < farRows.size(); ++k) at[size_t(farRows[k])] = int(k);
+    for (int i : farRows) {
+        for (int p = S.rp[size_t(i)]; p < S.rp[size_t(i) + 1]; ++p)
+            if (at[size_t(S.ci[size_t(p)])] >= 0) W.ci.push_back(at[size_t(S.ci[size_t(p)])]);
+        W.rp.push_back(int(W.ci.size()));
+    }
+    W.diag.assign(size_t(W.n), -1);
+    for (int i = 0; i < W.n; ++i)
+        for (int p = W.rp[size_t(i)]; p < W.rp[size_t(i) + 1]; ++p)
+            if (W.ci[size_t(p)] == i) W.diag[size_t(i)] = p;
+    std::vector<int> word;
+    T.farStart.assign(1, 0);
+    if (W.n > 0) levelOrder(W, word, T.farStart);
+    T.ord = bulk;
+    for (int k : word) T.ord.push_back(farRows[size_t(k)]);
+    return true;
+}
+
 }  // namespace rsh

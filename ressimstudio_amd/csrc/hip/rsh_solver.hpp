@@ -287,6 +287,16 @@ struct IluDev {
     // multicolour order: some rows finish their backward step in the forward launch (their lower slice lanes carry the
     // pivot block, dsrc >= 0) and are left out of the upper slices
     bool fuseBwd = false;
+    // red / black tiles instead of the cell colouring (rbDims = bx | by << 8 | bz << 16, 0: off; rbTileOrder): level 0
+    // holds the red tiles' rows, level 1 the black ones', the levels after them the rows behind the tiles; levels 0 and
+    // 1 are swept tile by tile (kIluTileWave: tilesL / tilesU = each tile's slices, red tiles first) and factorised by
+    // local level (factor levels fLevel0[l] .. fLevel0[l + 1] of sweep level l)
+    int rbDims = 0;
+    bool rbt = false;
+    RbTiles rb;
+    DBuf<int4> tilesL, tilesU;
+    int rbTiles = 0, rbRedTiles = 0, rbMaxRows = 0;
+    std::vector<int> fLevel0;
     // natural order: the long rows' update maps (maxLen 0: none, the long rows use kBilu0LongRows)
     DBuf<int> lmRow, lmPtr;
     DBuf<int2> lmMap;
@@ -335,7 +345,19 @@ struct IluDev {
             if (!tiled) naturalEntries(ord);
             sweepOrd = tiled ? tiles.ord : ord;
         } else {
-            colours = colourGraph(S, colour, ord, lstart);
+            rbt = false;
+            if (rbDims > 0 && !rowSys) {
+                const int dims[3] = {rbDims & 255, (rbDims >> 8) & 255, (rbDims >> 16) & 255};
+                rbt = rbTileOrder(S, dims, rb);
+            }
+            if (rbt) {
+                ord = rb.ord;
+                lstart.assign({0, rb.nRed, rb.nTile});
+                for (size_t k = 1; k < rb.farStart.size(); ++k) lstart.push_back(rb.nTile + rb.farStart[k]);
+                colours = int(lstart.size()) - 1;
+            } else {
+                colours = colourGraph(S, colour, ord, lstart);
+            }
             std::vector<int> bmap;
             permutePattern(S, ord, fpat, bmap);
             frp.upload(fpat.rp, s);
@@ -369,7 +391,7 @@ struct IluDev {
             // slices), and are left out of the upper slices
             std::vector<char> skipU;
             const int capL = entriesCap(EL);
-            if (mc && !rowSys) {
+            if (mc && !rowSys && !rbt) {
                 std::vector<char> read(static_cast<size_t>(n), 0);
                 for (int c : EL.col) if (c >= 0 && c < n) read[size_t(c)] = 1;
                 skipU.assign(size_t(n), 0);
@@ -388,8 +410,54 @@ struct IluDev {
                     skipU.clear();
                 }
             }
-            buildSlices(EL, lstart, capL, hL, sL);
-            buildSlices(EU, lstart, entriesCap(EU), hU, sU, nullptr, skipU.empty() ? nullptr : &skipU);
+            if (rbt) {
+                // slices per (tile, local level) unit, then per level of the rows behind the tiles; a tile row is never a
+                // long row (the caps cover every tile row); the sweep levels' slice ranges are the units' ranges joined
+                std::vector<int> bounds = rb.unitStart;
+                for (size_t k = 1; k < rb.farStart.size(); ++k) bounds.push_back(rb.nTile + rb.farStart[k]);
+                auto tileCap = [&](const RowEntries& E) {
+                    int c = entriesCap(E);
+                    for (int w = 0; w < rb.nTile; ++w) c = std::max(c, E.ptr[size_t(w) + 1] - E.ptr[size_t(w)]);
+                    return c;
+                };
+                LevelSched uL, uU;
+                buildSlices(EL, bounds, tileCap(EL), hL, uL);
+                buildSlices(EU, bounds, tileCap(EU), hU, uU);
+                const int nu = int(rb.unitStart.size()) - 1, redU = rb.tileUnit[size_t(rb.redTiles)];
+                for (int u = 0; u < nu; ++u)
+                    if (uL.qStart[size_t(u)] != uL.qStart[size_t(u) + 1] || uU.qStart[size_t(u)] != uU.qStart[size_t(u) + 1])
+                        throw Error(RSH_ERR_DEVICE, "red / black tiles: a tile row became a long row");
+                auto join = [&](const LevelSched& u, LevelSched& out) {
+                    out.sStart = {u.sStart[0], u.sStart[size_t(redU)], u.sStart[size_t(nu)]};
+                    out.qStart = {u.qStart[0], u.qStart[size_t(redU)], u.qStart[size_t(nu)]};
+                    for (size_t k = size_t(nu) + 1; k < u.sStart.size(); ++k) {
+                        out.sStart.push_back(u.sStart[k]);
+                        out.qStart.push_back(u.qStart[k]);
+                    }
+                };
+                join(uL, sL);
+                join(uU, sU);
+                rbTiles = int(rb.tileUnit.size()) - 1;
+                rbRedTiles = rb.redTiles;
+                std::vector<int4> tl, tu;
+                rbMaxRows = 0;
+                for (int t = 0; t < rbTiles; ++t) {
+                    const int u0 = rb.tileUnit[size_t(t)], u1 = rb.tileUnit[size_t(t) + 1];
+                    const int r0 = rb.unitStart[size_t(u0)], r1 = rb.unitStart[size_t(u1)];
+                    rbMaxRows = std::max(rbMaxRows, r1 - r0);
+                    tl.push_back(make_int4(uL.sStart[size_t(u0)], uL.sStart[size_t(u1)], r0, r1));
+                    tu.push_back(make_int4(uU.sStart[size_t(u0)], uU.sStart[size_t(u1)], r0, r1));
+                }
+                for (int sl = 0; sl < uL.sStart[size_t(nu)]; ++sl)
+                    if (hL.width[size_t(sl)] > kTileWaveCap) throw Error(RSH_ERR_DEVICE, "red / black tiles: a tile row has too many slots");
+                for (int sl = 0; sl < uU.sStart[size_t(nu)]; ++sl)
+                    if (hU.width[size_t(sl)] > kTileWaveCap) throw Error(RSH_ERR_DEVICE, "red / black tiles: a tile row has too many slots");
+                tilesL.upload(tl, s);
+                tilesU.upload(tu, s);
+            } else {
+                buildSlices(EL, lstart, capL, hL, sL);
+                buildSlices(EU, lstart, entriesCap(EU), hU, sU, nullptr, skipU.empty() ? nullptr : &skipU);
+            }
         }
         if (rowSys) {
             sliceOwners(hL, *rowSys);
@@ -408,7 +476,15 @@ struct IluDev {
             for (int l = 0; l < nl; ++l) rowsL[size_t(l)] = lstart[size_t(l) + 1] - lstart[size_t(l)];
             runsL.clear();
             runsU.clear();
-            if (!tiled) {
+            if (!tiled && rbt) {
+                // the two tile levels launch alone; the rows behind the tiles in runs as the multicolour order's
+                std::vector<int> wL2(wL.begin() + 2, wL.end()), wU2(wU.begin() + 2, wU.end()), rows2(rowsL.begin() + 2, rowsL.end());
+                for (auto* pr : {&runsL, &runsU}) {
+                    const std::vector<LevelRun> rest = levelRuns(pr == &runsL ? wL2 : wU2, fuseMax, &rows2, runRowsMax);
+                    pr->assign({{0, 1, false}, {1, 2, false}});
+                    for (const LevelRun& r2 : rest) pr->push_back({r2.l0 + 2, r2.l1 + 2, r2.fused});
+                }
+            } else if (!tiled) {
                 runsL = levelRuns(wL, fuseMax, &rowsL, runRowsMax);
                 runsU = levelRuns(wU, fuseMax, &rowsL, runRowsMax);
             }
@@ -437,8 +513,28 @@ struct IluDev {
             std::vector<int> sh, lo;
             fShortStart.assign(1, 0);
             fLongStart.assign(1, 0);
+            // factorisation levels: the sweep levels, except that a red / black tile level is factorised by local level
+            std::vector<std::vector<int>> flev;
+            fLevel0.assign(1, 0);
             for (int l = 0; l < nl; ++l) {
-                for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) {
+                if (rbt && l < 2) {
+                    const int L0 = l == 0 ? 0 : rb.redLevels, L1 = l == 0 ? rb.redLevels : rb.redLevels + rb.blackLevels;
+                    const size_t base = flev.size();
+                    flev.resize(base + size_t(L1 - L0));
+                    for (size_t u = 0; u + 1 < rb.unitStart.size(); ++u) {
+                        const int fl = rb.unitLevel[u];
+                        if (fl < L0 || fl >= L1) continue;
+                        for (int w = rb.unitStart[u]; w < rb.unitStart[u + 1]; ++w) flev[base + size_t(fl - L0)].push_back(w);
+                    }
+                } else {
+                    std::vector<int> rows;
+                    for (int w = lstart[size_t(l)]; w < lstart[size_t(l) + 1]; ++w) rows.push_back(w);
+                    flev.push_back(std::move(rows));
+                }
+                fLevel0.push_back(int(flev.size()));
+            }
+            for (const std::vector<int>& fr : flev) {
+                for (int w : fr) {
                     const int i = mc ? w : ord[size_t(w)];
                     const int lower = F.diag[size_t(i)] - F.rp[size_t(i)];
                     // natural order: a row whose pivot rows carry long upper parts (the neighbours of a well's perforated
@@ -641,7 +737,7 @@ struct IluDev {
     }
     // newRunRows > 0: also the rows a fused run may hold (its working vector's LDS tile, at most kRunRowsMax)
     void retune(int newFuseMax, hipStream_t s, int newRunRows = 0) {
-        if (!keepEntries) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a schedule built with keepEntries");
+        if (!keepEntries || rbt) throw Error(RSH_ERR_ARG, "level chunking can only be retuned on a batch member's schedule");
         fuseMax = newFuseMax;
         if (newRunRows > 0) runRowsMax = std::min(newRunRows, kRunRowsMax);
         runsL = levelRuns(workL, fuseMax, &rowsPerLevel, runRowsMax);
@@ -682,10 +778,11 @@ struct IluDev {
                 launchIluGroupFactor(s, nb, fgRun0[ri], fgRun1[ri], fgStart.p, fgRows.p, fgIdxOff.p, fgIdx.p, diag, lu.p, nreg.p, rp, lean);
                 continue;
             }
-            for (int l = run.l0; l < run.l1; ++l) {
-                launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(l)], fShortStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, -1, lean);
-                launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(l)], fLongStart[size_t(l) + 1], rp, ci, diag, lu.p, nreg.p, lean);
-            }
+            for (int l = run.l0; l < run.l1; ++l)
+                for (int fl = fLevel0[size_t(l)]; fl < fLevel0[size_t(l) + 1]; ++fl) {
+                    launchBilu0Level(s, nb, fShort.p, fShortStart[size_t(fl)], fShortStart[size_t(fl) + 1], rp, ci, diag, lu.p, nreg.p, -1, lean);
+                    launchBilu0LongRows(s, nb, fLong.p, fLongStart[size_t(fl)], fLongStart[size_t(fl) + 1], rp, ci, diag, lu.p, nreg.p, lean);
+                }
         }
         packSlices(s, nb, lu.p, L64, hL.nslots, false);
         packSlices(s, nb, lu.p, U64, hU.nslots, false);
@@ -743,6 +840,11 @@ struct IluDev {
             if (run.fused) { sweepRun(run, ri, true); continue; }
             const int l = run.l0;
             if (l == 0 && firstDone) continue;   // the caller wrote y of the first level
+            if (rbt && l < 2) {
+                const int t0 = l == 0 ? 0 : rbRedTiles, t1 = l == 0 ? rbRedTiles : rbTiles;
+                launchIluTileWave(s, nb, fp, Lp, true, t1 - t0, tilesL.p + t0, r, y, nullptr, nullptr);
+                continue;
+            }
             launchIluForward(s, nb, fp, Lp, sL.sStart[size_t(l)], sL.sStart[size_t(l) + 1], sL.qStart[size_t(l)], sL.qStart[size_t(l) + 1], r, y,
                              gate, fuseBwd ? z : nullptr, xp);
         }
@@ -750,6 +852,11 @@ struct IluDev {
             const LevelRun& run = runsU[k];
             if (run.fused) { sweepRun(run, k, false); continue; }
             const int l = run.l0;
+            if (rbt && l < 2) {
+                const int t0 = l == 0 ? 0 : rbRedTiles, t1 = l == 0 ? rbRedTiles : rbTiles;
+                launchIluTileWave(s, nb, fp, Up, false, t1 - t0, tilesU.p + t0, nullptr, y, z, xp);
+                continue;
+            }
             launchIluBackward(s, nb, fp, Up, sU.sStart[size_t(l)], sU.sStart[size_t(l) + 1], sU.qStart[size_t(l)], sU.qStart[size_t(l) + 1], y, z,
                               xp, gate, k != 0);
         }

@@ -108,6 +108,93 @@ void run(const Input& in, int id) {
     for (int v : hL.src) placed += v >= 0;
     for (int v : hL.lsrc) placed += v >= 0;
     check(placed == (long long)EL.col.size() && sL.levels() == int(start.size()) - 1, "lower slices hold every entry once", id);
+    // red / black tiles: a permutation; a bulk row's lower entries lie in its own tile on an earlier unit (red and
+    // black) or in a red tile (black rows); far rows come last; every lower entry of a bulk row has a smaller
+    // factorisation level; the tile units' slices hold no long rows
+    static const int rbDims[3][3] = {{4, 4, 4}, {8, 8, 2}, {3, 5, 2}};
+    int rbOrders = 0;
+    for (const int* dims : rbDims) {
+        RbTiles R;
+        if (!rbTileOrder(A, dims, R)) continue;
+        ++rbOrders;
+        const int n = A.n;
+        bool ok = int(R.ord.size()) == n && R.nRed > 0 && R.nRed < R.nTile && R.nTile <= n && R.unitStart.front() == 0 &&
+                  R.unitStart.back() == R.nTile && int(R.unitLevel.size()) == int(R.unitStart.size()) - 1 &&
+                  R.tileUnit.back() == int(R.unitStart.size()) - 1 && R.farStart.back() == n - R.nTile;
+        std::vector<int> seen(size_t(n), 0), wOf(size_t(n), -1), unitOf(size_t(n), -1), tileOf(size_t(n), -1);
+        for (int w = 0; w < n && ok; ++w) {
+            ok = R.ord[size_t(w)] >= 0 && R.ord[size_t(w)] < n && ++seen[size_t(R.ord[size_t(w)])] == 1;
+            if (ok) wOf[size_t(R.ord[size_t(w)])] = w;
+        }
+        for (size_t t = 0; t + 1 < R.tileUnit.size() && ok; ++t)
+            for (int u = R.tileUnit[t]; u < R.tileUnit[t + 1]; ++u)
+                for (int w = R.unitStart[size_t(u)]; w < R.unitStart[size_t(u) + 1]; ++w) {
+                    unitOf[size_t(w)] = u;
+                    tileOf[size_t(w)] = int(t);
+                }
+        for (int w = 0; w < R.nTile && ok; ++w) {
+            const int i = R.ord[size_t(w)];
+            ok = unitOf[size_t(w)] >= 0 && (w < R.nRed) == (tileOf[size_t(w)] < R.redTiles);
+            for (int p = A.rp[size_t(i)]; p < A.rp[size_t(i) + 1] && ok; ++p) {
+                const int c = wOf[size_t(A.ci[size_t(p)])];
+                if (c >= w) continue;
+                const bool sameTile = tileOf[size_t(c)] == tileOf[size_t(w)];
+                ok = sameTile ? unitOf[size_t(c)] < unitOf[size_t(w)] : (w >= R.nRed && c < R.nRed);
+                ok = ok && R.unitLevel[size_t(unitOf[size_t(c)])] < R.unitLevel[size_t(unitOf[size_t(w)])];
+            }
+        }
+        check(ok, "red / black tile order", id);
+        // the tile sweeps' slices as the ILU builds them: one range of slices per tile, every row of the range in the
+        // tile's working rows, no long rows among the tile units, every slot's column inside the vector (or the zero row n)
+        HostBsr PR;
+        std::vector<int> bmR;
+        permutePattern(A, R.ord, PR, bmR);
+        RowEntries RL, RU;
+        RL.ptr.assign(1, 0);
+        RU.ptr.assign(1, 0);
+        for (int w = 0; w < n; ++w) {
+            for (int p = PR.rp[size_t(w)]; p < PR.diag[size_t(w)]; ++p) { RL.col.push_back(PR.ci[size_t(p)]); RL.src.push_back(p); }
+            for (int p = PR.diag[size_t(w)] + 1; p < PR.rp[size_t(w) + 1]; ++p) { RU.col.push_back(PR.ci[size_t(p)]); RU.src.push_back(p); }
+            RL.ptr.push_back(int(RL.col.size()));
+            RU.ptr.push_back(int(RU.col.size()));
+            RU.dsrc.push_back(PR.diag[size_t(w)]);
+        }
+        RL.orig = R.ord;
+        RU.orig = R.ord;
+        std::vector<int> bounds = R.unitStart;
+        for (size_t k = 1; k < R.farStart.size(); ++k) bounds.push_back(R.nTile + R.farStart[k]);
+        for (const RowEntries* E : {&RL, &RU}) {
+            std::vector<int> lens(static_cast<size_t>(n));
+            for (int w = 0; w < n; ++w) lens[size_t(w)] = E->ptr[size_t(w) + 1] - E->ptr[size_t(w)];
+            int cap = rowCap(lens);
+            for (int w = 0; w < R.nTile; ++w) cap = std::max(cap, lens[size_t(w)]);
+            SliceHost h;
+            LevelSched u;
+            buildSlices(*E, bounds, cap, h, u);
+            const int nu = int(R.unitStart.size()) - 1;
+            bool okS = int(u.sStart.size()) == int(bounds.size());
+            for (int k = 0; k < nu && okS; ++k) okS = u.qStart[size_t(k)] == u.qStart[size_t(k) + 1];
+            for (size_t t = 0; t + 1 < R.tileUnit.size() && okS; ++t) {
+                const int s0 = u.sStart[size_t(R.tileUnit[t])], s1 = u.sStart[size_t(R.tileUnit[t + 1])];
+                const int r0 = R.unitStart[size_t(R.tileUnit[t])], r1 = R.unitStart[size_t(R.tileUnit[t + 1])];
+                int rows = 0;
+                okS = s0 >= 0 && s0 <= s1 && s1 <= h.nslices;
+                for (int sl = s0; sl < s1 && okS; ++sl)
+                    for (int lane = 0; lane < kSliceRows && okS; ++lane) {
+                        const int w = h.row[size_t(sl) * kSliceRows + size_t(lane)];
+                        if (w < 0) continue;
+                        ++rows;
+                        okS = w >= r0 && w < r1;
+                        for (int k = 0; k < h.width[size_t(sl)] && okS; ++k) {
+                            const int c = h.col[(size_t(h.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane)];
+                            okS = c >= 0 && c <= n;
+                        }
+                    }
+                okS = okS && rows == r1 - r0;
+            }
+            check(okS, "red / black tile slices", id);
+        }
+    }
     std::vector<int> work(size_t(sL.levels())), rows(size_t(sL.levels()));
     for (int l = 0; l < sL.levels(); ++l) {
         work[size_t(l)] = sL.sStart[size_t(l) + 1] - sL.sStart[size_t(l)] + sL.qStart[size_t(l) + 1] - sL.qStart[size_t(l)];
@@ -240,8 +327,8 @@ void run(const Input& in, int id) {
         wellplan::recover(W, in.jci.data(), jv, dInv, b.data(), bc.data(), x.data());
         fill = W.fill();
     }
-    std::printf("system %d: n %d nb %d levels %d colours %d L slices %d AMG levels %zu well fill %d grid offsets %d tile levels (4^3) %d\n", id,
-                A.n, A.nb, sL.levels(), ncol, hL.nslices, H.size(), fill, nd, ntl);
+    std::printf("system %d: n %d nb %d levels %d colours %d L slices %d AMG levels %zu well fill %d grid offsets %d tile levels (4^3) %d "
+                "red/black tile orders %d\n", id, A.n, A.nb, sL.levels(), ncol, hL.nslices, H.size(), fill, nd, ntl, rbOrders);
 }
 }  // namespace
 
