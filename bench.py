@@ -211,9 +211,12 @@ def batched_leg(rs, gen, args):
                      "outer": info["last_outer"], "max_rel_res": float(rel.max()), "all_converged": bool((st == 0).all()),
                      "chunk": info["chunk"], "fuse_max": [g.get("fuse_max") for g in info["groups"]],
                      "run_rows": [g.get("run_rows") for g in info["groups"]],
+                     "spmv_lanes": [g.get("spmv_lanes") for g in info["groups"]],
+                     "spmv_lanes_ms": [g.get("spmv_lanes_ms") for g in info["groups"]],
                      "amg": [[g.get("amg_levels"), g.get("amg_coarsest")] for g in info["groups"]] if pc.startswith("cpr") else None}
         log(f"{name}: {m / dt:.1f} solves/s ({dt * 1e3:.1f} ms per batch: setup {info['last_setup_ms']:.1f} + solve {info['last_solve_ms']:.1f}), "
-            f"latency p50 {out[name]['latency_p50_ms']:.1f} ms p99 {out[name]['latency_p99_ms']:.1f} ms, its mean {its.mean():.0f} max {its.max()}")
+            f"latency p50 {out[name]['latency_p50_ms']:.1f} ms p99 {out[name]['latency_p99_ms']:.1f} ms, its mean {its.mean():.0f} max {its.max()}, "
+            f"SpMV lanes per row {out[name]['spmv_lanes']} (tuner ms for 1/2/4: {out[name]['spmv_lanes_ms']})")
     B.close()
 
     def loop_leg(name, o, ref):

@@ -163,6 +163,12 @@ int rsh_solve_device_noalloc(rsh_system* sys, const double* b_dev, double* x_dev
 
 /* ---- kernel-level entry points (host vectors), for parity checks against BSR::spmv / Preconditioner::apply */
 int rsh_spmv(rsh_system* sys, const double* x, double* y);
+/* Rows per wavefront of the FP64 operator's SpMV (no counterpart in the reference, whose BSR::spmv, bsr.cpp:109-126,
+ * walks one row per thread): lanes = 1 (64 rows per wavefront, the default), 2 or 4 (32 or 16 rows, each row's
+ * blocks split over its lanes and their partial sums added pairwise; another summation order, within rounding of
+ * lanes = 1).  Used by rsh_spmv and the Krylov solves; the FP32 operator and the CPR second stage keep one lane per
+ * row.  Takes another copy of the matrix values in HBM while lanes > 1. */
+int rsh_system_set_spmv_lanes(rsh_system* sys, int lanes);
 int rsh_precond_apply(rsh_system* sys, const double* r, double* z);
 int rsh_amg_apply(rsh_system* sys, const double* r, double* z);          /* CPR only: one V-cycle, n doubles */
 int rsh_cpr_weights(rsh_system* sys, double* w);                         /* CPR only: n*nb quasi-IMPES weights */
@@ -225,6 +231,11 @@ int rsh_batch_solve(rsh_batch* batch, const double* const* b, double* const* x, 
 int rsh_batch_solve_device(rsh_batch* batch, const double* b_dev, double* x_dev, const double* tol, int max_it, int* iterations,
                            double* rel_res, int* status);
 int rsh_batch_latencies(rsh_batch* batch, double* ms);   /* per system: ms from the solve's start to its convergence */
+/* Rows per wavefront of each block-size group's SpMV (no counterpart in the reference): 0 = chosen by the batch
+ * autotuner at the next rsh_batch_setup (1, 2 or 4 lanes per row timed on the group's resident operator, cached per
+ * block size and size bucket; the default), or 1, 2, 4 fixed (see rsh_system_set_spmv_lanes).  Applies at the next
+ * rsh_batch_setup; rsh_batch_info reports each group's "spmv_lanes". */
+int rsh_batch_set_spmv_lanes(rsh_batch* batch, int lanes);
 int rsh_batch_info(rsh_batch* batch, char* json, int cap);
 
 /* ---- serving loop for ensemble callers (no counterpart in the reference; its ensemble drivers runBatch / runESMDA,

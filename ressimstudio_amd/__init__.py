@@ -42,7 +42,8 @@ EXPORTED = [
     "rsh_batch_set_values_device", "rsh_batch_setup", "rsh_batch_solve", "rsh_batch_solve_device", "rsh_batch_latencies",
     "rsh_batch_info", "rsh_server_create", "rsh_server_create_batched", "rsh_server_destroy", "rsh_server_add_model", "rsh_server_submit", "rsh_server_wait",
     "rsh_server_stats", "rsh_wells_create", "rsh_wells_destroy", "rsh_wells_system", "rsh_wells_set_values", "rsh_wells_solve",
-    "rsh_wells_reduced_values", "rsh_context_device_seconds", "rsh_solve_device_noalloc",
+    "rsh_wells_reduced_values", "rsh_context_device_seconds", "rsh_solve_device_noalloc", "rsh_system_set_spmv_lanes",
+    "rsh_batch_set_spmv_lanes",
 ]
 
 
@@ -101,6 +102,8 @@ def lib():
         L.rsh_solve_device.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.rsh_solve_device_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.rsh_spmv.argtypes = [_vp, _dp, _dp]
+        L.rsh_system_set_spmv_lanes.argtypes = [_vp, ctypes.c_int]
+        L.rsh_batch_set_spmv_lanes.argtypes = [_vp, ctypes.c_int]
         L.rsh_precond_apply.argtypes = [_vp, _dp, _dp]
         L.rsh_amg_apply.argtypes = [_vp, _dp, _dp]
         L.rsh_cpr_weights.argtypes = [_vp, _dp]
@@ -268,6 +271,11 @@ class System:
         _check(lib().rsh_spmv(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double)), "rsh_spmv")
         return y
 
+    def set_spmv_lanes(self, lanes: int):
+        """Rows per wavefront of the FP64 SpMV: 1, 2 or 4 lanes per row (64, 32 or 16 rows)."""
+        _check(lib().rsh_system_set_spmv_lanes(self.h, int(lanes)), "rsh_system_set_spmv_lanes")
+        return self
+
     def precond(self, r):
         r = np.ascontiguousarray(r, np.float64)
         z = np.empty(self.N)
@@ -373,6 +381,11 @@ class Batch:
         if rc < 0 and (raise_on_fail or rc not in (ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_NONFINITE)):
             _check(rc, "rsh_batch_solve")
         return xs, its, rel, st
+
+    def set_spmv_lanes(self, lanes: int):
+        """SpMV lanes per row of every group from the next setup: 0 = autotuned (default), or 1, 2, 4."""
+        _check(lib().rsh_batch_set_spmv_lanes(self.h, int(lanes)), "rsh_batch_set_spmv_lanes")
+        return self
 
     def latencies(self):
         ms = np.zeros(len(self.sizes))

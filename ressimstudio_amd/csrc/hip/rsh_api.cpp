@@ -681,6 +681,17 @@ int rsh_solve_device_noalloc(rsh_system* sys, const double* b_dev, double* x_dev
     });
 }
 
+int rsh_system_set_spmv_lanes(rsh_system* sys, int lanes) {
+    return guarded([&] {
+        if (!sys) return fail(RSH_ERR_ARG, "null system");
+        if (lanes != 1 && lanes != 2 && lanes != 4) return fail(RSH_ERR_ARG, "lanes per row must be 1, 2 or 4");
+        RSH_HIP(hipSetDevice(sys->ctx->c.device));
+        sys->setSpmvLanes(lanes);
+        RSH_HIP(hipStreamSynchronize(sys->s()));
+        return RSH_OK;
+    });
+}
+
 int rsh_spmv(rsh_system* sys, const double* x, double* y) {
     return guarded([&] {
         if (!sys || !x || !y) return fail(RSH_ERR_ARG, "null argument");
@@ -783,7 +794,7 @@ int rsh_system_info(rsh_system* sys, char* json, int cap) {
         std::ostringstream o;
         o << "{\"n\":" << S.n << ",\"nb\":" << S.nb << ",\"nnzb\":" << S.nnzb << ",\"spmv_slices\":" << S.spmvHost.nslices
           << ",\"spmv_slots\":" << S.spmvHost.nslots << ",\"spmv_long_rows\":" << S.spmvHost.lrow.size()
-          << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0) << ",\"cpr_true_impes\":" << (S.acc.n ? 1 : 0)
+          << ",\"spmv_col16\":" << (S.spmv64.dcol.n ? 1 : 0) << ",\"spmv_lanes\":" << S.spmvLanes() << ",\"cpr_true_impes\":" << (S.acc.n ? 1 : 0)
           << ",\"cpr_impes_fallbacks\":" << S.impesFallbacks << ",\"device_bytes\":" << S.deviceBytes();
         if (S.ilu) {
             const IluDev& I = *S.ilu;

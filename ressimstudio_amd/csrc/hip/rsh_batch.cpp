@@ -31,6 +31,7 @@ struct rsh_batch {
         long long off = 0, N = 0;
         rsh_system* S = nullptr;
         std::vector<int> members;
+        std::vector<double> lanesMs;   // the autotuner's SpMV time per candidate (1, 2, 4 lanes per row), ms
     };
     std::vector<Sys> sys;
     std::vector<Group> groups;
@@ -48,6 +49,8 @@ struct rsh_batch {
     rsh_options opt{};
     bool ready = false, tuned = false;
     std::vector<int> fuseChosen;
+    int lanesReq = 0;             // SpMV lanes per row: 0 = tuned per group at the next set-up, else 1, 2 or 4
+    bool lanesDone = false;       // the groups' SpMV layouts follow lanesReq
     hipGraphExec_t iter = nullptr;
     std::vector<int> graphKey;
     // last solve
@@ -340,6 +343,7 @@ struct rsh_batch {
         return double(ms) / reps;
     }
     void tune();
+    void tuneLanes();
 };
 
 namespace {
@@ -349,6 +353,7 @@ namespace {
 std::mutex g_tuneMu;
 std::map<std::pair<int, int>, std::pair<int, int>> g_fuseCache;   // -> (fuseMax, run rows)
 std::map<int, int> g_chunkCache;
+std::map<std::pair<int, int>, std::pair<int, std::vector<double>>> g_lanesCache;   // -> SpMV lanes per row, ms per candidate
 
 int log2Bucket(long long v) {
     int b = 0;
@@ -427,8 +432,44 @@ void rsh_batch::tune() {
             if (chunk != best) buildTasks(best);
         }
     }
+    tuneLanes();
     RSH_HIP(hipStreamSynchronize(st));
     tuned = true;
+}
+
+// (3) the rows per wavefront of each group's SpMV: 64, 32 or 16 (1, 2 or 4 lanes per row, laneSliceLayout), timed on
+// the group's resident operator (ungated) and cached per (block size, size bucket); or the caller's fixed choice.
+void rsh_batch::tuneLanes() {
+    if (lanesDone) return;
+    hipStream_t st = s();
+    bool changed = false;
+    for (auto& g : groups) {
+        int best = lanesReq;
+        g.lanesMs.clear();
+        if (best == 0) {
+            const auto key = std::make_pair(g.nb, log2Bucket(g.S->n));
+            auto it = g_lanesCache.find(key);
+            if (it != g_lanesCache.end()) {
+                best = it->second.first;
+                g.lanesMs = it->second.second;
+            } else {
+                launchFill(st, p.p + g.off, 1.0, g.N);
+                double bestMs = 1e300;
+                for (int cand : {1, 2, 4}) {
+                    if (g.S->spmvLanes() != cand) changed = true;
+                    g.S->setSpmvLanes(cand);
+                    const double ms = timeMs([&] { g.S->spmv(p.p + g.off, ph.p + g.off, false, false); }, 5);
+                    g.lanesMs.push_back(ms);
+                    if (ms < bestMs) { bestMs = ms; best = cand; }
+                }
+                g_lanesCache[key] = {best, g.lanesMs};
+            }
+        }
+        if (g.S->spmvLanes() != best) changed = true;
+        g.S->setSpmvLanes(best);
+    }
+    if (changed) dropGraph();
+    lanesDone = true;
 }
 
 extern "C" {
@@ -676,6 +717,19 @@ int rsh_batch_latencies(rsh_batch* batch, double* ms) {
     });
 }
 
+int rsh_batch_set_spmv_lanes(rsh_batch* batch, int lanes) {
+    return guarded([&] {
+        if (!batch) return fail(RSH_ERR_ARG, "null batch");
+        if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4) return fail(RSH_ERR_ARG, "lanes per row must be 0 (tuned), 1, 2 or 4");
+        if (lanes != batch->lanesReq) {
+            batch->lanesReq = lanes;
+            batch->lanesDone = false;
+            batch->ready = false;                    // applied by the next rsh_batch_setup
+        }
+        return RSH_OK;
+    });
+}
+
 int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
     return guarded([&] {
         if (!batch || !json || cap <= 0) return fail(RSH_ERR_ARG, "null argument");
@@ -690,6 +744,9 @@ int rsh_batch_info(rsh_batch* batch, char* json, int cap) {
                   << g.S->ilu->runRowsMax << ",\"dense_groups\":["
                   << g.S->ilu->denseL.ngroups << "," << g.S->ilu->denseU.ngroups << "]";
             }
+            o << ",\"spmv_lanes\":" << g.S->spmvLanes();
+            if (!g.lanesMs.empty())
+                o << ",\"spmv_lanes_ms\":[" << g.lanesMs[0] << "," << g.lanesMs[1] << "," << g.lanesMs[2] << "]";
             if (g.S->cpr && g.S->cpr->built)
                 o << ",\"amg_levels\":" << g.S->cpr->lv.size() << ",\"amg_coarsest\":" << g.S->cpr->m << ",\"amg_dense_coarsest\":"
                   << (g.S->cpr->dense ? 1 : 0);

@@ -118,6 +118,15 @@ struct SliceDev {
     DBuf<int> owner, lowner;
     // coarse AMG levels: the rows' sums are formed in G chains folded like the G-lane CSR step (0: one sum per row)
     int chains = 0;
+    int lanes = 1;                 // lanes per row of the SpMV slices (laneSliceLayout)
+    void release() {
+        for (DBuf<int>* b : {&row, &orig, &col, &src, &dsrc, &lrow, &lorig, &lptr, &lcol, &lsrc, &ldsrc, &owner, &lowner}) b->release();
+        info.release();
+        dcol.release();
+        for (DBuf<TA>* b : {&val, &dval, &lval, &ldval}) b->release();
+        nslices = nlong = chains = 0;
+        lanes = 1;
+    }
     size_t bytes() const {
         return info.bytes() + row.bytes() + orig.bytes() + col.bytes() + dcol.bytes() + src.bytes() + dsrc.bytes() + val.bytes() + dval.bytes() +
                lrow.bytes() + lorig.bytes() + lptr.bytes() + lcol.bytes() + lsrc.bytes() + ldsrc.bytes() + lval.bytes() + ldval.bytes();
@@ -171,6 +180,8 @@ void buildSlices(const RowEntries& E, const std::vector<int>& levelStart, int ca
 // AMG level): the bulk ends a quarter above the 99th percentile of the row lengths, and rows with a column beyond
 // 16-bit reach are long rows too
 void rowSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, SliceHost& h, bool coarse = false);
+// the same rows in slices of 64 / lanes rows, `lanes` lanes per row (1, 2 or 4) splitting each row's entries
+void laneSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, int lanes, SliceHost& h);
 
 // Tiled schedule of the natural-order triangular sweeps.  Rows are grouped into units: the rows of one tile of the grid
 // (boxes along the pattern's dominant lower offsets) that share a tile level, where tileLevel(i) is the maximum over

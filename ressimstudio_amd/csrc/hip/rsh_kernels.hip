@@ -160,8 +160,10 @@ __device__ inline int slotCol(const CT* __restrict__ col, long long at, int r) {
 
 // ---------------------------------------------------------------- BSR SpMV over slices (+ long rows)
 // y_i = sum_p A_p x_col(p) (bsr.cpp:109-126); per lane the blocks are visited in column order and each block
-// product is formed as s_e = sum_f a_ef x_f before y_e += s_e, as gemvAdd does.
-template <int NB, typename TA, typename CT>
+// product is formed as s_e = sum_f a_ef x_f before y_e += s_e, as gemvAdd does.  LN > 1 (laneSliceLayout): LN lanes
+// per row, each summing every LN-th block of the row in that way, then the LN partial sums added pairwise (lanes
+// xor 1, then xor 2), which every lane of the row ends with.
+template <int NB, typename TA, typename CT, int LN = 1>
 __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* __restrict__ info, const int* __restrict__ row,
                                                       const CT* __restrict__ col, const TA* __restrict__ val, int nlong,
                                                       const int* __restrict__ lrow, const int* __restrict__ lptr,
@@ -222,7 +224,13 @@ __global__ __launch_bounds__(kBlock) void kSpmvSliced(int nslices, const int2* _
                 acc[e] += s;
             }
         }
-        if (r >= 0) {
+        if constexpr (LN > 1) {
+#pragma unroll
+            for (int m = 1; m < LN; m <<= 1)
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[e] += __shfl_xor(acc[e], m, 64);
+        }
+        if (r >= 0 && (lane & (LN - 1)) == 0) {
 #pragma unroll
             for (int e = 0; e < NB; ++e) y[(long long)r * NB + e] = acc[e];
         }
@@ -2723,17 +2731,23 @@ __global__ void kMultiAxpy(long long n, int k, const double* __restrict__ V, lon
     }
 }
 
-template <int NB, typename TA>
-void spmvLayoutT(hipStream_t s, const SliceDev<TA>& d, const double* x, double* y, const KScalars* gate) {
-    const int waves = d.nslices + d.nlong;
-    if (!waves) return;
-    const dim3 grid((waves + 3) / 4);
+template <int NB, typename TA, int LN>
+void spmvLanesT(hipStream_t s, const SliceDev<TA>& d, const double* x, double* y, const KScalars* gate) {
+    const dim3 grid((d.nslices + d.nlong + 3) / 4);
     if (d.dcol.n)
-        hipLaunchKernelGGL((kSpmvSliced<NB, TA, short>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.dcol.p, d.val.p, d.nlong,
+        hipLaunchKernelGGL((kSpmvSliced<NB, TA, short, LN>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.dcol.p, d.val.p, d.nlong,
                            d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
     else
-        hipLaunchKernelGGL((kSpmvSliced<NB, TA, int>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p, d.val.p, d.nlong,
+        hipLaunchKernelGGL((kSpmvSliced<NB, TA, int, LN>), grid, dim3(kBlock), 0, s, d.nslices, d.info.p, d.row.p, d.col.p, d.val.p, d.nlong,
                            d.lrow.p, d.lptr.p, d.lcol.p, d.lval.p, x, y, d.owner.p, d.lowner.p, gate);
+}
+template <int NB, typename TA>
+void spmvLayoutT(hipStream_t s, const SliceDev<TA>& d, const double* x, double* y, const KScalars* gate) {
+    if (!(d.nslices + d.nlong)) return;
+    if (d.lanes == 4) spmvLanesT<NB, TA, 4>(s, d, x, y, gate);
+    else if (d.lanes == 2) spmvLanesT<NB, TA, 2>(s, d, x, y, gate);
+    else if (d.lanes == 1) spmvLanesT<NB, TA, 1>(s, d, x, y, gate);
+    else throw Error(-1, "SpMV: lanes per row must be 1, 2 or 4");
 }
 template <int NB>
 void spmvT(hipStream_t s, bool f32, const SliceDev<double>* d64, const SliceDev<float>* d32, const double* x, double* y, const KScalars* gate) {

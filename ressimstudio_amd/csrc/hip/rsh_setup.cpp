@@ -266,6 +266,67 @@ void rowSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& c
     h.dcol = std::move(d);
 }
 
+// Row slices with `lanes` lanes per row (1, 2 or 4): a slice holds 64 / lanes rows in natural order, lane
+// j * lanes + p of the slice takes entries p, p + lanes, p + 2 lanes, ... of the slice's j-th row (slot k holds entry
+// k * lanes + p), so the slice is ceil(longest row / lanes) slots wide; the lanes' partial sums are added afterwards.
+// Rows longer than the bulk are long rows, as in rowSliceLayout; padding points at the lane's own row (idle lanes at
+// row 0) with source -1 (a zero block).
+void laneSliceLayout(int n, const std::vector<int>& rp, const std::vector<int>& ci, int lanes, SliceHost& h) {
+    if (lanes != 1 && lanes != 2 && lanes != 4) throw Error(-1, "slice layout: lanes per row must be 1, 2 or 4");
+    std::vector<int> len(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) len[size_t(i)] = rp[size_t(i) + 1] - rp[size_t(i)];
+    const int cap = rowCap(len);
+    const int per = kSliceRows / lanes;
+    h = SliceHost{};
+    h.lptr.push_back(0);
+    std::vector<int> bulk;
+    bulk.reserve(size_t(n));
+    for (int i = 0; i < n; ++i) {
+        if (len[size_t(i)] > cap) {
+            h.lrow.push_back(i);
+            h.lorig.push_back(i);
+            for (int q = rp[size_t(i)]; q < rp[size_t(i) + 1]; ++q) { h.lcol.push_back(ci[size_t(q)]); h.lsrc.push_back(q); }
+            h.lptr.push_back(int(h.lcol.size()));
+        } else bulk.push_back(i);
+    }
+    bool rel16 = true;
+    std::vector<short> d;
+    for (size_t c0 = 0; c0 < bulk.size(); c0 += size_t(per)) {
+        const size_t c1 = std::min(bulk.size(), c0 + size_t(per));
+        int width = 0;
+        for (size_t t = c0; t < c1; ++t) width = std::max(width, (len[size_t(bulk[t])] + lanes - 1) / lanes);
+        h.width.push_back(width);
+        h.off.push_back(int(h.nslots));
+        const size_t base = h.row.size();
+        h.row.resize(base + kSliceRows, -1);
+        h.orig.resize(base + kSliceRows, -1);
+        const size_t cbase = h.col.size();
+        h.col.resize(cbase + size_t(width) * kSliceRows, 0);
+        h.src.resize(cbase + size_t(width) * kSliceRows, -1);
+        d.resize(cbase + size_t(width) * kSliceRows, short(1));    // idle lanes: column 0 relative to row -1
+        for (size_t t = c0; t < c1; ++t) {
+            const int w = bulk[t], p0 = rp[size_t(w)];
+            for (int part = 0; part < lanes; ++part) {
+                const int lane = int(t - c0) * lanes + part;
+                h.row[base + size_t(lane)] = w;
+                h.orig[base + size_t(lane)] = w;
+                for (int k = 0; k < width; ++k) {
+                    const size_t at = cbase + size_t(k) * kSliceRows + size_t(lane);
+                    const int e = k * lanes + part;
+                    if (e < len[size_t(w)]) { h.col[at] = ci[size_t(p0 + e)]; h.src[at] = p0 + e; }
+                    else h.col[at] = w;
+                    const long long delta = (long long)h.col[at] - w;
+                    if (delta < -32768 || delta > 32767) rel16 = false;
+                    else d[at] = short(delta);
+                }
+            }
+        }
+        h.nslots += width;
+        h.nslices++;
+    }
+    if (rel16) h.dcol = std::move(d);
+}
+
 // ------------------------------------------------------------------ aggregation AMG set-up (amg.cpp:137-252)
 namespace {
 void csrDiag(HostCsr& A) {

@@ -1234,6 +1234,9 @@ struct rsh_system {
     SliceDev<double> spmv64;
     SliceDev<float> spmv32;
     bool have32 = false;
+    // the FP64 operator in slices of several lanes per row (setSpmvLanes; empty: the one-lane slices above)
+    SliceHost spmvWHost;
+    SliceDev<double> spmvW64;
     DBuf<double> acc;                   // accumulation blocks for true-IMPES CPR weights (n*nb*nb; empty: quasi-IMPES)
     int impesFallbacks = 0;             // cells of the last set-up whose accumulation block fell back to quasi-IMPES
     DBuf<double> colP;                  // pressure column of S in the SpMV slice layout (CPR second stage)
@@ -1278,7 +1281,7 @@ struct rsh_system {
     void ensureVec(DBuf<double>& b) { if (b.n != size_t(N)) b.alloc(size_t(N)); }
     // device memory the system holds: pattern, values and their slice layouts, preconditioner, Krylov vectors
     size_t deviceBytes() const {
-        size_t b = val.bytes() + spmv64.bytes() + spmv32.bytes() + colP.bytes() + colP32.bytes() + bjInv.bytes() + bjInv32.bytes() + rp.bytes() +
+        size_t b = val.bytes() + spmv64.bytes() + spmv32.bytes() + spmvW64.bytes() + colP.bytes() + colP32.bytes() + bjInv.bytes() + bjInv32.bytes() + rp.bytes() +
                    ci.bytes() + diag.bytes();
         if (ilu) b += ilu->bytes();
         if (cpr) b += cpr->bytes();
@@ -1290,8 +1293,24 @@ struct rsh_system {
     void valuesChanged() {
         packSlices(s(), nb, val.p, spmv64, spmvHost.nslots, false);
         if (have32) packSlices(s(), nb, val.p, spmv32, spmvHost.nslots, true);
+        if (spmvW64.lanes > 1) packSlices(s(), nb, val.p, spmvW64, spmvWHost.nslots, false);
         valuesSet = true;
         ready = false;
+    }
+    int spmvLanes() const { return spmvW64.lanes > 1 ? spmvW64.lanes : 1; }
+    // The FP64 operator's SpMV with `lanes` lanes per row (1, 2 or 4): 64 / lanes rows per wavefront, each row's
+    // blocks split over its lanes (another summation order than one lane per row; the FP32 operator and the CPR
+    // second stage keep the one-lane slices).  Batch members' slices carry their systems for the gating.
+    void setSpmvLanes(int lanes) {
+        if (lanes == spmvLanes()) return;
+        spmvWHost = SliceHost{};
+        spmvW64.release();
+        if (lanes == 1) return;
+        laneSliceLayout(n, pat.rp, pat.ci, lanes, spmvWHost);
+        if (!rowSys.empty()) sliceOwners(spmvWHost, rowSys);
+        uploadSlices(spmvWHost, spmvW64, nb, s());
+        spmvW64.lanes = lanes;
+        if (valuesSet) packSlices(s(), nb, val.p, spmvW64, spmvWHost.nslots, false);
     }
 
     // Preconditioner applies are captured once per (input, output, precision) into a HIP graph and replayed: a
@@ -1336,7 +1355,7 @@ struct rsh_system {
 
     // gated: skip the rows of stopped batched systems (their outputs are then stale)
     void spmv(const double* x, double* y, bool f32, bool gated = true) {
-        launchSpmvSliced(s(), nb, f32, &spmv64, &spmv32, x, y, gated ? gate : nullptr);
+        launchSpmvSliced(s(), nb, f32, spmvW64.lanes > 1 ? &spmvW64 : &spmv64, &spmv32, x, y, gated ? gate : nullptr);
     }
     // preconditioner z = M^-1 r (device vectors)
     void precond(const double* rr, double* z, bool f32) {

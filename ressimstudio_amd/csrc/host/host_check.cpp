@@ -304,6 +304,41 @@ void run(const Input& in, int id) {
         }
         check(okS && ys == yc, "coarse-level row slices", id);
     }
+    // SpMV slices with 1, 2 or 4 lanes per row: each bulk row on `lanes` aligned lanes of one slice, entry e of the row
+    // on lane part e % lanes at slot e / lanes, padding a zero block at the row's own column, long rows whole
+    for (int lanes : {1, 2, 4}) {
+        SliceHost sh;
+        laneSliceLayout(A.n, A.rp, A.ci, lanes, sh);
+        std::vector<int> seen(size_t(A.n), 0);
+        bool okW = sh.nslices == int(sh.width.size()) && sh.row.size() == size_t(sh.nslices) * kSliceRows;
+        for (int sl = 0; sl < sh.nslices && okW; ++sl)
+            for (int g = 0; g < kSliceRows / lanes && okW; ++g) {
+                const int i = sh.row[size_t(sl) * kSliceRows + size_t(g * lanes)];
+                for (int part = 0; part < lanes && okW; ++part) {
+                    const int lane = g * lanes + part;
+                    okW = sh.row[size_t(sl) * kSliceRows + size_t(lane)] == i;
+                    for (int k = 0; k < sh.width[size_t(sl)] && okW; ++k) {
+                        const size_t at = (size_t(sh.off[size_t(sl)]) + size_t(k)) * kSliceRows + size_t(lane);
+                        const int c = sh.col[at];
+                        okW = c >= 0 && c < A.n && (sh.dcol.empty() || (i < 0 ? -1 : i) + sh.dcol[at] == c);
+                        if (i < 0) { okW = okW && sh.src[at] < 0; continue; }
+                        const int e = k * lanes + part, len = A.rp[size_t(i) + 1] - A.rp[size_t(i)];
+                        okW = okW && (e < len ? sh.src[at] == A.rp[size_t(i)] + e && c == A.ci[size_t(A.rp[size_t(i)] + e)] : sh.src[at] < 0 && c == i);
+                    }
+                }
+                if (i >= 0 && okW) {
+                    okW = seen[size_t(i)]++ == 0 && A.rp[size_t(i) + 1] - A.rp[size_t(i)] <= sh.width[size_t(sl)] * lanes;
+                }
+            }
+        for (size_t q = 0; q < sh.lrow.size() && okW; ++q) {
+            const int i = sh.lrow[q];
+            okW = i >= 0 && i < A.n && seen[size_t(i)]++ == 0 && sh.lptr[q + 1] - sh.lptr[q] == A.rp[size_t(i) + 1] - A.rp[size_t(i)];
+            for (int u = sh.lptr[q]; u < sh.lptr[q + 1] && okW; ++u)
+                okW = sh.lsrc[size_t(u)] == A.rp[size_t(i)] + (u - sh.lptr[q]) && sh.lcol[size_t(u)] == A.ci[size_t(sh.lsrc[size_t(u)])];
+        }
+        for (int i = 0; i < A.n && okW; ++i) okW = seen[size_t(i)] == 1;
+        check(okW, lanes == 1 ? "SpMV slices, 1 lane per row" : lanes == 2 ? "SpMV slices, 2 lanes per row" : "SpMV slices, 4 lanes per row", id);
+    }
     // the well Schur plan of the Jacobian the system was reduced from: the same pattern and values
     int fill = 0;
     if (in.jac) {
