@@ -366,6 +366,11 @@ def cpu_sample(models, count, what):
                       f"one at a time"}
 
 
+# the CPR second stage of the variants that do not use the multicolour BILU0 (the workload text of a run led by them)
+SECOND_STAGE = {"cpr": "natural-order BILU0", "cpr_mixed": "natural-order BILU0", "cpr_exact_gs": "natural-order BILU0",
+                "cpr_bj": "block-Jacobi", "cpr_rbt": "BILU0 of red / black 10x10x6 tiles"}
+
+
 def serving_process(args):
     """The serving leg in a process of its own, as a serving loop runs (a child of this one, which keeps its own
     device allocations meanwhile): after the other legs in one process its saturation rate read about a quarter lower
@@ -620,7 +625,7 @@ def main():
         "config": {"workload": (f"engine Jacobian dump {os.path.basename(args.dump)}, {s.n} cells x nb {s.nb}" if args.dump else
                                 f"black-oil Jacobian {s.n} cells 60x220x85 x nb 3, 5 wells") + f"; variant {head}: BiCGSTAB to 1e-8 "
                                f"preconditioned by CPR ({'true' if acc is not None else 'quasi'}-IMPES weights, aggregation AMG V-cycle with two Jacobi steps (weight 1) per level) + "
-                               f"multicolour BILU0; numeric setup + solve per step",
+                               f"{SECOND_STAGE.get(head, 'multicolour BILU0')}; numeric setup + solve per step",
                    "cells": s.n, "block": s.nb, "blocks": s.nnzb, "tol": args.tol, "scale": args.scale},
         "iterations": main_r["iterations"],
         "variants": results,
