@@ -234,7 +234,8 @@ int rsh_batch_latencies(rsh_batch* batch, double* ms);   /* per system: ms from 
 /* Rows per wavefront of each block-size group's SpMV (no counterpart in the reference): 0 = chosen by the batch
  * autotuner at the next rsh_batch_setup (1, 2 or 4 lanes per row timed on the group's resident operator, cached per
  * block size and size bucket; the default), or 1, 2, 4 fixed (see rsh_system_set_spmv_lanes).  Applies at the next
- * rsh_batch_setup; rsh_batch_info reports each group's "spmv_lanes". */
+ * rsh_batch_setup (a changed choice makes rsh_batch_solve wait for that set-up: it is refused until then);
+ * rsh_batch_info reports each group's "spmv_lanes" and, when they were timed, "spmv_lanes_ms". */
 int rsh_batch_set_spmv_lanes(rsh_batch* batch, int lanes);
 int rsh_batch_info(rsh_batch* batch, char* json, int cap);
 
